@@ -1,0 +1,171 @@
+/*
+ * mbqc_hip.h - C ABI of the MI355X measurement-pattern simulator.
+ *
+ * Every entry point takes plain pointers and sizes.  Device pointers are
+ * device (HBM) addresses of complex64 data stored as interleaved float pairs;
+ * `stream` is a hipStream_t passed as void*.  Launches are stream-ordered and
+ * asynchronous; a plan is immutable after creation, so concurrent launches of
+ * one plan on different streams are safe.  Status codes are negative on error
+ * and the message is available from mbqc_last_error() (thread-local).
+ *
+ * The reference functions each entry point replaces are cited below as
+ * file:line into mentpy 0.1.0a9.
+ */
+#ifndef MBQC_HIP_H
+#define MBQC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  VALUE/TYPE/NOT_IMPLEMENTED map to the Python exceptions the
+ * reference raises for the same conditions (ValueError, TypeError,
+ * NotImplementedError); RUNTIME is a HIP runtime failure. */
+#define MBQC_OK 0
+#define MBQC_ERR_VALUE (-1)
+#define MBQC_ERR_TYPE (-2)
+#define MBQC_ERR_NOT_IMPLEMENTED (-3)
+#define MBQC_ERR_RUNTIME (-4)
+
+/* Launch flags. */
+#define MBQC_FLAG_REF_PHASE 1u    /* reference global phase: prod_k sgn(cos(t_k/2)) e^{i t_k/2} */
+#define MBQC_FLAG_RENORMALIZE 2u  /* divide the output by its norm (SV) / trace (DM) */
+#define MBQC_FLAG_WRITE_PHASE 4u  /* also write the per-sim global phase (SV forward) */
+
+/* Host description of a pattern; all arrays are host memory, copied by
+ * mbqc_plan_create.  Node ids are 0..n_nodes-1. */
+typedef struct mbqc_pattern_desc {
+  int32_t n_nodes;
+  int32_t n_edges;
+  const int32_t* edges;        /* [n_edges][2] undirected resource-graph edges */
+  const int32_t* schedule;     /* [n_nodes] full window order (measured nodes first) */
+  int32_t n_measure;           /* schedule[0..n_measure) are measured, in this order */
+  int32_t window;              /* live-window size w (>= 1) */
+  int32_t n_inputs;
+  const int32_t* input_nodes;  /* input-state qubit order, MSB first */
+  int32_t n_outputs;
+  const int32_t* output_nodes; /* output-state qubit order, MSB first */
+  const int32_t* angle_index;  /* [n_nodes] index into the angle vector, or -1 */
+  const float* fixed_angle;    /* [n_nodes] XY angle used when angle_index < 0 */
+  int32_t n_angles;            /* length of one angle vector */
+} mbqc_pattern_desc;
+
+typedef struct mbqc_plan mbqc_plan;
+
+typedef struct mbqc_plan_info {
+  int32_t n_slots;        /* register slots of the live state (w-1 or w) */
+  int32_t lag;            /* 1: appended qubit entangled before the oldest is measured */
+  int32_t n_steps;        /* measured nodes */
+  int32_t n_inputs, n_outputs, n_angles, n_nodes, window;
+  int32_t n_appends;      /* |+> qubits appended over the run (padding included) */
+  int32_t n_dropped_edges;/* edges never applied because the window was too small */
+  int32_t min_window;     /* smallest window that applies every edge */
+} mbqc_plan_info;
+
+/* Per-step table entry (16 bytes) as stored on the device. */
+typedef struct mbqc_step {
+  int32_t angle_index;    /* >= 0: angle vector entry; < 0: use fixed_angle */
+  uint32_t flags;         /* bit0 append, bit1 measured qubit is a CZ neighbour of
+                             the appended one, bits 8..15 measured slot */
+  uint32_t nmask;         /* slots that are CZ neighbours of the appended qubit */
+  float fixed_angle;
+} mbqc_step;
+
+/* Noise of the density-matrix path: single-qubit depolarizing `p_depol` on
+ * every qubit after all of its CZs, and a readout bit flip `p_readout` on
+ * every measured qubit (post-selected on reported outcome 0). */
+typedef struct mbqc_noise {
+  float p_depol;
+  float p_readout;
+} mbqc_noise;
+
+/* Compile a pattern into per-step tables; replaces the set-up of
+ * NumpySimulatorSV.__init__ / NumpySimulatorDM.__init__
+ * (mentpy/simulators/np_simulator_sv.py:38-128, np_simulator_dm.py:33-115):
+ * window, input reordering, |+> padding, initial CZ block.  Host only. */
+int mbqc_plan_create(const mbqc_pattern_desc* desc, mbqc_plan** out);
+void mbqc_plan_destroy(mbqc_plan* plan);
+int mbqc_plan_get_info(const mbqc_plan* plan, mbqc_plan_info* info);
+/* Copy host tables out (tests): steps[n_steps], in_index[2^n_slots],
+ * init_sign[2^n_slots] (0/1), out_pos[2^n_slots] (-1 for dead slots). */
+int mbqc_plan_get_tables(const mbqc_plan* plan, mbqc_step* steps, int32_t* in_index,
+                         int32_t* init_sign, int32_t* out_pos);
+/* Growth-control schedule (tests): per block of n_slots steps, whether the
+ * state is multiplied by 2^-32 at its start (SV and DM variants), and the
+ * final sqrt(2)-exponent (SV) / 2-exponent (DM) of the unnormalized state. */
+int mbqc_plan_get_rescale(const mbqc_plan* plan, int32_t* sv_flags, int32_t* dm_flags,
+                          int32_t* e_final);
+
+/* Statevector forward, complex64: replaces NumpySimulatorSV.run / measure /
+ * measure_ment (np_simulator_sv.py:164-297, 322-358) and the partial trace it
+ * calls (mentpy/calculator/state_ops.py:42-74).  Simulation g = b*S + s uses
+ * angle vector b (angles + b*angle_stride, angle_stride 0 = shared) and input
+ * s (psi_in + s*in_stride amplitudes, in_stride 0 = shared).  out is
+ * [B*S][2^n_outputs]; phase_out (optional, MBQC_FLAG_WRITE_PHASE) is [B*S]. */
+int mbqc_sv_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
+                        const void* psi_in, int64_t in_stride, void* out, void* phase_out,
+                        int64_t B, int64_t S, uint32_t flags, void* stream);
+
+/* Density-matrix forward, complex64: replaces NumpySimulatorDM.run / measure /
+ * measure_ment (np_simulator_dm.py:151-283, 307-346) and
+ * partial_trace_mixed_state (state_ops.py:77-119); the noise channels follow
+ * the placement of the PennyLane backend (pennylane_simulator.py:118-136).
+ * Inputs are state vectors (the reference purifies them, np_simulator_dm.py:89-92).
+ * out is [B*S][2^n_outputs][2^n_outputs]. */
+int mbqc_dm_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
+                        const void* psi_in, int64_t in_stride, void* out, int64_t B, int64_t S,
+                        const mbqc_noise* noise, uint32_t flags, void* stream);
+
+/* Adjoint angle gradient (replaces the 2n cost evaluations of fd_gradient /
+ * psr_gradient, mentpy/gradients/_finite_difference.py:9-24,
+ * _parameter_shift.py:9-25, behind get_gradient, grad.py:13-33).
+ * Forward pass: like mbqc_sv_forward_c64 in the raw phase frame, unnormalized,
+ * also storing one checkpoint of the live state per block of n_slots steps in
+ * `ckpt` (mbqc_sv_adjoint_workspace_bytes bytes).  Columns c = 0..C-1 of
+ * parameter set b are simulations b*C + c. */
+size_t mbqc_sv_adjoint_workspace_bytes(const mbqc_plan* plan, int64_t n_columns);
+int mbqc_sv_adjoint_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
+                                const void* psi_in, int64_t in_stride, void* out,
+                                int64_t B, int64_t C, void* ckpt, size_t ckpt_bytes, void* stream);
+/* Backward sweep: seed[B*C][2^n_outputs] is dL/d(conj out) in the convention
+ * dL = Re sum conj(seed) d(out).  Writes grad_partial[B][P][n_angles] with
+ * P = mbqc_sv_adjoint_partials(C); the gradient is the sum over P.
+ * Entries of non-trainable angles are left untouched. */
+int32_t mbqc_sv_adjoint_partials(int64_t C);
+int mbqc_sv_adjoint_backward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
+                                 const void* seed, int64_t B, int64_t C, const void* ckpt,
+                                 size_t ckpt_bytes, float* grad_partial, void* stream);
+
+/* Batched forward of many plans in one persistent launch (no reference
+ * counterpart: the reference runs one simulation per Python call).  Plans are
+ * first bound into a batch, which packs their tables on the device. */
+typedef struct mbqc_batch mbqc_batch;
+typedef struct mbqc_batch_job {
+  const mbqc_plan* plan;
+  const float* angles;   /* device [B][n_angles] */
+  int64_t angle_stride;
+  const void* psi_in;    /* device [S][2^n_inputs] */
+  int64_t in_stride;
+  void* out;             /* device [B*S][2^n_outputs] */
+  int64_t B, S;
+} mbqc_batch_job;
+int mbqc_batch_create(const mbqc_batch_job* jobs, int32_t n_jobs, int32_t sims_per_item,
+                      mbqc_batch** out);
+void mbqc_batch_destroy(mbqc_batch* batch);
+int64_t mbqc_batch_items(const mbqc_batch* batch);
+/* Launch the persistent kernel: `waves` resident wavefronts pull work items
+ * from a device counter (reset by a kernel, so the launch pair can be captured
+ * in a hipGraph), `wg_waves` wavefronts per workgroup. */
+int mbqc_batch_launch(mbqc_batch* batch, int32_t waves, int32_t wg_waves, uint32_t flags,
+                      void* stream);
+
+const char* mbqc_last_error(void);
+int32_t mbqc_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MBQC_HIP_H */

@@ -1,0 +1,962 @@
+// HIP kernels (gfx950) of the measurement-pattern simulator and their C-ABI
+// launchers.
+//
+// Execution model: one simulation per lane.  The live state of a simulation
+// (2^n_slots complex64 amplitudes for SV, 4^n_slots entries for DM) lives in
+// VGPRs for the whole run; a step is an in-place pass over amplitude pairs
+// (quartets for DM) whose register indices are compile-time constants because
+// the measured slot rotates with period n_slots and the step loop is unrolled
+// by n_slots.  Per-step control (angle index, append/CZ flags, CZ neighbour
+// mask) is wave-uniform and read with scalar loads, so a wave of 64
+// simulations of one pattern runs with no divergence and no cross-lane
+// traffic.  HBM sees only the angles, the input states and the outputs.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <utility>
+
+#include "plan.h"
+
+namespace mbqc {
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// flip the sign of both components when sbit == 0x80000000
+__device__ __forceinline__ float2 flip(float2 v, uint32_t sbit) {
+  return make_float2(__uint_as_float(__float_as_uint(v.x) ^ sbit),
+                     __uint_as_float(__float_as_uint(v.y) ^ sbit));
+}
+// CZ sign of slot-index j against the neighbour mask: parity(j & mask) << 31
+__device__ __forceinline__ uint32_t czbit(uint32_t j, uint32_t mask) {
+  return uint32_t(__builtin_popcount(j & mask) & 1) << 31;
+}
+
+// e^{-i theta}; with the reference phase also the per-step global phase
+// sgn(cos(theta/2)) e^{i theta/2} (the factor the reference's amplitude-sum
+// partial trace introduces, mentpy/calculator/state_ops.py:68-73).
+template <bool REF>
+__device__ __forceinline__ float2 step_coeff(float th, float2* ph) {
+  if constexpr (REF) {
+    float s2, c2;
+    sincosf(0.5f * th, &s2, &c2);
+    const float2 b = make_float2(fmaf(c2, c2, -s2 * s2), -2.0f * s2 * c2);
+    const float sg = c2 < 0.0f ? -1.0f : 1.0f;
+    *ph = cmul(*ph, make_float2(fabsf(c2), sg * s2));
+    return b;
+  } else {
+    float s, c;
+    sincosf(th, &s, &c);
+    return make_float2(c, -s);
+  }
+}
+
+// ---------------------------------------------------------------- SV step
+// Measure slot Q at XY angle theta (b = e^{-i theta}) and, if the append flag
+// is set, put the next scheduled qubit (|+> with CZs to `nmask`, and to the
+// measured qubit when flag bit1 is set) into slot Q.  With x/y the halves of
+// the state where slot Q is 0/1:  u0 = x + b y,  u1 = x -/+ b y,
+// new[Q=0] = u0, new[Q=1] = sign(nmask) u1.  The 1/sqrt(2) of the |+> and of
+// the projector are left out (tracked as an exponent, applied at the end).
+template <int NS, int Q>
+__device__ __forceinline__ void sv_step(float2 (&t)[1 << NS], float2 b, uint32_t flags,
+                                        uint32_t nmask) {
+  constexpr int H = 1 << NS;
+  constexpr int QB = 1 << Q;
+  if (flags & 1u) {
+    if (flags & 2u) {
+#pragma unroll
+      for (int j = 0; j < H; ++j) {
+        if (j & QB) continue;
+        const float2 x = t[j], y = t[j | QB];
+        float2 u0;
+        u0.x = fmaf(-b.y, y.y, fmaf(b.x, y.x, x.x));
+        u0.y = fmaf(b.x, y.y, fmaf(b.y, y.x, x.y));
+        const float2 u1 = make_float2(fmaf(2.0f, x.x, -u0.x), fmaf(2.0f, x.y, -u0.y));
+        t[j] = u0;
+        t[j | QB] = flip(u1, czbit(j, nmask));
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < H; ++j) {
+        if (j & QB) continue;
+        const float2 x = t[j], y = t[j | QB];
+        float2 u0;
+        u0.x = fmaf(-b.y, y.y, fmaf(b.x, y.x, x.x));
+        u0.y = fmaf(b.x, y.y, fmaf(b.y, y.x, x.y));
+        t[j] = u0;
+        t[j | QB] = flip(u0, czbit(j, nmask));
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      if (j & QB) continue;
+      const float2 x = t[j], y = t[j | QB];
+      float2 u0;
+      u0.x = fmaf(-b.y, y.y, fmaf(b.x, y.x, x.x));
+      u0.y = fmaf(b.x, y.y, fmaf(b.y, y.x, x.y));
+      t[j] = u0;
+      t[j | QB] = make_float2(0.0f, 0.0f);
+    }
+  }
+}
+
+// Job description shared by the SV / DM / adjoint kernels (passed by value
+// or read from global memory by the persistent kernel).
+struct Job {
+  const int32_t* tab;       // packed device tables of the plan
+  int32_t off_in, off_sign, off_out, off_svres, off_dmres, e_sv, e_dm;
+  int32_t ns, n_steps, n_out, live_mask;
+  const float* angles;
+  int64_t angle_stride;
+  const float2* psi;
+  int64_t in_stride;
+  float2* out;
+  float2* phase_out;
+  int64_t S, total;
+};
+
+template <int NS, bool REF, int SUB>
+__device__ __forceinline__ void sv_sub(float2 (&t)[1 << NS], const mbqc_step* steps, int k0,
+                                       int n_steps, const float (&th)[NS], float2* ph) {
+  const int k = k0 + SUB;
+  if (k < n_steps) {
+    const uint32_t fl = steps[k].flags;
+    const uint32_t nm = steps[k].nmask;
+    const float2 b = step_coeff<REF>(th[SUB], ph);
+    sv_step<NS, NS - 1 - SUB>(t, b, fl, nm);
+  }
+}
+
+template <int NS, bool REF, int... SUBS>
+__device__ __forceinline__ void sv_block(float2 (&t)[1 << NS], const mbqc_step* steps, int k0,
+                                         int n_steps, const float (&th)[NS], float2* ph,
+                                         std::integer_sequence<int, SUBS...>) {
+  (sv_sub<NS, REF, SUBS>(t, steps, k0, n_steps, th, ph), ...);
+}
+
+template <int NS>
+__device__ __forceinline__ void load_block_angles(const mbqc_step* steps, int k0, int n_steps,
+                                                  const float* ang, float (&th)[NS]) {
+#pragma unroll
+  for (int sub = 0; sub < NS; ++sub) {
+    const int k = k0 + sub;
+    th[sub] = 0.0f;
+    if (k < n_steps) {
+      const int ai = steps[k].angle_index;
+      th[sub] = ai >= 0 ? ang[ai] : steps[k].fixed_angle;
+    }
+  }
+}
+
+template <int NS>
+__device__ __forceinline__ void sv_load_input(const Job& J, const float2* psi, float2 (&t)[1 << NS]) {
+  const int32_t* in_index = J.tab + J.off_in;
+  const int32_t* sign = J.tab + J.off_sign;
+#pragma unroll
+  for (int j = 0; j < (1 << NS); ++j) {
+    const float2 v = psi[in_index[j]];
+    t[j] = flip(v, sign[j] ? 0x80000000u : 0u);
+  }
+}
+
+__device__ __forceinline__ float sv_final_scale(int e) {
+  float s = ldexpf(1.0f, -(e >> 1));
+  return (e & 1) ? s * 0.70710678118654752440f : s;
+}
+
+// One SV simulation on this lane (g = b*S + s).  Inactive lanes run on
+// simulation 0 and write nothing.
+template <int NS, bool REF>
+__device__ __forceinline__ void sv_sim(const Job& J, int64_t g, bool active, uint32_t flags) {
+  constexpr int H = 1 << NS;
+  const int64_t gg = active ? g : 0;
+  const int64_t bi = gg / J.S, si = gg - bi * J.S;
+  const float* ang = J.angles + bi * J.angle_stride;
+  const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
+  const int32_t* resc = J.tab + J.off_svres;
+  float2 t[H];
+  sv_load_input<NS>(J, J.psi + si * J.in_stride, t);
+  float2 ph = make_float2(1.0f, 0.0f);
+  const int n = J.n_steps;
+  const int nblk = (n + NS - 1) / NS;
+  for (int blk = 0; blk < nblk; ++blk) {
+    if (resc[blk]) {
+#pragma unroll
+      for (int j = 0; j < H; ++j) t[j] = cscale(t[j], 0x1p-32f);
+    }
+    float th[NS];
+    load_block_angles<NS>(steps, blk * NS, n, ang, th);
+    sv_block<NS, REF>(t, steps, blk * NS, n, th, &ph, std::make_integer_sequence<int, NS>{});
+  }
+  // output gather: slot-index j -> output index out_pos[j] (dead slots skipped)
+  const int32_t* out_pos = J.tab + J.off_out;
+  float scale = sv_final_scale(J.e_sv);
+  if (flags & MBQC_FLAG_RENORMALIZE) {
+    float n2 = 0.0f;
+#pragma unroll
+    for (int j = 0; j < H; ++j)
+      if (out_pos[j] >= 0) n2 = fmaf(t[j].x, t[j].x, fmaf(t[j].y, t[j].y, n2));
+    scale = 1.0f / sqrtf(n2);
+  }
+  const float2 f = cscale(ph, scale);
+  if (active) {
+    float2* o = J.out + g * (int64_t(1) << J.n_out);
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      const int p = out_pos[j];
+      if (p >= 0) o[p] = cmul(t[j], f);
+    }
+    if ((flags & MBQC_FLAG_WRITE_PHASE) && J.phase_out) J.phase_out[g] = ph;
+  }
+}
+
+template <int NS, bool REF>
+__global__ __launch_bounds__(256) void k_sv_forward(Job J, uint32_t flags) {
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  sv_sim<NS, REF>(J, g, g < J.total, flags);
+}
+
+// ---------------------------------------------------------------- DM step
+// Quartet form of  rho -> Tr_Q[(D^dag P_theta) CZ (rho (x) |+><+|) CZ]:
+// with B_ab the blocks of rho where slot Q is a (row) / b (column), and
+// gb = gamma e^{-i theta} (gamma = coherence damping of the measured qubit),
+//   P = B00 + B11, M = B00 - B11, U = conj(gb) B01, V = gb B10,
+//   r00 = P+U+V, r01 = M-U+V, r10 = M+U-V, r11 = P-U-V   (CZ to the measured qubit)
+//   r_cd = P+U+V                                       (no CZ to the measured qubit)
+// new[(c,j),(d,k)] = sign_j^c sign_k^d r_cd.  The factor 1/2 of |+><+| and
+// the division by the outcome probability (1/2) are tracked as an exponent.
+template <int NS, int Q>
+__device__ __forceinline__ void dm_step(float2 (&r)[1 << NS][1 << NS], float2 gb, uint32_t flags,
+                                        uint32_t nmask) {
+  constexpr int H = 1 << NS;
+  constexpr int QB = 1 << Q;
+  const float2 gbc = cconj(gb);
+  const bool app = flags & 1u, sneg = flags & 2u;
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    if (j & QB) continue;
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      if (k & QB) continue;
+      const float2 B00 = r[j][k], B01 = r[j][k | QB], B10 = r[j | QB][k], B11 = r[j | QB][k | QB];
+      const float2 U = cmul(gbc, B01), V = cmul(gb, B10);
+      const float2 P = cadd(B00, B11);
+      const float2 r00 = cadd(P, cadd(U, V));
+      if (app) {
+        const uint32_t sj = czbit(j, nmask), sk = czbit(k, nmask);
+        if (sneg) {
+          const float2 M = csub(B00, B11);
+          r[j][k] = r00;
+          r[j][k | QB] = flip(cadd(M, csub(V, U)), sk);
+          r[j | QB][k] = flip(cadd(M, csub(U, V)), sj);
+          r[j | QB][k | QB] = flip(csub(P, cadd(U, V)), sj ^ sk);
+        } else {
+          r[j][k] = r00;
+          r[j][k | QB] = flip(r00, sk);
+          r[j | QB][k] = flip(r00, sj);
+          r[j | QB][k | QB] = flip(r00, sj ^ sk);
+        }
+      } else {
+        r[j][k] = r00;
+        r[j][k | QB] = make_float2(0.f, 0.f);
+        r[j | QB][k] = make_float2(0.f, 0.f);
+        r[j | QB][k | QB] = make_float2(0.f, 0.f);
+      }
+    }
+  }
+}
+
+// single-qubit depolarizing on slot M of a density matrix
+template <int NS, int M>
+__device__ __forceinline__ void dm_depolarize(float2 (&r)[1 << NS][1 << NS], float p) {
+  constexpr int H = 1 << NS;
+  constexpr int MB = 1 << M;
+  const float keep = 1.0f - 2.0f * p / 3.0f, mix = 2.0f * p / 3.0f, coh = 1.0f - 4.0f * p / 3.0f;
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    if (j & MB) continue;
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      if (k & MB) continue;
+      const float2 a = r[j][k], d = r[j | MB][k | MB];
+      r[j][k] = make_float2(keep * a.x + mix * d.x, keep * a.y + mix * d.y);
+      r[j | MB][k | MB] = make_float2(keep * d.x + mix * a.x, keep * d.y + mix * a.y);
+      r[j][k | MB] = cscale(r[j][k | MB], coh);
+      r[j | MB][k] = cscale(r[j | MB][k], coh);
+    }
+  }
+}
+
+template <int NS, bool REF, int SUB>
+__device__ __forceinline__ void dm_sub(float2 (&r)[1 << NS][1 << NS], const mbqc_step* steps, int k0,
+                                       int n_steps, const float (&th)[NS], float gamma) {
+  const int k = k0 + SUB;
+  if (k < n_steps) {
+    float2 ph = make_float2(1.f, 0.f);
+    const float2 b = step_coeff<false>(th[SUB], &ph);
+    dm_step<NS, NS - 1 - SUB>(r, cscale(b, gamma), steps[k].flags, steps[k].nmask);
+  }
+}
+
+template <int NS, int... SUBS>
+__device__ __forceinline__ void dm_block(float2 (&r)[1 << NS][1 << NS], const mbqc_step* steps, int k0,
+                                         int n_steps, const float (&th)[NS], float gamma,
+                                         std::integer_sequence<int, SUBS...>) {
+  (dm_sub<NS, false, SUBS>(r, steps, k0, n_steps, th, gamma), ...);
+}
+
+template <int NS, int... MS>
+__device__ __forceinline__ void dm_depolarize_outputs(float2 (&r)[1 << NS][1 << NS], uint32_t live,
+                                                      float p, std::integer_sequence<int, MS...>) {
+  ((live & (1u << MS) ? dm_depolarize<NS, MS>(r, p) : void()), ...);
+}
+
+template <int NS>
+__global__ __launch_bounds__(256) void k_dm_forward(Job J, float gamma, float p_out, uint32_t flags) {
+  constexpr int H = 1 << NS;
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool active = g < J.total;
+  const int64_t gg = active ? g : 0;
+  const int64_t bi = gg / J.S, si = gg - bi * J.S;
+  const float* ang = J.angles + bi * J.angle_stride;
+  const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
+  const int32_t* resc = J.tab + J.off_dmres;
+  float2 v[H];
+  sv_load_input<NS>(J, J.psi + si * J.in_stride, v);
+  float2 r[H][H];
+#pragma unroll
+  for (int j = 0; j < H; ++j)
+#pragma unroll
+    for (int k = 0; k < H; ++k) r[j][k] = cmul(v[j], cconj(v[k]));
+  const int n = J.n_steps;
+  const int nblk = (n + NS - 1) / NS;
+  for (int blk = 0; blk < nblk; ++blk) {
+    if (resc[blk]) {
+#pragma unroll
+      for (int j = 0; j < H; ++j)
+#pragma unroll
+        for (int k = 0; k < H; ++k) r[j][k] = cscale(r[j][k], 0x1p-32f);
+    }
+    float th[NS];
+    load_block_angles<NS>(steps, blk * NS, n, ang, th);
+    dm_block<NS>(r, steps, blk * NS, n, th, gamma, std::make_integer_sequence<int, NS>{});
+  }
+  dm_depolarize_outputs<NS>(r, uint32_t(J.live_mask), p_out, std::make_integer_sequence<int, NS>{});
+  const int32_t* out_pos = J.tab + J.off_out;
+  float scale = ldexpf(1.0f, -J.e_dm);
+  if (flags & MBQC_FLAG_RENORMALIZE) {
+    float tr = 0.0f;
+#pragma unroll
+    for (int j = 0; j < H; ++j)
+      if (out_pos[j] >= 0) tr += r[j][j].x;
+    scale = 1.0f / tr;
+  }
+  if (active) {
+    const int64_t d = int64_t(1) << J.n_out;
+    float2* o = J.out + g * d * d;
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      const int pj = out_pos[j];
+      if (pj < 0) continue;
+#pragma unroll
+      for (int k = 0; k < H; ++k) {
+        const int pk = out_pos[k];
+        if (pk >= 0) o[int64_t(pj) * d + pk] = cscale(r[j][k], scale);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- adjoint
+// Forward pass of the adjoint: raw phase, unnormalized output, one checkpoint
+// of the live state at the start of every block of NS steps (after the
+// block's rescale), stored column-fastest for coalesced access.
+template <int NS>
+__global__ __launch_bounds__(256) void k_sv_adj_forward(Job J, float2* ckpt) {
+  constexpr int H = 1 << NS;
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool active = g < J.total;
+  const int64_t gg = active ? g : 0;
+  const int64_t bi = gg / J.S, si = gg - bi * J.S;
+  const float* ang = J.angles + bi * J.angle_stride;
+  const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
+  const int32_t* resc = J.tab + J.off_svres;
+  float2 t[H];
+  sv_load_input<NS>(J, J.psi + si * J.in_stride, t);
+  float2 ph = make_float2(1.f, 0.f);
+  const int n = J.n_steps;
+  const int nblk = (n + NS - 1) / NS;
+  for (int blk = 0; blk < nblk; ++blk) {
+    if (resc[blk]) {
+#pragma unroll
+      for (int j = 0; j < H; ++j) t[j] = cscale(t[j], 0x1p-32f);
+    }
+    if (active) {
+#pragma unroll
+      for (int j = 0; j < H; ++j) ckpt[(int64_t(blk) * H + j) * J.total + g] = t[j];
+    }
+    float th[NS];
+    load_block_angles<NS>(steps, blk * NS, n, ang, th);
+    sv_block<NS, false>(t, steps, blk * NS, n, th, &ph, std::make_integer_sequence<int, NS>{});
+  }
+  const int32_t* out_pos = J.tab + J.off_out;
+  const float scale = sv_final_scale(J.e_sv);
+  if (active) {
+    float2* o = J.out + g * (int64_t(1) << J.n_out);
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      const int p = out_pos[j];
+      if (p >= 0) o[p] = cscale(t[j], scale);
+    }
+  }
+}
+
+// Adjoint of one SV step (slot Q).  Forward: u0 = x + b y, u1 = x + s b y,
+// new0 = u0, new1 = sigma u1 (append) or new1 = 0.  Given the cotangent lam of
+// the new state (dL = Re sum conj(lam) d(new)):
+//   g0 = lam0, g1 = sigma lam1, h = g0 + s g1,
+//   lam_x = g0 + g1, lam_y = conj(b) h,  dL/dtheta += Im(b sum y conj(h)).
+template <int NS, int Q>
+__device__ __forceinline__ float sv_adj_step(float2 (&lam)[1 << NS], const float2 (&y)[1 << (NS - 1)],
+                                             float2 b, uint32_t flags, uint32_t nmask) {
+  constexpr int H = 1 << NS;
+  constexpr int QB = 1 << Q;
+  const bool app = flags & 1u, sneg = flags & 2u;
+  const float2 bc = cconj(b);
+  float ar = 0.f, ai = 0.f;
+  int i = 0;
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    if (j & QB) continue;
+    const float2 g0 = lam[j];
+    float2 g1 = make_float2(0.f, 0.f);
+    if (app) g1 = flip(lam[j | QB], czbit(j, nmask));
+    const float2 h = sneg ? csub(g0, g1) : cadd(g0, g1);
+    const float2 yy = y[i++];
+    // acc += y * conj(h)
+    ar = fmaf(yy.x, h.x, fmaf(yy.y, h.y, ar));
+    ai = fmaf(yy.y, h.x, fmaf(-yy.x, h.y, ai));
+    lam[j] = cadd(g0, g1);
+    lam[j | QB] = cmul(bc, h);
+  }
+  // Im(b * acc)
+  return fmaf(b.x, ai, b.y * ar);
+}
+
+template <int NS, int Q>
+__device__ __forceinline__ void take_y(const float2 (&t)[1 << NS], float2 (&y)[1 << (NS - 1)]) {
+  constexpr int QB = 1 << Q;
+  int i = 0;
+#pragma unroll
+  for (int j = 0; j < (1 << NS); ++j) {
+    if (j & QB) continue;
+    y[i++] = t[j | QB];
+  }
+}
+
+struct AdjCtx {
+  float* gpart;     // [B][P][n_angles]
+  int64_t C;        // columns per parameter set
+  int32_t P;        // partial sums per parameter set
+  int32_t red;      // lanes reduced together (1, pow2 <= 64)
+  int32_t n_angles;
+};
+
+template <int NS, int SUB>
+__device__ __forceinline__ void adj_recompute(float2 (&t)[1 << NS], float2 (&ys)[NS][1 << (NS - 1)],
+                                              float2 (&bs)[NS], const mbqc_step* steps, int k0, int n,
+                                              const float (&th)[NS]) {
+  const int k = k0 + SUB;
+  if (k < n) {
+    float2 ph = make_float2(1.f, 0.f);
+    bs[SUB] = step_coeff<false>(th[SUB], &ph);
+    take_y<NS, NS - 1 - SUB>(t, ys[SUB]);
+    if (SUB + 1 < NS) sv_step<NS, NS - 1 - SUB>(t, bs[SUB], steps[k].flags, steps[k].nmask);
+  }
+}
+
+template <int NS, int SUB>
+__device__ __forceinline__ void adj_back(float2 (&lam)[1 << NS], const float2 (&ys)[NS][1 << (NS - 1)],
+                                         const float2 (&bs)[NS], const mbqc_step* steps, int k0, int n,
+                                         const AdjCtx& A, int64_t bi, int64_t ci, bool active) {
+  const int k = k0 + SUB;
+  if (k < n) {
+    float d = sv_adj_step<NS, NS - 1 - SUB>(lam, ys[SUB], bs[SUB], steps[k].flags, steps[k].nmask);
+    const int ai = steps[k].angle_index;
+    if (ai >= 0) {
+      for (int m = 1; m < A.red; m <<= 1) d += __shfl_xor(d, m, 64);
+      const int64_t p = A.P == 1 ? 0 : (A.red == 64 ? ci / 64 : ci);
+      const bool writer = active && (A.red == 1 || (ci % A.red) == 0);
+      if (writer) A.gpart[(bi * A.P + p) * A.n_angles + ai] = d;
+    }
+  }
+}
+
+template <int NS, int... SUBS>
+__device__ __forceinline__ void adj_block(float2 (&t)[1 << NS], float2 (&lam)[1 << NS],
+                                          const mbqc_step* steps, int k0, int n, const float (&th)[NS],
+                                          const AdjCtx& A, int64_t bi, int64_t ci, bool active,
+                                          std::integer_sequence<int, SUBS...>) {
+  float2 ys[NS][1 << (NS - 1)];
+  float2 bs[NS];
+  (adj_recompute<NS, SUBS>(t, ys, bs, steps, k0, n, th), ...);
+  (adj_back<NS, NS - 1 - SUBS>(lam, ys, bs, steps, k0, n, A, bi, ci, active), ...);
+}
+
+template <int NS>
+__global__ __launch_bounds__(64) void k_sv_adj_backward(Job J, const float2* ckpt, const float2* seed,
+                                                        AdjCtx A) {
+  constexpr int H = 1 << NS;
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool active = g < J.total;
+  const int64_t gg = active ? g : 0;
+  const int64_t bi = gg / A.C, ci = gg - bi * A.C;
+  const float* ang = J.angles + bi * J.angle_stride;
+  const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
+  const int32_t* resc = J.tab + J.off_svres;
+  const int32_t* out_pos = J.tab + J.off_out;
+  const float scale = sv_final_scale(J.e_sv);
+  float2 lam[H];
+  const float2* sd = seed + gg * (int64_t(1) << J.n_out);
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    const int p = out_pos[j];
+    lam[j] = (active && p >= 0) ? cscale(sd[p], scale) : make_float2(0.f, 0.f);
+  }
+  const int n = J.n_steps;
+  const int nblk = (n + NS - 1) / NS;
+  for (int blk = nblk - 1; blk >= 0; --blk) {
+    float2 t[H];
+#pragma unroll
+    for (int j = 0; j < H; ++j)
+      t[j] = active ? ckpt[(int64_t(blk) * H + j) * J.total + g] : make_float2(0.f, 0.f);
+    float th[NS];
+    load_block_angles<NS>(steps, blk * NS, n, ang, th);
+    adj_block<NS>(t, lam, steps, blk * NS, n, th, A, bi, ci, active, std::make_integer_sequence<int, NS>{});
+    if (resc[blk]) {
+#pragma unroll
+      for (int j = 0; j < H; ++j) lam[j] = cscale(lam[j], 0x1p-32f);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- batched
+struct BatchItem {
+  int32_t job;
+  int32_t count;   // simulations in this item
+  int64_t first;   // first simulation index within the job
+};
+
+template <bool REF>
+__device__ __forceinline__ void run_item(const Job& J, int64_t g, bool active, uint32_t flags) {
+  switch (J.ns) {
+    case 1: sv_sim<1, REF>(J, g, active, flags); break;
+    case 2: sv_sim<2, REF>(J, g, active, flags); break;
+    case 3: sv_sim<3, REF>(J, g, active, flags); break;
+    case 4: sv_sim<4, REF>(J, g, active, flags); break;
+    case 5: sv_sim<5, REF>(J, g, active, flags); break;
+    case 6: sv_sim<6, REF>(J, g, active, flags); break;
+    default: break;
+  }
+}
+
+// Persistent kernel: every wave pulls work items (runs of simulations of one
+// job) from a device counter until none are left.  Items are sorted by
+// decreasing cost on the host, so long patterns start first.
+template <bool REF>
+__global__ __launch_bounds__(256) void k_batch(const Job* __restrict__ jobs,
+                                               const BatchItem* __restrict__ items, int64_t n_items,
+                                               int* counter, uint32_t flags) {
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    int it = 0;
+    if (lane == 0) it = atomicAdd(counter, 1);
+    it = __builtin_amdgcn_readfirstlane(it);
+    if (it >= n_items) break;
+    const BatchItem I = items[it];
+    const Job J = jobs[I.job];
+    for (int off = 0; off < I.count; off += 64) {
+      const int64_t g = I.first + off + lane;
+      run_item<REF>(J, g, (off + lane) < I.count && g < J.total, flags);
+    }
+  }
+}
+
+__global__ void k_reset_counter(int* counter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+}
+
+}  // namespace mbqc
+
+// ==================================================================== C ABI
+using namespace mbqc;
+
+namespace {
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(MBQC_ERR_RUNTIME, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Upload (once per device) and return the plan's packed tables.
+int device_tables(Plan* p, const int32_t** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return fail(MBQC_ERR_RUNTIME, "device id out of range");
+  std::lock_guard<std::mutex> lk(p->mu);
+  if (!p->dev_tab[dev]) {
+    std::vector<int32_t> t = pack_device_tables(*p);
+    void* d = nullptr;
+    e = hipMalloc(&d, t.size() * sizeof(int32_t));
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(plan tables)");
+    e = hipMemcpy(d, t.data(), t.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(d);
+      return hip_fail(e, "hipMemcpy(plan tables)");
+    }
+    p->dev_tab[dev] = d;
+  }
+  *out = static_cast<const int32_t*>(p->dev_tab[dev]);
+  return MBQC_OK;
+}
+
+int make_job(Plan* p, const float* angles, int64_t angle_stride, const void* psi, int64_t in_stride,
+             void* out, int64_t B, int64_t S, Job* J) {
+  if (!angles && p->n_angles > 0) return fail(MBQC_ERR_VALUE, "angles pointer is null");
+  if (!psi) return fail(MBQC_ERR_VALUE, "input pointer is null");
+  if (!out) return fail(MBQC_ERR_VALUE, "output pointer is null");
+  if (B < 0 || S <= 0) return fail(MBQC_ERR_VALUE, "batch sizes must be B >= 0, S >= 1");
+  if (angle_stride < 0 || in_stride < 0) return fail(MBQC_ERR_VALUE, "negative stride");
+  if (angle_stride > 0 && angle_stride < p->n_angles)
+    return fail(MBQC_ERR_VALUE, "angle stride shorter than the angle vector");
+  if (in_stride > 0 && in_stride < (int64_t(1) << p->n_in))
+    return fail(MBQC_ERR_VALUE, "input stride shorter than the input state");
+  const int32_t* tab = nullptr;
+  int rc = device_tables(p, &tab);
+  if (rc != MBQC_OK) return rc;
+  TableOffsets o = table_offsets(*p);
+  Job j{};
+  j.tab = tab;
+  j.off_in = o.in_index;
+  j.off_sign = o.init_sign;
+  j.off_out = o.out_pos;
+  j.off_svres = o.sv_rescale;
+  j.off_dmres = o.dm_rescale;
+  j.e_sv = p->e_sv;
+  j.e_dm = p->e_dm;
+  j.ns = p->ns;
+  j.n_steps = p->n_steps;
+  j.n_out = p->n_out;
+  uint32_t live = 0;
+  for (int s : p->out_slot) live |= 1u << s;
+  j.live_mask = int32_t(live);
+  j.angles = angles;
+  j.angle_stride = angle_stride;
+  j.psi = static_cast<const float2*>(psi);
+  j.in_stride = in_stride;
+  j.out = static_cast<float2*>(out);
+  j.phase_out = nullptr;
+  j.S = S;
+  j.total = B * S;
+  *J = j;
+  return MBQC_OK;
+}
+
+template <int NS>
+hipError_t launch_sv(const Job& J, uint32_t flags, hipStream_t st) {
+  const int threads = 256;
+  const int64_t blocks = (J.total + threads - 1) / threads;
+  if (flags & MBQC_FLAG_REF_PHASE)
+    hipLaunchKernelGGL((k_sv_forward<NS, true>), dim3(blocks), dim3(threads), 0, st, J, flags);
+  else
+    hipLaunchKernelGGL((k_sv_forward<NS, false>), dim3(blocks), dim3(threads), 0, st, J, flags);
+  return hipGetLastError();
+}
+
+template <int NS>
+hipError_t launch_dm(const Job& J, float gamma, float p_out, uint32_t flags, hipStream_t st) {
+  const int threads = 256;
+  const int64_t blocks = (J.total + threads - 1) / threads;
+  hipLaunchKernelGGL((k_dm_forward<NS>), dim3(blocks), dim3(threads), 0, st, J, gamma, p_out, flags);
+  return hipGetLastError();
+}
+
+template <int NS>
+hipError_t launch_adj_fwd(const Job& J, float2* ckpt, hipStream_t st) {
+  const int threads = 256;
+  const int64_t blocks = (J.total + threads - 1) / threads;
+  hipLaunchKernelGGL((k_sv_adj_forward<NS>), dim3(blocks), dim3(threads), 0, st, J, ckpt);
+  return hipGetLastError();
+}
+
+template <int NS>
+hipError_t launch_adj_bwd(const Job& J, const float2* ckpt, const float2* seed, const AdjCtx& A,
+                          hipStream_t st) {
+  const int threads = 64;
+  const int64_t blocks = (J.total + threads - 1) / threads;
+  hipLaunchKernelGGL((k_sv_adj_backward<NS>), dim3(blocks), dim3(threads), 0, st, J, ckpt, seed, A);
+  return hipGetLastError();
+}
+
+constexpr int kMaxSvSlots = 6;
+constexpr int kMaxDmSlots = 3;
+constexpr int kMaxAdjSlots = 5;
+
+}  // namespace
+
+struct mbqc_batch {
+  std::vector<Job> jobs;
+  std::vector<BatchItem> items;
+  Job* d_jobs = nullptr;
+  BatchItem* d_items = nullptr;
+  int* d_counter = nullptr;
+  int device = -1;
+};
+
+extern "C" {
+
+void mbqc_plan_destroy(mbqc_plan* plan) {
+  if (!plan) return;
+  Plan* p = reinterpret_cast<Plan*>(plan);
+  for (int d = 0; d < kMaxDevices; ++d)
+    if (p->dev_tab[d]) (void)hipFree(p->dev_tab[d]);
+  delete p;
+}
+
+int mbqc_sv_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
+                        const void* psi_in, int64_t in_stride, void* out, void* phase_out, int64_t B,
+                        int64_t S, uint32_t flags, void* stream) {
+  if (!plan) return fail(MBQC_ERR_VALUE, "null plan");
+  Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(plan));
+  if (p->ns > kMaxSvSlots)
+    return fail(MBQC_ERR_NOT_IMPLEMENTED, "statevector kernels support live windows of at most " +
+                                              std::to_string(kMaxSvSlots) + " qubits");
+  Job J;
+  int rc = make_job(p, angles, angle_stride, psi_in, in_stride, out, B, S, &J);
+  if (rc != MBQC_OK) return rc;
+  if ((flags & MBQC_FLAG_WRITE_PHASE) && !phase_out) return fail(MBQC_ERR_VALUE, "phase_out is null");
+  J.phase_out = static_cast<float2*>(phase_out);
+  if (J.total == 0) return MBQC_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipError_t e = hipSuccess;
+  switch (p->ns) {
+    case 1: e = launch_sv<1>(J, flags, st); break;
+    case 2: e = launch_sv<2>(J, flags, st); break;
+    case 3: e = launch_sv<3>(J, flags, st); break;
+    case 4: e = launch_sv<4>(J, flags, st); break;
+    case 5: e = launch_sv<5>(J, flags, st); break;
+    case 6: e = launch_sv<6>(J, flags, st); break;
+  }
+  return e == hipSuccess ? MBQC_OK : hip_fail(e, "sv_forward launch");
+}
+
+int mbqc_dm_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
+                        const void* psi_in, int64_t in_stride, void* out, int64_t B, int64_t S,
+                        const mbqc_noise* noise, uint32_t flags, void* stream) {
+  if (!plan) return fail(MBQC_ERR_VALUE, "null plan");
+  Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(plan));
+  if (p->ns > kMaxDmSlots)
+    return fail(MBQC_ERR_NOT_IMPLEMENTED, "density-matrix kernels support live windows of at most " +
+                                              std::to_string(kMaxDmSlots) + " qubits");
+  float pd = 0.f, pr = 0.f;
+  if (noise) {
+    pd = noise->p_depol;
+    pr = noise->p_readout;
+  }
+  if (!(pd >= 0.f && pd <= 1.f) || !(pr >= 0.f && pr <= 1.f))
+    return fail(MBQC_ERR_VALUE, "noise probabilities must lie in [0, 1]");
+  const float gamma = (1.0f - 4.0f * pd / 3.0f) * (1.0f - 2.0f * pr);
+  Job J;
+  int rc = make_job(p, angles, angle_stride, psi_in, in_stride, out, B, S, &J);
+  if (rc != MBQC_OK) return rc;
+  if (J.total == 0) return MBQC_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipError_t e = hipSuccess;
+  switch (p->ns) {
+    case 1: e = launch_dm<1>(J, gamma, pd, flags, st); break;
+    case 2: e = launch_dm<2>(J, gamma, pd, flags, st); break;
+    case 3: e = launch_dm<3>(J, gamma, pd, flags, st); break;
+  }
+  return e == hipSuccess ? MBQC_OK : hip_fail(e, "dm_forward launch");
+}
+
+size_t mbqc_sv_adjoint_workspace_bytes(const mbqc_plan* plan, int64_t n_columns) {
+  if (!plan || n_columns < 0) return 0;
+  const Plan* p = reinterpret_cast<const Plan*>(plan);
+  return size_t(p->n_blocks()) * (size_t(1) << p->ns) * size_t(n_columns) * 8u;
+}
+
+int32_t mbqc_sv_adjoint_partials(int64_t C) {
+  if (C <= 0) return 1;
+  if (C <= 64 && (C & (C - 1)) == 0) return 1;
+  if (C % 64 == 0) return int32_t(C / 64);
+  return int32_t(C);
+}
+
+int mbqc_sv_adjoint_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
+                                const void* psi_in, int64_t in_stride, void* out, int64_t B, int64_t C,
+                                void* ckpt, size_t ckpt_bytes, void* stream) {
+  if (!plan) return fail(MBQC_ERR_VALUE, "null plan");
+  Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(plan));
+  if (p->ns > kMaxAdjSlots)
+    return fail(MBQC_ERR_NOT_IMPLEMENTED, "adjoint kernels support live windows of at most " +
+                                              std::to_string(kMaxAdjSlots) + " qubits");
+  Job J;
+  int rc = make_job(p, angles, angle_stride, psi_in, in_stride, out, B, C, &J);
+  if (rc != MBQC_OK) return rc;
+  if (!ckpt || ckpt_bytes < mbqc_sv_adjoint_workspace_bytes(plan, B * C))
+    return fail(MBQC_ERR_VALUE, "checkpoint workspace too small");
+  if (J.total == 0) return MBQC_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  float2* ck = static_cast<float2*>(ckpt);
+  hipError_t e = hipSuccess;
+  switch (p->ns) {
+    case 1: e = launch_adj_fwd<1>(J, ck, st); break;
+    case 2: e = launch_adj_fwd<2>(J, ck, st); break;
+    case 3: e = launch_adj_fwd<3>(J, ck, st); break;
+    case 4: e = launch_adj_fwd<4>(J, ck, st); break;
+    case 5: e = launch_adj_fwd<5>(J, ck, st); break;
+  }
+  return e == hipSuccess ? MBQC_OK : hip_fail(e, "adjoint forward launch");
+}
+
+int mbqc_sv_adjoint_backward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
+                                 const void* seed, int64_t B, int64_t C, const void* ckpt,
+                                 size_t ckpt_bytes, float* grad_partial, void* stream) {
+  if (!plan) return fail(MBQC_ERR_VALUE, "null plan");
+  Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(plan));
+  if (p->ns > kMaxAdjSlots)
+    return fail(MBQC_ERR_NOT_IMPLEMENTED, "adjoint kernels support live windows of at most " +
+                                              std::to_string(kMaxAdjSlots) + " qubits");
+  if (!seed || !grad_partial) return fail(MBQC_ERR_VALUE, "null seed or gradient pointer");
+  if (!ckpt || ckpt_bytes < mbqc_sv_adjoint_workspace_bytes(plan, B * C))
+    return fail(MBQC_ERR_VALUE, "checkpoint workspace too small");
+  Job J;
+  // psi is unused by the backward sweep; pass the seed as a valid pointer
+  int rc = make_job(p, angles, angle_stride, seed, 0, const_cast<void*>(seed), B, C, &J);
+  if (rc != MBQC_OK) return rc;
+  if (J.total == 0) return MBQC_OK;
+  AdjCtx A{};
+  A.gpart = grad_partial;
+  A.C = C;
+  A.P = mbqc_sv_adjoint_partials(C);
+  A.red = (C <= 64 && (C & (C - 1)) == 0) ? int32_t(C) : (C % 64 == 0 ? 64 : 1);
+  A.n_angles = p->n_angles;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const float2* ck = static_cast<const float2*>(ckpt);
+  const float2* sd = static_cast<const float2*>(seed);
+  hipError_t e = hipSuccess;
+  switch (p->ns) {
+    case 1: e = launch_adj_bwd<1>(J, ck, sd, A, st); break;
+    case 2: e = launch_adj_bwd<2>(J, ck, sd, A, st); break;
+    case 3: e = launch_adj_bwd<3>(J, ck, sd, A, st); break;
+    case 4: e = launch_adj_bwd<4>(J, ck, sd, A, st); break;
+    case 5: e = launch_adj_bwd<5>(J, ck, sd, A, st); break;
+  }
+  return e == hipSuccess ? MBQC_OK : hip_fail(e, "adjoint backward launch");
+}
+
+int mbqc_batch_create(const mbqc_batch_job* jobs, int32_t n_jobs, int32_t sims_per_item,
+                      mbqc_batch** out) {
+  if (!out) return fail(MBQC_ERR_VALUE, "out is null");
+  *out = nullptr;
+  if (!jobs || n_jobs <= 0) return fail(MBQC_ERR_VALUE, "no jobs");
+  if (sims_per_item <= 0 || sims_per_item % 64 != 0)
+    return fail(MBQC_ERR_VALUE, "sims_per_item must be a positive multiple of 64");
+  auto* bt = new mbqc_batch();
+  std::vector<std::pair<int64_t, size_t>> order;
+  for (int i = 0; i < n_jobs; ++i) {
+    const mbqc_batch_job& bj = jobs[i];
+    if (!bj.plan) {
+      delete bt;
+      return fail(MBQC_ERR_VALUE, "job has a null plan");
+    }
+    Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(bj.plan));
+    if (p->ns > kMaxSvSlots) {
+      delete bt;
+      return fail(MBQC_ERR_NOT_IMPLEMENTED, "batched kernel supports live windows of at most " +
+                                                std::to_string(kMaxSvSlots) + " qubits");
+    }
+    Job J;
+    int rc = make_job(p, bj.angles, bj.angle_stride, bj.psi_in, bj.in_stride, bj.out, bj.B, bj.S, &J);
+    if (rc != MBQC_OK) {
+      delete bt;
+      return rc;
+    }
+    bt->jobs.push_back(J);
+    const int64_t cost = int64_t(p->n_steps) << p->ns;
+    for (int64_t f = 0; f < J.total; f += sims_per_item) {
+      BatchItem it{};
+      it.job = i;
+      it.first = f;
+      it.count = int32_t(std::min<int64_t>(sims_per_item, J.total - f));
+      order.push_back({cost, bt->items.size()});
+      bt->items.push_back(it);
+    }
+  }
+  // longest patterns first (stable within a job)
+  std::stable_sort(order.begin(), order.end(),
+                   [](const std::pair<int64_t, size_t>& a, const std::pair<int64_t, size_t>& b) {
+                     return a.first > b.first;
+                   });
+  std::vector<BatchItem> sorted;
+  sorted.reserve(order.size());
+  for (auto& o : order) sorted.push_back(bt->items[o.second]);
+  bt->items.swap(sorted);
+  hipError_t e = hipGetDevice(&bt->device);
+  if (e == hipSuccess) e = hipMalloc(&bt->d_jobs, sizeof(Job) * bt->jobs.size());
+  if (e == hipSuccess && !bt->items.empty())
+    e = hipMalloc(&bt->d_items, sizeof(BatchItem) * bt->items.size());
+  if (e == hipSuccess) e = hipMalloc(&bt->d_counter, sizeof(int) * 64);
+  if (e == hipSuccess)
+    e = hipMemcpy(bt->d_jobs, bt->jobs.data(), sizeof(Job) * bt->jobs.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !bt->items.empty())
+    e = hipMemcpy(bt->d_items, bt->items.data(), sizeof(BatchItem) * bt->items.size(),
+                  hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    mbqc_batch_destroy(bt);
+    return hip_fail(e, "batch allocation");
+  }
+  *out = bt;
+  return MBQC_OK;
+}
+
+void mbqc_batch_destroy(mbqc_batch* bt) {
+  if (!bt) return;
+  if (bt->d_jobs) (void)hipFree(bt->d_jobs);
+  if (bt->d_items) (void)hipFree(bt->d_items);
+  if (bt->d_counter) (void)hipFree(bt->d_counter);
+  delete bt;
+}
+
+int64_t mbqc_batch_items(const mbqc_batch* bt) { return bt ? int64_t(bt->items.size()) : -1; }
+
+int mbqc_batch_launch(mbqc_batch* bt, int32_t waves, int32_t wg_waves, uint32_t flags, void* stream) {
+  if (!bt) return fail(MBQC_ERR_VALUE, "null batch");
+  if (wg_waves < 1 || wg_waves > 4) return fail(MBQC_ERR_VALUE, "wg_waves must be 1..4");
+  if (waves < wg_waves) return fail(MBQC_ERR_VALUE, "waves must be at least wg_waves");
+  if (bt->items.empty()) return MBQC_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, bt->d_counter);
+  const int blocks = waves / wg_waves;
+  const int threads = 64 * wg_waves;
+  const int64_t n = int64_t(bt->items.size());
+  if (flags & MBQC_FLAG_REF_PHASE)
+    hipLaunchKernelGGL(k_batch<true>, dim3(blocks), dim3(threads), 0, st, bt->d_jobs, bt->d_items, n,
+                       bt->d_counter, flags);
+  else
+    hipLaunchKernelGGL(k_batch<false>, dim3(blocks), dim3(threads), 0, st, bt->d_jobs, bt->d_items, n,
+                       bt->d_counter, flags);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? MBQC_OK : hip_fail(e, "batch launch");
+}
+
+}  // extern "C"

@@ -1,0 +1,47 @@
+// Internal plan representation shared by the host compiler and the launchers.
+#pragma once
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mbqc_hip.h"
+
+namespace mbqc {
+
+constexpr int kMaxSlots = 12;   // host tables; kernels support fewer (see launchers)
+constexpr int kMaxDevices = 16;
+
+// Growth control: the kernels skip the 1/sqrt(2) of every appended |+>, so the
+// SV state grows by sqrt(2) (the DM by 2) per append; at the start of a block
+// whose running exponent reached the threshold the state is multiplied by
+// 2^-32, an exact power of two.
+constexpr int kSvRescaleAppends = 64;   // 2^(64/2) growth -> rescale by 2^-32
+constexpr int kDmRescaleAppends = 32;   // 2^32 growth     -> rescale by 2^-32
+
+struct Plan {
+  int ns = 0, lag = 0, w = 0, n_nodes = 0, n_steps = 0, n_in = 0, n_out = 0, n_angles = 0;
+  int n_appends = 0, n_edges = 0, n_dropped = 0, min_window = 0;
+  int e_sv = 0, e_dm = 0;   // final growth exponents after the rescale schedule
+  std::vector<mbqc_step> steps;
+  std::vector<int32_t> in_index, init_sign, out_pos, out_slot;
+
+  // device copies, one per device id
+  std::mutex mu;
+  void* dev_tab[kMaxDevices] = {};
+  int n_blocks() const { return (n_steps + ns - 1) / ns; }
+};
+
+// Device table layout (int32 words), built by pack_device_tables():
+//   [steps: 4*n_steps][in_index: H][init_sign: H][out_pos: H]
+//   [sv_rescale: n_blocks][dm_rescale: n_blocks][sv_e_final, dm_e_final]
+struct TableOffsets {
+  int steps, in_index, init_sign, out_pos, sv_rescale, dm_rescale, e_final, total;
+};
+TableOffsets table_offsets(const Plan& p);
+std::vector<int32_t> pack_device_tables(const Plan& p);
+
+extern thread_local std::string g_last_error;
+int fail(int code, const std::string& msg);
+int compile_plan(const mbqc_pattern_desc* d, Plan* p);
+
+}  // namespace mbqc

@@ -1,0 +1,365 @@
+"""MI355X backends ``hip-sv`` and ``hip-dm``.
+
+They keep the constructor keywords, ``run``/``reset``/``measure`` semantics,
+error types and return forms of the reference's numpy backends
+(`mentpy/simulators/np_simulator_sv.py:38-384`,
+`mentpy/simulators/np_simulator_dm.py:33-380`): the schedule is the
+pattern's measurement order (or ``schedule=``), the window defaults to
+``|I|+1`` when a flow exists, every outcome is post-selected to 0, ``run``
+returns a numpy density matrix (SV: ``output_form="dm"`` default, or
+``"sv"``), and a second ``run`` without ``reset`` raises ``ValueError``.
+
+The arithmetic runs in the HIP kernels of ``libmbqc_hip.so`` in complex64;
+``run_batch`` exposes the batched device-resident call (torch tensors in HBM,
+no host copies).  There is no CPU path: without a GPU the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from typing import Optional, Sequence
+
+import numpy as np
+
+from .. import _native as nat
+from ..operators.ment import Ment
+from .base_simulator import BaseSimulator
+
+__all__ = ["HipSimulatorSV", "HipSimulatorDM", "compile_pattern", "sv_forward", "dm_forward"]
+
+
+def _torch():
+    import torch
+    return torch
+
+
+# ----------------------------------------------------------------- compile
+def compile_pattern(mbqcircuit, schedule=None, window_size: int = 1, exclude: str = "output",
+                    xy_only_error=ValueError):
+    """Flatten an ``MBQCircuit`` into the arrays of the C plan and compile it.
+
+    ``exclude`` selects the unmeasured nodes: ``"output"`` (statevector
+    backend, `np_simulator_sv.py:62-75`) or ``"quantum_output"`` (density
+    matrix backend, `np_simulator_dm.py:48-61`).  Returns
+    ``(NativePlan, schedule, schedule_measure, window)``.
+    """
+    graph = mbqcircuit.graph
+    nodes = list(graph.nodes())
+    idx = {v: i for i, v in enumerate(nodes)}
+    excl = mbqcircuit.output_nodes if exclude == "output" else mbqcircuit.quantum_output_nodes
+    if schedule is not None:
+        schedule = list(schedule)
+        sched_meas = [i for i in schedule if i not in excl]
+    elif mbqcircuit.measurement_order is not None:
+        schedule = list(mbqcircuit.measurement_order)
+        sched_meas = [i for i in schedule if i not in excl]
+        if window_size == 1 and mbqcircuit.flow is not None:
+            window_size = len(mbqcircuit.input_nodes) + 1
+    else:
+        raise ValueError("Schedule must be provided for numpy simulator as the MBQCircuit "
+                         "does not have a flow.")
+    if len(schedule) != len(nodes) or set(schedule) != set(nodes):
+        raise ValueError("schedule must contain every node of the pattern exactly once")
+    n_meas = len(sched_meas)
+    if schedule[:n_meas] != sched_meas:
+        raise NotImplementedError(
+            "schedules that place unmeasured nodes before measured ones are not supported")
+    trainable = list(mbqcircuit.trainable_nodes)
+    tpos = {v: k for k, v in enumerate(trainable)}
+    angle_index = np.full(len(nodes), -1, np.int32)
+    fixed = np.zeros(len(nodes), np.float32)
+    for v in sched_meas:
+        m = mbqcircuit[v]
+        if not isinstance(m, Ment):
+            raise ValueError(f"Ment has no matrix representation at qubit {v}")
+        if m.plane not in ("X", "Y", "XY"):
+            raise xy_only_error(
+                f"Node {v} has plane {m.plane}, but only XY plane is supported.")
+        if v in tpos:
+            angle_index[idx[v]] = tpos[v]
+        else:
+            fixed[idx[v]] = m.xy_angle()
+    edges = np.array([(idx[a], idx[b]) for a, b in graph.edges()], np.int32).reshape(-1, 2)
+    outputs = [v for v in schedule[n_meas:]]
+    target_out = list(mbqcircuit.output_nodes if exclude == "output"
+                      else mbqcircuit.quantum_output_nodes)
+    if set(target_out) != set(outputs):
+        raise ValueError("output nodes must be the unmeasured nodes of the schedule")
+    plan = nat.NativePlan(
+        n_nodes=len(nodes), edges=edges, schedule=[idx[v] for v in schedule], n_measure=n_meas,
+        window=int(window_size), input_nodes=[idx[v] for v in mbqcircuit.input_nodes],
+        output_nodes=[idx[v] for v in target_out], angle_index=angle_index, fixed_angle=fixed,
+        n_angles=len(trainable))
+    return plan, schedule, sched_meas, int(window_size)
+
+
+# ----------------------------------------------------------------- launches
+def _stream_ptr(device):
+    torch = _torch()
+    return C.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _as_angles(angles, n_angles: int, device):
+    torch = _torch()
+    a = torch.as_tensor(angles, dtype=torch.float32, device=device)
+    if a.dim() == 1:
+        a = a.unsqueeze(0)
+    if a.dim() != 2 or a.shape[1] != n_angles:
+        raise ValueError(f"angles must have shape [B, {n_angles}], got {tuple(a.shape)}")
+    return a.contiguous()
+
+
+def _as_states(states, n_qubits: int, device):
+    torch = _torch()
+    s = torch.as_tensor(states, device=device)
+    if not torch.is_complex(s):
+        s = s.to(torch.float32)
+    s = s.to(torch.complex64)
+    if s.dim() == 1:
+        s = s.unsqueeze(0)
+    if s.dim() != 2 or s.shape[1] != (1 << n_qubits):
+        raise ValueError(f"input states must have shape [S, {1 << n_qubits}], got {tuple(s.shape)}")
+    return s.contiguous()
+
+
+def _require_gpu(device):
+    torch = _torch()
+    if device.type != "cuda" or not torch.cuda.is_available():
+        raise RuntimeError("hip backends need a ROCm GPU device (device='cuda')")
+
+
+def sv_forward(plan: nat.NativePlan, angles, states, *, ref_phase: bool = True,
+               renormalize: bool = True, want_phase: bool = False, out=None):
+    """Batched statevector forward: ``angles [B, n]`` x ``states [S, 2^|I|]`` ->
+    ``[B, S, 2^|O|]`` complex64 on the device (and the per-simulation global
+    phase ``[B, S]`` if ``want_phase``)."""
+    torch = _torch()
+    device = states.device
+    _require_gpu(device)
+    info = plan.info
+    B, S = angles.shape[0], states.shape[0]
+    if out is None:
+        out = torch.empty((B, S, 1 << info["n_outputs"]), dtype=torch.complex64, device=device)
+    phase = torch.empty((B, S), dtype=torch.complex64, device=device) if want_phase else None
+    flags = ((nat.FLAG_REF_PHASE if ref_phase else 0) | (nat.FLAG_RENORMALIZE if renormalize else 0)
+             | (nat.FLAG_WRITE_PHASE if want_phase else 0))
+    with torch.cuda.device(device):
+        nat.check(nat.lib().mbqc_sv_forward_c64(
+            plan.handle, C.c_void_p(angles.data_ptr()), info["n_angles"],
+            C.c_void_p(states.data_ptr()), states.shape[1], C.c_void_p(out.data_ptr()),
+            C.c_void_p(phase.data_ptr() if phase is not None else 0), B, S, flags,
+            _stream_ptr(device)))
+    return (out, phase) if want_phase else out
+
+
+def dm_forward(plan: nat.NativePlan, angles, states, *, p_depol: float = 0.0,
+               p_readout: float = 0.0, renormalize: bool = True, out=None):
+    """Batched density-matrix forward -> ``[B, S, 2^|O|, 2^|O|]`` complex64."""
+    torch = _torch()
+    device = states.device
+    _require_gpu(device)
+    info = plan.info
+    B, S = angles.shape[0], states.shape[0]
+    d = 1 << info["n_outputs"]
+    if out is None:
+        out = torch.empty((B, S, d, d), dtype=torch.complex64, device=device)
+    noise = nat.Noise(p_depol=float(p_depol), p_readout=float(p_readout))
+    flags = nat.FLAG_RENORMALIZE if renormalize else 0
+    with torch.cuda.device(device):
+        nat.check(nat.lib().mbqc_dm_forward_c64(
+            plan.handle, C.c_void_p(angles.data_ptr()), info["n_angles"],
+            C.c_void_p(states.data_ptr()), states.shape[1], C.c_void_p(out.data_ptr()), B, S,
+            C.byref(noise), flags, _stream_ptr(device)))
+    return out
+
+
+# ----------------------------------------------------------------- backends
+class _HipBackend(BaseSimulator):
+    _exclude = "output"
+
+    def __init__(self, mbqcircuit, input_state=None, **kwargs):
+        super().__init__(mbqcircuit, input_state)
+        self.window_size = kwargs.pop("window_size", 1)
+        self.schedule = kwargs.pop("schedule", None)
+        self.force0 = kwargs.pop("force0", True)
+        self.dev_mode = kwargs.pop("dev_mode", False)
+        self.wires = kwargs.pop("wires", None)
+        self._device_arg = kwargs.pop("device", "cuda")
+        self._backend_kwargs(kwargs)
+        if kwargs:
+            raise TypeError(f"unexpected keyword arguments {sorted(kwargs)}")
+        if not self.force0:
+            raise NotImplementedError("Numpy simulator does not support force0=False.")
+        if self.dev_mode:
+            raise NotImplementedError("dev_mode scheduling is not supported by the hip backends.")
+        self._check_planes(mbqcircuit)
+        plan, schedule, sched_meas, window = compile_pattern(
+            mbqcircuit, self.schedule, self.window_size, self._exclude, self._plane_error)
+        self.plan = plan
+        self.schedule = schedule
+        self.schedule_measure = sched_meas
+        self.window_size = window
+        n_in = len(mbqcircuit.input_nodes)
+        if input_state is None:
+            input_state = np.ones(1)
+            for _ in range(n_in):
+                input_state = np.kron(input_state, np.array([1, 1]) / np.sqrt(2))
+        self._set_input(input_state)
+        self.current_measurement = 0
+        self.outcomes = {}
+
+    # hooks
+    _plane_error = ValueError
+
+    def _backend_kwargs(self, kwargs):
+        pass
+
+    def _check_planes(self, mbqcircuit):
+        pass
+
+    @property
+    def device(self):
+        torch = _torch()
+        return torch.device(self._device_arg)
+
+    def _set_input(self, state):
+        n_in = len(self.mbqcircuit.input_nodes)
+        arr = state.detach().cpu().numpy() if hasattr(state, "detach") else np.asarray(state)
+        arr = arr.reshape(-1)
+        if arr.shape[0] != (1 << n_in):
+            raise ValueError(f"Input state has {int(math.log2(max(arr.shape[0], 1)))} qubits, "
+                             f"but the pattern has {n_in} inputs.")
+        self.input_state = state
+        self._input_np = arr.astype(np.complex128)
+
+    def reset(self, input_state=None):
+        self.current_measurement = 0
+        if input_state is not None:
+            self._set_input(input_state)
+        self.outcomes = {}
+
+    def measure(self, angle: float, **kwargs):
+        raise NotImplementedError(
+            "step-by-step measure() is not provided by the hip backends; use run()")
+
+    def _check_run(self, angles, input_state):
+        if input_state is not None:
+            self.reset(input_state=input_state)
+        if len(angles) != len(self.mbqcircuit.trainable_nodes):
+            raise ValueError(f"Number of angles ({len(angles)}) does not match number of "
+                             f"trainable nodes ({len(self.mbqcircuit.trainable_nodes)}).")
+        if self.current_measurement >= len(self.schedule_measure) and self.schedule_measure:
+            raise ValueError("No more measurements to be done.")
+
+    def _finish_run(self):
+        self.current_measurement = len(self.schedule_measure)
+        self.outcomes = {v: 0 for v in self.schedule_measure}
+
+    def _device_inputs(self, angles):
+        torch = _torch()
+        dev = self.device
+        _require_gpu(dev)
+        a = _as_angles(np.asarray(angles, dtype=np.float64).reshape(1, -1),
+                       self.plan.info["n_angles"], dev)
+        s = _as_states(self._input_np, self.plan.info["n_inputs"], dev)
+        return a, s
+
+
+class HipSimulatorSV(_HipBackend):
+    """Statevector backend (post-selected, complex64 on the GPU).
+
+    Extra keywords: ``device`` (default ``"cuda"``), ``phase`` (``"reference"``:
+    the reference's global phase convention; ``"raw"``: the plain projector
+    contraction), ``renormalize`` (default True, as the reference normalizes
+    every step).
+    """
+
+    _exclude = "output"
+
+    def _backend_kwargs(self, kwargs):
+        self.phase = kwargs.pop("phase", "reference")
+        if self.phase not in ("reference", "raw"):
+            raise ValueError("phase must be 'reference' or 'raw'")
+        self.renormalize = kwargs.pop("renormalize", True)
+
+    def _check_planes(self, mbqcircuit):
+        for node in mbqcircuit.graph.nodes:
+            m = mbqcircuit[node]
+            if m is not None and m.plane not in ("X", "Y", "XY"):
+                raise ValueError(f"Node {node} has plane {m.plane}, but only XY plane is supported.")
+        if mbqcircuit.classical_output_nodes:
+            raise NotImplementedError("measured output nodes are not supported by hip-sv")
+
+    def run(self, angles, output_form: str = "dm", **kwargs):
+        self._check_run(angles, kwargs.get("input_state"))
+        form = output_form.lower()
+        if form not in ("dm", "densitymatrix", "sv", "statevector"):
+            raise ValueError(f"Output form {output_form} is not supported.")
+        a, s = self._device_inputs(angles)
+        out = sv_forward(self.plan, a, s, ref_phase=self.phase == "reference",
+                         renormalize=self.renormalize)
+        psi = out[0, 0].cpu().numpy().astype(np.complex128)
+        self._finish_run()
+        self.qstate = psi
+        if form in ("dm", "densitymatrix"):
+            return np.outer(psi, np.conj(psi))
+        return psi
+
+    def run_batch(self, angles, input_states=None, *, phase: Optional[str] = None,
+                  renormalize: Optional[bool] = None, want_phase: bool = False):
+        """All-pairs batch: ``angles [B, n]`` x ``input_states [S, 2^|I|]`` ->
+        device tensor ``[B, S, 2^|O|]`` (torch complex64, stays in HBM)."""
+        dev = self.device
+        _require_gpu(dev)
+        a = _as_angles(angles, self.plan.info["n_angles"], dev)
+        st = self._input_np if input_states is None else input_states
+        s = _as_states(st, self.plan.info["n_inputs"], dev)
+        ph = self.phase if phase is None else phase
+        rn = self.renormalize if renormalize is None else renormalize
+        return sv_forward(self.plan, a, s, ref_phase=ph == "reference", renormalize=rn,
+                          want_phase=want_phase)
+
+
+class HipSimulatorDM(_HipBackend):
+    """Density-matrix backend with optional noise.
+
+    Extra keywords: ``device``, ``noise`` (a ``mentpy_amd.noise.NoiseModel``)
+    or ``p_depol`` / ``p_readout`` floats, ``renormalize`` (default True: the
+    reference divides by the outcome probability each step).
+    """
+
+    _exclude = "quantum_output"
+    _plane_error = NotImplementedError
+
+    def _backend_kwargs(self, kwargs):
+        from ..noise import NoiseModel
+        noise = kwargs.pop("noise", None)
+        pd = kwargs.pop("p_depol", None)
+        pr = kwargs.pop("p_readout", None)
+        if noise is None:
+            noise = NoiseModel(depolarizing=pd or 0.0, readout_flip=pr or 0.0)
+        elif pd is not None or pr is not None:
+            raise ValueError("pass either noise= or p_depol/p_readout, not both")
+        self.noise = noise
+        self.renormalize = kwargs.pop("renormalize", True)
+
+    def run(self, angles, mode: str = "sample", input_state=None):
+        self._check_run(angles, input_state)
+        a, s = self._device_inputs(angles)
+        rho = dm_forward(self.plan, a, s, p_depol=self.noise.depolarizing,
+                         p_readout=self.noise.readout_flip, renormalize=self.renormalize)
+        out = rho[0, 0].cpu().numpy().astype(np.complex128)
+        self._finish_run()
+        self.qstate = out
+        return out
+
+    def run_batch(self, angles, input_states=None, renormalize: Optional[bool] = None):
+        """``angles [B, n]`` x pure ``input_states [S, 2^|I|]`` -> ``[B, S, d, d]``."""
+        dev = self.device
+        _require_gpu(dev)
+        a = _as_angles(angles, self.plan.info["n_angles"], dev)
+        st = self._input_np if input_states is None else input_states
+        s = _as_states(st, self.plan.info["n_inputs"], dev)
+        rn = self.renormalize if renormalize is None else renormalize
+        return dm_forward(self.plan, a, s, p_depol=self.noise.depolarizing,
+                          p_readout=self.noise.readout_flip, renormalize=rn)

@@ -1,0 +1,118 @@
+"""GPU parity of the hip-sv / hip-dm forward kernels against the complex128 oracle.
+
+Tolerances (complex64 arithmetic, float32 angles): max amplitude error 2e-5
+in the reference phase convention, output infidelity 1e-5 (the north-star
+bar), DM max entry error 2e-5 and trace distance 1e-5.
+"""
+import numpy as np
+import pytest
+
+from conftest import haar_state, infidelity, phase_aligned_max_err
+
+pytestmark = pytest.mark.gpu
+
+AMP_TOL = 2e-5
+FID_TOL = 1e-5
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+GRIDS = [(1, 3), (2, 3), (2, 5), (3, 4), (4, 6), (5, 7), (6, 8)]
+
+
+@pytest.mark.parametrize("rows,cols", GRIDS)
+def test_sv_grid_matches_oracle(rows, cols, rng):
+    torch = _torch()
+    import mentpy_amd as mp
+    from oracle.vectorized import sv_batch
+    g = mp.templates.grid_cluster(rows, cols) if rows > 1 else mp.templates.linear_cluster(cols)
+    sim = mp.PatternSimulator(g, backend="hip-sv")
+    B, S = 37, 3
+    th = rng.uniform(0, 2 * np.pi, (B, len(g.trainable_nodes)))
+    st = np.stack([haar_state(len(g.input_nodes), rng) for _ in range(S)])
+    out = sim.run_batch(th, st).cpu().numpy()
+    th32 = th.astype(np.float32).astype(np.float64)
+    ref = sv_batch(g, th32, st)
+    print(f"grid {rows}x{cols}: max amp err {np.abs(out - ref).max():.3e}", flush=True)
+    assert np.abs(out - ref).max() < AMP_TOL
+    assert infidelity(out, ref).max() < FID_TOL
+
+
+@pytest.mark.parametrize("n", [3, 5, 9])
+def test_teleportation_known_answer(n, rng):
+    """mentpy tests/test_simulators.py:15-30: all-zero angles teleport the input."""
+    _torch()
+    import mentpy_amd as mp
+    g = mp.templates.linear_cluster(n)
+    for backend in ("hip-sv", "hip-dm"):
+        ps = mp.PatternSimulator(g, backend=backend)
+        for _ in range(3):
+            st = haar_state(1, rng)
+            ps.reset(input_state=st)
+            out = ps([0] * (n - 1))
+            assert np.allclose(out, np.outer(st, st.conj()), atol=1e-5)
+
+
+def test_single_run_matches_reference_sv(rng):
+    _torch()
+    import mentpy_amd as mp
+    from oracle.reference_sv import ReferenceSV
+    g = mp.templates.grid_cluster(3, 5)
+    th = rng.uniform(0, 2 * np.pi, len(g.trainable_nodes))
+    psi = haar_state(3, rng)
+    ps = mp.PatternSimulator(g, input_state=psi, backend="hip-sv")
+    sv = ps(th, output_form="sv")
+    ref = ReferenceSV(g, psi).run(th.astype(np.float32).astype(np.float64), output_form="sv")
+    assert np.abs(sv - ref).max() < AMP_TOL
+    with pytest.raises(ValueError):
+        ps(th)   # second run without reset
+    ps.reset()
+    dm = ps(th)
+    assert dm.shape == (8, 8)
+
+
+@pytest.mark.parametrize("rows,cols,w", [(2, 4, 5), (2, 5, 5), (3, 3, 5), (2, 4, 4)])
+def test_sv_window_sizes(rows, cols, w, rng):
+    """Cross-backend case of mentpy tests/test_simulators.py:33-64 (window_size given)."""
+    _torch()
+    import mentpy_amd as mp
+    from oracle.reference_sv import ReferenceSV
+    g = mp.templates.grid_cluster(rows, cols)
+    nodes = g.outputc
+    g[nodes[1]] = mp.Ment("X")
+    g[nodes[-2]] = mp.Ment("Y")
+    th = rng.uniform(0, 2 * np.pi, len(g.trainable_nodes))
+    psi = haar_state(rows, rng)
+    ps = mp.PatternSimulator(g, input_state=psi, backend="hip-sv", window_size=w)
+    out = ps(th, output_form="sv")
+    ref = ReferenceSV(g, psi, window_size=w).run(th.astype(np.float32).astype(np.float64), output_form="sv")
+    assert np.abs(out - ref).max() < AMP_TOL
+
+
+GRIDS_DM = [(1, 4), (2, 3), (2, 6), (3, 4), (3, 12)]
+
+
+@pytest.mark.parametrize("rows,cols", GRIDS_DM)
+@pytest.mark.parametrize("noise", [(0.0, 0.0), (0.01, 0.005), (0.2, 0.1)])
+def test_dm_grid_matches_oracle(rows, cols, noise, rng):
+    _torch()
+    import mentpy_amd as mp
+    from oracle.vectorized import dm_batch
+    g = mp.templates.grid_cluster(rows, cols) if rows > 1 else mp.templates.linear_cluster(cols)
+    sim = mp.PatternSimulator(g, backend="hip-dm", p_depol=noise[0], p_readout=noise[1])
+    B, S = 29, 2
+    th = rng.uniform(0, 2 * np.pi, (B, len(g.trainable_nodes)))
+    st = np.stack([haar_state(len(g.input_nodes), rng) for _ in range(S)])
+    out = sim.run_batch(th, st).cpu().numpy()
+    ref = dm_batch(g, th.astype(np.float32).astype(np.float64), st, p_depol=noise[0], p_readout=noise[1])
+    err = np.abs(out - ref).max()
+    td = max(0.5 * np.abs(np.linalg.eigvalsh(out[b, s] - ref[b, s])).sum()
+             for b in range(B) for s in range(S))
+    print(f"dm {rows}x{cols} noise {noise}: max err {err:.3e} trace dist {td:.3e}", flush=True)
+    assert err < AMP_TOL
+    assert td < 1e-5
