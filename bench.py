@@ -67,6 +67,84 @@ def cpu_baseline(circ, psi, seconds=12.0, seed=7):
     return count / el, count, el
 
 
+def adjoint_section(circ, sim, dev, steps, stream, sets=(1024, 16384), S=256, seed=99):
+    """Adjoint angle-gradients/s on the MB-QML infidelity loss (one graph-replayed
+    step: adjoint forward, Gram-matrix loss/seed GEMM, adjoint backward) against
+    the plain GPU finite-difference loop (2 batched forward launches per angle);
+    gradient error vs the complex128 oracle's exact shift rule."""
+    import torch
+    from mentpy_amd.gradients import MBQMLStep, fd_loop_infidelity_grad
+    from mentpy_amd.operators.gates import haar_unitary
+    rng = np.random.default_rng(seed)
+    n_in = len(circ.input_nodes)
+    U = haar_unitary(2 ** n_in, rng)
+    X = np.stack([haar_unitary(2 ** n_in, rng)[:, 0] for _ in range(S)])
+    Y = X @ U.T
+    n = len(circ.trainable_nodes)
+    Xd = torch.as_tensor(X, dtype=torch.complex64, device=dev)
+    Yd = torch.as_tensor(Y, dtype=torch.complex64, device=dev)
+    res = {}
+    for B in sets:
+        th = torch.tensor(rng.uniform(0, 2 * np.pi, (B, n)), dtype=torch.float32, device=dev)
+        row = {}
+        for mode in ("gram", "direct"):
+            step = MBQMLStep(sim, Xd, Yd, n_sets=B, mode=mode)
+            step.angles.copy_(th)
+            graph = step.capture()
+            secs = time_launches(graph.replay, steps, 2, stream)
+            row[mode] = B * steps / secs
+            del graph, step
+        secs = time_launches(lambda: fd_loop_infidelity_grad(sim, th, Xd, Yd, h=1e-3), 1, 1, stream)
+        row["fd"] = B / secs
+        res[B] = row
+    from oracle.gradients import exact_shift_gradient
+    from oracle.vectorized import sv_batch
+    th0 = torch.tensor(np.random.default_rng(seed + 1).uniform(0, 2 * np.pi, (sets[0], n)),
+                       dtype=torch.float32, device=dev)
+    step = MBQMLStep(sim, Xd, Yd, n_sets=sets[0], mode="gram")
+    _, g = step.run(th0)
+    t0 = th0[0].double().cpu().numpy()
+
+    def cost(t):
+        o = sv_batch(circ, t[None], X)[0]
+        return 1.0 - np.mean(np.abs(np.sum(np.conj(Y) * o, -1)) ** 2)
+    ref = exact_shift_gradient(cost, t0)
+    err = float(np.abs(g[0].double().cpu().numpy() - ref).max())
+    big, small = sets[-1], sets[0]
+    out = {"adjoint_grads_per_s": res[big]["gram"], "fd_loop_grads_per_s": res[big]["fd"],
+           "adjoint_vs_fd_loop": res[big]["gram"] / res[big]["fd"],
+           "adjoint_direct_grads_per_s": res[big]["direct"],
+           "adjoint_direct_vs_fd_loop": res[big]["direct"] / res[big]["fd"],
+           "adjoint_grads_per_s_1024": res[small]["gram"], "fd_loop_grads_per_s_1024": res[small]["fd"],
+           "adjoint_vs_fd_loop_1024": res[small]["gram"] / res[small]["fd"],
+           "grad_max_abs_err_vs_c128": err,
+           "grad_workload": f"4x20 grid ({n} angles), loss 1-mean_s|<U psi_s|out_s>|^2 over {S} Haar "
+                            f"inputs; adjoint = one graph-replayed step (transfer matrix + data Gram "
+                            f"matrix GEMM); {big} parameter sets (and {small})"}
+    return out, (X, Y, t0, cost)
+
+
+def cpu_gradient_baseline(circ, X, Y, t0, seconds_per_eval_cap=30.0):
+    """mentpy's central FD gradient with the step-by-step numpy restatement:
+    2 cost evaluations (S sims each) of one angle timed, scaled to all angles."""
+    from oracle.reference_sv import ReferenceSV
+    sim = ReferenceSV(circ, X[0])
+    n = len(circ.trainable_nodes)
+
+    def cost(t):
+        acc = 0.0
+        for x, y in zip(X, Y):
+            o = sim.run(t, output_form="sv", input_state=x)
+            acc += 1 - abs(np.vdot(y, o)) ** 2
+        return acc / len(X)
+    e = np.zeros(n)
+    e[0] = 1e-5
+    tic = time.perf_counter()
+    (cost(t0 + e) - cost(t0 - e)) / 2e-5
+    el = time.perf_counter() - tic
+    return 1.0 / (el * n), el
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +153,9 @@ def main():
     ap.add_argument("--batch", type=int, default=1 << 21)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--grad-sets", type=int, nargs="+", default=[1024, 16384])
+    ap.add_argument("--grad-inputs", type=int, default=256)
+    ap.add_argument("--cpu-grad-inputs", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -96,8 +177,12 @@ def main():
     n_ang = len(circ.trainable_nodes)
     psi = haar_unitary(16, rng)[:, 0]
     B = args.batch
-    angles = torch.rand((B, n_ang), generator=torch.Generator().manual_seed(11),
-                        dtype=torch.float32).mul_(2 * np.pi).to(dev)
+    # angle-major storage [n_angles, B] (each step's angles are contiguous across
+    # the batch, so every angle load of a wavefront is one coalesced 256-byte read);
+    # the kernels see it as the [B, n_angles] view angles_t.t()
+    angles_t = torch.rand((n_ang, B), generator=torch.Generator().manual_seed(11),
+                          dtype=torch.float32).mul_(2 * np.pi).to(dev)
+    angles = angles_t.t()
     states = torch.as_tensor(psi, dtype=torch.complex64, device=dev).reshape(1, -1)
     out = torch.empty((B, 1, 16), dtype=torch.complex64, device=dev)
 
@@ -130,15 +215,26 @@ def main():
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "c64", "data": "synthetic (seeded uniform angles, Haar input)",
         "config": {"workload": "grid_cluster(4,20) hip-sv forward, B angle vectors x 1 Haar input",
-                   "batch": B, "n_angles": n_ang, "window": plan.info["window"]},
+                   "batch": B, "n_angles": n_ang, "window": plan.info["window"],
+                   "angle_layout": "angle-major [n_angles, B]"},
         "max_amp_err_vs_c128": amp_err, "max_infidelity_vs_c128": fid_err,
         "roofline": roofline,
     }
+    adj, (X, Y, t0, _) = adjoint_section(circ, sim, dev, max(2, args.steps // 4), stream,
+                                         sets=tuple(args.grad_sets), S=args.grad_inputs)
+    line.update(adj)
     if not args.no_cpu:
         rate, count, el = cpu_baseline(circ, psi, args.cpu_seconds)
         line["cpu_baseline"] = {"value": rate, "unit": "sims/s", "cores": 1, "kind": "port",
                                 "sample": f"{count} sims of grid_cluster(4,20) in {el:.1f} s, "
                                           "step-by-step numpy complex128 restatement, 1 thread"}
+        grate, gel = cpu_gradient_baseline(circ, X[:args.cpu_grad_inputs], Y[:args.cpu_grad_inputs], t0)
+        grate = grate * args.cpu_grad_inputs / len(X)
+        line["cpu_baseline_grads"] = {
+            "value": grate, "unit": "grads/s", "cores": 1, "kind": "port",
+            "sample": f"central FD of one angle over {args.cpu_grad_inputs} of the {len(X)} inputs "
+                      f"({gel:.1f} s), scaled linearly to all angles and inputs"}
+        line["adjoint_vs_cpu"] = line["adjoint_grads_per_s"] / grate
     print(json.dumps(line), flush=True)
 
 

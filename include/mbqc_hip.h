@@ -102,12 +102,14 @@ int mbqc_plan_get_rescale(const mbqc_plan* plan, int32_t* sv_flags, int32_t* dm_
 /* Statevector forward, complex64: replaces NumpySimulatorSV.run / measure /
  * measure_ment (np_simulator_sv.py:164-297, 322-358) and the partial trace it
  * calls (mentpy/calculator/state_ops.py:42-74).  Simulation g = b*S + s uses
- * angle vector b (angles + b*angle_stride, angle_stride 0 = shared) and input
- * s (psi_in + s*in_stride amplitudes, in_stride 0 = shared).  out is
+ * angle a of vector b at angles[b*angle_stride + a*angle_stride_a]
+ * (angle_stride 0 = one shared vector; angle-major storage, angle_stride 1 and
+ * angle_stride_a = B, gives coalesced loads) and input s at
+ * psi_in + s*in_stride amplitudes (in_stride 0 = shared).  out is
  * [B*S][2^n_outputs]; phase_out (optional, MBQC_FLAG_WRITE_PHASE) is [B*S]. */
 int mbqc_sv_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
-                        const void* psi_in, int64_t in_stride, void* out, void* phase_out,
-                        int64_t B, int64_t S, uint32_t flags, void* stream);
+                        int64_t angle_stride_a, const void* psi_in, int64_t in_stride, void* out,
+                        void* phase_out, int64_t B, int64_t S, uint32_t flags, void* stream);
 
 /* Density-matrix forward, complex64: replaces NumpySimulatorDM.run / measure /
  * measure_ment (np_simulator_dm.py:151-283, 307-346) and
@@ -116,8 +118,8 @@ int mbqc_sv_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
  * Inputs are state vectors (the reference purifies them, np_simulator_dm.py:89-92).
  * out is [B*S][2^n_outputs][2^n_outputs]. */
 int mbqc_dm_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
-                        const void* psi_in, int64_t in_stride, void* out, int64_t B, int64_t S,
-                        const mbqc_noise* noise, uint32_t flags, void* stream);
+                        int64_t angle_stride_a, const void* psi_in, int64_t in_stride, void* out,
+                        int64_t B, int64_t S, const mbqc_noise* noise, uint32_t flags, void* stream);
 
 /* Adjoint angle gradient (replaces the 2n cost evaluations of fd_gradient /
  * psr_gradient, mentpy/gradients/_finite_difference.py:9-24,
@@ -128,16 +130,18 @@ int mbqc_dm_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
  * parameter set b are simulations b*C + c. */
 size_t mbqc_sv_adjoint_workspace_bytes(const mbqc_plan* plan, int64_t n_columns);
 int mbqc_sv_adjoint_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
-                                const void* psi_in, int64_t in_stride, void* out,
-                                int64_t B, int64_t C, void* ckpt, size_t ckpt_bytes, void* stream);
+                                int64_t angle_stride_a, const void* psi_in, int64_t in_stride,
+                                void* out, int64_t B, int64_t C, void* ckpt, size_t ckpt_bytes,
+                                void* stream);
 /* Backward sweep: seed[B*C][2^n_outputs] is dL/d(conj out) in the convention
  * dL = Re sum conj(seed) d(out).  Writes grad_partial[B][P][n_angles] with
  * P = mbqc_sv_adjoint_partials(C); the gradient is the sum over P.
  * Entries of non-trainable angles are left untouched. */
 int32_t mbqc_sv_adjoint_partials(int64_t C);
 int mbqc_sv_adjoint_backward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
-                                 const void* seed, int64_t B, int64_t C, const void* ckpt,
-                                 size_t ckpt_bytes, float* grad_partial, void* stream);
+                                 int64_t angle_stride_a, const void* seed, int64_t B, int64_t C,
+                                 const void* ckpt, size_t ckpt_bytes, float* grad_partial,
+                                 void* stream);
 
 /* Batched forward of many plans in one persistent launch (no reference
  * counterpart: the reference runs one simulation per Python call).  Plans are
@@ -145,8 +149,9 @@ int mbqc_sv_adjoint_backward_c64(const mbqc_plan* plan, const float* angles, int
 typedef struct mbqc_batch mbqc_batch;
 typedef struct mbqc_batch_job {
   const mbqc_plan* plan;
-  const float* angles;   /* device [B][n_angles] */
+  const float* angles;   /* device, angle a of vector b at b*angle_stride + a*angle_stride_a */
   int64_t angle_stride;
+  int64_t angle_stride_a;
   const void* psi_in;    /* device [S][2^n_inputs] */
   int64_t in_stride;
   void* out;             /* device [B*S][2^n_outputs] */

@@ -4,7 +4,7 @@ Keeps the ``PatternSimulator`` / ``MBQCircuit`` call surface of mentpy and runs
 the simulate-with-angles path in hand-written HIP kernels (gfx950) behind a C
 ABI (``include/mbqc_hip.h``).
 """
-from . import mbqc, noise, operators
+from . import gradients, mbqc, noise, operators, optimizers, utils
 from .mbqc import GraphState, MBQCircuit, hstack, templates, vstack
 from .mbqc.flow import find_cflow
 from .noise import NoiseModel

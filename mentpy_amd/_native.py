@@ -64,7 +64,7 @@ class Noise(C.Structure):
 
 class BatchJob(C.Structure):
     _fields_ = [("plan", C.c_void_p), ("angles", C.c_void_p), ("angle_stride", C.c_int64),
-                ("psi_in", C.c_void_p), ("in_stride", C.c_int64), ("out", C.c_void_p),
+                ("angle_stride_a", C.c_int64), ("psi_in", C.c_void_p), ("in_stride", C.c_int64), ("out", C.c_void_p),
                 ("B", C.c_int64), ("S", C.c_int64)]
 
 
@@ -99,17 +99,17 @@ def lib() -> C.CDLL:
     L.mbqc_plan_get_tables.restype = C.c_int
     L.mbqc_plan_get_rescale.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
     L.mbqc_plan_get_rescale.restype = C.c_int
-    L.mbqc_sv_forward_c64.argtypes = [vp, vp, i64, vp, i64, vp, vp, i64, i64, u32, vp]
+    L.mbqc_sv_forward_c64.argtypes = [vp, vp, i64, i64, vp, i64, vp, vp, i64, i64, u32, vp]
     L.mbqc_sv_forward_c64.restype = C.c_int
-    L.mbqc_dm_forward_c64.argtypes = [vp, vp, i64, vp, i64, vp, i64, i64, C.POINTER(Noise), u32, vp]
+    L.mbqc_dm_forward_c64.argtypes = [vp, vp, i64, i64, vp, i64, vp, i64, i64, C.POINTER(Noise), u32, vp]
     L.mbqc_dm_forward_c64.restype = C.c_int
     L.mbqc_sv_adjoint_workspace_bytes.argtypes = [vp, i64]
     L.mbqc_sv_adjoint_workspace_bytes.restype = sz
-    L.mbqc_sv_adjoint_forward_c64.argtypes = [vp, vp, i64, vp, i64, vp, i64, i64, vp, sz, vp]
+    L.mbqc_sv_adjoint_forward_c64.argtypes = [vp, vp, i64, i64, vp, i64, vp, i64, i64, vp, sz, vp]
     L.mbqc_sv_adjoint_forward_c64.restype = C.c_int
     L.mbqc_sv_adjoint_partials.argtypes = [i64]
     L.mbqc_sv_adjoint_partials.restype = i32
-    L.mbqc_sv_adjoint_backward_c64.argtypes = [vp, vp, i64, vp, i64, i64, vp, sz, vp, vp]
+    L.mbqc_sv_adjoint_backward_c64.argtypes = [vp, vp, i64, i64, vp, i64, i64, vp, sz, vp, vp]
     L.mbqc_sv_adjoint_backward_c64.restype = C.c_int
     L.mbqc_batch_create.argtypes = [C.POINTER(BatchJob), i32, i32, C.POINTER(vp)]
     L.mbqc_batch_create.restype = C.c_int

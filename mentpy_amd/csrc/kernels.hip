@@ -38,6 +38,32 @@ __device__ __forceinline__ uint32_t czbit(uint32_t j, uint32_t mask) {
   return uint32_t(__builtin_popcount(j & mask) & 1) << 31;
 }
 
+// sin and cos of x for |x| up to a few thousand: x = k pi/2 + r with a two-term
+// Cody-Waite split of pi/2 (hi part exact in float), |r| <= pi/4, then the
+// Taylor series of sin (to r^9) and cos (to r^10), whose truncation error on
+// [-pi/4, pi/4] is below 2e-9, and the quadrant of k.
+__device__ __forceinline__ void sincos_poly(float x, float* sn, float* cs) {
+  const float k = rintf(x * 0.63661977236758134308f);
+  float r = fmaf(-k, 1.57079637050628662109f, x);
+  r = fmaf(-k, -4.37113900018624283e-8f, r);
+  const float r2 = r * r;
+  float ps = fmaf(r2, 2.75573192239858907e-6f, -1.98412698412698413e-4f);
+  ps = fmaf(r2, ps, 8.33333333333333333e-3f);
+  ps = fmaf(r2, ps, -1.66666666666666667e-1f);
+  const float sr = fmaf(r * r2, ps, r);
+  float pc = fmaf(r2, -2.75573192239858907e-7f, 2.48015873015873016e-5f);
+  pc = fmaf(r2, pc, -1.38888888888888889e-3f);
+  pc = fmaf(r2, pc, 4.16666666666666667e-2f);
+  pc = fmaf(r2, pc, -0.5f);
+  const float cr = fmaf(r2, pc, 1.0f);
+  const int q = int(k);
+  const bool swap = q & 1;
+  const float s0 = swap ? cr : sr;
+  const float c0 = swap ? sr : cr;
+  *sn = __uint_as_float(__float_as_uint(s0) ^ (uint32_t(q & 2) << 30));
+  *cs = __uint_as_float(__float_as_uint(c0) ^ (uint32_t((q + 1) & 2) << 30));
+}
+
 // e^{-i theta}; with the reference phase also the per-step global phase
 // sgn(cos(theta/2)) e^{i theta/2} (the factor the reference's amplitude-sum
 // partial trace introduces, mentpy/calculator/state_ops.py:68-73).
@@ -45,66 +71,42 @@ template <bool REF>
 __device__ __forceinline__ float2 step_coeff(float th, float2* ph) {
   if constexpr (REF) {
     float s2, c2;
-    sincosf(0.5f * th, &s2, &c2);
+    sincos_poly(0.5f * th, &s2, &c2);
     const float2 b = make_float2(fmaf(c2, c2, -s2 * s2), -2.0f * s2 * c2);
     const float sg = c2 < 0.0f ? -1.0f : 1.0f;
     *ph = cmul(*ph, make_float2(fabsf(c2), sg * s2));
     return b;
   } else {
     float s, c;
-    sincosf(th, &s, &c);
+    sincos_poly(th, &s, &c);
     return make_float2(c, -s);
   }
 }
 
 // ---------------------------------------------------------------- SV step
-// Measure slot Q at XY angle theta (b = e^{-i theta}) and, if the append flag
-// is set, put the next scheduled qubit (|+> with CZs to `nmask`, and to the
-// measured qubit when flag bit1 is set) into slot Q.  With x/y the halves of
-// the state where slot Q is 0/1:  u0 = x + b y,  u1 = x -/+ b y,
-// new[Q=0] = u0, new[Q=1] = sign(nmask) u1.  The 1/sqrt(2) of the |+> and of
-// the projector are left out (tracked as an exponent, applied at the end).
+// Measure slot Q at XY angle theta (b = e^{-i theta}) and put the next
+// scheduled qubit (|+> with its CZs) into slot Q.  With x / y the halves of the
+// state where slot Q is 0 / 1:  u0 = x + b y  is the new Q=0 half and
+// alpha u0 + beta x  the new Q=1 half, where the per-step, per-pair
+// coefficients (host-computed) fold in the CZ signs, the CZ with the measured
+// qubit and the "no append" case (alpha = beta = 0).  The 1/sqrt(2) factors of
+// the |+> and of the projector are left out (tracked as an exponent).
 template <int NS, int Q>
-__device__ __forceinline__ void sv_step(float2 (&t)[1 << NS], float2 b, uint32_t flags,
-                                        uint32_t nmask) {
+__device__ __forceinline__ void sv_step(float2 (&t)[1 << NS], float2 b, const float* coef) {
   constexpr int H = 1 << NS;
   constexpr int QB = 1 << Q;
-  if (flags & 1u) {
-    if (flags & 2u) {
+  int i = 0;
 #pragma unroll
-      for (int j = 0; j < H; ++j) {
-        if (j & QB) continue;
-        const float2 x = t[j], y = t[j | QB];
-        float2 u0;
-        u0.x = fmaf(-b.y, y.y, fmaf(b.x, y.x, x.x));
-        u0.y = fmaf(b.x, y.y, fmaf(b.y, y.x, x.y));
-        const float2 u1 = make_float2(fmaf(2.0f, x.x, -u0.x), fmaf(2.0f, x.y, -u0.y));
-        t[j] = u0;
-        t[j | QB] = flip(u1, czbit(j, nmask));
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < H; ++j) {
-        if (j & QB) continue;
-        const float2 x = t[j], y = t[j | QB];
-        float2 u0;
-        u0.x = fmaf(-b.y, y.y, fmaf(b.x, y.x, x.x));
-        u0.y = fmaf(b.x, y.y, fmaf(b.y, y.x, x.y));
-        t[j] = u0;
-        t[j | QB] = flip(u0, czbit(j, nmask));
-      }
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < H; ++j) {
-      if (j & QB) continue;
-      const float2 x = t[j], y = t[j | QB];
-      float2 u0;
-      u0.x = fmaf(-b.y, y.y, fmaf(b.x, y.x, x.x));
-      u0.y = fmaf(b.x, y.y, fmaf(b.y, y.x, x.y));
-      t[j] = u0;
-      t[j | QB] = make_float2(0.0f, 0.0f);
-    }
+  for (int j = 0; j < H; ++j) {
+    if (j & QB) continue;
+    const float al = coef[2 * i], be = coef[2 * i + 1];
+    ++i;
+    const float2 x = t[j], y = t[j | QB];
+    float2 u0;
+    u0.x = fmaf(-b.y, y.y, fmaf(b.x, y.x, x.x));
+    u0.y = fmaf(b.x, y.y, fmaf(b.y, y.x, x.y));
+    t[j] = u0;
+    t[j | QB] = make_float2(fmaf(al, u0.x, be * x.x), fmaf(al, u0.y, be * x.y));
   }
 }
 
@@ -113,9 +115,10 @@ __device__ __forceinline__ void sv_step(float2 (&t)[1 << NS], float2 b, uint32_t
 struct Job {
   const int32_t* tab;       // packed device tables of the plan
   int32_t off_in, off_sign, off_out, off_svres, off_dmres, e_sv, e_dm;
-  int32_t ns, n_steps, n_out, live_mask;
+  int32_t ns, n_steps, n_out, live_mask, off_coef;
   const float* angles;
-  int64_t angle_stride;
+  int64_t angle_stride;     // elements between angle vectors (0: shared)
+  int64_t angle_stride_a;   // elements between the angles of one vector
   const float2* psi;
   int64_t in_stride;
   float2* out;
@@ -123,35 +126,85 @@ struct Job {
   int64_t S, total;
 };
 
-template <int NS, bool REF, int SUB>
-__device__ __forceinline__ void sv_sub(float2 (&t)[1 << NS], const mbqc_step* steps, int k0,
+template <int NS, bool REF, bool FULL, int SUB>
+__device__ __forceinline__ void sv_sub(float2 (&t)[1 << NS], const float* coef, int k0,
                                        int n_steps, const float (&th)[NS], float2* ph) {
   const int k = k0 + SUB;
-  if (k < n_steps) {
-    const uint32_t fl = steps[k].flags;
-    const uint32_t nm = steps[k].nmask;
+  if (FULL || k < n_steps) {
     const float2 b = step_coeff<REF>(th[SUB], ph);
-    sv_step<NS, NS - 1 - SUB>(t, b, fl, nm);
+    sv_step<NS, NS - 1 - SUB>(t, b, coef + (k << NS));
   }
 }
 
-template <int NS, bool REF, int... SUBS>
-__device__ __forceinline__ void sv_block(float2 (&t)[1 << NS], const mbqc_step* steps, int k0,
+template <int NS, bool REF, bool FULL, int... SUBS>
+__device__ __forceinline__ void sv_block(float2 (&t)[1 << NS], const float* coef, int k0,
                                          int n_steps, const float (&th)[NS], float2* ph,
                                          std::integer_sequence<int, SUBS...>) {
-  (sv_sub<NS, REF, SUBS>(t, steps, k0, n_steps, th, ph), ...);
+  (sv_sub<NS, REF, FULL, SUBS>(t, coef, k0, n_steps, th, ph), ...);
+}
+
+// Angles of the NS steps of a block (steps past the end get 0).  The index is
+// clamped so the load is always issued from a valid address; fixed-angle
+// steps select their stored angle afterwards.
+template <int NS>
+__device__ __forceinline__ void fetch_block_angles(const mbqc_step* steps, int k0, int n_steps,
+                                                   const float* ang, int64_t sa, float (&th)[NS]) {
+#pragma unroll
+  for (int sub = 0; sub < NS; ++sub) {
+    const int k = k0 + sub;
+    float v = 0.0f;
+    if (k < n_steps) {
+      const int ai = steps[k].angle_index;
+      const float a = ang[int64_t(ai >= 0 ? ai : 0) * sa];
+      v = ai >= 0 ? a : steps[k].fixed_angle;
+    }
+    th[sub] = v;
+  }
+}
+
+// Run all steps: full blocks of NS unrolled steps with the next block's
+// angles in flight, then the partial tail block.  `on_block(blk, t)` runs at
+// the start of every block (after its rescale).
+template <int NS, bool REF, typename OnBlock>
+__device__ __forceinline__ void sv_run_steps(float2 (&t)[1 << NS], const Job& J, const float* ang,
+                                             float2* ph, OnBlock on_block) {
+  constexpr int H = 1 << NS;
+  const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
+  const int32_t* resc = J.tab + J.off_svres;
+  const float* coef = reinterpret_cast<const float*>(J.tab + J.off_coef);
+  const int n = J.n_steps;
+  const int nfull = n / NS;
+  const int nblk = (n + NS - 1) / NS;
+  float th_next[NS];
+  const int64_t sa = J.angle_stride_a;
+  fetch_block_angles<NS>(steps, 0, n, ang, sa, th_next);
+  for (int blk = 0; blk < nblk; ++blk) {
+    float th[NS];
+#pragma unroll
+    for (int s2 = 0; s2 < NS; ++s2) th[s2] = th_next[s2];
+    if (blk + 1 < nblk) fetch_block_angles<NS>(steps, (blk + 1) * NS, n, ang, sa, th_next);
+    if (resc[blk]) {
+#pragma unroll
+      for (int j = 0; j < H; ++j) t[j] = cscale(t[j], 0x1p-32f);
+    }
+    on_block(blk, t);
+    if (blk < nfull)
+      sv_block<NS, REF, true>(t, coef, blk * NS, n, th, ph, std::make_integer_sequence<int, NS>{});
+    else
+      sv_block<NS, REF, false>(t, coef, blk * NS, n, th, ph, std::make_integer_sequence<int, NS>{});
+  }
 }
 
 template <int NS>
 __device__ __forceinline__ void load_block_angles(const mbqc_step* steps, int k0, int n_steps,
-                                                  const float* ang, float (&th)[NS]) {
+                                                  const float* ang, int64_t sa, float (&th)[NS]) {
 #pragma unroll
   for (int sub = 0; sub < NS; ++sub) {
     const int k = k0 + sub;
     th[sub] = 0.0f;
     if (k < n_steps) {
       const int ai = steps[k].angle_index;
-      th[sub] = ai >= 0 ? ang[ai] : steps[k].fixed_angle;
+      th[sub] = ai >= 0 ? ang[int64_t(ai) * sa] : steps[k].fixed_angle;
     }
   }
 }
@@ -180,22 +233,12 @@ __device__ __forceinline__ void sv_sim(const Job& J, int64_t g, bool active, uin
   const int64_t gg = active ? g : 0;
   const int64_t bi = gg / J.S, si = gg - bi * J.S;
   const float* ang = J.angles + bi * J.angle_stride;
-  const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
-  const int32_t* resc = J.tab + J.off_svres;
   float2 t[H];
   sv_load_input<NS>(J, J.psi + si * J.in_stride, t);
   float2 ph = make_float2(1.0f, 0.0f);
-  const int n = J.n_steps;
-  const int nblk = (n + NS - 1) / NS;
-  for (int blk = 0; blk < nblk; ++blk) {
-    if (resc[blk]) {
-#pragma unroll
-      for (int j = 0; j < H; ++j) t[j] = cscale(t[j], 0x1p-32f);
-    }
-    float th[NS];
-    load_block_angles<NS>(steps, blk * NS, n, ang, th);
-    sv_block<NS, REF>(t, steps, blk * NS, n, th, &ph, std::make_integer_sequence<int, NS>{});
-  }
+  sv_run_steps<NS, REF>(t, J, ang, &ph, [](int, const float2 (&)[H]) {});
+  // the accumulated phase drifts off unit modulus in float; renormalise it
+  ph = cscale(ph, 1.0f / sqrtf(fmaf(ph.x, ph.x, ph.y * ph.y)));
   // output gather: slot-index j -> output index out_pos[j] (dead slots skipped)
   const int32_t* out_pos = J.tab + J.off_out;
   float scale = sv_final_scale(J.e_sv);
@@ -346,7 +389,7 @@ __global__ __launch_bounds__(256) void k_dm_forward(Job J, float gamma, float p_
         for (int k = 0; k < H; ++k) r[j][k] = cscale(r[j][k], 0x1p-32f);
     }
     float th[NS];
-    load_block_angles<NS>(steps, blk * NS, n, ang, th);
+    load_block_angles<NS>(steps, blk * NS, n, ang, J.angle_stride_a, th);
     dm_block<NS>(r, steps, blk * NS, n, th, gamma, std::make_integer_sequence<int, NS>{});
   }
   dm_depolarize_outputs<NS>(r, uint32_t(J.live_mask), p_out, std::make_integer_sequence<int, NS>{});
@@ -387,26 +430,16 @@ __global__ __launch_bounds__(256) void k_sv_adj_forward(Job J, float2* ckpt) {
   const int64_t gg = active ? g : 0;
   const int64_t bi = gg / J.S, si = gg - bi * J.S;
   const float* ang = J.angles + bi * J.angle_stride;
-  const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
-  const int32_t* resc = J.tab + J.off_svres;
   float2 t[H];
   sv_load_input<NS>(J, J.psi + si * J.in_stride, t);
   float2 ph = make_float2(1.f, 0.f);
-  const int n = J.n_steps;
-  const int nblk = (n + NS - 1) / NS;
-  for (int blk = 0; blk < nblk; ++blk) {
-    if (resc[blk]) {
-#pragma unroll
-      for (int j = 0; j < H; ++j) t[j] = cscale(t[j], 0x1p-32f);
-    }
+  const int64_t total = J.total;
+  sv_run_steps<NS, false>(t, J, ang, &ph, [&](int blk, const float2 (&tt)[H]) {
     if (active) {
 #pragma unroll
-      for (int j = 0; j < H; ++j) ckpt[(int64_t(blk) * H + j) * J.total + g] = t[j];
+      for (int j = 0; j < H; ++j) ckpt[(int64_t(blk) * H + j) * total + g] = tt[j];
     }
-    float th[NS];
-    load_block_angles<NS>(steps, blk * NS, n, ang, th);
-    sv_block<NS, false>(t, steps, blk * NS, n, th, &ph, std::make_integer_sequence<int, NS>{});
-  }
+  });
   const int32_t* out_pos = J.tab + J.off_out;
   const float scale = sv_final_scale(J.e_sv);
   if (active) {
@@ -419,32 +452,28 @@ __global__ __launch_bounds__(256) void k_sv_adj_forward(Job J, float2* ckpt) {
   }
 }
 
-// Adjoint of one SV step (slot Q).  Forward: u0 = x + b y, u1 = x + s b y,
-// new0 = u0, new1 = sigma u1 (append) or new1 = 0.  Given the cotangent lam of
-// the new state (dL = Re sum conj(lam) d(new)):
-//   g0 = lam0, g1 = sigma lam1, h = g0 + s g1,
-//   lam_x = g0 + g1, lam_y = conj(b) h,  dL/dtheta += Im(b sum y conj(h)).
+// Adjoint of one SV step (slot Q).  Forward: u0 = x + b y, new0 = u0,
+// new1 = alpha u0 + beta x.  Given the cotangent lam of the new state
+// (dL = Re sum conj(lam) d(new)):  h = lam0 + alpha lam1 (cotangent of u0),
+//   lam_x = h + beta lam1,  lam_y = conj(b) h,  dL/dtheta += Im(b sum y conj(h)).
 template <int NS, int Q>
 __device__ __forceinline__ float sv_adj_step(float2 (&lam)[1 << NS], const float2 (&y)[1 << (NS - 1)],
-                                             float2 b, uint32_t flags, uint32_t nmask) {
+                                             float2 b, const float* coef) {
   constexpr int H = 1 << NS;
   constexpr int QB = 1 << Q;
-  const bool app = flags & 1u, sneg = flags & 2u;
   const float2 bc = cconj(b);
   float ar = 0.f, ai = 0.f;
   int i = 0;
 #pragma unroll
   for (int j = 0; j < H; ++j) {
     if (j & QB) continue;
-    const float2 g0 = lam[j];
-    float2 g1 = make_float2(0.f, 0.f);
-    if (app) g1 = flip(lam[j | QB], czbit(j, nmask));
-    const float2 h = sneg ? csub(g0, g1) : cadd(g0, g1);
+    const float al = coef[2 * i], be = coef[2 * i + 1];
+    const float2 l0 = lam[j], l1 = lam[j | QB];
+    const float2 h = make_float2(fmaf(al, l1.x, l0.x), fmaf(al, l1.y, l0.y));
     const float2 yy = y[i++];
-    // acc += y * conj(h)
     ar = fmaf(yy.x, h.x, fmaf(yy.y, h.y, ar));
     ai = fmaf(yy.y, h.x, fmaf(-yy.x, h.y, ai));
-    lam[j] = cadd(g0, g1);
+    lam[j] = make_float2(fmaf(be, l1.x, h.x), fmaf(be, l1.y, h.y));
     lam[j | QB] = cmul(bc, h);
   }
   // Im(b * acc)
@@ -472,24 +501,25 @@ struct AdjCtx {
 
 template <int NS, int SUB>
 __device__ __forceinline__ void adj_recompute(float2 (&t)[1 << NS], float2 (&ys)[NS][1 << (NS - 1)],
-                                              float2 (&bs)[NS], const mbqc_step* steps, int k0, int n,
+                                              float2 (&bs)[NS], const float* coef, int k0, int n,
                                               const float (&th)[NS]) {
   const int k = k0 + SUB;
   if (k < n) {
     float2 ph = make_float2(1.f, 0.f);
     bs[SUB] = step_coeff<false>(th[SUB], &ph);
     take_y<NS, NS - 1 - SUB>(t, ys[SUB]);
-    if (SUB + 1 < NS) sv_step<NS, NS - 1 - SUB>(t, bs[SUB], steps[k].flags, steps[k].nmask);
+    if (SUB + 1 < NS) sv_step<NS, NS - 1 - SUB>(t, bs[SUB], coef + (k << NS));
   }
 }
 
 template <int NS, int SUB>
 __device__ __forceinline__ void adj_back(float2 (&lam)[1 << NS], const float2 (&ys)[NS][1 << (NS - 1)],
-                                         const float2 (&bs)[NS], const mbqc_step* steps, int k0, int n,
-                                         const AdjCtx& A, int64_t bi, int64_t ci, bool active) {
+                                         const float2 (&bs)[NS], const mbqc_step* steps,
+                                         const float* coef, int k0, int n, const AdjCtx& A,
+                                         int64_t bi, int64_t ci, bool active) {
   const int k = k0 + SUB;
   if (k < n) {
-    float d = sv_adj_step<NS, NS - 1 - SUB>(lam, ys[SUB], bs[SUB], steps[k].flags, steps[k].nmask);
+    float d = sv_adj_step<NS, NS - 1 - SUB>(lam, ys[SUB], bs[SUB], coef + (k << NS));
     const int ai = steps[k].angle_index;
     if (ai >= 0) {
       for (int m = 1; m < A.red; m <<= 1) d += __shfl_xor(d, m, 64);
@@ -502,13 +532,13 @@ __device__ __forceinline__ void adj_back(float2 (&lam)[1 << NS], const float2 (&
 
 template <int NS, int... SUBS>
 __device__ __forceinline__ void adj_block(float2 (&t)[1 << NS], float2 (&lam)[1 << NS],
-                                          const mbqc_step* steps, int k0, int n, const float (&th)[NS],
-                                          const AdjCtx& A, int64_t bi, int64_t ci, bool active,
-                                          std::integer_sequence<int, SUBS...>) {
+                                          const mbqc_step* steps, const float* coef, int k0, int n,
+                                          const float (&th)[NS], const AdjCtx& A, int64_t bi,
+                                          int64_t ci, bool active, std::integer_sequence<int, SUBS...>) {
   float2 ys[NS][1 << (NS - 1)];
   float2 bs[NS];
-  (adj_recompute<NS, SUBS>(t, ys, bs, steps, k0, n, th), ...);
-  (adj_back<NS, NS - 1 - SUBS>(lam, ys, bs, steps, k0, n, A, bi, ci, active), ...);
+  (adj_recompute<NS, SUBS>(t, ys, bs, coef, k0, n, th), ...);
+  (adj_back<NS, NS - 1 - SUBS>(lam, ys, bs, steps, coef, k0, n, A, bi, ci, active), ...);
 }
 
 template <int NS>
@@ -523,6 +553,7 @@ __global__ __launch_bounds__(64) void k_sv_adj_backward(Job J, const float2* ckp
   const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
   const int32_t* resc = J.tab + J.off_svres;
   const int32_t* out_pos = J.tab + J.off_out;
+  const float* coef = reinterpret_cast<const float*>(J.tab + J.off_coef);
   const float scale = sv_final_scale(J.e_sv);
   float2 lam[H];
   const float2* sd = seed + gg * (int64_t(1) << J.n_out);
@@ -539,8 +570,9 @@ __global__ __launch_bounds__(64) void k_sv_adj_backward(Job J, const float2* ckp
     for (int j = 0; j < H; ++j)
       t[j] = active ? ckpt[(int64_t(blk) * H + j) * J.total + g] : make_float2(0.f, 0.f);
     float th[NS];
-    load_block_angles<NS>(steps, blk * NS, n, ang, th);
-    adj_block<NS>(t, lam, steps, blk * NS, n, th, A, bi, ci, active, std::make_integer_sequence<int, NS>{});
+    load_block_angles<NS>(steps, blk * NS, n, ang, J.angle_stride_a, th);
+    adj_block<NS>(t, lam, steps, coef, blk * NS, n, th, A, bi, ci, active,
+                  std::make_integer_sequence<int, NS>{});
     if (resc[blk]) {
 #pragma unroll
       for (int j = 0; j < H; ++j) lam[j] = cscale(lam[j], 0x1p-32f);
@@ -555,15 +587,15 @@ struct BatchItem {
   int64_t first;   // first simulation index within the job
 };
 
-template <bool REF>
+template <bool REF, int MAXNS>
 __device__ __forceinline__ void run_item(const Job& J, int64_t g, bool active, uint32_t flags) {
   switch (J.ns) {
     case 1: sv_sim<1, REF>(J, g, active, flags); break;
     case 2: sv_sim<2, REF>(J, g, active, flags); break;
     case 3: sv_sim<3, REF>(J, g, active, flags); break;
-    case 4: sv_sim<4, REF>(J, g, active, flags); break;
-    case 5: sv_sim<5, REF>(J, g, active, flags); break;
-    case 6: sv_sim<6, REF>(J, g, active, flags); break;
+    case 4: if constexpr (MAXNS >= 4) sv_sim<4, REF>(J, g, active, flags); break;
+    case 5: if constexpr (MAXNS >= 5) sv_sim<5, REF>(J, g, active, flags); break;
+    case 6: if constexpr (MAXNS >= 6) sv_sim<6, REF>(J, g, active, flags); break;
     default: break;
   }
 }
@@ -571,7 +603,7 @@ __device__ __forceinline__ void run_item(const Job& J, int64_t g, bool active, u
 // Persistent kernel: every wave pulls work items (runs of simulations of one
 // job) from a device counter until none are left.  Items are sorted by
 // decreasing cost on the host, so long patterns start first.
-template <bool REF>
+template <bool REF, int MAXNS>
 __global__ __launch_bounds__(256) void k_batch(const Job* __restrict__ jobs,
                                                const BatchItem* __restrict__ items, int64_t n_items,
                                                int* counter, uint32_t flags) {
@@ -585,7 +617,7 @@ __global__ __launch_bounds__(256) void k_batch(const Job* __restrict__ jobs,
     const Job J = jobs[I.job];
     for (int off = 0; off < I.count; off += 64) {
       const int64_t g = I.first + off + lane;
-      run_item<REF>(J, g, (off + lane) < I.count && g < J.total, flags);
+      run_item<REF, MAXNS>(J, g, (off + lane) < I.count && g < J.total, flags);
     }
   }
 }
@@ -628,15 +660,16 @@ int device_tables(Plan* p, const int32_t** out) {
   return MBQC_OK;
 }
 
-int make_job(Plan* p, const float* angles, int64_t angle_stride, const void* psi, int64_t in_stride,
-             void* out, int64_t B, int64_t S, Job* J) {
+int make_job(Plan* p, const float* angles, int64_t angle_stride, int64_t angle_stride_a,
+             const void* psi, int64_t in_stride, void* out, int64_t B, int64_t S, Job* J) {
   if (!angles && p->n_angles > 0) return fail(MBQC_ERR_VALUE, "angles pointer is null");
   if (!psi) return fail(MBQC_ERR_VALUE, "input pointer is null");
   if (!out) return fail(MBQC_ERR_VALUE, "output pointer is null");
   if (B < 0 || S <= 0) return fail(MBQC_ERR_VALUE, "batch sizes must be B >= 0, S >= 1");
-  if (angle_stride < 0 || in_stride < 0) return fail(MBQC_ERR_VALUE, "negative stride");
-  if (angle_stride > 0 && angle_stride < p->n_angles)
-    return fail(MBQC_ERR_VALUE, "angle stride shorter than the angle vector");
+  if (angle_stride < 0 || angle_stride_a < 0 || in_stride < 0)
+    return fail(MBQC_ERR_VALUE, "negative stride");
+  if (p->n_angles > 1 && angle_stride_a == 0)
+    return fail(MBQC_ERR_VALUE, "angle element stride must be positive");
   if (in_stride > 0 && in_stride < (int64_t(1) << p->n_in))
     return fail(MBQC_ERR_VALUE, "input stride shorter than the input state");
   const int32_t* tab = nullptr;
@@ -650,6 +683,7 @@ int make_job(Plan* p, const float* angles, int64_t angle_stride, const void* psi
   j.off_out = o.out_pos;
   j.off_svres = o.sv_rescale;
   j.off_dmres = o.dm_rescale;
+  j.off_coef = o.coef;
   j.e_sv = p->e_sv;
   j.e_dm = p->e_dm;
   j.ns = p->ns;
@@ -660,6 +694,7 @@ int make_job(Plan* p, const float* angles, int64_t angle_stride, const void* psi
   j.live_mask = int32_t(live);
   j.angles = angles;
   j.angle_stride = angle_stride;
+  j.angle_stride_a = angle_stride_a;
   j.psi = static_cast<const float2*>(psi);
   j.in_stride = in_stride;
   j.out = static_cast<float2*>(out);
@@ -713,6 +748,7 @@ constexpr int kMaxAdjSlots = 5;
 }  // namespace
 
 struct mbqc_batch {
+  int max_ns = 1;
   std::vector<Job> jobs;
   std::vector<BatchItem> items;
   Job* d_jobs = nullptr;
@@ -732,15 +768,15 @@ void mbqc_plan_destroy(mbqc_plan* plan) {
 }
 
 int mbqc_sv_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
-                        const void* psi_in, int64_t in_stride, void* out, void* phase_out, int64_t B,
-                        int64_t S, uint32_t flags, void* stream) {
+                        int64_t angle_stride_a, const void* psi_in, int64_t in_stride, void* out,
+                        void* phase_out, int64_t B, int64_t S, uint32_t flags, void* stream) {
   if (!plan) return fail(MBQC_ERR_VALUE, "null plan");
   Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(plan));
   if (p->ns > kMaxSvSlots)
     return fail(MBQC_ERR_NOT_IMPLEMENTED, "statevector kernels support live windows of at most " +
                                               std::to_string(kMaxSvSlots) + " qubits");
   Job J;
-  int rc = make_job(p, angles, angle_stride, psi_in, in_stride, out, B, S, &J);
+  int rc = make_job(p, angles, angle_stride, angle_stride_a, psi_in, in_stride, out, B, S, &J);
   if (rc != MBQC_OK) return rc;
   if ((flags & MBQC_FLAG_WRITE_PHASE) && !phase_out) return fail(MBQC_ERR_VALUE, "phase_out is null");
   J.phase_out = static_cast<float2*>(phase_out);
@@ -759,8 +795,8 @@ int mbqc_sv_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
 }
 
 int mbqc_dm_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
-                        const void* psi_in, int64_t in_stride, void* out, int64_t B, int64_t S,
-                        const mbqc_noise* noise, uint32_t flags, void* stream) {
+                        int64_t angle_stride_a, const void* psi_in, int64_t in_stride, void* out,
+                        int64_t B, int64_t S, const mbqc_noise* noise, uint32_t flags, void* stream) {
   if (!plan) return fail(MBQC_ERR_VALUE, "null plan");
   Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(plan));
   if (p->ns > kMaxDmSlots)
@@ -775,7 +811,7 @@ int mbqc_dm_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
     return fail(MBQC_ERR_VALUE, "noise probabilities must lie in [0, 1]");
   const float gamma = (1.0f - 4.0f * pd / 3.0f) * (1.0f - 2.0f * pr);
   Job J;
-  int rc = make_job(p, angles, angle_stride, psi_in, in_stride, out, B, S, &J);
+  int rc = make_job(p, angles, angle_stride, angle_stride_a, psi_in, in_stride, out, B, S, &J);
   if (rc != MBQC_OK) return rc;
   if (J.total == 0) return MBQC_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -802,15 +838,16 @@ int32_t mbqc_sv_adjoint_partials(int64_t C) {
 }
 
 int mbqc_sv_adjoint_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
-                                const void* psi_in, int64_t in_stride, void* out, int64_t B, int64_t C,
-                                void* ckpt, size_t ckpt_bytes, void* stream) {
+                                int64_t angle_stride_a, const void* psi_in, int64_t in_stride,
+                                void* out, int64_t B, int64_t C, void* ckpt, size_t ckpt_bytes,
+                                void* stream) {
   if (!plan) return fail(MBQC_ERR_VALUE, "null plan");
   Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(plan));
   if (p->ns > kMaxAdjSlots)
     return fail(MBQC_ERR_NOT_IMPLEMENTED, "adjoint kernels support live windows of at most " +
                                               std::to_string(kMaxAdjSlots) + " qubits");
   Job J;
-  int rc = make_job(p, angles, angle_stride, psi_in, in_stride, out, B, C, &J);
+  int rc = make_job(p, angles, angle_stride, angle_stride_a, psi_in, in_stride, out, B, C, &J);
   if (rc != MBQC_OK) return rc;
   if (!ckpt || ckpt_bytes < mbqc_sv_adjoint_workspace_bytes(plan, B * C))
     return fail(MBQC_ERR_VALUE, "checkpoint workspace too small");
@@ -829,8 +866,9 @@ int mbqc_sv_adjoint_forward_c64(const mbqc_plan* plan, const float* angles, int6
 }
 
 int mbqc_sv_adjoint_backward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
-                                 const void* seed, int64_t B, int64_t C, const void* ckpt,
-                                 size_t ckpt_bytes, float* grad_partial, void* stream) {
+                                 int64_t angle_stride_a, const void* seed, int64_t B, int64_t C,
+                                 const void* ckpt, size_t ckpt_bytes, float* grad_partial,
+                                 void* stream) {
   if (!plan) return fail(MBQC_ERR_VALUE, "null plan");
   Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(plan));
   if (p->ns > kMaxAdjSlots)
@@ -841,7 +879,8 @@ int mbqc_sv_adjoint_backward_c64(const mbqc_plan* plan, const float* angles, int
     return fail(MBQC_ERR_VALUE, "checkpoint workspace too small");
   Job J;
   // psi is unused by the backward sweep; pass the seed as a valid pointer
-  int rc = make_job(p, angles, angle_stride, seed, 0, const_cast<void*>(seed), B, C, &J);
+  int rc = make_job(p, angles, angle_stride, angle_stride_a, seed, 0, const_cast<void*>(seed), B, C,
+                    &J);
   if (rc != MBQC_OK) return rc;
   if (J.total == 0) return MBQC_OK;
   AdjCtx A{};
@@ -886,12 +925,14 @@ int mbqc_batch_create(const mbqc_batch_job* jobs, int32_t n_jobs, int32_t sims_p
                                                 std::to_string(kMaxSvSlots) + " qubits");
     }
     Job J;
-    int rc = make_job(p, bj.angles, bj.angle_stride, bj.psi_in, bj.in_stride, bj.out, bj.B, bj.S, &J);
+    int rc = make_job(p, bj.angles, bj.angle_stride, bj.angle_stride_a, bj.psi_in, bj.in_stride,
+                      bj.out, bj.B, bj.S, &J);
     if (rc != MBQC_OK) {
       delete bt;
       return rc;
     }
     bt->jobs.push_back(J);
+    bt->max_ns = std::max(bt->max_ns, p->ns);
     const int64_t cost = int64_t(p->n_steps) << p->ns;
     for (int64_t f = 0; f < J.total; f += sims_per_item) {
       BatchItem it{};
@@ -949,12 +990,23 @@ int mbqc_batch_launch(mbqc_batch* bt, int32_t waves, int32_t wg_waves, uint32_t 
   const int blocks = waves / wg_waves;
   const int threads = 64 * wg_waves;
   const int64_t n = int64_t(bt->items.size());
-  if (flags & MBQC_FLAG_REF_PHASE)
-    hipLaunchKernelGGL(k_batch<true>, dim3(blocks), dim3(threads), 0, st, bt->d_jobs, bt->d_items, n,
-                       bt->d_counter, flags);
-  else
-    hipLaunchKernelGGL(k_batch<false>, dim3(blocks), dim3(threads), 0, st, bt->d_jobs, bt->d_items, n,
-                       bt->d_counter, flags);
+  const bool ref = flags & MBQC_FLAG_REF_PHASE;
+  const int mx = bt->max_ns <= 3 ? 3 : bt->max_ns;
+#define MBQC_BATCH_LAUNCH(R, M)                                                                  \
+  hipLaunchKernelGGL((k_batch<R, M>), dim3(blocks), dim3(threads), 0, st, bt->d_jobs, bt->d_items, \
+                     n, bt->d_counter, flags)
+  switch (mx * 2 + (ref ? 1 : 0)) {
+    case 6: MBQC_BATCH_LAUNCH(false, 3); break;
+    case 7: MBQC_BATCH_LAUNCH(true, 3); break;
+    case 8: MBQC_BATCH_LAUNCH(false, 4); break;
+    case 9: MBQC_BATCH_LAUNCH(true, 4); break;
+    case 10: MBQC_BATCH_LAUNCH(false, 5); break;
+    case 11: MBQC_BATCH_LAUNCH(true, 5); break;
+    default:
+      if (ref) MBQC_BATCH_LAUNCH(true, 6); else MBQC_BATCH_LAUNCH(false, 6);
+      break;
+  }
+#undef MBQC_BATCH_LAUNCH
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "batch launch");
 }

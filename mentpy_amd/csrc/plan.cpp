@@ -218,6 +218,28 @@ int compile_plan(const mbqc_pattern_desc* d, Plan* p) {
     p->out_pos[j] = o;
   }
   p->out_slot = live_slot;
+  // pair coefficients: u0 = x + b y is the new bit-0 half; the bit-1 half is
+  // c (x + s b y) with s = -1 when the measured qubit is a CZ neighbour of the
+  // appended one, c = CZ sign of the pair (0 without an append).  Written as
+  // alpha u0 + beta x:  alpha = c s,  beta = c (1 - s).
+  if (ns <= kMaxCoefSlots) {
+    p->coef.assign(size_t(nm) << ns, 0.0f);
+    for (int k = 0; k < nm; ++k) {
+      const mbqc_step& st = p->steps[k];
+      const int q = int((st.flags >> 8) & 0xffu);
+      const bool app = st.flags & 1u, sneg = st.flags & 2u;
+      int i = 0;
+      for (int j = 0; j < H; ++j) {
+        if (j & (1 << q)) continue;
+        const float sgn = (__builtin_popcount(unsigned(j) & st.nmask) & 1) ? -1.0f : 1.0f;
+        const float c = app ? sgn : 0.0f;
+        const float s = sneg ? -1.0f : 1.0f;
+        p->coef[(size_t(k) << ns) + 2 * i] = c * s;
+        p->coef[(size_t(k) << ns) + 2 * i + 1] = c * (1.0f - s);
+        ++i;
+      }
+    }
+  }
   std::vector<int32_t> flags(p->n_blocks());
   rescale_schedule(*p, kSvRescaleAppends, flags.data(), &p->e_sv);
   rescale_schedule(*p, kDmRescaleAppends, flags.data(), &p->e_dm);
@@ -234,7 +256,8 @@ TableOffsets table_offsets(const Plan& p) {
   o.sv_rescale = o.out_pos + H;
   o.dm_rescale = o.sv_rescale + p.n_blocks();
   o.e_final = o.dm_rescale + p.n_blocks();
-  o.total = o.e_final + 2;
+  o.coef = o.e_final + 2;
+  o.total = o.coef + int(p.coef.size());
   return o;
 }
 
@@ -250,6 +273,7 @@ std::vector<int32_t> pack_device_tables(const Plan& p) {
   }
   rescale_schedule(p, kSvRescaleAppends, t.data() + o.sv_rescale, t.data() + o.e_final);
   rescale_schedule(p, kDmRescaleAppends, t.data() + o.dm_rescale, t.data() + o.e_final + 1);
+  if (!p.coef.empty()) std::memcpy(t.data() + o.coef, p.coef.data(), sizeof(float) * p.coef.size());
   return t;
 }
 

@@ -10,6 +10,7 @@ namespace mbqc {
 
 constexpr int kMaxSlots = 12;   // host tables; kernels support fewer (see launchers)
 constexpr int kMaxDevices = 16;
+constexpr int kMaxCoefSlots = 6;
 
 // Growth control: the kernels skip the 1/sqrt(2) of every appended |+>, so the
 // SV state grows by sqrt(2) (the DM by 2) per append; at the start of a block
@@ -24,6 +25,10 @@ struct Plan {
   int e_sv = 0, e_dm = 0;   // final growth exponents after the rescale schedule
   std::vector<mbqc_step> steps;
   std::vector<int32_t> in_index, init_sign, out_pos, out_slot;
+  // Per-step pair coefficients (statevector kernels, n_slots <= kMaxCoefSlots):
+  // for step k and pair i (the i-th slot index with the measured bit clear),
+  // new[bit=1] = alpha * u0 + beta * x, stored as coef[(k * 2^ns) + 2i + {0,1}].
+  std::vector<float> coef;
 
   // device copies, one per device id
   std::mutex mu;
@@ -34,8 +39,9 @@ struct Plan {
 // Device table layout (int32 words), built by pack_device_tables():
 //   [steps: 4*n_steps][in_index: H][init_sign: H][out_pos: H]
 //   [sv_rescale: n_blocks][dm_rescale: n_blocks][sv_e_final, dm_e_final]
+//   [coef: n_steps * 2^ns floats, only when ns <= kMaxCoefSlots]
 struct TableOffsets {
-  int steps, in_index, init_sign, out_pos, sv_rescale, dm_rescale, e_final, total;
+  int steps, in_index, init_sign, out_pos, sv_rescale, dm_rescale, e_final, coef, total;
 };
 TableOffsets table_offsets(const Plan& p);
 std::vector<int32_t> pack_device_tables(const Plan& p);

@@ -1,0 +1,325 @@
+"""Adjoint angle gradients on the GPU.
+
+mentpy differentiates a cost by evaluating it 2n times (`mentpy/gradients/grad.py:13-33`,
+`_finite_difference.py:9-24`, `_parameter_shift.py:9-25`).  Here one forward
+launch (with a checkpoint of the live state every n_slots steps) and one
+backward launch give the gradient with respect to every measurement angle:
+
+* ``AdjointEngine`` drives the two C-ABI entry points for one compiled plan and
+  keeps the checkpoint workspace between calls.
+* ``PatternSVFunction`` wraps them as a ``torch.autograd.Function``: angles
+  ``[B, n]`` and input columns ``[C, 2^|I|]`` to raw-frame outputs
+  ``[B, C, 2^|O|]`` (no reference phase, no renormalisation: the post-selected
+  pattern is linear in its input, and every fidelity-type loss is phase
+  invariant).
+* ``mbqml_infidelity`` is the MB-QML loss of mentpy's tutorial
+  (`docs/tutorials/intro-to-mbqml.rst:35-54`: mean over inputs of
+  ``1 - F(U psi_s, out_s)``) with its gradient.  With more inputs S than
+  2^|I| basis states it propagates the 2^|I| basis columns (the transfer
+  matrix T of the pattern) and applies T to the data set with batched complex
+  GEMMs; otherwise it propagates the inputs directly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from .. import _native as nat
+from ..simulators.hip_simulator import _angle_strides
+
+__all__ = ["AdjointEngine", "PatternSVFunction", "pattern_sv", "MBQMLStep", "mbqml_infidelity",
+           "fd_loop_infidelity_grad"]
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class AdjointEngine:
+    """Forward-with-checkpoints and backward sweep for one plan on one device."""
+
+    def __init__(self, plan: nat.NativePlan, device="cuda"):
+        torch = _torch()
+        self.plan = plan
+        self.device = torch.device(device)
+        self._ws = None
+        self._ws_cols = -1
+
+    def _workspace(self, n_cols: int):
+        torch = _torch()
+        need = int(nat.lib().mbqc_sv_adjoint_workspace_bytes(self.plan.handle, n_cols))
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = torch.empty(max(need, 8), dtype=torch.uint8, device=self.device)
+        return self._ws, need
+
+    def forward(self, angles, cols):
+        """``angles [B, n]`` (float32) and input columns ``cols [C, 2^|I|]``
+        shared by every parameter set -> raw outputs ``[B, C, 2^|O|]``
+        (simulation b*C + c uses angle vector b and column c)."""
+        torch = _torch()
+        info = self.plan.info
+        B, C_ = angles.shape[0], cols.shape[0]
+        out = torch.empty((B, C_, 1 << info["n_outputs"]), dtype=torch.complex64, device=self.device)
+        ws, need = self._workspace(B * C_)
+        stream = C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        with torch.cuda.device(self.device):
+            sb, sa = _angle_strides(angles)
+            nat.check(nat.lib().mbqc_sv_adjoint_forward_c64(
+                self.plan.handle, C.c_void_p(angles.data_ptr()), sb, sa,
+                C.c_void_p(cols.data_ptr()), cols.shape[1], C.c_void_p(out.data_ptr()), B, C_,
+                C.c_void_p(ws.data_ptr()), need, stream))
+        self._B, self._C = B, C_
+        self._angles = angles
+        return out
+
+    def backward(self, seed):
+        """``seed [B, C, 2^|O|]`` = dL/d(conj out) -> ``grad [B, n]`` (float32)."""
+        torch = _torch()
+        info = self.plan.info
+        B, C_ = self._B, self._C
+        seed = seed.to(torch.complex64).contiguous()
+        P = int(nat.lib().mbqc_sv_adjoint_partials(C_))
+        part = torch.zeros((B, P, info["n_angles"]), dtype=torch.float32, device=self.device)
+        ws, need = self._workspace(B * C_)
+        stream = C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        with torch.cuda.device(self.device):
+            sb, sa = _angle_strides(self._angles)
+            nat.check(nat.lib().mbqc_sv_adjoint_backward_c64(
+                self.plan.handle, C.c_void_p(self._angles.data_ptr()), sb, sa,
+                C.c_void_p(seed.data_ptr()), B, C_, C.c_void_p(ws.data_ptr()), need,
+                C.c_void_p(part.data_ptr()), stream))
+        return part.sum(1) if P > 1 else part[:, 0]
+
+
+def _engine_for(sim) -> AdjointEngine:
+    core = getattr(sim, "simulator", sim)
+    eng = getattr(core, "_adjoint_engine", None)
+    if eng is None:
+        eng = AdjointEngine(core.plan, core.device)
+        core._adjoint_engine = eng
+    return eng
+
+
+class PatternSVFunction:
+    """``torch.autograd.Function`` over the hip-sv adjoint (built lazily so the
+    module imports without torch)."""
+
+    _cls = None
+
+    @classmethod
+    def get(cls):
+        if cls._cls is None:
+            torch = _torch()
+
+            class _Fn(torch.autograd.Function):
+                @staticmethod
+                def forward(ctx, angles, cols, engine):
+                    out = engine.forward(angles.detach().to(torch.float32).contiguous(),
+                                         cols.detach().to(torch.complex64).contiguous())
+                    ctx.engine = engine
+                    return out
+
+                @staticmethod
+                def backward(ctx, grad_out):
+                    g = ctx.engine.backward(grad_out)
+                    return g, None, None
+
+            cls._cls = _Fn
+        return cls._cls
+
+
+def pattern_sv(sim, angles, cols):
+    """Differentiable raw-frame outputs ``[B, C, 2^|O|]`` of a hip-sv simulator."""
+    return PatternSVFunction.get().apply(angles, cols, _engine_for(sim))
+
+
+def _basis(n_in, device):
+    torch = _torch()
+    return torch.eye(1 << n_in, dtype=torch.complex64, device=device)
+
+
+class MBQMLStep:
+    """Fixed-shape MB-QML loss-and-gradient step for ``B`` parameter sets.
+
+    Loss ``L_b = 1 - (1/S) sum_s |<target_s | out_{b,s}>|^2`` (mentpy's tutorial
+    cost, `docs/tutorials/intro-to-mbqml.rst:35-54`) and ``dL_b/dtheta_b``.
+
+    Modes:
+    * ``"gram"`` (transfer matrix + data Gram matrix): the adjoint forward
+      propagates the 2^|I| basis columns, giving ``t_b = vec(T_b)``; then
+      ``L_b = 1 - t_b^H M t_b / S`` with the data-only Hermitian matrix
+      ``M = D^H D``, ``D[s, (c, o)] = psi_s[c] conj(phi_s[o])`` (computed once
+      per data set), and the backward seed is ``-2 M t_b / S``: one dense
+      complex GEMM ``[B, K] x [K, K]`` per step.
+    * ``"transfer"``: transfer matrix, then ``T X`` and ``(Phi f) X^H`` GEMMs.
+    * ``"direct"``: the inputs themselves are the propagated columns.
+
+    All buffers are allocated once, so a step can be captured in a HIP graph
+    (``capture``); write new angles into ``self.angles`` before each replay.
+    """
+
+    def __init__(self, sim, inputs, targets, n_sets: int, mode: str = "auto"):
+        torch = _torch()
+        self.engine = _engine_for(sim)
+        eng = self.engine
+        dev = eng.device
+        info = eng.plan.info
+        self.X = torch.as_tensor(inputs, device=dev).to(torch.complex64).contiguous()
+        self.Y = torch.as_tensor(targets, device=dev).to(torch.complex64).contiguous()
+        S = self.X.shape[0]
+        din, dout = 1 << info["n_inputs"], 1 << info["n_outputs"]
+        K = din * dout
+        if mode == "auto":
+            if S <= din:
+                mode = "direct"
+            else:
+                mode = "gram" if K * K <= 2 * S * K else "transfer"
+        self.mode = mode
+        self.S, self.B, self.K = S, int(n_sets), K
+        self.angles = torch.zeros((self.B, info["n_angles"]), dtype=torch.float32, device=dev)
+        self.cols = self.X if mode == "direct" else _basis(info["n_inputs"], dev)
+        C_ = self.cols.shape[0]
+        self.out = torch.empty((self.B, C_, dout), dtype=torch.complex64, device=dev)
+        self.seed = torch.empty_like(self.out)
+        self.P = int(nat.lib().mbqc_sv_adjoint_partials(C_))
+        self.part = torch.zeros((self.B, self.P, info["n_angles"]), dtype=torch.float32, device=dev)
+        self.loss = torch.empty(self.B, dtype=torch.float32, device=dev)
+        if mode == "gram":
+            D = (self.X.unsqueeze(2) * self.Y.conj().unsqueeze(1)).reshape(S, K)
+            M = torch.matmul(D.conj().transpose(0, 1).contiguous(), D)
+            self.MT = M.transpose(0, 1).contiguous()               # Z = t @ M^T = (M t)^T
+            self.Z = torch.empty((self.B, K), dtype=torch.complex64, device=dev)
+        self._ws, self._need = eng._workspace(self.B * C_)
+
+    def _launch_forward(self):
+        torch = _torch()
+        eng, info = self.engine, self.engine.plan.info
+        stream = C.c_void_p(torch.cuda.current_stream(eng.device).cuda_stream)
+        sb, sa = _angle_strides(self.angles)
+        nat.check(nat.lib().mbqc_sv_adjoint_forward_c64(
+            eng.plan.handle, C.c_void_p(self.angles.data_ptr()), sb, sa,
+            C.c_void_p(self.cols.data_ptr()), self.cols.shape[1], C.c_void_p(self.out.data_ptr()),
+            self.B, self.cols.shape[0], C.c_void_p(self._ws.data_ptr()), self._need, stream))
+
+    def _launch_backward(self):
+        torch = _torch()
+        eng = self.engine
+        stream = C.c_void_p(torch.cuda.current_stream(eng.device).cuda_stream)
+        sb, sa = _angle_strides(self.angles)
+        nat.check(nat.lib().mbqc_sv_adjoint_backward_c64(
+            eng.plan.handle, C.c_void_p(self.angles.data_ptr()), sb, sa,
+            C.c_void_p(self.seed.data_ptr()), self.B, self.cols.shape[0],
+            C.c_void_p(self._ws.data_ptr()), self._need, C.c_void_p(self.part.data_ptr()), stream))
+
+    def run(self, angles=None, grad: bool = True):
+        """One step; returns ``(loss [B], grad [B, n])`` device tensors (views
+        of internal buffers, overwritten by the next step)."""
+        torch = _torch()
+        if angles is not None:
+            self.angles.copy_(torch.as_tensor(angles, dtype=torch.float32, device=self.angles.device)
+                              .reshape(self.angles.shape))
+        with torch.cuda.device(self.engine.device):
+            self._launch_forward()
+            S = self.S
+            if self.mode == "gram":
+                t = self.out.reshape(self.B, self.K)
+                torch.matmul(t, self.MT, out=self.Z)
+                self.loss.copy_(1.0 - (t.conj() * self.Z).sum(1).real / S)
+                if grad:
+                    torch.mul(self.Z.reshape(self.seed.shape), -2.0 / S, out=self.seed)
+            elif self.mode == "transfer":
+                Tm = self.out.transpose(1, 2)                                # [B, dout, din]
+                outs = torch.matmul(Tm, self.X.transpose(0, 1))             # [B, dout, S]
+                Phi = self.Y.transpose(0, 1)
+                f = (Phi.conj().unsqueeze(0) * outs).sum(1)                 # [B, S]
+                self.loss.copy_(1.0 - (f.abs() ** 2).mean(1))
+                if grad:
+                    G = torch.matmul(Phi.unsqueeze(0) * f.unsqueeze(1), self.X.conj())
+                    self.seed.copy_(G.transpose(1, 2) * (-2.0 / S))
+            else:
+                f = (self.Y.conj().unsqueeze(0) * self.out).sum(2)          # [B, S]
+                self.loss.copy_(1.0 - (f.abs() ** 2).mean(1))
+                if grad:
+                    torch.mul(f.unsqueeze(2) * self.Y.unsqueeze(0), -2.0 / S, out=self.seed)
+            if not grad:
+                return self.loss, None
+            self._launch_backward()
+        g = self.part.sum(1) if self.P > 1 else self.part[:, 0]
+        return self.loss, g
+
+    def capture(self):
+        """Capture one step (both kernels and the loss/seed GEMM) in a HIP graph."""
+        torch = _torch()
+        self.run()
+        torch.cuda.synchronize(self.engine.device)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.run()
+        torch.cuda.synchronize(self.engine.device)
+        return g
+
+
+def mbqml_infidelity(sim, angles, inputs, targets, mode: str = "auto", grad: bool = True):
+    """Loss ``L_b = 1 - mean_s |<target_s | out_{b,s}>|^2`` and ``dL_b/dtheta_b``.
+
+    ``angles [B, n]``, ``inputs [S, 2^|I|]``, ``targets [S, 2^|O|]``.  Returns
+    ``(loss [B], grad [B, n])`` float32 device tensors (``grad`` None when
+    ``grad=False``).  ``mode`` as in ``MBQMLStep`` (``"auto"`` picks
+    ``"direct"`` when S <= 2^|I|, else the cheaper of ``"gram"`` and
+    ``"transfer"``).  Steps (with their Gram matrix) are cached per shape and
+    data set on the simulator.
+    """
+    torch = _torch()
+    eng = _engine_for(sim)
+    a = torch.as_tensor(angles, dtype=torch.float32, device=eng.device)
+    if a.dim() == 1:
+        a = a.unsqueeze(0)
+    X = torch.as_tensor(inputs, device=eng.device)
+    Y = torch.as_tensor(targets, device=eng.device)
+    core = getattr(sim, "simulator", sim)
+    cache = core.__dict__.setdefault("_mbqml_steps", {})
+    key = (a.shape[0], mode, X.data_ptr(), tuple(X.shape), Y.data_ptr(), tuple(Y.shape))
+    step = cache.get(key)
+    if step is None:
+        if len(cache) > 8:
+            cache.clear()
+        step = MBQMLStep(sim, X, Y, a.shape[0], mode)
+        cache[key] = step
+    loss, g = step.run(a, grad=grad)
+    return loss.clone(), (None if g is None else g.clone())
+
+
+def fd_loop_infidelity_grad(sim, angles, inputs, targets, h: float = 1e-3):
+    """The plain GPU finite-difference loop: for every angle, two batched
+    hip-sv forward launches over all parameter sets and inputs (central
+    differences, as mentpy's ``fd_gradient`` with the cost on the GPU)."""
+    torch = _torch()
+    from ..simulators.hip_simulator import sv_forward
+    core = getattr(sim, "simulator", sim)
+    dev = core.device
+    a = torch.as_tensor(angles, dtype=torch.float32, device=dev)
+    if a.dim() == 1:
+        a = a.unsqueeze(0)
+    X = torch.as_tensor(inputs, device=dev).to(torch.complex64).contiguous()
+    Y = torch.as_tensor(targets, device=dev).to(torch.complex64).contiguous()
+    B, n = a.shape
+    out = torch.empty((B, X.shape[0], Y.shape[1]), dtype=torch.complex64, device=dev)
+    grad = torch.empty((B, n), dtype=torch.float32, device=dev)
+    work = a.clone()
+
+    def cost():
+        sv_forward(core.plan, work, X, ref_phase=False, renormalize=True, out=out)
+        f = (Y.conj().unsqueeze(0) * out).sum(2)
+        return 1.0 - (f.abs() ** 2).mean(1)
+
+    for i in range(n):
+        work[:, i] = a[:, i] + h
+        cp = cost()
+        work[:, i] = a[:, i] - h
+        cm = cost()
+        work[:, i] = a[:, i]
+        grad[:, i] = (cp - cm) / (2 * h)
+    return grad

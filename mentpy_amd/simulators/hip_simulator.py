@@ -100,13 +100,21 @@ def _stream_ptr(device):
 
 
 def _as_angles(angles, n_angles: int, device):
+    """float32 device view ``[B, n]``; any strides are kept (an angle-major
+    ``[n, B]`` tensor passed as ``.t()`` gives coalesced loads)."""
     torch = _torch()
     a = torch.as_tensor(angles, dtype=torch.float32, device=device)
     if a.dim() == 1:
         a = a.unsqueeze(0)
     if a.dim() != 2 or a.shape[1] != n_angles:
         raise ValueError(f"angles must have shape [B, {n_angles}], got {tuple(a.shape)}")
-    return a.contiguous()
+    if a.stride(1) == 0 and n_angles > 1:
+        a = a.contiguous()
+    return a
+
+
+def _angle_strides(a):
+    return (a.stride(0) if a.shape[0] > 1 else 0), max(a.stride(1), 1)
 
 
 def _as_states(states, n_qubits: int, device):
@@ -143,9 +151,10 @@ def sv_forward(plan: nat.NativePlan, angles, states, *, ref_phase: bool = True,
     phase = torch.empty((B, S), dtype=torch.complex64, device=device) if want_phase else None
     flags = ((nat.FLAG_REF_PHASE if ref_phase else 0) | (nat.FLAG_RENORMALIZE if renormalize else 0)
              | (nat.FLAG_WRITE_PHASE if want_phase else 0))
+    sb, sa = _angle_strides(angles)
     with torch.cuda.device(device):
         nat.check(nat.lib().mbqc_sv_forward_c64(
-            plan.handle, C.c_void_p(angles.data_ptr()), info["n_angles"],
+            plan.handle, C.c_void_p(angles.data_ptr()), sb, sa,
             C.c_void_p(states.data_ptr()), states.shape[1], C.c_void_p(out.data_ptr()),
             C.c_void_p(phase.data_ptr() if phase is not None else 0), B, S, flags,
             _stream_ptr(device)))
@@ -165,9 +174,10 @@ def dm_forward(plan: nat.NativePlan, angles, states, *, p_depol: float = 0.0,
         out = torch.empty((B, S, d, d), dtype=torch.complex64, device=device)
     noise = nat.Noise(p_depol=float(p_depol), p_readout=float(p_readout))
     flags = nat.FLAG_RENORMALIZE if renormalize else 0
+    sb, sa = _angle_strides(angles)
     with torch.cuda.device(device):
         nat.check(nat.lib().mbqc_dm_forward_c64(
-            plan.handle, C.c_void_p(angles.data_ptr()), info["n_angles"],
+            plan.handle, C.c_void_p(angles.data_ptr()), sb, sa,
             C.c_void_p(states.data_ptr()), states.shape[1], C.c_void_p(out.data_ptr()), B, S,
             C.byref(noise), flags, _stream_ptr(device)))
     return out

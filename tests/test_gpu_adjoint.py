@@ -1,0 +1,178 @@
+"""GPU parity of the adjoint angle gradient.
+
+References: the exact shift rule (s = pi/2) of the complex128 oracle cost
+(exact for these costs, which are a + b cos t + c sin t in every angle), the
+GPU central finite-difference loop (h = 1e-3), and the oracle's central
+finite differences (h = 1e-5, mentpy's default) on the 4x16 grid.
+Tolerance: 2e-4 absolute on gradient entries of order 1e-2..1 (complex64
+forward/backward over up to 60 steps; FD in complex64 with h = 1e-3 carries
+~1e-4 of cancellation error).
+"""
+import numpy as np
+import pytest
+
+from conftest import haar_state
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def _dataset(rows, S, rng):
+    from mentpy_amd.operators.gates import haar_unitary
+    U = haar_unitary(2 ** rows, rng)
+    X = np.stack([haar_state(rows, rng) for _ in range(S)])
+    return X, X @ U.T
+
+
+def _oracle_cost(g, X, Y):
+    from oracle.vectorized import sv_batch
+
+    def cost(th):
+        out = sv_batch(g, np.asarray(th)[None], X)[0]
+        f = np.sum(np.conj(Y) * out, -1)
+        return 1.0 - np.mean(np.abs(f) ** 2)
+    return cost
+
+
+@pytest.mark.parametrize("rows,cols,S,mode", [
+    (1, 5, 3, "direct"), (2, 4, 3, "direct"), (2, 4, 9, "transfer"), (3, 5, 16, "transfer"),
+    (3, 5, 4, "direct"), (4, 6, 20, "transfer"), (5, 4, 8, "direct"), (2, 4, 9, "gram"),
+    (3, 5, 40, "gram"), (4, 6, 300, "gram")])
+def test_adjoint_matches_exact_shift(rows, cols, S, mode, rng):
+    torch = _torch()
+    import mentpy_amd as mp
+    from mentpy_amd.gradients import mbqml_infidelity
+    from oracle.gradients import exact_shift_gradient
+    g = mp.templates.grid_cluster(rows, cols) if rows > 1 else mp.templates.linear_cluster(cols)
+    sim = mp.PatternSimulator(g, backend="hip-sv")
+    X, Y = _dataset(rows, S, rng)
+    B = 3
+    th = rng.uniform(0, 2 * np.pi, (B, len(g.trainable_nodes))).astype(np.float32)
+    loss, grad = mbqml_infidelity(sim, th, X, Y, mode=mode)
+    loss, grad = loss.cpu().numpy(), grad.cpu().numpy()
+    for b in range(B):
+        cost = _oracle_cost(g, X, Y)
+        ref_g = exact_shift_gradient(cost, th[b].astype(np.float64))
+        ref_l = cost(th[b].astype(np.float64))
+        err = np.abs(grad[b] - ref_g).max()
+        print(f"{rows}x{cols} S={S} {mode} b={b}: loss err {abs(loss[b]-ref_l):.2e} grad err {err:.2e}",
+              flush=True)
+        assert abs(loss[b] - ref_l) < 2e-5
+        assert err < 2e-4
+
+
+def test_adjoint_matches_gpu_fd_loop(rng):
+    torch = _torch()
+    import mentpy_amd as mp
+    from mentpy_amd.gradients import fd_loop_infidelity_grad, mbqml_infidelity
+    g = mp.templates.grid_cluster(3, 6)
+    sim = mp.PatternSimulator(g, backend="hip-sv")
+    X, Y = _dataset(3, 24, rng)
+    th = rng.uniform(0, 2 * np.pi, (8, len(g.trainable_nodes))).astype(np.float32)
+    _, ga = mbqml_infidelity(sim, th, X, Y)
+    gf = fd_loop_infidelity_grad(sim, th, X, Y, h=1e-3)
+    err = (ga - gf).abs().max().item()
+    print(f"adjoint vs GPU FD (h=1e-3): {err:.2e}", flush=True)
+    assert err < 5e-3
+
+
+def test_adjoint_4x16_mbqml_vs_oracle_fd(rng):
+    """The MB-QML step configuration (4x16 grid, 60 angles, 256 Haar inputs),
+    checked on 2 parameter sets against the oracle's central FD (h = 1e-5)."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from mentpy_amd.gradients import mbqml_infidelity
+    from oracle.gradients import fd_gradient
+    g = mp.templates.grid_cluster(4, 16)
+    sim = mp.PatternSimulator(g, backend="hip-sv")
+    X, Y = _dataset(4, 256, rng)
+    th = rng.uniform(0, 2 * np.pi, (2, 60)).astype(np.float32)
+    loss_t, g_t = mbqml_infidelity(sim, th, X, Y, mode="transfer")
+    loss_d, g_d = mbqml_infidelity(sim, th, X, Y, mode="direct")
+    loss_g, g_g = mbqml_infidelity(sim, th, X, Y, mode="gram")
+    assert (g_t - g_d).abs().max().item() < 1e-4
+    assert (g_g - g_d).abs().max().item() < 1e-4
+    assert (loss_t - loss_d).abs().max().item() < 1e-5
+    assert (loss_g - loss_d).abs().max().item() < 1e-5
+    cost = _oracle_cost(g, X, Y)
+    ref = fd_gradient(cost, th[0].astype(np.float64), h=1e-5)
+    err = np.abs(g_t[0].cpu().numpy() - ref).max()
+    print(f"4x16 adjoint vs oracle FD(h=1e-5): {err:.2e}", flush=True)
+    assert err < 2e-4
+
+
+def test_mbqml_step_graph_replay(rng):
+    """HIP-graph capture of a whole loss+gradient step, replayed with new angles
+    and eager work in between; every replay equals the eager step."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from mentpy_amd.gradients import MBQMLStep
+    g = mp.templates.grid_cluster(4, 7)
+    sim = mp.PatternSimulator(g, backend="hip-sv")
+    X, Y = _dataset(4, 64, rng)
+    for mode in ("gram", "direct"):
+        step = MBQMLStep(sim, X, Y, n_sets=8, mode=mode)
+        graph = step.capture()
+        for rep in range(3):
+            th = torch.tensor(rng.uniform(0, 2 * np.pi, (8, len(g.trainable_nodes))),
+                              dtype=torch.float32, device="cuda")
+            step.angles.copy_(th)
+            graph.replay()
+            torch.cuda.synchronize()
+            l_graph = step.loss.clone()
+            g_graph = (step.part.sum(1) if step.P > 1 else step.part[:, 0]).clone()
+            l_eager, g_eager = step.run(th)
+            torch.cuda.synchronize()
+            assert torch.allclose(l_graph, l_eager, atol=1e-6), (mode, rep)
+            assert torch.allclose(g_graph, g_eager, atol=1e-6), (mode, rep)
+
+
+def test_autograd_function_custom_loss(rng):
+    """Gradient of an arbitrary (phase-invariant) loss through autograd."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from mentpy_amd.gradients import pattern_sv
+    from oracle.gradients import exact_shift_gradient
+    from oracle.vectorized import sv_batch
+    g = mp.templates.grid_cluster(2, 5)
+    sim = mp.PatternSimulator(g, backend="hip-sv")
+    cols = np.stack([haar_state(2, rng) for _ in range(3)])
+    w = rng.uniform(0.5, 1.5, 4)
+    th = torch.tensor(rng.uniform(0, 2 * np.pi, (1, len(g.trainable_nodes))), dtype=torch.float32,
+                      device="cuda", requires_grad=True)
+    out = pattern_sv(sim, th, torch.as_tensor(cols, dtype=torch.complex64, device="cuda"))
+    wt = torch.as_tensor(w, dtype=torch.float32, device="cuda")
+    loss = ((out.abs() ** 2) * wt).sum()
+    loss.backward()
+
+    def cost(x):
+        o = sv_batch(g, x[None], cols)[0]
+        return float(((np.abs(o) ** 2) * w).sum())
+    ref = exact_shift_gradient(cost, th.detach().cpu().numpy()[0].astype(np.float64))
+    err = np.abs(th.grad.cpu().numpy()[0] - ref).max()
+    print(f"autograd custom loss grad err {err:.2e}", flush=True)
+    assert err < 2e-4
+
+
+def test_adam_with_adjoint_cost_decreases_loss(rng):
+    torch = _torch()
+    import mentpy_amd as mp
+    from mentpy_amd.gradients import AdjointCost
+    g = mp.templates.grid_cluster(2, 6)
+    g[3] = mp.Ment("X")
+    sim = mp.PatternSimulator(g, backend="hip-sv")
+    X, Y = _dataset(2, 12, rng)
+    cost = AdjointCost(sim, X, Y)
+    x0 = rng.uniform(0, 2 * np.pi, len(g.trainable_nodes))
+    opt = mp.optimizers.AdamOptimizer(step_size=0.08)
+    c0 = cost(x0)
+    x = opt.optimize(cost, x0, num_iters=40)
+    c1 = cost(x)
+    print(f"Adam: cost {c0:.4f} -> {c1:.4f}", flush=True)
+    assert c1 < c0

@@ -43,6 +43,26 @@ def test_sv_grid_matches_oracle(rows, cols, rng):
     assert infidelity(out, ref).max() < FID_TOL
 
 
+def test_angle_major_layout_matches_row_major(rng):
+    """Angles given as an angle-major [n, B] tensor viewed as [B, n] (strided
+    kernel path) give the same outputs as contiguous [B, n] angles."""
+    torch = _torch()
+    import mentpy_amd as mp
+    g = mp.templates.grid_cluster(4, 9)
+    sim = mp.PatternSimulator(g, backend="hip-sv")
+    B = 333
+    th = torch.rand((len(g.trainable_nodes), B), device="cuda") * 6.28
+    st = haar_state(4, rng)
+    a = sim.run_batch(th.t(), st)
+    b = sim.run_batch(th.t().contiguous(), st)
+    assert torch.equal(a, b)
+    g3 = mp.templates.grid_cluster(3, 5)
+    dm = mp.PatternSimulator(g3, backend="hip-dm")
+    th3 = torch.rand((len(g3.trainable_nodes), 7), device="cuda") * 6.28
+    st3 = haar_state(3, rng)
+    assert torch.equal(dm.run_batch(th3.t(), st3), dm.run_batch(th3.t().contiguous(), st3))
+
+
 @pytest.mark.parametrize("n", [3, 5, 9])
 def test_teleportation_known_answer(n, rng):
     """mentpy tests/test_simulators.py:15-30: all-zero angles teleport the input."""

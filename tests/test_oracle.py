@@ -1,0 +1,180 @@
+"""Pin the complex128 oracle (CPU only).
+
+Each check is independent of the formula it checks:
+* mentpy's own known-answer tests: teleportation (tests/test_simulators.py:15-30),
+  partial trace / purification (tests/test_calculator.py:12-54), Ment matrices
+  (tests/operators/test_ment.py:31-61);
+* the chain closed form (gate model H P(-t)) including the reference phase;
+* brute-force full-register simulation in gate order (no window logic) on
+  small grids, with fixed X / Y nodes, for SV and for noisy DM;
+* invariants: outcome probability 1/2 at every step, window-size independence,
+  unitarity of the post-selected map, trace one;
+* gradients: the exact shift rule against finite differences, and the known
+  bias sin(1.5)/1.5 of mentpy's default parameter shift.
+"""
+import numpy as np
+import pytest
+
+import mentpy_amd as mp
+from conftest import haar_state
+from oracle.brute import brute_dm, brute_sv
+from oracle.closed_forms import chain_output, reference_phase
+from oracle.reference_dm import ReferenceDM
+from oracle.reference_sv import (ReferenceSV, partial_trace, pure2density)
+from oracle.vectorized import dm_batch, sv_batch
+
+
+def _measured_angles(g, th):
+    out = []
+    for v in g.measurement_order:
+        if v in g.output_nodes:
+            continue
+        if v in g.trainable_nodes:
+            out.append(th[g.trainable_nodes.index(v)])
+        else:
+            m = g[v]
+            out.append(0.0 if m.plane == "X" else (np.pi / 2 if m.plane == "Y" else m.angle))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("i", [1, 2, 3, 4])
+def test_teleportation_known_answer(i, rng):
+    g = mp.templates.linear_cluster(2 * i + 1)
+    for _ in range(3):
+        st = haar_state(1, rng)
+        for cls in (ReferenceSV, ReferenceDM):
+            sim = cls(g, st)
+            assert len(g.trainable_nodes) == 2 * i
+            out = sim.run([0] * (2 * i))
+            assert np.allclose(out, np.outer(st, st.conj()), atol=1e-3)
+
+
+def test_calculator_known_answers():
+    assert np.allclose(pure2density(np.array([1, 0])), [[1, 0], [0, 0]])
+    assert np.allclose(pure2density(np.array([1, 1]) / np.sqrt(2)), [[0.5, 0.5], [0.5, 0.5]])
+    rho = np.array([[0.5, 0], [0, 0.5]])
+    assert np.allclose(partial_trace(rho, [0]), 1)
+    plus = np.array([1, 1]) / np.sqrt(2)
+    sigma = np.outer(plus, plus)
+    prod = np.kron(rho, sigma)
+    assert np.allclose(partial_trace(prod, [0]), sigma)
+    assert np.allclose(partial_trace(prod, [1]), rho)
+    psi = np.array([1, 1]) / np.sqrt(2)
+    assert np.allclose(partial_trace(psi, [0]), 1)
+    prod = np.kron(psi, plus)
+    assert np.allclose(partial_trace(prod, [0]), plus)
+    assert np.allclose(partial_trace(prod, [1]), psi)
+
+
+def test_ment_known_answers():
+    m = mp.Ment(0, "X")
+    assert m.plane == "X" and m.angle == 0
+    assert repr(mp.Ment(None, "X")) == "Ment(0, X)"
+    assert repr(mp.Ment(np.pi / 4, "XY")) == f"Ment({round(np.pi / 4, 4)}, XY)"
+    assert not mp.Ment(None, "X").is_trainable()
+    assert mp.Ment(None, "YZ").is_trainable()
+    assert not mp.Ment(np.pi / 4, "XY").is_trainable()
+    X, Y, Z = mp.gates.PauliX, mp.gates.PauliY, mp.gates.PauliZ
+    assert np.allclose(mp.Ment(np.pi / 4, "XY").matrix(), np.cos(np.pi / 4) * X + np.sin(np.pi / 4) * Y)
+    assert np.allclose(mp.Ment(0, "X").matrix(), X)
+    assert np.allclose(mp.Ment(0, "Y").matrix(), Y)
+    assert np.allclose(mp.Ment(0, "Z").matrix(), Z)
+    assert np.allclose(mp.Ment(plane="Y").matrix(), Y)
+    with pytest.raises(ValueError):
+        mp.Ment(None, "A")
+    with pytest.raises(ValueError):
+        mp.Ment(np.pi, "X")
+
+
+def test_window_larger_than_schedule_raises():
+    with pytest.raises(ValueError):
+        ReferenceSV(mp.templates.linear_cluster(2))
+
+
+@pytest.mark.parametrize("n", [3, 6, 9])
+def test_chain_closed_form_with_reference_phase(n, rng):
+    g = mp.templates.linear_cluster(n)
+    th = rng.uniform(0, 2 * np.pi, n - 1)
+    psi = haar_state(1, rng)
+    out = ReferenceSV(g, psi).run(th, output_form="sv")
+    want = chain_output(th, psi) * reference_phase(th)
+    assert np.abs(out - want).max() < 1e-12
+
+
+@pytest.mark.parametrize("rows,cols", [(2, 3), (2, 4), (3, 3), (2, 5)])
+def test_grid_matches_brute_force(rows, cols, rng):
+    g = mp.templates.grid_cluster(rows, cols)
+    oc = g.outputc
+    g[oc[1]] = mp.Ment("X")
+    g[oc[-1]] = mp.Ment("Y")
+    th = rng.uniform(0, 2 * np.pi, len(g.trainable_nodes))
+    psi = haar_state(rows, rng)
+    out = ReferenceSV(g, psi).run(th, output_form="sv")
+    want = brute_sv(g, th, psi) * reference_phase(_measured_angles(g, th))
+    assert np.abs(out - want).max() < 1e-12
+    vec = sv_batch(g, th[None], psi[None])[0, 0]
+    assert np.abs(vec - out).max() < 1e-12
+
+
+def test_brute_force_catches_a_dropped_cz(rng):
+    """A plausible mistake planted in the oracle's input (one CZ missing) is
+    visible against brute force."""
+    g = mp.templates.grid_cluster(2, 4)
+    th = rng.uniform(0, 2 * np.pi, len(g.trainable_nodes))
+    psi = haar_state(2, rng)
+    good = ReferenceSV(g, psi).run(th, output_form="sv")
+    g2 = mp.templates.grid_cluster(2, 4)
+    g2.graph.remove_edge(1, 5)
+    bad = ReferenceSV(g2, psi).run(th, output_form="sv")
+    assert np.abs(good - bad).max() > 1e-3
+
+
+@pytest.mark.parametrize("rows,cols", [(2, 4), (3, 4)])
+def test_window_size_independence_and_probability(rows, cols, rng):
+    g = mp.templates.grid_cluster(rows, cols)
+    th = rng.uniform(0, 2 * np.pi, len(g.trainable_nodes))
+    psi = haar_state(rows, rng)
+    base = ReferenceSV(g, psi).run(th, output_form="sv")
+    for w in range(rows + 2, min(len(g.trainable_nodes), rows + 4) + 1):
+        assert np.abs(ReferenceSV(g, psi, window_size=w).run(th, output_form="sv") - base).max() < 1e-12
+    sim = ReferenceSV(g, psi)
+    for v in sim.schedule_measure:
+        sim.measure(th[g.trainable_nodes.index(v)])
+        assert abs(sim.prob0 - 0.5) < 1e-12
+
+
+def test_transfer_map_is_unitary(rng):
+    g = mp.templates.grid_cluster(3, 4)
+    th = rng.uniform(0, 2 * np.pi, len(g.trainable_nodes))
+    T = sv_batch(g, th[None], np.eye(8), phase="raw")[0].T
+    assert np.abs(T.conj().T @ T - np.eye(8)).max() < 1e-12
+
+
+@pytest.mark.parametrize("rows,cols", [(2, 3), (3, 3)])
+def test_noisy_dm_matches_brute_force(rows, cols, rng):
+    g = mp.templates.grid_cluster(rows, cols)
+    th = rng.uniform(0, 2 * np.pi, len(g.trainable_nodes))
+    psi = haar_state(rows, rng)
+    sv = ReferenceSV(g, psi).run(th, output_form="sv")
+    assert np.abs(ReferenceDM(g, psi).run(th) - np.outer(sv, sv.conj())).max() < 1e-12
+    for p, pm in ((0.01, 0.005), (0.3, 0.2)):
+        rho = ReferenceDM(g, psi, p_depol=p, p_readout=pm).run(th)
+        assert np.abs(rho - brute_dm(g, th, psi, p, pm)).max() < 1e-12
+        assert np.abs(rho - dm_batch(g, th[None], psi[None], p_depol=p, p_readout=pm)[0, 0]).max() < 1e-12
+        assert abs(np.trace(rho) - 1) < 1e-12
+
+
+def test_gradient_rules(rng):
+    from oracle.gradients import exact_shift_gradient, fd_gradient, psr_gradient
+    g = mp.templates.grid_cluster(2, 4)
+    X = np.stack([haar_state(2, rng) for _ in range(4)])
+    U = mp.gates.random_su(2, rng)
+    Y = X @ U.T
+
+    def cost(t):
+        o = sv_batch(g, t[None], X)[0]
+        return 1 - np.mean(np.abs(np.sum(np.conj(Y) * o, -1)) ** 2)
+    th = rng.uniform(0, 2 * np.pi, len(g.trainable_nodes))
+    ex = exact_shift_gradient(cost, th)
+    assert np.abs(ex - fd_gradient(cost, th, h=1e-5)).max() < 1e-8
+    assert np.abs(psr_gradient(cost, th) - np.sin(1.5) / 1.5 * ex).max() < 1e-12

@@ -1,0 +1,165 @@
+"""Host side (CPU only): pattern model, plan compiler, C ABI.
+
+* The product's schedule / trainable order equals the oracle's restatement of
+  mentpy's causal flow and calculate_order, and the grid closed form.
+* The plan tables, executed by a numpy model of the kernel arithmetic
+  (slot layout, pair coefficients, growth exponent, output gather), reproduce
+  the oracle: this checks every table the GPU reads, without a GPU.
+* The native library loads and exports every entry point of include/mbqc_hip.h,
+  and its validation errors map to mentpy's exception types.
+"""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import mentpy_amd as mp
+from conftest import haar_state
+from oracle.pattern import cflow_layers, grid_order_closed_form, measurement_order
+from oracle.reference_sv import ReferenceSV
+from oracle.vectorized import sv_batch
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+@pytest.fixture(scope="module")
+def native():
+    from mentpy_amd import _native
+    if not _native.library_available():
+        from mentpy_amd.csrc.build import build
+        build()
+    return _native
+
+
+@pytest.mark.parametrize("maker", [
+    lambda: mp.templates.grid_cluster(4, 20), lambda: mp.templates.grid_cluster(3, 12),
+    lambda: mp.templates.grid_cluster(2, 5), lambda: mp.templates.linear_cluster(9),
+    lambda: mp.templates.many_wires([3, 4, 2])])
+def test_schedule_matches_oracle_restatement(maker):
+    g = maker()
+    flow, layer = cflow_layers(g.graph, g.input_nodes, g.output_nodes)
+    assert flow is not None
+    assert g.measurement_order == measurement_order(g.graph, g.input_nodes, layer)
+    assert g.trainable_nodes == [v for v in g.graph.nodes() if v not in g.output_nodes]
+    for v, f in flow.items():
+        assert g.flow(v) == f
+
+
+@pytest.mark.parametrize("n,m", [(4, 20), (2, 3), (5, 40), (3, 12)])
+def test_grid_schedule_closed_form(n, m):
+    g = mp.templates.grid_cluster(n, m)
+    assert g.measurement_order == grid_order_closed_form(n, m)
+    assert g.input_nodes == [i * m for i in range(n)]
+    assert g.output_nodes == [i * m + m - 1 for i in range(n)]
+
+
+def _slot_model(plan, angles, psi):
+    """numpy complex128 model of the kernel arithmetic driven by the plan tables."""
+    st, in_index, sign, out_pos = plan.tables()
+    sv_res, _, e = plan.rescale()
+    ns = plan.info["n_slots"]
+    H = 1 << ns
+    t = np.where(sign.astype(bool), -1, 1) * np.asarray(psi)[in_index]
+    n = plan.info["n_steps"]
+    for k in range(n):
+        if k % ns == 0 and sv_res[k // ns]:
+            t = t * 2.0 ** -32
+        ai, fl, nm, fx = st[k]
+        th = angles[int(ai)] if ai >= 0 else fx
+        b = np.exp(-1j * th)
+        q = (int(fl) >> 8) & 0xFF
+        app, sneg = int(fl) & 1, int(fl) & 2
+        new = t.copy()
+        for j in range(H):
+            if j & (1 << q):
+                continue
+            x, y = t[j], t[j | (1 << q)]
+            u0 = x + b * y
+            s = -1.0 if sneg else 1.0
+            sg = -1.0 if bin(j & int(nm)).count("1") % 2 else 1.0
+            new[j] = u0
+            new[j | (1 << q)] = sg * (x + s * b * y) if app else 0.0
+        t = new
+    scale = 2.0 ** (-e[0] / 2)
+    out = np.zeros(1 << plan.info["n_outputs"], complex)
+    for j in range(H):
+        if out_pos[j] >= 0:
+            out[out_pos[j]] = t[j] * scale
+    return out
+
+
+@pytest.mark.parametrize("rows,cols,w", [(1, 5, 1), (2, 4, 1), (3, 5, 1), (4, 20, 1), (2, 5, 5),
+                                         (2, 4, 4), (3, 4, 6)])
+def test_plan_tables_reproduce_oracle(native, rows, cols, w, rng):
+    from mentpy_amd.simulators import compile_pattern
+    g = mp.templates.grid_cluster(rows, cols) if rows > 1 else mp.templates.linear_cluster(cols)
+    if rows > 1:
+        g[g.outputc[1]] = mp.Ment("Y")
+    plan, *_ = compile_pattern(g, window_size=w)
+    th = rng.uniform(0, 2 * np.pi, len(g.trainable_nodes))
+    psi = haar_state(rows, rng)
+    got = _slot_model(plan, th, psi)
+    want = sv_batch(g, th[None], psi[None], window_size=w, phase="raw")[0, 0]
+    # fixed angles are stored as float32 in the step table (pi/2 to ~4e-8)
+    assert np.abs(got - want).max() < 1e-6
+    assert plan.info["n_dropped_edges"] == 0
+    assert plan.info["min_window"] <= plan.info["window"]
+
+
+def test_too_small_window_drops_edges_like_the_reference(native, rng):
+    from mentpy_amd.simulators import compile_pattern
+    g = mp.templates.grid_cluster(3, 4)
+    plan, *_ = compile_pattern(g, window_size=3)
+    assert plan.info["min_window"] == 4
+    assert plan.info["n_dropped_edges"] > 0
+    th = rng.uniform(0, 2 * np.pi, len(g.trainable_nodes))
+    psi = haar_state(3, rng)
+    want = ReferenceSV(g, psi, window_size=3).run(th, output_form="sv")
+    got = _slot_model(plan, th, psi)
+    ov = np.vdot(want, got)
+    assert abs(abs(ov) / np.linalg.norm(got) - 1) < 1e-10
+
+
+def test_library_exports_every_header_symbol(native):
+    header = (ROOT / "include" / "mbqc_hip.h").read_text()
+    names = set(re.findall(r"\b(mbqc_\w+)\s*\(", header))
+    assert names, "no entry points found in the header"
+    L = native.lib()
+    for nm in sorted(names):
+        assert hasattr(L, nm), f"{nm} not exported"
+    assert set(native.EXPORTED_SYMBOLS) == names
+    assert L.mbqc_abi_version() == 1
+
+
+def test_validation_errors_map_to_reference_exceptions(native):
+    from mentpy_amd.simulators import compile_pattern
+    g = mp.templates.grid_cluster(2, 3)
+    with pytest.raises(ValueError, match="Window size"):
+        compile_pattern(g, window_size=10)
+    with pytest.raises(ValueError, match="Input state has"):
+        compile_pattern(mp.templates.grid_cluster(3, 4), window_size=2)
+    with pytest.raises(ValueError):
+        mp.PatternSimulator(g, backend="no-such-backend")
+    with pytest.raises(TypeError):
+        mp.PatternSimulator(g, input_state=[1, 0, 0, 0], backend="hip-sv")
+    with pytest.raises(NotImplementedError):
+        mp.PatternSimulator(g, backend="hip-sv", force0=False)
+    g2 = mp.templates.grid_cluster(2, 3)
+    g2[1] = mp.Ment("XZ")
+    with pytest.raises(ValueError, match="only XY plane"):
+        mp.PatternSimulator(g2, backend="hip-sv")
+    with pytest.raises(NotImplementedError):
+        mp.PatternSimulator(g2, backend="hip-dm")
+    with pytest.raises(ValueError, match="Number of angles"):
+        mp.PatternSimulator(g, backend="hip-sv").run([0.1])
+
+
+def test_batched_kernel_limits_are_reported(native):
+    from mentpy_amd.simulators import compile_pattern
+    plan, *_ = compile_pattern(mp.templates.grid_cluster(8, 9))
+    assert plan.info["n_slots"] == 8
+    L = native.lib()
+    import ctypes as C
+    rc = L.mbqc_sv_forward_c64(plan.handle, None, 0, 1, None, 0, None, None, 1, 1, 0, None)
+    assert rc == native.MBQC_ERR_NOT_IMPLEMENTED
