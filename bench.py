@@ -28,6 +28,10 @@ sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3     # MI355X published FP32 (vector and matrix) peak
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per simulation of the headline forward kernel, from rocprofv3
+# FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) per dispatch, the scales read off a
+# 1 GiB device copy (profiles/sv_forward_grid4x20.md)
+TRAFFIC_BYTES_PER_SIM_4X20 = 432.1
 
 
 def flops_per_sim(n_steps, window):
@@ -145,6 +149,142 @@ def cpu_gradient_baseline(circ, X, Y, t0, seconds_per_eval_cap=30.0):
     return 1.0 / (el * n), el
 
 
+def config_batched(args, dev, stream):
+    """64 grids (rows 2..5, cols 10..40) x 1024 angle vectors x 16 Haar inputs
+    = 1,048,576 simulations in one autotuned, graph-captured persistent launch."""
+    import torch
+    import mentpy_amd as mp
+    from mentpy_amd.batched import BatchedForward, autotune
+    from mentpy_amd.operators.gates import haar_unitary
+    from oracle.vectorized import sv_batch
+    rng = np.random.default_rng(3)
+    jobs, pats, flops = [], [], 0
+    for _ in range(64):
+        r, c = int(rng.integers(2, 6)), int(rng.integers(10, 41))
+        g = mp.templates.grid_cluster(r, c)
+        sim = mp.PatternSimulator(g, backend="hip-sv")
+        n = len(g.trainable_nodes)
+        th = torch.rand((n, 1024), device=dev) * (2 * np.pi)
+        st = np.stack([haar_unitary(2 ** r, rng)[:, 0] for _ in range(16)])
+        jobs.append((sim, th.t(), st))
+        pats.append((g, th, st))
+        flops += 1024 * 16 * flops_per_sim(sim.simulator.plan.info["n_steps"], r + 1)
+    bf = BatchedForward(jobs)
+    cfg = autotune(bf, reps=2)
+    graph = bf.capture_graph()
+    secs = time_launches(graph.replay, args.steps, args.warmup, stream)
+    ms = secs / args.steps * 1e3
+    total = bf.total_sims
+    errs = []
+    for (g, th, st), out in zip(pats[:16], bf.outputs[:16]):
+        rows = np.random.default_rng(1).choice(1024, 8, replace=False)
+        ref = sv_batch(g, th.t()[rows].double().cpu().numpy(), st)
+        errs.append(float(np.abs(out[rows].cpu().numpy() - ref).max()))
+    achieved = flops / (ms / 1e3) / 1e12
+    return {"metric": "batched pattern sims/s, 64 uneven grids x 1024 angles x 16 inputs",
+            "value": total / (ms / 1e3), "unit": "sims/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "c64", "data": "synthetic (seeded)",
+            "config": {"workload": "64 grid_cluster(r in 2..5, c in 10..40), 1024 angle vectors x 16 "
+                                   "Haar inputs each, one persistent graph-replayed launch",
+                       "sims": total, "autotune": cfg},
+            "max_amp_err_vs_c128": max(errs),
+            "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                         "traffic": None, "kernel": "k_batch"}}
+
+
+def config_dm(args, dev, stream):
+    """Density-matrix 3x12 grid, depolarizing 0.01 + readout flip 0.005, 4096 angle vectors."""
+    import torch
+    import mentpy_amd as mp
+    from mentpy_amd.operators.gates import haar_unitary
+    from mentpy_amd.simulators import dm_forward
+    from oracle.vectorized import dm_batch
+    rng = np.random.default_rng(12)
+    g = mp.templates.grid_cluster(3, 12)
+    sim = mp.PatternSimulator(g, backend="hip-dm", p_depol=0.01, p_readout=0.005)
+    n = len(g.trainable_nodes)
+    B = args.dm_batch
+    th = torch.rand((n, B), device=dev) * (2 * np.pi)
+    psi = haar_unitary(8, rng)[:, 0]
+    st = torch.as_tensor(psi, dtype=torch.complex64, device=dev).reshape(1, -1)
+    out = torch.empty((B, 1, 8, 8), dtype=torch.complex64, device=dev)
+    plan = sim.simulator.plan
+
+    def run():
+        dm_forward(plan, th.t(), st, p_depol=0.01, p_readout=0.005, out=out)
+    secs = time_launches(run, args.steps, args.warmup, stream)
+    ms = secs / args.steps * 1e3
+    k = min(B, 4096)
+    ref = dm_batch(g, th.t()[:k].double().cpu().numpy(), psi[None], p_depol=0.01, p_readout=0.005)[:, 0]
+    td = 0.5 * np.abs(np.linalg.eigvalsh(out[:k, 0].cpu().numpy() - ref)).sum(-1)
+    f_sim = plan.info["n_steps"] * 20 * 4 ** (plan.info["window"] - 1)
+    achieved = f_sim * B / (ms / 1e3) / 1e12
+    return {"metric": "density-matrix pattern sims/s, 3x12 grid, noisy, trace distance vs c128",
+            "value": B / (ms / 1e3), "unit": "sims/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "c64", "data": "synthetic (seeded)",
+            "config": {"workload": "grid_cluster(3,12) hip-dm, depolarizing 0.01, readout flip 0.005",
+                       "batch": B},
+            "max_trace_distance_vs_c128": float(td.max()),
+            "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": FP32_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
+                         "traffic": None, "kernel": "k_dm_forward<3>", "flops_per_sim": f_sim}}
+
+
+def config_mbqml(args, dev, stream):
+    """MB-QML training step: 4x16 grid (60 angles), 1024 parameter sets x 256 Haar inputs."""
+    import torch
+    import mentpy_amd as mp
+    from mentpy_amd.gradients import MBQMLStep, fd_loop_infidelity_grad
+    from mentpy_amd.operators.gates import haar_unitary
+    from oracle.gradients import exact_shift_gradient
+    from oracle.vectorized import sv_batch
+    rng = np.random.default_rng(16)
+    g = mp.templates.grid_cluster(4, 16)
+    sim = mp.PatternSimulator(g, backend="hip-sv")
+    U = haar_unitary(16, rng)
+    X = np.stack([haar_unitary(16, rng)[:, 0] for _ in range(256)])
+    Y = X @ U.T
+    B = 1024
+    th = torch.tensor(rng.uniform(0, 2 * np.pi, (B, 60)), dtype=torch.float32, device=dev)
+    step = MBQMLStep(sim, X, Y, n_sets=B, mode="gram")
+    step.angles.copy_(th)
+    graph = step.capture()
+    secs = time_launches(graph.replay, args.steps, args.warmup, stream)
+    ms = secs / args.steps * 1e3
+    fd_secs = time_launches(lambda: fd_loop_infidelity_grad(sim, th, step.X, step.Y, h=1e-3), 1, 1, stream)
+    _, gr = step.run(th)
+
+    def cost(t):
+        o = sv_batch(g, t[None], X)[0]
+        return 1.0 - np.mean(np.abs(np.sum(np.conj(Y) * o, -1)) ** 2)
+    ref = exact_shift_gradient(cost, th[0].double().cpu().numpy())
+    err = float(np.abs(gr[0].double().cpu().numpy() - ref).max())
+    return {"metric": "MB-QML adjoint angle-grads/s, 4x16 grid, 1024 sets x 256 inputs",
+            "value": B / (ms / 1e3), "unit": "grads/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "c64", "data": "synthetic (seeded Haar data, Haar target U)",
+            "config": {"workload": "grid_cluster(4,16), 60 angles, loss 1-mean|<U psi|out>|^2",
+                       "parameter_sets": B, "inputs": 256},
+            "fd_loop_grads_per_s": B / fd_secs, "adjoint_vs_fd_loop": (B / (ms / 1e3)) / (B / fd_secs),
+            "grad_max_abs_err_vs_c128": err}
+
+
+def config_copy(args, dev, stream):
+    """HBM calibration probe: device-to-device copy of a 1 GiB buffer (the
+    reference kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE counters)."""
+    import torch
+    n = (1 << 30) // 4
+    x = torch.rand(n, device=dev)
+    y = torch.empty_like(x)
+    secs = time_launches(lambda: y.copy_(x), args.steps, args.warmup, stream)
+    ms = secs / args.steps * 1e3
+    return {"metric": "device copy GB/s (1 GiB read + 1 GiB write)", "value": 2 * 4 * n / (ms / 1e3) / 1e9,
+            "unit": "GB/s", "ms_per_step": ms, "bytes_read": 4 * n, "bytes_written": 4 * n}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +296,9 @@ def main():
     ap.add_argument("--grad-sets", type=int, nargs="+", default=[1024, 16384])
     ap.add_argument("--grad-inputs", type=int, default=256)
     ap.add_argument("--cpu-grad-inputs", type=int, default=32)
+    ap.add_argument("--config", default="grid4x20", choices=["grid4x20", "batched", "dm", "mbqml", "copy"])
+    ap.add_argument("--no-grad", action="store_true", help="skip the adjoint section (profiling runs)")
+    ap.add_argument("--dm-batch", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -170,6 +313,11 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream(dev)
+    if args.config != "grid4x20":
+        fn = {"batched": config_batched, "dm": config_dm, "mbqml": config_mbqml,
+              "copy": config_copy}[args.config]
+        print(json.dumps(fn(args, dev, stream)), flush=True)
+        return
     rng = np.random.default_rng(2024)
     circ = mp.templates.grid_cluster(4, 20)
     sim = mp.PatternSimulator(circ, backend="hip-sv")
@@ -206,7 +354,8 @@ def main():
     f_sim = flops_per_sim(plan.info["n_steps"], plan.info["window"])
     achieved = f_sim * B / (ms / 1e3) / 1e12
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                "traffic": int(TRAFFIC_BYTES_PER_SIM_4X20 * B),
                 "kernel": "k_sv_forward<4,true>", "flops_per_sim": f_sim}
 
     line = {
@@ -220,21 +369,24 @@ def main():
         "max_amp_err_vs_c128": amp_err, "max_infidelity_vs_c128": fid_err,
         "roofline": roofline,
     }
-    adj, (X, Y, t0, _) = adjoint_section(circ, sim, dev, max(2, args.steps // 4), stream,
-                                         sets=tuple(args.grad_sets), S=args.grad_inputs)
-    line.update(adj)
+    X = None
+    if not args.no_grad:
+        adj, (X, Y, t0, _) = adjoint_section(circ, sim, dev, max(2, args.steps // 4), stream,
+                                             sets=tuple(args.grad_sets), S=args.grad_inputs)
+        line.update(adj)
     if not args.no_cpu:
         rate, count, el = cpu_baseline(circ, psi, args.cpu_seconds)
         line["cpu_baseline"] = {"value": rate, "unit": "sims/s", "cores": 1, "kind": "port",
                                 "sample": f"{count} sims of grid_cluster(4,20) in {el:.1f} s, "
                                           "step-by-step numpy complex128 restatement, 1 thread"}
-        grate, gel = cpu_gradient_baseline(circ, X[:args.cpu_grad_inputs], Y[:args.cpu_grad_inputs], t0)
-        grate = grate * args.cpu_grad_inputs / len(X)
-        line["cpu_baseline_grads"] = {
-            "value": grate, "unit": "grads/s", "cores": 1, "kind": "port",
-            "sample": f"central FD of one angle over {args.cpu_grad_inputs} of the {len(X)} inputs "
-                      f"({gel:.1f} s), scaled linearly to all angles and inputs"}
-        line["adjoint_vs_cpu"] = line["adjoint_grads_per_s"] / grate
+        if X is not None:
+            grate, gel = cpu_gradient_baseline(circ, X[:args.cpu_grad_inputs], Y[:args.cpu_grad_inputs], t0)
+            grate = grate * args.cpu_grad_inputs / len(X)
+            line["cpu_baseline_grads"] = {
+                "value": grate, "unit": "grads/s", "cores": 1, "kind": "port",
+                "sample": f"central FD of one angle over {args.cpu_grad_inputs} of the {len(X)} "
+                          f"inputs ({gel:.1f} s), scaled linearly to all angles and inputs"}
+            line["adjoint_vs_cpu"] = line["adjoint_grads_per_s"] / grate
     print(json.dumps(line), flush=True)
 
 

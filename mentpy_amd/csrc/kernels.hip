@@ -227,7 +227,11 @@ __device__ __forceinline__ float sv_final_scale(int e) {
 
 // One SV simulation on this lane (g = b*S + s).  Inactive lanes run on
 // simulation 0 and write nothing.
-template <int NS, bool REF>
+// One SV simulation on this lane (g = b*S + s).  Inactive lanes run on
+// simulation 0 and write nothing.  With STAGE (kernels whose waves cover 64
+// consecutive simulations and outputs of at most 2^NS <= 16 amplitudes) the
+// wave's output rows go through LDS and leave as contiguous 512-byte stores.
+template <int NS, bool REF, bool STAGE = false>
 __device__ __forceinline__ void sv_sim(const Job& J, int64_t g, bool active, uint32_t flags) {
   constexpr int H = 1 << NS;
   const int64_t gg = active ? g : 0;
@@ -250,21 +254,43 @@ __device__ __forceinline__ void sv_sim(const Job& J, int64_t g, bool active, uin
     scale = 1.0f / sqrtf(n2);
   }
   const float2 f = cscale(ph, scale);
-  if (active) {
-    float2* o = J.out + g * (int64_t(1) << J.n_out);
+  if constexpr (STAGE) {
+    static_assert(H <= 16, "staged output supports at most 16 amplitudes");
+    __shared__ float2 stage[4][64 * (H + 1)];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int dout = 1 << J.n_out;
+    float2* sw = stage[w];
 #pragma unroll
     for (int j = 0; j < H; ++j) {
       const int p = out_pos[j];
-      if (p >= 0) o[p] = cmul(t[j], f);
+      if (p >= 0) sw[lane * (dout + 1) + p] = cmul(t[j], f);
     }
-    if ((flags & MBQC_FLAG_WRITE_PHASE) && J.phase_out) J.phase_out[g] = ph;
+    __syncthreads();
+    const int64_t g0 = g - lane;
+    float2* obase = J.out + g0 * dout;
+    const int shift = J.n_out;
+    for (int e = lane; e < 64 * dout; e += 64) {
+      const int r = e >> shift, c = e & (dout - 1);
+      if (g0 + r < J.total) obase[e] = sw[r * (dout + 1) + c];
+    }
+    if (active && (flags & MBQC_FLAG_WRITE_PHASE) && J.phase_out) J.phase_out[g] = ph;
+  } else {
+    if (active) {
+      float2* o = J.out + g * (int64_t(1) << J.n_out);
+#pragma unroll
+      for (int j = 0; j < H; ++j) {
+        const int p = out_pos[j];
+        if (p >= 0) o[p] = cmul(t[j], f);
+      }
+      if ((flags & MBQC_FLAG_WRITE_PHASE) && J.phase_out) J.phase_out[g] = ph;
+    }
   }
 }
 
 template <int NS, bool REF>
 __global__ __launch_bounds__(256) void k_sv_forward(Job J, uint32_t flags) {
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  sv_sim<NS, REF>(J, g, g < J.total, flags);
+  sv_sim<NS, REF, (NS <= 4)>(J, g, g < J.total, flags);
 }
 
 // ---------------------------------------------------------------- DM step

@@ -1,0 +1,144 @@
+"""The configured workloads at their full sizes on the GPU.
+
+Sizes the oracle cannot cover completely are checked on a seeded sample
+against the complex128 oracle plus size-independent properties on every
+output (unit norm, unit trace, Hermiticity, agreement between two
+independent kernel paths).
+"""
+import numpy as np
+import pytest
+
+from conftest import haar_state
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def test_grid_4x20_full_batch():
+    """Single pattern: 4x20 grid, 65536 angle vectors, one Haar input."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from oracle.vectorized import sv_batch
+    rng = np.random.default_rng(20)
+    g = mp.templates.grid_cluster(4, 20)
+    sim = mp.PatternSimulator(g, backend="hip-sv")
+    B = 65536
+    th = torch.rand((76, B), generator=torch.Generator().manual_seed(3)).mul_(2 * np.pi).cuda()
+    psi = haar_state(4, rng)
+    out = sim.run_batch(th.t(), psi)[:, 0]
+    norms = (out.abs() ** 2).sum(-1)
+    assert (norms - 1).abs().max().item() < 2e-6
+    idx = rng.choice(B, 512, replace=False)
+    ref = sv_batch(g, th.t()[idx].double().cpu().numpy(), psi[None])[:, 0]
+    err = np.abs(out[idx].cpu().numpy() - ref).max()
+    fid = np.abs(np.sum(np.conj(ref) * out[idx].cpu().numpy(), -1)) ** 2
+    print(f"4x20 B={B}: max amp err {err:.2e}, max 1-F {np.max(1 - fid):.2e}", flush=True)
+    assert err < 2e-5
+    assert np.max(np.abs(1 - fid)) < 1e-5
+
+
+def _c3_jobs(seed=3, n_pat=64, B=1024, S=16):
+    import mentpy_amd as mp
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for _ in range(n_pat):
+        r = int(rng.integers(2, 6))
+        c = int(rng.integers(10, 41))
+        g = mp.templates.grid_cluster(r, c)
+        sim = mp.PatternSimulator(g, backend="hip-sv")
+        th = rng.uniform(0, 2 * np.pi, (B, len(g.trainable_nodes))).astype(np.float32)
+        st = np.stack([haar_state(r, rng) for _ in range(S)])
+        jobs.append((sim, th, st, g))
+    return jobs
+
+
+def test_batched_uneven_patterns_full():
+    """64 grids (rows 2..5, cols 10..40) x 1024 angle vectors x 16 inputs =
+    1,048,576 simulations in one graph-captured persistent launch; >1% spot
+    check against the oracle, unit norm everywhere, equality with per-pattern
+    launches."""
+    torch = _torch()
+    from mentpy_amd.batched import BatchedForward, autotune
+    from oracle.vectorized import sv_batch
+    jobs = _c3_jobs()
+    bf = BatchedForward([(s, th, st) for s, th, st, _ in jobs], sims_per_item=64)
+    assert bf.total_sims == 64 * 1024 * 16
+    autotune(bf, reps=1, candidates=[{"waves": 4096, "wg_waves": 4, "sims_per_item": 64},
+                                     {"waves": 8192, "wg_waves": 4, "sims_per_item": 128}])
+    graph = bf.capture_graph()
+    graph.replay()
+    torch.cuda.synchronize()
+    checked, worst = 0, 0.0
+    rng = np.random.default_rng(5)
+    for (sim, th, st, g), out in zip(jobs, bf.outputs):
+        norms = (out.abs() ** 2).sum(-1)
+        assert (norms - 1).abs().max().item() < 5e-6
+        rows = rng.choice(th.shape[0], 11, replace=False)
+        ref = sv_batch(g, th[rows].astype(np.float64), st)
+        got = out[rows].cpu().numpy()
+        worst = max(worst, float(np.abs(got - ref).max()))
+        checked += ref.shape[0] * ref.shape[1]
+    single = jobs[0][0].run_batch(jobs[0][1], jobs[0][2])
+    assert (single - bf.outputs[0]).abs().max().item() < 1e-6
+    print(f"batched 1M sims: spot-checked {checked} ({100 * checked / bf.total_sims:.2f}%), "
+          f"max amp err {worst:.2e}", flush=True)
+    assert checked >= 0.01 * bf.total_sims
+    assert worst < 2e-5
+
+
+def test_dm_3x12_noisy_full():
+    """3x12 grid, depolarizing 0.01 + readout flip 0.005, 4096 angle vectors:
+    every output against the oracle (trace distance), unit trace, Hermitian."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from oracle.vectorized import dm_batch
+    rng = np.random.default_rng(12)
+    g = mp.templates.grid_cluster(3, 12)
+    sim = mp.PatternSimulator(g, backend="hip-dm", p_depol=0.01, p_readout=0.005)
+    th = rng.uniform(0, 2 * np.pi, (4096, len(g.trainable_nodes))).astype(np.float32)
+    psi = haar_state(3, rng)
+    out = sim.run_batch(th, psi)[:, 0].cpu().numpy()
+    ref = dm_batch(g, th.astype(np.float64), psi[None], p_depol=0.01, p_readout=0.005)[:, 0]
+    td = 0.5 * np.abs(np.linalg.eigvalsh(out - ref)).sum(-1)
+    tr = np.trace(out, axis1=1, axis2=2)
+    print(f"dm 3x12 x4096: max trace distance {td.max():.2e}", flush=True)
+    assert td.max() < 1e-5
+    assert np.abs(tr - 1).max() < 1e-6
+    assert np.abs(out - np.conj(np.transpose(out, (0, 2, 1)))).max() < 1e-6
+
+
+def test_mbqml_4x16_full_step():
+    """MB-QML step: 4x16 grid, 1024 parameter sets x 256 Haar inputs; the Gram
+    (transfer-matrix) path against the direct path on every parameter set,
+    and parameter set 0 against the oracle's exact shift rule."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from mentpy_amd.gradients import mbqml_infidelity
+    from mentpy_amd.operators.gates import haar_unitary
+    from oracle.gradients import exact_shift_gradient
+    from oracle.vectorized import sv_batch
+    rng = np.random.default_rng(16)
+    g = mp.templates.grid_cluster(4, 16)
+    sim = mp.PatternSimulator(g, backend="hip-sv")
+    U = haar_unitary(16, rng)
+    X = np.stack([haar_state(4, rng) for _ in range(256)])
+    Y = X @ U.T
+    th = rng.uniform(0, 2 * np.pi, (1024, 60)).astype(np.float32)
+    l_g, g_g = mbqml_infidelity(sim, th, X, Y, mode="gram")
+    l_d, g_d = mbqml_infidelity(sim, th, X, Y, mode="direct")
+    assert (l_g - l_d).abs().max().item() < 2e-6
+    assert (g_g - g_d).abs().max().item() < 5e-5
+
+    def cost(t):
+        o = sv_batch(g, t[None], X)[0]
+        return 1.0 - np.mean(np.abs(np.sum(np.conj(Y) * o, -1)) ** 2)
+    ref = exact_shift_gradient(cost, th[0].astype(np.float64))
+    err = np.abs(g_g[0].cpu().numpy() - ref).max()
+    print(f"4x16 MB-QML step: grad err vs c128 {err:.2e}", flush=True)
+    assert err < 2e-5
