@@ -444,6 +444,191 @@ __global__ __launch_bounds__(256) void k_dm_forward(Job J, float gamma, float p_
   }
 }
 
+// ------------------------------------------------------- LDS-resident kernels
+// Larger live windows (SV with 7..12 slots, DM with 4..6 slots): one
+// workgroup per simulation keeps the state in LDS; the measured slot is a
+// runtime value, each thread updates a strided subset of the amplitude pairs
+// (DM: quartets) and the workgroup synchronises between steps.
+__device__ __forceinline__ int insert_zero(int v, int q) {
+  return ((v >> q) << (q + 1)) | (v & ((1 << q) - 1));
+}
+
+template <int T>
+__device__ __forceinline__ float block_sum(float v, float* red) {
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  if constexpr (T > 64) {
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    v = 0.0f;
+#pragma unroll
+    for (int i = 0; i < T / 64; ++i) v += red[i];
+  }
+  return v;
+}
+
+template <int NS, int T>
+__global__ __launch_bounds__(T) void k_sv_forward_lds(Job J, uint32_t flags) {
+  constexpr int H = 1 << NS;
+  __shared__ float2 st[H];
+  __shared__ float red[4];
+  const int64_t g = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int64_t bi = g / J.S, si = g - bi * J.S;
+  const float* ang = J.angles + bi * J.angle_stride;
+  const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
+  const int32_t* in_index = J.tab + J.off_in;
+  const int32_t* sign = J.tab + J.off_sign;
+  const int32_t* resc = J.tab + J.off_svres;
+  const int32_t* out_pos = J.tab + J.off_out;
+  const float2* psi = J.psi + si * J.in_stride;
+  for (int j = tid; j < H; j += T) st[j] = flip(psi[in_index[j]], sign[j] ? 0x80000000u : 0u);
+  float2 ph = make_float2(1.0f, 0.0f);
+  const bool ref = flags & MBQC_FLAG_REF_PHASE;
+  for (int k = 0; k < J.n_steps; ++k) {
+    if (k % NS == 0 && resc[k / NS]) {
+      __syncthreads();
+      for (int j = tid; j < H; j += T) st[j] = cscale(st[j], 0x1p-32f);
+    }
+    const mbqc_step step = steps[k];
+    const float th = step.angle_index >= 0 ? ang[int64_t(step.angle_index) * J.angle_stride_a]
+                                           : step.fixed_angle;
+    const float2 b = ref ? step_coeff<true>(th, &ph) : step_coeff<false>(th, &ph);
+    const int q = int((step.flags >> 8) & 0xffu);
+    const int QB = 1 << q;
+    const float s = (step.flags & 2u) ? -1.0f : 1.0f;
+    const float app = (step.flags & 1u) ? 1.0f : 0.0f;
+    __syncthreads();
+    for (int i = tid; i < H / 2; i += T) {
+      const int j = insert_zero(i, q);
+      const float2 x = st[j], y = st[j | QB];
+      const float c = (__builtin_popcount(unsigned(j) & step.nmask) & 1) ? -app : app;
+      const float al = c * s, be = c * (1.0f - s);
+      float2 u0;
+      u0.x = fmaf(-b.y, y.y, fmaf(b.x, y.x, x.x));
+      u0.y = fmaf(b.x, y.y, fmaf(b.y, y.x, x.y));
+      st[j] = u0;
+      st[j | QB] = make_float2(fmaf(al, u0.x, be * x.x), fmaf(al, u0.y, be * x.y));
+    }
+  }
+  __syncthreads();
+  ph = cscale(ph, 1.0f / sqrtf(fmaf(ph.x, ph.x, ph.y * ph.y)));
+  float scale = sv_final_scale(J.e_sv);
+  if (flags & MBQC_FLAG_RENORMALIZE) {
+    float n2 = 0.0f;
+    for (int j = tid; j < H; j += T)
+      if (out_pos[j] >= 0) n2 = fmaf(st[j].x, st[j].x, fmaf(st[j].y, st[j].y, n2));
+    n2 = block_sum<T>(n2, red);
+    scale = 1.0f / sqrtf(n2);
+  }
+  const float2 f = cscale(ph, scale);
+  float2* o = J.out + g * (int64_t(1) << J.n_out);
+  for (int j = tid; j < H; j += T) {
+    const int p = out_pos[j];
+    if (p >= 0) o[p] = cmul(st[j], f);
+  }
+  if (tid == 0 && (flags & MBQC_FLAG_WRITE_PHASE) && J.phase_out) J.phase_out[g] = ph;
+}
+
+template <int NS, int T>
+__global__ __launch_bounds__(T) void k_dm_forward_lds(Job J, float gamma, float p_out, uint32_t flags) {
+  constexpr int H = 1 << NS;
+  constexpr int HQ = H / 2;
+  __shared__ float2 r[H * H];
+  __shared__ float red[4];
+  const int64_t g = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int64_t bi = g / J.S, si = g - bi * J.S;
+  const float* ang = J.angles + bi * J.angle_stride;
+  const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
+  const int32_t* in_index = J.tab + J.off_in;
+  const int32_t* sign = J.tab + J.off_sign;
+  const int32_t* resc = J.tab + J.off_dmres;
+  const int32_t* out_pos = J.tab + J.off_out;
+  const float2* psi = J.psi + si * J.in_stride;
+  for (int e = tid; e < H * H; e += T) {
+    const int j = e / H, k = e % H;
+    const float2 vj = flip(psi[in_index[j]], sign[j] ? 0x80000000u : 0u);
+    const float2 vk = flip(psi[in_index[k]], sign[k] ? 0x80000000u : 0u);
+    r[e] = cmul(vj, cconj(vk));
+  }
+  for (int k = 0; k < J.n_steps; ++k) {
+    if (k % NS == 0 && resc[k / NS]) {
+      __syncthreads();
+      for (int e = tid; e < H * H; e += T) r[e] = cscale(r[e], 0x1p-32f);
+    }
+    const mbqc_step step = steps[k];
+    const float th = step.angle_index >= 0 ? ang[int64_t(step.angle_index) * J.angle_stride_a]
+                                           : step.fixed_angle;
+    float2 ph = make_float2(1.f, 0.f);
+    const float2 gb = cscale(step_coeff<false>(th, &ph), gamma);
+    const float2 gbc = cconj(gb);
+    const int q = int((step.flags >> 8) & 0xffu);
+    const int QB = 1 << q;
+    const bool app = step.flags & 1u, sneg = step.flags & 2u;
+    __syncthreads();
+    for (int i = tid; i < HQ * HQ; i += T) {
+      const int j = insert_zero(i / HQ, q), kk = insert_zero(i % HQ, q);
+      const float2 B00 = r[j * H + kk], B01 = r[j * H + (kk | QB)];
+      const float2 B10 = r[(j | QB) * H + kk], B11 = r[(j | QB) * H + (kk | QB)];
+      const float2 U = cmul(gbc, B01), V = cmul(gb, B10);
+      const float2 P = cadd(B00, B11);
+      const float2 r00 = cadd(P, cadd(U, V));
+      if (app) {
+        const uint32_t sj = czbit(j, step.nmask), sk = czbit(kk, step.nmask);
+        if (sneg) {
+          const float2 M = csub(B00, B11);
+          r[j * H + kk] = r00;
+          r[j * H + (kk | QB)] = flip(cadd(M, csub(V, U)), sk);
+          r[(j | QB) * H + kk] = flip(cadd(M, csub(U, V)), sj);
+          r[(j | QB) * H + (kk | QB)] = flip(csub(P, cadd(U, V)), sj ^ sk);
+        } else {
+          r[j * H + kk] = r00;
+          r[j * H + (kk | QB)] = flip(r00, sk);
+          r[(j | QB) * H + kk] = flip(r00, sj);
+          r[(j | QB) * H + (kk | QB)] = flip(r00, sj ^ sk);
+        }
+      } else {
+        r[j * H + kk] = r00;
+        r[j * H + (kk | QB)] = make_float2(0.f, 0.f);
+        r[(j | QB) * H + kk] = make_float2(0.f, 0.f);
+        r[(j | QB) * H + (kk | QB)] = make_float2(0.f, 0.f);
+      }
+    }
+  }
+  // depolarizing of every output qubit
+  const float keep = 1.0f - 2.0f * p_out / 3.0f, mix = 2.0f * p_out / 3.0f, coh = 1.0f - 4.0f * p_out / 3.0f;
+  for (int m = 0; m < NS; ++m) {
+    if (!(uint32_t(J.live_mask) & (1u << m)) || p_out == 0.0f) continue;
+    const int MB = 1 << m;
+    __syncthreads();
+    for (int i = tid; i < HQ * HQ; i += T) {
+      const int j = insert_zero(i / HQ, m), kk = insert_zero(i % HQ, m);
+      const float2 a = r[j * H + kk], d = r[(j | MB) * H + (kk | MB)];
+      r[j * H + kk] = make_float2(keep * a.x + mix * d.x, keep * a.y + mix * d.y);
+      r[(j | MB) * H + (kk | MB)] = make_float2(keep * d.x + mix * a.x, keep * d.y + mix * a.y);
+      r[j * H + (kk | MB)] = cscale(r[j * H + (kk | MB)], coh);
+      r[(j | MB) * H + kk] = cscale(r[(j | MB) * H + kk], coh);
+    }
+  }
+  __syncthreads();
+  float scale = ldexpf(1.0f, -J.e_dm);
+  if (flags & MBQC_FLAG_RENORMALIZE) {
+    float tr = 0.0f;
+    for (int j = tid; j < H; j += T)
+      if (out_pos[j] >= 0) tr += r[j * H + j].x;
+    tr = block_sum<T>(tr, red);
+    scale = 1.0f / tr;
+  }
+  const int64_t d = int64_t(1) << J.n_out;
+  float2* o = J.out + g * d * d;
+  for (int e = tid; e < H * H; e += T) {
+    const int pj = out_pos[e / H], pk = out_pos[e % H];
+    if (pj >= 0 && pk >= 0) o[int64_t(pj) * d + pk] = cscale(r[e], scale);
+  }
+}
+
 // ---------------------------------------------------------------- adjoint
 // Forward pass of the adjoint: raw phase, unnormalized output, one checkpoint
 // of the live state at the start of every block of NS steps (after the
@@ -767,9 +952,26 @@ hipError_t launch_adj_bwd(const Job& J, const float2* ckpt, const float2* seed, 
   return hipGetLastError();
 }
 
-constexpr int kMaxSvSlots = 6;
-constexpr int kMaxDmSlots = 3;
+constexpr int kMaxSvSlots = 6;       // register-resident SV kernels
+constexpr int kMaxSvLdsSlots = 12;   // LDS-resident SV kernel
+constexpr int kMaxDmSlots = 3;       // register-resident DM kernel
+constexpr int kMaxDmLdsSlots = 6;    // LDS-resident DM kernel
 constexpr int kMaxAdjSlots = 5;
+
+template <int NS>
+hipError_t launch_sv_lds(const Job& J, uint32_t flags, hipStream_t st) {
+  constexpr int T = (1 << (NS - 1)) < 256 ? (1 << (NS - 1)) : 256;
+  hipLaunchKernelGGL((k_sv_forward_lds<NS, T>), dim3(J.total), dim3(T), 0, st, J, flags);
+  return hipGetLastError();
+}
+
+template <int NS>
+hipError_t launch_dm_lds(const Job& J, float gamma, float p_out, uint32_t flags, hipStream_t st) {
+  constexpr int Q = 1 << (2 * (NS - 1));
+  constexpr int T = Q < 256 ? Q : 256;
+  hipLaunchKernelGGL((k_dm_forward_lds<NS, T>), dim3(J.total), dim3(T), 0, st, J, gamma, p_out, flags);
+  return hipGetLastError();
+}
 
 }  // namespace
 
@@ -798,9 +1000,9 @@ int mbqc_sv_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
                         void* phase_out, int64_t B, int64_t S, uint32_t flags, void* stream) {
   if (!plan) return fail(MBQC_ERR_VALUE, "null plan");
   Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(plan));
-  if (p->ns > kMaxSvSlots)
+  if (p->ns > kMaxSvLdsSlots)
     return fail(MBQC_ERR_NOT_IMPLEMENTED, "statevector kernels support live windows of at most " +
-                                              std::to_string(kMaxSvSlots) + " qubits");
+                                              std::to_string(kMaxSvLdsSlots) + " qubits");
   Job J;
   int rc = make_job(p, angles, angle_stride, angle_stride_a, psi_in, in_stride, out, B, S, &J);
   if (rc != MBQC_OK) return rc;
@@ -816,6 +1018,12 @@ int mbqc_sv_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
     case 4: e = launch_sv<4>(J, flags, st); break;
     case 5: e = launch_sv<5>(J, flags, st); break;
     case 6: e = launch_sv<6>(J, flags, st); break;
+    case 7: e = launch_sv_lds<7>(J, flags, st); break;
+    case 8: e = launch_sv_lds<8>(J, flags, st); break;
+    case 9: e = launch_sv_lds<9>(J, flags, st); break;
+    case 10: e = launch_sv_lds<10>(J, flags, st); break;
+    case 11: e = launch_sv_lds<11>(J, flags, st); break;
+    case 12: e = launch_sv_lds<12>(J, flags, st); break;
   }
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "sv_forward launch");
 }
@@ -825,9 +1033,9 @@ int mbqc_dm_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
                         int64_t B, int64_t S, const mbqc_noise* noise, uint32_t flags, void* stream) {
   if (!plan) return fail(MBQC_ERR_VALUE, "null plan");
   Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(plan));
-  if (p->ns > kMaxDmSlots)
+  if (p->ns > kMaxDmLdsSlots)
     return fail(MBQC_ERR_NOT_IMPLEMENTED, "density-matrix kernels support live windows of at most " +
-                                              std::to_string(kMaxDmSlots) + " qubits");
+                                              std::to_string(kMaxDmLdsSlots) + " qubits");
   float pd = 0.f, pr = 0.f;
   if (noise) {
     pd = noise->p_depol;
@@ -846,6 +1054,9 @@ int mbqc_dm_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
     case 1: e = launch_dm<1>(J, gamma, pd, flags, st); break;
     case 2: e = launch_dm<2>(J, gamma, pd, flags, st); break;
     case 3: e = launch_dm<3>(J, gamma, pd, flags, st); break;
+    case 4: e = launch_dm_lds<4>(J, gamma, pd, flags, st); break;
+    case 5: e = launch_dm_lds<5>(J, gamma, pd, flags, st); break;
+    case 6: e = launch_dm_lds<6>(J, gamma, pd, flags, st); break;
   }
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "dm_forward launch");
 }

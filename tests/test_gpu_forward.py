@@ -43,6 +43,64 @@ def test_sv_grid_matches_oracle(rows, cols, rng):
     assert infidelity(out, ref).max() < FID_TOL
 
 
+@pytest.mark.parametrize("rows,cols", [(7, 3), (8, 3), (9, 3), (10, 3), (12, 3)])
+def test_sv_large_windows_lds_kernel(rows, cols, rng):
+    """Live windows of 7..12 qubits run in the LDS-resident kernel."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from oracle.vectorized import sv_batch
+    g = mp.templates.grid_cluster(rows, cols)
+    sim = mp.PatternSimulator(g, backend="hip-sv")
+    assert sim.simulator.plan.info["n_slots"] == rows
+    th = rng.uniform(0, 2 * np.pi, (3, len(g.trainable_nodes)))
+    st = np.stack([haar_state(rows, rng) for _ in range(2)])
+    out = sim.run_batch(th, st).cpu().numpy()
+    ref = sv_batch(g, th.astype(np.float32).astype(np.float64), st)
+    err = np.abs(out - ref).max()
+    print(f"grid {rows}x{cols} (LDS kernel): max amp err {err:.2e}", flush=True)
+    assert err < AMP_TOL
+
+
+@pytest.mark.parametrize("rows,cols", [(4, 4), (5, 3), (6, 3)])
+def test_dm_large_windows_lds_kernel(rows, cols, rng):
+    torch = _torch()
+    import mentpy_amd as mp
+    from oracle.vectorized import dm_batch
+    g = mp.templates.grid_cluster(rows, cols)
+    sim = mp.PatternSimulator(g, backend="hip-dm", p_depol=0.02, p_readout=0.01)
+    assert sim.simulator.plan.info["n_slots"] == rows
+    th = rng.uniform(0, 2 * np.pi, (3, len(g.trainable_nodes)))
+    st = np.stack([haar_state(rows, rng) for _ in range(2)])
+    out = sim.run_batch(th, st).cpu().numpy()
+    ref = dm_batch(g, th.astype(np.float32).astype(np.float64), st, p_depol=0.02, p_readout=0.01)
+    err = np.abs(out - ref).max()
+    print(f"dm grid {rows}x{cols} (LDS kernel): max err {err:.2e}", flush=True)
+    assert err < AMP_TOL
+
+
+@pytest.mark.parametrize("rows,cols", [(2, 4), (2, 5)])
+def test_random_measurements_equal_across_backends(rows, cols, rng):
+    """mentpy tests/test_simulators.py:33-64: window_size=min(5, #trainable),
+    two nodes fixed to X, hip-sv and hip-dm agree with the oracle."""
+    _torch()
+    import mentpy_amd as mp
+    from oracle.reference_sv import ReferenceSV
+    g = mp.templates.grid_cluster(rows, cols)
+    idx = rng.choice(g.outputc, 2, replace=False)
+    g[int(idx[0])] = mp.Ment("X")
+    g[int(idx[1])] = mp.Ment("X")
+    th = 2 * np.pi * rng.random(len(g.trainable_nodes))
+    w = min(5, len(g.trainable_nodes))
+    ref = ReferenceSV(g, window_size=w)
+    ref.reset()
+    want = ref.run(th.astype(np.float32).astype(np.float64))
+    for backend in ("hip-sv", "hip-dm"):
+        ps = mp.PatternSimulator(g, backend=backend, window_size=w)
+        ps.reset()
+        got = ps.run(th)
+        assert np.allclose(got, want, atol=2e-5), backend
+
+
 def test_angle_major_layout_matches_row_major(rng):
     """Angles given as an angle-major [n, B] tensor viewed as [B, n] (strided
     kernel path) give the same outputs as contiguous [B, n] angles."""

@@ -155,11 +155,16 @@ def test_validation_errors_map_to_reference_exceptions(native):
         mp.PatternSimulator(g, backend="hip-sv").run([0.1])
 
 
-def test_batched_kernel_limits_are_reported(native):
+def test_kernel_limits_are_reported(native):
+    """Windows beyond the kernels' reach fail with NotImplementedError before
+    any device work (the reference has no limit, but no configuration needs more)."""
     from mentpy_amd.simulators import compile_pattern
-    plan, *_ = compile_pattern(mp.templates.grid_cluster(8, 9))
-    assert plan.info["n_slots"] == 8
+    plan, *_ = compile_pattern(mp.templates.grid_cluster(7, 8), exclude="quantum_output")
+    assert plan.info["n_slots"] == 7
     L = native.lib()
-    import ctypes as C
-    rc = L.mbqc_sv_forward_c64(plan.handle, None, 0, 1, None, 0, None, None, 1, 1, 0, None)
+    rc = L.mbqc_dm_forward_c64(plan.handle, None, 0, 1, None, 0, None, 1, 1, None, 0, None)
     assert rc == native.MBQC_ERR_NOT_IMPLEMENTED
+    rc = L.mbqc_sv_adjoint_forward_c64(plan.handle, None, 0, 1, None, 0, None, 1, 1, None, 0, None)
+    assert rc == native.MBQC_ERR_NOT_IMPLEMENTED
+    with pytest.raises(NotImplementedError):
+        compile_pattern(mp.templates.grid_cluster(13, 14))
