@@ -51,6 +51,8 @@ typedef struct mbqc_pattern_desc {
   const int32_t* angle_index;  /* [n_nodes] index into the angle vector, or -1 */
   const float* fixed_angle;    /* [n_nodes] XY angle used when angle_index < 0 */
   int32_t n_angles;            /* length of one angle vector */
+  const int32_t* flow;         /* optional [n_nodes]: causal-flow successor, -1 for outputs;
+                                  needed only for sampled outcomes */
 } mbqc_pattern_desc;
 
 typedef struct mbqc_plan mbqc_plan;
@@ -63,6 +65,7 @@ typedef struct mbqc_plan_info {
   int32_t n_appends;      /* |+> qubits appended over the run (padding included) */
   int32_t n_dropped_edges;/* edges never applied because the window was too small */
   int32_t min_window;     /* smallest window that applies every edge */
+  int32_t can_sample;     /* 1: mbqc_sv_sample_c64 accepts this plan */
 } mbqc_plan_info;
 
 /* Per-step table entry (16 bytes) as stored on the device. */
@@ -110,6 +113,24 @@ int mbqc_plan_get_rescale(const mbqc_plan* plan, int32_t* sv_flags, int32_t* dm_
 int mbqc_sv_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
                         int64_t angle_stride_a, const void* psi_in, int64_t in_stride, void* out,
                         void* phase_out, int64_t B, int64_t S, uint32_t flags, void* stream);
+
+/* Sampled outcomes with flow corrections (force0=False), complex64: the
+ * semantics of the PennyLane backend (mentpy/simulators/pennylane_simulator.py:138-153):
+ * every measurement outcome is drawn (probability 1/2 each on patterns with a
+ * causal flow and no dropped CZ), an outcome 1 on node i applies X to flow(i)
+ * and Z to the neighbours of flow(i) measured after i.  The corrections are
+ * tracked as a Pauli frame (no state update) and the residual frame of the
+ * output qubits is applied to the output at the end, so the output equals the
+ * post-selected one up to a global phase.  outcomes is [B*S][W+2] int32 with
+ * W = (n_steps+31)/32: words 0..W-1 hold the physical outcomes (bit k%32 of
+ * word k/32 = outcome of the k-th measured node), words W and W+1 the X and Z
+ * frame applied to the output (bit n_outputs-1-m = output node m).  Draws are
+ * a counter-based hash of (seed, simulation, step).  Needs the flow in the
+ * pattern description, no dropped edges and n_slots <= 6 (else
+ * MBQC_ERR_NOT_IMPLEMENTED with the reason). */
+int mbqc_sv_sample_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
+                       int64_t angle_stride_a, const void* psi_in, int64_t in_stride, void* out,
+                       int32_t* outcomes, int64_t B, int64_t S, uint64_t seed, void* stream);
 
 /* Density-matrix forward, complex64: replaces NumpySimulatorDM.run / measure /
  * measure_ment (np_simulator_dm.py:151-283, 307-346) and
@@ -166,6 +187,17 @@ int64_t mbqc_batch_items(const mbqc_batch* batch);
  * in a hipGraph), `wg_waves` wavefronts per workgroup. */
 int mbqc_batch_launch(mbqc_batch* batch, int32_t waves, int32_t wg_waves, uint32_t flags,
                       void* stream);
+
+/* Device-resident optimizer steps on n float32 parameters (all pointers device
+ * memory, updated in place), replacing the host updates of
+ * AdamOptimizer.step (mentpy/optimizers/adam.py:54-66) and SGDOptimizer.step
+ * (mentpy/optimizers/sgd.py:49-59).  `step` is a device int32 holding the
+ * number of Adam steps taken (bias correction uses step+1); the call advances
+ * it, so a captured graph of training iterations replays correctly. */
+int mbqc_adam_step_f32(float* x, const float* grad, float* m, float* v, int64_t n, float lr, float b1,
+                       float b2, float eps, int32_t* step, void* stream);
+int mbqc_sgd_step_f32(float* x, const float* grad, float* v, int64_t n, float lr, float momentum,
+                      int32_t nesterov, void* stream);
 
 const char* mbqc_last_error(void);
 int32_t mbqc_abi_version(void);

@@ -28,11 +28,11 @@ FLAG_WRITE_PHASE = 4
 
 EXPORTED_SYMBOLS = (
     "mbqc_plan_create", "mbqc_plan_destroy", "mbqc_plan_get_info", "mbqc_plan_get_tables",
-    "mbqc_plan_get_rescale", "mbqc_sv_forward_c64", "mbqc_dm_forward_c64",
+    "mbqc_plan_get_rescale", "mbqc_sv_forward_c64", "mbqc_sv_sample_c64", "mbqc_dm_forward_c64",
     "mbqc_sv_adjoint_workspace_bytes", "mbqc_sv_adjoint_forward_c64",
     "mbqc_sv_adjoint_partials", "mbqc_sv_adjoint_backward_c64", "mbqc_batch_create",
-    "mbqc_batch_destroy", "mbqc_batch_items", "mbqc_batch_launch", "mbqc_last_error",
-    "mbqc_abi_version",
+    "mbqc_batch_destroy", "mbqc_batch_items", "mbqc_batch_launch", "mbqc_adam_step_f32",
+    "mbqc_sgd_step_f32", "mbqc_last_error", "mbqc_abi_version",
 )
 
 
@@ -43,14 +43,14 @@ class PatternDesc(C.Structure):
         ("n_inputs", C.c_int32), ("input_nodes", C.POINTER(C.c_int32)),
         ("n_outputs", C.c_int32), ("output_nodes", C.POINTER(C.c_int32)),
         ("angle_index", C.POINTER(C.c_int32)), ("fixed_angle", C.POINTER(C.c_float)),
-        ("n_angles", C.c_int32),
+        ("n_angles", C.c_int32), ("flow", C.POINTER(C.c_int32)),
     ]
 
 
 class PlanInfo(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "n_slots", "lag", "n_steps", "n_inputs", "n_outputs", "n_angles", "n_nodes", "window",
-        "n_appends", "n_dropped_edges", "min_window")]
+        "n_appends", "n_dropped_edges", "min_window", "can_sample")]
 
 
 class Step(C.Structure):
@@ -101,6 +101,8 @@ def lib() -> C.CDLL:
     L.mbqc_plan_get_rescale.restype = C.c_int
     L.mbqc_sv_forward_c64.argtypes = [vp, vp, i64, i64, vp, i64, vp, vp, i64, i64, u32, vp]
     L.mbqc_sv_forward_c64.restype = C.c_int
+    L.mbqc_sv_sample_c64.argtypes = [vp, vp, i64, i64, vp, i64, vp, vp, i64, i64, C.c_uint64, vp]
+    L.mbqc_sv_sample_c64.restype = C.c_int
     L.mbqc_dm_forward_c64.argtypes = [vp, vp, i64, i64, vp, i64, vp, i64, i64, C.POINTER(Noise), u32, vp]
     L.mbqc_dm_forward_c64.restype = C.c_int
     L.mbqc_sv_adjoint_workspace_bytes.argtypes = [vp, i64]
@@ -119,6 +121,11 @@ def lib() -> C.CDLL:
     L.mbqc_batch_items.restype = i64
     L.mbqc_batch_launch.argtypes = [vp, i32, i32, u32, vp]
     L.mbqc_batch_launch.restype = C.c_int
+    f32 = C.c_float
+    L.mbqc_adam_step_f32.argtypes = [vp, vp, vp, vp, i64, f32, f32, f32, f32, vp, vp]
+    L.mbqc_adam_step_f32.restype = C.c_int
+    L.mbqc_sgd_step_f32.argtypes = [vp, vp, vp, i64, f32, f32, i32, vp]
+    L.mbqc_sgd_step_f32.restype = C.c_int
     L.mbqc_last_error.argtypes = []
     L.mbqc_last_error.restype = C.c_char_p
     L.mbqc_abi_version.argtypes = []
@@ -149,13 +156,15 @@ class NativePlan:
     """Owner of an ``mbqc_plan*`` compiled from flat pattern arrays."""
 
     def __init__(self, n_nodes: int, edges, schedule, n_measure: int, window: int,
-                 input_nodes, output_nodes, angle_index, fixed_angle, n_angles: int):
+                 input_nodes, output_nodes, angle_index, fixed_angle, n_angles: int,
+                 flow=None):
         L = lib()
         self._keep = dict(
             edges=_i32(np.asarray(edges, dtype=np.int32).reshape(-1, 2) if len(edges) else np.zeros((0, 2))),
             schedule=_i32(schedule), inputs=_i32(input_nodes), outputs=_i32(output_nodes),
             angle_index=_i32(angle_index),
             fixed=np.ascontiguousarray(np.asarray(fixed_angle, dtype=np.float32)),
+            flow=None if flow is None else _i32(flow),
         )
         k = self._keep
         p32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
@@ -165,7 +174,8 @@ class NativePlan:
             n_inputs=len(k["inputs"]), input_nodes=p32(k["inputs"]),
             n_outputs=len(k["outputs"]), output_nodes=p32(k["outputs"]),
             angle_index=p32(k["angle_index"]),
-            fixed_angle=k["fixed"].ctypes.data_as(C.POINTER(C.c_float)), n_angles=n_angles)
+            fixed_angle=k["fixed"].ctypes.data_as(C.POINTER(C.c_float)), n_angles=n_angles,
+            flow=None if k["flow"] is None else p32(k["flow"]))
         handle = C.c_void_p()
         check(L.mbqc_plan_create(C.byref(desc), C.byref(handle)))
         self.handle = handle
@@ -175,8 +185,9 @@ class NativePlan:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value and _lib is not None:
-            _lib.mbqc_plan_destroy(h)
+        lib_ = globals().get("_lib")
+        if h is not None and h.value and lib_ is not None:
+            lib_.mbqc_plan_destroy(h)
             self.handle = None
 
     def tables(self):

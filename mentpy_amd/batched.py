@@ -84,8 +84,9 @@ class BatchedForward:
 
     def __del__(self):
         h = getattr(self, "_handle", None)
-        if h is not None and nat._lib is not None:
-            nat._lib.mbqc_batch_destroy(h)
+        lib_ = getattr(nat, "_lib", None) if nat is not None else None
+        if h is not None and lib_ is not None:
+            lib_.mbqc_batch_destroy(h)
             self._handle = None
 
     @property

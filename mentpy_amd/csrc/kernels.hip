@@ -126,21 +126,26 @@ struct Job {
   int64_t S, total;
 };
 
-template <int NS, bool REF, bool FULL, int SUB>
+// Identity modifier of the per-step projection coefficient b = e^{-i theta}.
+struct NoModify {
+  __device__ __forceinline__ float2 operator()(int, float2 b) const { return b; }
+};
+
+template <int NS, bool REF, bool FULL, int SUB, typename Mod>
 __device__ __forceinline__ void sv_sub(float2 (&t)[1 << NS], const float* coef, int k0,
-                                       int n_steps, const float (&th)[NS], float2* ph) {
+                                       int n_steps, const float (&th)[NS], float2* ph, Mod& mod) {
   const int k = k0 + SUB;
   if (FULL || k < n_steps) {
-    const float2 b = step_coeff<REF>(th[SUB], ph);
+    const float2 b = mod(k, step_coeff<REF>(th[SUB], ph));
     sv_step<NS, NS - 1 - SUB>(t, b, coef + (k << NS));
   }
 }
 
-template <int NS, bool REF, bool FULL, int... SUBS>
+template <int NS, bool REF, bool FULL, typename Mod, int... SUBS>
 __device__ __forceinline__ void sv_block(float2 (&t)[1 << NS], const float* coef, int k0,
-                                         int n_steps, const float (&th)[NS], float2* ph,
+                                         int n_steps, const float (&th)[NS], float2* ph, Mod& mod,
                                          std::integer_sequence<int, SUBS...>) {
-  (sv_sub<NS, REF, FULL, SUBS>(t, coef, k0, n_steps, th, ph), ...);
+  (sv_sub<NS, REF, FULL, SUBS>(t, coef, k0, n_steps, th, ph, mod), ...);
 }
 
 // Angles of the NS steps of a block (steps past the end get 0).  The index is
@@ -165,9 +170,9 @@ __device__ __forceinline__ void fetch_block_angles(const mbqc_step* steps, int k
 // Run all steps: full blocks of NS unrolled steps with the next block's
 // angles in flight, then the partial tail block.  `on_block(blk, t)` runs at
 // the start of every block (after its rescale).
-template <int NS, bool REF, typename OnBlock>
+template <int NS, bool REF, typename OnBlock, typename Mod = NoModify>
 __device__ __forceinline__ void sv_run_steps(float2 (&t)[1 << NS], const Job& J, const float* ang,
-                                             float2* ph, OnBlock on_block) {
+                                             float2* ph, OnBlock on_block, Mod&& mod = Mod()) {
   constexpr int H = 1 << NS;
   const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
   const int32_t* resc = J.tab + J.off_svres;
@@ -189,9 +194,9 @@ __device__ __forceinline__ void sv_run_steps(float2 (&t)[1 << NS], const Job& J,
     }
     on_block(blk, t);
     if (blk < nfull)
-      sv_block<NS, REF, true>(t, coef, blk * NS, n, th, ph, std::make_integer_sequence<int, NS>{});
+      sv_block<NS, REF, true>(t, coef, blk * NS, n, th, ph, mod, std::make_integer_sequence<int, NS>{});
     else
-      sv_block<NS, REF, false>(t, coef, blk * NS, n, th, ph, std::make_integer_sequence<int, NS>{});
+      sv_block<NS, REF, false>(t, coef, blk * NS, n, th, ph, mod, std::make_integer_sequence<int, NS>{});
   }
 }
 
@@ -442,6 +447,115 @@ __global__ __launch_bounds__(256) void k_dm_forward(Job J, float gamma, float p_
       }
     }
   }
+}
+
+// ------------------------------------------------------- sampled outcomes
+// 32 pseudo-random bits per (seed, simulation, block of 32 steps): the
+// splitmix64 finaliser (Steele, Lea, Flood, "Fast splittable pseudorandom
+// number generators", OOPSLA 2014) applied to a Weyl-sequence key.
+__device__ __forceinline__ uint32_t draw32(uint64_t seed, uint64_t sim, uint64_t block) {
+  uint64_t z = seed + sim * 0x9E3779B97F4A7C15ull + (block + 1) * 0xD1B54A32D192ED03ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return uint32_t(z ^ (z >> 31));
+}
+
+// Per-lane Pauli frame and outcome draw of the sampled mode.  Before step k,
+// bit 0 of fx / fz is the X / Z frame of schedule[k]: the node is measured at
+// (-1)^x theta, the drawn projection m' (probability 1/2 each) is onto
+// |+/- theta'>, and the physical outcome is m' xor z.  An outcome 1 flips the
+// frames of the flow successor and of its later neighbours (xmask/zmask).
+struct SampleMod {
+  uint64_t fx = 0, fz = 0, seed = 0, sim = 0;
+  const uint64_t* xm;
+  const uint64_t* zm;
+  int32_t* out_words;   // this simulation's outcome words (null for inactive lanes)
+  int n;
+  uint32_t rbits = 0, word = 0;
+  __device__ __forceinline__ float2 operator()(int k, float2 b) {
+    const int kb = k & 31;
+    if (kb == 0) rbits = draw32(seed, sim, uint64_t(k >> 5));
+    const uint32_t mprime = (rbits >> kb) & 1u;
+    const uint32_t xk = uint32_t(fx & 1u), zk = uint32_t(fz & 1u);
+    const uint32_t mk = mprime ^ zk;
+    word |= mk << kb;
+    if (kb == 31 || k == n - 1) {
+      if (out_words) out_words[k >> 5] = int32_t(word);
+      word = 0;
+    }
+    const uint64_t sel = uint64_t(0) - uint64_t(mk);
+    fx = (fx ^ (xm[k] & sel)) >> 1;
+    fz = (fz ^ (zm[k] & sel)) >> 1;
+    float2 bb = xk ? cconj(b) : b;
+    return mprime ? make_float2(-bb.x, -bb.y) : bb;
+  }
+};
+
+template <int NS>
+__global__ __launch_bounds__(256) void k_sv_sample(Job J, const uint64_t* xm, const uint64_t* zm,
+                                                   const int32_t* out_off, int32_t* outcomes,
+                                                   int32_t words, uint64_t seed) {
+  constexpr int H = 1 << NS;
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool active = g < J.total;
+  const int64_t gg = active ? g : 0;
+  const int64_t bi = gg / J.S, si = gg - bi * J.S;
+  const float* ang = J.angles + bi * J.angle_stride;
+  float2 t[H];
+  sv_load_input<NS>(J, J.psi + si * J.in_stride, t);
+  float2 ph = make_float2(1.0f, 0.0f);
+  SampleMod mod;
+  mod.seed = seed;
+  mod.sim = uint64_t(gg);
+  mod.xm = xm;
+  mod.zm = zm;
+  mod.n = J.n_steps;
+  int32_t* row = outcomes + gg * (words + 2);
+  mod.out_words = active ? row : nullptr;
+  sv_run_steps<NS, false>(t, J, ang, &ph, [](int, const float2 (&)[H]) {}, mod);
+  // residual frame of the output qubits, in output-index bit order
+  uint32_t xo = 0, zo = 0;
+  for (int m = 0; m < J.n_out; ++m) {
+    const int off = out_off[m];
+    xo |= uint32_t((mod.fx >> off) & 1u) << (J.n_out - 1 - m);
+    zo |= uint32_t((mod.fz >> off) & 1u) << (J.n_out - 1 - m);
+  }
+  const int32_t* out_pos = J.tab + J.off_out;
+  float n2 = 0.0f;
+#pragma unroll
+  for (int j = 0; j < H; ++j)
+    if (out_pos[j] >= 0) n2 = fmaf(t[j].x, t[j].x, fmaf(t[j].y, t[j].y, n2));
+  const float scale = 1.0f / sqrtf(n2);
+  if (active) {
+    float2* o = J.out + g * (int64_t(1) << J.n_out);
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      const int p = out_pos[j];
+      if (p >= 0) o[p] = cscale(t[j], scale);
+    }
+    row[words] = int32_t(xo);
+    row[words + 1] = int32_t(zo);
+  }
+}
+
+// Apply the output frame Z^z X^x of every simulation in place:
+// new[o] = (-1)^{popcount(o & z)} old[o ^ x]; each index pair is handled once.
+__global__ void k_apply_frame(float2* out, const int32_t* outcomes, int32_t words, int64_t total,
+                              int32_t n_out) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t d = int64_t(1) << n_out;
+  if (e >= total * d) return;
+  const int64_t g = e >> n_out;
+  const int o = int(e & (d - 1));
+  const int32_t* row = outcomes + g * (words + 2);
+  const int x = row[words], z = row[words + 1];
+  const int o2 = o ^ x;
+  if (o2 < o) return;
+  float2* r = out + g * d;
+  const float2 a = r[o], c = r[o2];
+  const uint32_t s1 = czbit(uint32_t(o), uint32_t(z)), s2 = czbit(uint32_t(o2), uint32_t(z));
+  r[o] = flip(c, s1);
+  if (o2 != o) r[o2] = flip(a, s2);
 }
 
 // ------------------------------------------------------- LDS-resident kernels
@@ -791,6 +905,43 @@ __global__ __launch_bounds__(64) void k_sv_adj_backward(Job J, const float2* ckp
   }
 }
 
+// ---------------------------------------------------------------- optimizer
+// Adam update of mentpy's AdamOptimizer.step (mentpy/optimizers/adam.py:54-66)
+// for a batch of parameter vectors, in place on the device:
+//   m = b1 m + (1-b1) g,  v = b2 v + (1-b2) g^2,
+//   x -= lr * (m / (1-b1^t)) / (sqrt(v / (1-b2^t)) + eps).
+// With b1 = b2 = 0 and eps = 0 it is not SGD; SGD (mentpy/optimizers/sgd.py:49-59)
+// has its own kernel: v = momentum v - lr g; x += v (or the Nesterov form).
+__global__ void k_adam(float* __restrict__ x, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, int64_t n, float lr, float b1, float b2, float eps,
+                       const int32_t* __restrict__ step) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float t = float(*step + 1);
+  const float c1 = 1.0f / (1.0f - powf(b1, t));
+  const float c2 = 1.0f / (1.0f - powf(b2, t));
+  const float gi = g[i];
+  const float mi = fmaf(b1, m[i], (1.0f - b1) * gi);
+  const float vi = fmaf(b2, v[i], (1.0f - b2) * gi * gi);
+  m[i] = mi;
+  v[i] = vi;
+  x[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+__global__ void k_count(int32_t* step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
+}
+
+__global__ void k_sgd(float* __restrict__ x, const float* __restrict__ g, float* __restrict__ v,
+                      int64_t n, float lr, float momentum, int nesterov) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float vi = fmaf(momentum, v[i], -lr * gi);
+  v[i] = vi;
+  x[i] += nesterov ? fmaf(momentum, vi, -lr * gi) : vi;
+}
+
 // ---------------------------------------------------------------- batched
 struct BatchItem {
   int32_t job;
@@ -959,6 +1110,23 @@ constexpr int kMaxDmLdsSlots = 6;    // LDS-resident DM kernel
 constexpr int kMaxAdjSlots = 5;
 
 template <int NS>
+hipError_t launch_sample(const Job& J, const int32_t* tab, const TableOffsets& o, int32_t* outcomes,
+                         int32_t words, uint64_t seed, hipStream_t st) {
+  const int threads = 256;
+  const int64_t blocks = (J.total + threads - 1) / threads;
+  const uint64_t* xm = reinterpret_cast<const uint64_t*>(tab + o.xmask);
+  const uint64_t* zm = reinterpret_cast<const uint64_t*>(tab + o.zmask);
+  hipLaunchKernelGGL((k_sv_sample<NS>), dim3(blocks), dim3(threads), 0, st, J, xm, zm,
+                     tab + o.out_offset, outcomes, words, seed);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int64_t elems = J.total << J.n_out;
+  hipLaunchKernelGGL(k_apply_frame, dim3((elems + threads - 1) / threads), dim3(threads), 0, st,
+                     J.out, outcomes, words, J.total, J.n_out);
+  return hipGetLastError();
+}
+
+template <int NS>
 hipError_t launch_sv_lds(const Job& J, uint32_t flags, hipStream_t st) {
   constexpr int T = (1 << (NS - 1)) < 256 ? (1 << (NS - 1)) : 256;
   hipLaunchKernelGGL((k_sv_forward_lds<NS, T>), dim3(J.total), dim3(T), 0, st, J, flags);
@@ -1026,6 +1194,35 @@ int mbqc_sv_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
     case 12: e = launch_sv_lds<12>(J, flags, st); break;
   }
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "sv_forward launch");
+}
+
+int mbqc_sv_sample_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
+                       int64_t angle_stride_a, const void* psi_in, int64_t in_stride, void* out,
+                       int32_t* outcomes, int64_t B, int64_t S, uint64_t seed, void* stream) {
+  if (!plan) return fail(MBQC_ERR_VALUE, "null plan");
+  Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(plan));
+  if (!p->can_sample)
+    return fail(MBQC_ERR_NOT_IMPLEMENTED, "sampled outcomes unavailable: " + p->sample_reason);
+  if (!outcomes) return fail(MBQC_ERR_VALUE, "outcomes pointer is null");
+  Job J;
+  int rc = make_job(p, angles, angle_stride, angle_stride_a, psi_in, in_stride, out, B, S, &J);
+  if (rc != MBQC_OK) return rc;
+  if (J.total == 0) return MBQC_OK;
+  const int32_t* tab = J.tab;
+  TableOffsets o = table_offsets(*p);
+  const int32_t words = (p->n_steps + 31) / 32;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipError_t e = hipSuccess;
+  switch (p->ns) {
+    case 1: e = launch_sample<1>(J, tab, o, outcomes, words, seed, st); break;
+    case 2: e = launch_sample<2>(J, tab, o, outcomes, words, seed, st); break;
+    case 3: e = launch_sample<3>(J, tab, o, outcomes, words, seed, st); break;
+    case 4: e = launch_sample<4>(J, tab, o, outcomes, words, seed, st); break;
+    case 5: e = launch_sample<5>(J, tab, o, outcomes, words, seed, st); break;
+    case 6: e = launch_sample<6>(J, tab, o, outcomes, words, seed, st); break;
+    default: return fail(MBQC_ERR_NOT_IMPLEMENTED, "sampled outcomes need n_slots <= 6");
+  }
+  return e == hipSuccess ? MBQC_OK : hip_fail(e, "sv_sample launch");
 }
 
 int mbqc_dm_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
@@ -1216,6 +1413,32 @@ void mbqc_batch_destroy(mbqc_batch* bt) {
 }
 
 int64_t mbqc_batch_items(const mbqc_batch* bt) { return bt ? int64_t(bt->items.size()) : -1; }
+
+int mbqc_adam_step_f32(float* x, const float* grad, float* m, float* v, int64_t n, float lr, float b1,
+                       float b2, float eps, int32_t* step, void* stream) {
+  if (!x || !grad || !m || !v || !step) return fail(MBQC_ERR_VALUE, "null pointer");
+  if (n < 0) return fail(MBQC_ERR_VALUE, "negative size");
+  if (n == 0) return MBQC_OK;
+  const int threads = 256;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_adam, dim3((n + threads - 1) / threads), dim3(threads), 0, st, x, grad, m, v, n,
+                     lr, b1, b2, eps, step);
+  hipLaunchKernelGGL(k_count, dim3(1), dim3(64), 0, st, step);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? MBQC_OK : hip_fail(e, "adam launch");
+}
+
+int mbqc_sgd_step_f32(float* x, const float* grad, float* v, int64_t n, float lr, float momentum,
+                      int32_t nesterov, void* stream) {
+  if (!x || !grad || !v) return fail(MBQC_ERR_VALUE, "null pointer");
+  if (n < 0) return fail(MBQC_ERR_VALUE, "negative size");
+  if (n == 0) return MBQC_OK;
+  const int threads = 256;
+  hipLaunchKernelGGL(k_sgd, dim3((n + threads - 1) / threads), dim3(threads), 0,
+                     static_cast<hipStream_t>(stream), x, grad, v, n, lr, momentum, nesterov);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? MBQC_OK : hip_fail(e, "sgd launch");
+}
 
 int mbqc_batch_launch(mbqc_batch* bt, int32_t waves, int32_t wg_waves, uint32_t flags, void* stream) {
   if (!bt) return fail(MBQC_ERR_VALUE, "null batch");

@@ -240,6 +240,57 @@ int compile_plan(const mbqc_pattern_desc* d, Plan* p) {
       }
     }
   }
+  // Pauli-frame tables for sampled outcomes (PennyLane backend order,
+  // pennylane_simulator.py:146-153): outcome 1 on schedule[k] flips the X
+  // frame of its flow successor and the Z frame of the successor's neighbours
+  // measured after schedule[k].  Frames are kept relative to the current
+  // step in 64-bit words, so every offset must be below 64.
+  p->can_sample = false;
+  if (!d->flow) {
+    p->sample_reason = "the pattern description has no flow";
+  } else if (p->n_dropped > 0) {
+    p->sample_reason = "the window drops CZ edges";
+  } else if (ns > kMaxCoefSlots) {
+    p->sample_reason = "sampled outcomes support live windows of at most 6 qubits";
+  } else {
+    bool ok = true;
+    p->xmask.assign(nm, 0);
+    p->zmask.assign(nm, 0);
+    for (int k = 0; k < nm && ok; ++k) {
+      const int v = d->schedule[k];
+      const int f = d->flow[v];
+      if (f < 0 || f >= N || pos[f] <= k || !has_edge(v, f)) {
+        ok = false;
+        p->sample_reason = "flow successor of a measured node is invalid";
+        break;
+      }
+      const int dx = pos[f] - k;
+      if (dx >= 64) { ok = false; break; }
+      p->xmask[k] |= uint64_t(1) << dx;
+      for (int nb : nbr[f]) {
+        if (nb == v || pos[nb] <= k) continue;
+        const int dz = pos[nb] - k;
+        if (dz >= 64) { ok = false; break; }
+        p->zmask[k] |= uint64_t(1) << dz;
+      }
+    }
+    if (ok) {
+      p->out_offset.assign(nout, 0);
+      for (int m = 0; m < nout; ++m) {
+        const int off = pos[d->output_nodes[m]] - nm;
+        if (off < 0 || off >= 64) { ok = false; break; }
+        p->out_offset[m] = off;
+      }
+    }
+    if (ok) {
+      p->can_sample = true;
+    } else {
+      if (p->sample_reason.empty()) p->sample_reason = "flow dependencies span more than 64 positions";
+      p->xmask.clear();
+      p->zmask.clear();
+      p->out_offset.clear();
+    }
+  }
   std::vector<int32_t> flags(p->n_blocks());
   rescale_schedule(*p, kSvRescaleAppends, flags.data(), &p->e_sv);
   rescale_schedule(*p, kDmRescaleAppends, flags.data(), &p->e_dm);
@@ -257,7 +308,11 @@ TableOffsets table_offsets(const Plan& p) {
   o.dm_rescale = o.sv_rescale + p.n_blocks();
   o.e_final = o.dm_rescale + p.n_blocks();
   o.coef = o.e_final + 2;
-  o.total = o.coef + int(p.coef.size());
+  o.xmask = o.coef + int(p.coef.size());
+  o.xmask += o.xmask & 1;  // 8-byte alignment of the 64-bit masks
+  o.zmask = o.xmask + 2 * int(p.xmask.size());
+  o.out_offset = o.zmask + 2 * int(p.zmask.size());
+  o.total = o.out_offset + int(p.out_offset.size());
   return o;
 }
 
@@ -274,6 +329,11 @@ std::vector<int32_t> pack_device_tables(const Plan& p) {
   rescale_schedule(p, kSvRescaleAppends, t.data() + o.sv_rescale, t.data() + o.e_final);
   rescale_schedule(p, kDmRescaleAppends, t.data() + o.dm_rescale, t.data() + o.e_final + 1);
   if (!p.coef.empty()) std::memcpy(t.data() + o.coef, p.coef.data(), sizeof(float) * p.coef.size());
+  if (!p.xmask.empty()) {
+    std::memcpy(t.data() + o.xmask, p.xmask.data(), sizeof(uint64_t) * p.xmask.size());
+    std::memcpy(t.data() + o.zmask, p.zmask.data(), sizeof(uint64_t) * p.zmask.size());
+    std::memcpy(t.data() + o.out_offset, p.out_offset.data(), sizeof(int32_t) * p.out_offset.size());
+  }
   return t;
 }
 
@@ -310,6 +370,7 @@ int mbqc_plan_get_info(const mbqc_plan* plan, mbqc_plan_info* info) {
   info->n_appends = p->n_appends;
   info->n_dropped_edges = p->n_dropped;
   info->min_window = p->min_window;
+  info->can_sample = p->can_sample ? 1 : 0;
   return MBQC_OK;
 }
 

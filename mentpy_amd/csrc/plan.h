@@ -29,6 +29,14 @@ struct Plan {
   // for step k and pair i (the i-th slot index with the measured bit clear),
   // new[bit=1] = alpha * u0 + beta * x, stored as coef[(k * 2^ns) + 2i + {0,1}].
   std::vector<float> coef;
+  // Pauli-frame tables of the sampled mode (empty when not available): per step
+  // the 64-bit masks, relative to the step's schedule position, of the nodes
+  // whose X (flow successor) / Z (its later neighbours) frame bit an outcome 1
+  // flips; for each output node its offset from position n_steps.
+  bool can_sample = false;
+  std::string sample_reason;
+  std::vector<uint64_t> xmask, zmask;
+  std::vector<int32_t> out_offset;
 
   // device copies, one per device id
   std::mutex mu;
@@ -40,8 +48,10 @@ struct Plan {
 //   [steps: 4*n_steps][in_index: H][init_sign: H][out_pos: H]
 //   [sv_rescale: n_blocks][dm_rescale: n_blocks][sv_e_final, dm_e_final]
 //   [coef: n_steps * 2^ns floats, only when ns <= kMaxCoefSlots]
+//   [xmask, zmask: n_steps uint64 each, out_offset: n_out, only when sampling is possible]
 struct TableOffsets {
-  int steps, in_index, init_sign, out_pos, sv_rescale, dm_rescale, e_final, coef, total;
+  int steps, in_index, init_sign, out_pos, sv_rescale, dm_rescale, e_final, coef, xmask, zmask,
+      out_offset, total;
 };
 TableOffsets table_offsets(const Plan& p);
 std::vector<int32_t> pack_device_tables(const Plan& p);

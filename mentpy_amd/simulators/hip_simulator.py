@@ -25,7 +25,8 @@ from .. import _native as nat
 from ..operators.ment import Ment
 from .base_simulator import BaseSimulator
 
-__all__ = ["HipSimulatorSV", "HipSimulatorDM", "compile_pattern", "sv_forward", "dm_forward"]
+__all__ = ["HipSimulatorSV", "HipSimulatorDM", "compile_pattern", "sv_forward", "sv_sample",
+           "decode_outcomes", "dm_forward"]
 
 
 def _torch():
@@ -85,11 +86,22 @@ def compile_pattern(mbqcircuit, schedule=None, window_size: int = 1, exclude: st
                       else mbqcircuit.quantum_output_nodes)
     if set(target_out) != set(outputs):
         raise ValueError("output nodes must be the unmeasured nodes of the schedule")
+    flow = None
+    if mbqcircuit.flow is not None:
+        # flow successors of the measured nodes (-1 elsewhere): the byproduct
+        # corrections of sampled outcomes (pennylane_simulator.py:146-153)
+        flow = np.full(len(nodes), -1, np.int32)
+        try:
+            for v in sched_meas:
+                f = mbqcircuit.flow(v)
+                flow[idx[v]] = idx[f] if f in idx else -1
+        except (KeyError, IndexError, TypeError, ValueError):
+            flow = None
     plan = nat.NativePlan(
         n_nodes=len(nodes), edges=edges, schedule=[idx[v] for v in schedule], n_measure=n_meas,
         window=int(window_size), input_nodes=[idx[v] for v in mbqcircuit.input_nodes],
         output_nodes=[idx[v] for v in target_out], angle_index=angle_index, fixed_angle=fixed,
-        n_angles=len(trainable))
+        n_angles=len(trainable), flow=flow)
     return plan, schedule, sched_meas, int(window_size)
 
 
@@ -161,6 +173,43 @@ def sv_forward(plan: nat.NativePlan, angles, states, *, ref_phase: bool = True,
     return (out, phase) if want_phase else out
 
 
+def sv_sample(plan: nat.NativePlan, angles, states, seed: int, *, out=None, outcomes=None):
+    """Sampled-outcome statevector forward with flow corrections.
+
+    Returns ``(out [B, S, 2^|O|], rows [B, S, W + 2])``: the corrected output
+    (equal to the post-selected output up to a global phase) and per
+    simulation the outcome bit words (W = ceil(n_steps / 32), bit k % 32 of
+    word k // 32 = outcome of the k-th measured node) followed by the X and Z
+    frame applied to the output qubits.
+    """
+    torch = _torch()
+    device = states.device
+    _require_gpu(device)
+    info = plan.info
+    B, S = angles.shape[0], states.shape[0]
+    W = (info["n_steps"] + 31) // 32
+    if out is None:
+        out = torch.empty((B, S, 1 << info["n_outputs"]), dtype=torch.complex64, device=device)
+    if outcomes is None:
+        outcomes = torch.empty((B, S, W + 2), dtype=torch.int32, device=device)
+    sb, sa = _angle_strides(angles)
+    with torch.cuda.device(device):
+        nat.check(nat.lib().mbqc_sv_sample_c64(
+            plan.handle, C.c_void_p(angles.data_ptr()), sb, sa,
+            C.c_void_p(states.data_ptr()), states.shape[1], C.c_void_p(out.data_ptr()),
+            C.c_void_p(outcomes.data_ptr()), B, S, C.c_uint64(int(seed) & (2 ** 64 - 1)),
+            _stream_ptr(device)))
+    return out, outcomes
+
+
+def decode_outcomes(rows, n_steps: int):
+    """Outcome words ``[..., W (+2)]`` -> bits ``[..., n_steps]`` (numpy uint8)."""
+    r = rows.cpu().numpy() if hasattr(rows, "cpu") else np.asarray(rows)
+    w = r[..., : (n_steps + 31) // 32].astype(np.int64) & 0xFFFFFFFF
+    k = np.arange(n_steps)
+    return ((w[..., k // 32] >> (k % 32)) & 1).astype(np.uint8)
+
+
 def dm_forward(plan: nat.NativePlan, angles, states, *, p_depol: float = 0.0,
                p_readout: float = 0.0, renormalize: bool = True, out=None):
     """Batched density-matrix forward -> ``[B, S, 2^|O|, 2^|O|]`` complex64."""
@@ -192,13 +241,14 @@ class _HipBackend(BaseSimulator):
         self.window_size = kwargs.pop("window_size", 1)
         self.schedule = kwargs.pop("schedule", None)
         self.force0 = kwargs.pop("force0", True)
+        seed = kwargs.pop("seed", None)
         self.dev_mode = kwargs.pop("dev_mode", False)
         self.wires = kwargs.pop("wires", None)
         self._device_arg = kwargs.pop("device", "cuda")
         self._backend_kwargs(kwargs)
         if kwargs:
             raise TypeError(f"unexpected keyword arguments {sorted(kwargs)}")
-        if not self.force0:
+        if not self.force0 and not self._can_sample:
             raise NotImplementedError("Numpy simulator does not support force0=False.")
         if self.dev_mode:
             raise NotImplementedError("dev_mode scheduling is not supported by the hip backends.")
@@ -209,6 +259,12 @@ class _HipBackend(BaseSimulator):
         self.schedule = schedule
         self.schedule_measure = sched_meas
         self.window_size = window
+        if not self.force0 and not plan.info["can_sample"]:
+            raise NotImplementedError(
+                "force0=False needs a pattern with a causal flow, no dropped CZ edges and a "
+                "live window of at most 6 qubits")
+        self._rng = np.random.default_rng(seed)
+        self.last_outcomes = None
         n_in = len(mbqcircuit.input_nodes)
         if input_state is None:
             input_state = np.ones(1)
@@ -220,6 +276,7 @@ class _HipBackend(BaseSimulator):
 
     # hooks
     _plane_error = ValueError
+    _can_sample = False
 
     def _backend_kwargs(self, kwargs):
         pass
@@ -261,9 +318,15 @@ class _HipBackend(BaseSimulator):
         if self.current_measurement >= len(self.schedule_measure) and self.schedule_measure:
             raise ValueError("No more measurements to be done.")
 
-    def _finish_run(self):
+    def _finish_run(self, bits=None):
         self.current_measurement = len(self.schedule_measure)
-        self.outcomes = {v: 0 for v in self.schedule_measure}
+        if bits is None:
+            self.outcomes = {v: 0 for v in self.schedule_measure}
+        else:
+            self.outcomes = {v: int(b) for v, b in zip(self.schedule_measure, bits)}
+
+    def _next_seed(self) -> int:
+        return int(self._rng.integers(0, 2 ** 63, dtype=np.int64))
 
     def _device_inputs(self, angles):
         torch = _torch()
@@ -276,15 +339,19 @@ class _HipBackend(BaseSimulator):
 
 
 class HipSimulatorSV(_HipBackend):
-    """Statevector backend (post-selected, complex64 on the GPU).
+    """Statevector backend (complex64 on the GPU).
 
     Extra keywords: ``device`` (default ``"cuda"``), ``phase`` (``"reference"``:
     the reference's global phase convention; ``"raw"``: the plain projector
     contraction), ``renormalize`` (default True, as the reference normalizes
-    every step).
+    every step).  ``force0=False`` draws every measurement outcome and applies
+    the flow byproduct corrections (the PennyLane backend's semantics,
+    `pennylane_simulator.py:138-153`); ``seed`` seeds the draws and
+    ``outcomes`` holds the last run's outcomes per node.
     """
 
     _exclude = "output"
+    _can_sample = True
 
     def _backend_kwargs(self, kwargs):
         self.phase = kwargs.pop("phase", "reference")
@@ -306,10 +373,16 @@ class HipSimulatorSV(_HipBackend):
         if form not in ("dm", "densitymatrix", "sv", "statevector"):
             raise ValueError(f"Output form {output_form} is not supported.")
         a, s = self._device_inputs(angles)
-        out = sv_forward(self.plan, a, s, ref_phase=self.phase == "reference",
-                         renormalize=self.renormalize)
+        if self.force0:
+            out = sv_forward(self.plan, a, s, ref_phase=self.phase == "reference",
+                             renormalize=self.renormalize)
+            bits = None
+        else:
+            out, rows = sv_sample(self.plan, a, s, self._next_seed())
+            self.last_outcomes = rows
+            bits = decode_outcomes(rows[0, 0], self.plan.info["n_steps"])
         psi = out[0, 0].cpu().numpy().astype(np.complex128)
-        self._finish_run()
+        self._finish_run(bits)
         self.qstate = psi
         if form in ("dm", "densitymatrix"):
             return np.outer(psi, np.conj(psi))
@@ -318,12 +391,20 @@ class HipSimulatorSV(_HipBackend):
     def run_batch(self, angles, input_states=None, *, phase: Optional[str] = None,
                   renormalize: Optional[bool] = None, want_phase: bool = False):
         """All-pairs batch: ``angles [B, n]`` x ``input_states [S, 2^|I|]`` ->
-        device tensor ``[B, S, 2^|O|]`` (torch complex64, stays in HBM)."""
+        device tensor ``[B, S, 2^|O|]`` (torch complex64, stays in HBM).
+        With ``force0=False`` the outcomes are sampled (``last_outcomes`` holds
+        the ``[B, S, W + 2]`` outcome words and output frame)."""
         dev = self.device
         _require_gpu(dev)
         a = _as_angles(angles, self.plan.info["n_angles"], dev)
         st = self._input_np if input_states is None else input_states
         s = _as_states(st, self.plan.info["n_inputs"], dev)
+        if not self.force0:
+            if want_phase:
+                raise ValueError("want_phase is not available with sampled outcomes")
+            out, rows = sv_sample(self.plan, a, s, self._next_seed())
+            self.last_outcomes = rows
+            return out
         ph = self.phase if phase is None else phase
         rn = self.renormalize if renormalize is None else renormalize
         return sv_forward(self.plan, a, s, ref_phase=ph == "reference", renormalize=rn,

@@ -51,6 +51,54 @@ def brute_sv(circ, angles, input_state, measured=None):
     return np.transpose(full, out_perm).reshape(-1)
 
 
+def brute_sv_sampled(circ, angles, input_state, outcomes):
+    """The PennyLane backend's circuit with given outcomes, literally
+    (`mentpy/simulators/pennylane_simulator.py:138-153`): node v is rotated by
+    RZ(-theta), H, and projected on |m_v>; an outcome 1 applies X to flow(v)
+    and Z to every neighbour of flow(v) that comes after v in the measurement
+    order (output nodes included).  ``outcomes`` maps node -> bit.
+
+    Returns ``(state, probability, frame_x, frame_z)``: the normalised output
+    state in ``output_nodes`` order, the probability of the outcome string, and
+    the X / Z corrections that landed on the output qubits as bit masks
+    (bit n_out-1-m for output m), counted classically from the outcomes.
+    """
+    nodes = list(circ.graph.nodes())
+    full = brute_graph_state(circ, input_state)
+    order = list(circ.measurement_order)
+    measured = [v for v in order if v not in circ.output_nodes]
+    live = list(nodes)
+    prob = 1.0
+    outs = list(circ.output_nodes)
+    fx = fz = 0
+    for v in measured:
+        th = _node_angle(circ, v, angles)
+        m = int(outcomes[v])
+        ax = live.index(v)
+        bra = np.array([1.0, (-1) ** m * np.exp(-1j * th)]) / np.sqrt(2)
+        full = np.tensordot(bra, full, axes=([0], [ax]))
+        live.pop(ax)
+        p = float(np.linalg.norm(full)) ** 2
+        prob *= p
+        full /= np.sqrt(p)
+        if m:
+            f = circ.flow(v)
+            ka = live.index(f)
+            full = np.flip(full, axis=ka).copy()                     # X on flow(v)
+            if f in outs:
+                fx ^= 1 << (len(outs) - 1 - outs.index(f))
+            for nb in circ.graph.neighbors(f):
+                if nb != v and order.index(nb) > order.index(v):
+                    kb = live.index(nb)
+                    idx = [slice(None)] * full.ndim
+                    idx[kb] = 1
+                    full[tuple(idx)] *= -1                          # Z on nb
+                    if nb in outs:
+                        fz ^= 1 << (len(outs) - 1 - outs.index(nb))
+    out_perm = [live.index(v) for v in outs]
+    return np.transpose(full, out_perm).reshape(-1), prob, fx, fz
+
+
 def brute_dm(circ, angles, input_state, p_depol=0.0, p_readout=0.0):
     """Output density matrix (quantum_output_nodes order) with noise."""
     nodes = list(circ.graph.nodes())

@@ -176,3 +176,54 @@ def test_adam_with_adjoint_cost_decreases_loss(rng):
     c1 = cost(x)
     print(f"Adam: cost {c0:.4f} -> {c1:.4f}", flush=True)
     assert c1 < c0
+
+
+def test_device_adam_matches_host_adam(rng):
+    """Device Adam/SGD updates equal mentpy's host formulas (adam.py:54-66, sgd.py:49-59)."""
+    torch = _torch()
+    from mentpy_amd.optimizers import DeviceAdam, DeviceSGD
+    x0 = rng.uniform(0, 6, (4, 9))
+    grads = [rng.normal(size=(4, 9)) for _ in range(5)]
+    x = torch.tensor(x0, dtype=torch.float32, device="cuda")
+    opt = DeviceAdam(x, step_size=0.08)
+    m = np.zeros_like(x0)
+    v = np.zeros_like(x0)
+    xr = x0.copy()
+    for i, g in enumerate(grads):
+        opt.step(torch.tensor(g, dtype=torch.float32, device="cuda"))
+        m = 0.9 * m + 0.1 * g
+        v = 0.999 * v + 0.001 * g ** 2
+        xr = xr - 0.08 * (m / (1 - 0.9 ** (i + 1))) / (np.sqrt(v / (1 - 0.999 ** (i + 1))) + 1e-8)
+    assert np.abs(x.cpu().numpy() - xr).max() < 1e-5
+    y = torch.tensor(x0, dtype=torch.float32, device="cuda")
+    sgd = DeviceSGD(y, step_size=0.1, momentum=0.5, nesterov=True)
+    vv = np.zeros_like(x0)
+    yr = x0.copy()
+    for g in grads:
+        sgd.step(torch.tensor(g, dtype=torch.float32, device="cuda"))
+        vv = 0.5 * vv - 0.1 * g
+        yr = yr + 0.5 * vv - 0.1 * g
+    assert np.abs(y.cpu().numpy() - yr).max() < 1e-5
+
+
+def test_mbqml_trainer_graph_training_reduces_loss(rng):
+    """A whole training iteration (adjoint step + device Adam) replayed as a
+    HIP graph gives the same losses as the eager iteration and trains."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from mentpy_amd.optimizers import MBQMLTrainer
+    g = mp.templates.grid_cluster(2, 6)
+    sim = mp.PatternSimulator(g, backend="hip-sv")
+    X, Y = _dataset(2, 32, rng)
+    p0 = rng.uniform(0, 2 * np.pi, (16, len(g.trainable_nodes)))
+    eager = MBQMLTrainer(sim, X, Y, p0, step_size=0.08)
+    h_eager = eager.run(30, graph=False).cpu().numpy()
+    graphed = MBQMLTrainer(sim, X, Y, p0, step_size=0.08)
+    graphed.step_obj.angles.copy_(torch.tensor(p0, dtype=torch.float32, device="cuda"))
+    first = graphed.run(1, graph=False)
+    graphed.capture()
+    h_graph = torch.cat([first, graphed.run(28)]).cpu().numpy()
+    # the capture warm-up advanced one extra iteration: compare shifted histories
+    assert np.abs(h_graph[0] - h_eager[0]).max() < 1e-6
+    assert np.abs(h_graph[1:] - h_eager[2:]).max() < 1e-4
+    assert (h_eager[-1] < h_eager[0]).mean() > 0.9

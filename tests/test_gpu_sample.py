@@ -1,0 +1,118 @@
+"""Sampled outcomes with flow corrections (force0=False) on the GPU.
+
+Checks: every simulation's output against the oracle's PennyLane-order
+circuit run with the same outcome string (fidelity), the output frame the
+kernel reports against the frame the oracle counts from those outcomes (bit
+exact), the output against the post-selected output (up to a global phase),
+outcome statistics (each bit 1/2), seed reproducibility, and the API errors.
+"""
+import numpy as np
+import pytest
+
+from conftest import haar_state, infidelity
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.mark.parametrize("rows,cols", [(1, 5), (2, 4), (2, 7), (3, 4), (3, 5), (4, 3)])
+def test_sampled_matches_pennylane_order_oracle(rows, cols, rng):
+    torch = _torch()
+    import mentpy_amd as mp
+    from mentpy_amd.simulators.hip_simulator import (decode_outcomes, sv_forward, sv_sample,
+                                                      _as_angles, _as_states)
+    from oracle.brute import brute_sv_sampled
+    g = mp.templates.linear_cluster(cols) if rows == 1 else mp.templates.grid_cluster(rows, cols)
+    sim = mp.PatternSimulator(g, backend="hip-sv", force0=False, seed=7)
+    core = sim.simulator
+    plan = core.plan
+    assert plan.info["can_sample"] == 1
+    B, S = 6, 4
+    th = rng.uniform(0, 2 * np.pi, (B, len(g.trainable_nodes)))
+    X = np.stack([haar_state(rows, rng) for _ in range(S)])
+    a = _as_angles(th, plan.info["n_angles"], torch.device("cuda"))
+    s = _as_states(X, plan.info["n_inputs"], torch.device("cuda"))
+    out, rows_t = sv_sample(plan, a, s, seed=1234)
+    ps = sv_forward(plan, a, s, ref_phase=False)
+    torch.cuda.synchronize()
+    n = plan.info["n_steps"]
+    W = (n + 31) // 32
+    bits = decode_outcomes(rows_t, n)
+    frames = rows_t[..., W:].cpu().numpy()
+    out_np = out.cpu().numpy()
+    assert np.max(infidelity(out_np, ps.cpu().numpy())) < 1e-5
+    meas = core.schedule_measure
+    worst = 0.0
+    for b in range(B):
+        for si in range(S):
+            want, prob, fx, fz = brute_sv_sampled(
+                g, th[b], X[si], {v: int(bits[b, si, k]) for k, v in enumerate(meas)})
+            worst = max(worst, float(infidelity(out_np[b, si], want)))
+            assert (int(frames[b, si, 0]), int(frames[b, si, 1])) == (fx, fz), (b, si)
+    print(f"{rows}x{cols} sampled: max 1-F vs oracle {worst:.2e}, "
+          f"mean outcome {bits.mean():.3f}", flush=True)
+    assert worst < 1e-5
+
+
+def test_sampled_outcome_statistics_and_seeds():
+    """4x20 grid, 8192 simulations x 76 outcomes: every outcome bit has mean
+    1/2 (binomial 6 sigma), the draws depend on the seed only."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from mentpy_amd.simulators.hip_simulator import decode_outcomes, sv_sample, _as_angles, _as_states
+    rng = np.random.default_rng(5)
+    g = mp.templates.grid_cluster(4, 20)
+    sim = mp.PatternSimulator(g, backend="hip-sv", force0=False)
+    plan = sim.simulator.plan
+    dev = torch.device("cuda")
+    th = _as_angles(rng.uniform(0, 2 * np.pi, (2048, 76)), 76, dev)
+    st = _as_states(np.stack([haar_state(4, rng) for _ in range(4)]), 4, dev)
+    out1, r1 = sv_sample(plan, th, st, seed=99)
+    out2, r2 = sv_sample(plan, th, st, seed=99)
+    out3, r3 = sv_sample(plan, th, st, seed=100)
+    torch.cuda.synchronize()
+    assert torch.equal(r1, r2) and torch.equal(out1, out2)
+    b1 = decode_outcomes(r1, plan.info["n_steps"]).reshape(-1, plan.info["n_steps"])
+    b3 = decode_outcomes(r3, plan.info["n_steps"]).reshape(-1, plan.info["n_steps"])
+    assert (b1 != b3).mean() > 0.45
+    n = b1.shape[0]
+    col = b1.mean(0)
+    assert np.abs(col - 0.5).max() < 6 * 0.5 / np.sqrt(n)
+    # neighbouring steps uncorrelated
+    c = np.corrcoef(b1[:, :-1].reshape(-1), b1[:, 1:].reshape(-1))[0, 1]
+    assert abs(c) < 0.02
+    norms = (out1.abs() ** 2).sum(-1)
+    assert (norms - 1).abs().max().item() < 2e-6
+
+
+def test_force0_false_api(rng):
+    torch = _torch()
+    import mentpy_amd as mp
+    g = mp.templates.grid_cluster(2, 5)
+    th = rng.uniform(0, 2 * np.pi, len(g.trainable_nodes))
+    psi = haar_state(2, rng)
+    post = mp.PatternSimulator(g, backend="hip-sv", input_state=psi)
+    samp = mp.PatternSimulator(g, backend="hip-sv", input_state=psi, force0=False, seed=3)
+    rho0 = post(th)
+    seen = set()
+    for _ in range(6):
+        samp.reset()
+        rho = samp(th)
+        assert np.abs(rho - rho0).max() < 2e-5
+        oc = samp.simulator.outcomes
+        assert set(oc) == set(samp.simulator.schedule_measure)
+        seen.add(tuple(oc[v] for v in samp.simulator.schedule_measure))
+    assert len(seen) > 1
+    with pytest.raises(NotImplementedError):
+        mp.PatternSimulator(g, backend="hip-dm", force0=False)
+    # a window that drops CZ edges cannot carry the flow corrections
+    g3 = mp.templates.grid_cluster(3, 4)
+    with pytest.raises(NotImplementedError):
+        mp.PatternSimulator(g3, backend="hip-sv", force0=False, window_size=3,
+                            schedule=list(g3.measurement_order))

@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 import mentpy_amd as mp
-from conftest import haar_state
+from conftest import haar_state, infidelity
 from oracle.brute import brute_dm, brute_sv
 from oracle.closed_forms import chain_output, reference_phase
 from oracle.reference_dm import ReferenceDM
@@ -178,3 +178,97 @@ def test_gradient_rules(rng):
     ex = exact_shift_gradient(cost, th)
     assert np.abs(ex - fd_gradient(cost, th, h=1e-5)).max() < 1e-8
     assert np.abs(psr_gradient(cost, th) - np.sin(1.5) / 1.5 * ex).max() < 1e-12
+
+
+@pytest.mark.parametrize("rows,cols", [(1, 4), (1, 7), (2, 3), (2, 4), (3, 3)])
+def test_sampled_outcomes_are_deterministic_with_flow_corrections(rows, cols, rng):
+    """With the flow corrections of the PennyLane circuit every outcome string
+    has probability 2^-n and leaves the same output (up to a global phase) as
+    the all-zero string; without the corrections the output would depend on
+    the outcomes (checked on the same strings)."""
+    from oracle.brute import brute_sv_sampled
+    g = mp.templates.linear_cluster(cols) if rows == 1 else mp.templates.grid_cluster(rows, cols)
+    th = rng.uniform(0, 2 * np.pi, len(g.trainable_nodes))
+    psi = haar_state(rows, rng)
+    meas = [v for v in g.measurement_order if v not in g.output_nodes]
+    zero = brute_sv(g, th, psi)
+    out0, p0, fx0, fz0 = brute_sv_sampled(g, th, psi, {v: 0 for v in meas})
+    assert np.abs(out0 - zero).max() < 1e-12 and (fx0, fz0) == (0, 0)
+    assert abs(p0 - 0.5 ** len(meas)) < 1e-12
+    n_diff = 0
+    for _ in range(12):
+        bits = {v: int(b) for v, b in zip(meas, rng.integers(0, 2, len(meas)))}
+        out, p, fx, fz = brute_sv_sampled(g, th, psi, bits)
+        assert abs(p - 0.5 ** len(meas)) < 1e-12
+        assert infidelity(out, zero) < 1e-12
+        # the corrections matter: undoing the output frame changes the state
+        # unless the frame is trivial
+        n_out = len(g.output_nodes)
+        raw = out.reshape([2] * n_out).copy()
+        for m in range(n_out):
+            if (fz >> (n_out - 1 - m)) & 1:
+                idx = [slice(None)] * n_out
+                idx[m] = 1
+                raw[tuple(idx)] *= -1
+        for m in range(n_out):
+            if (fx >> (n_out - 1 - m)) & 1:
+                raw = np.flip(raw, axis=m)
+        if fx or fz:
+            n_diff += infidelity(raw.reshape(-1), zero) > 1e-6
+    assert n_diff > 0
+
+
+def test_sampled_outcome_corrections_fail_without_z(rng, monkeypatch):
+    """A planted mistake in the oracle (Z byproducts dropped) breaks the
+    determinism the test above relies on."""
+    import oracle.brute as ob
+    g = mp.templates.grid_cluster(2, 4)
+    th = rng.uniform(0, 2 * np.pi, len(g.trainable_nodes))
+    psi = haar_state(2, rng)
+    meas = [v for v in g.measurement_order if v not in g.output_nodes]
+    zero = brute_sv(g, th, psi)
+
+    class NoNeighbours:
+        def __init__(self, graph):
+            self._g = graph
+        def neighbors(self, v):
+            return []
+        def __getattr__(self, k):
+            return getattr(self._g, k)
+
+    g_bad = mp.templates.grid_cluster(2, 4)
+    worst = 0.0
+    for _ in range(8):
+        bits = {v: int(b) for v, b in zip(meas, rng.integers(0, 2, len(meas)))}
+        full_graph = g_bad.graph
+        monkeypatch.setattr(type(g_bad), "graph", property(lambda self: NoNeighbours(full_graph)),
+                            raising=True)
+        out, _, _, _ = ob.brute_sv_sampled(g_bad, th, psi, bits)
+        monkeypatch.undo()
+        worst = max(worst, infidelity(out, zero))
+    assert worst > 1e-3
+
+
+def test_plan_reports_sampling_capability():
+    from mentpy_amd.simulators.hip_simulator import compile_pattern
+    plan, *_ = compile_pattern(mp.templates.grid_cluster(3, 6))
+    assert plan.info["can_sample"] == 1
+    # a window too small for the schedule drops CZ edges: sampling refused
+    g = mp.templates.grid_cluster(3, 4)
+    plan2, *_ = compile_pattern(g, schedule=list(g.measurement_order), window_size=3)
+    assert plan2.info["n_dropped_edges"] > 0 and plan2.info["can_sample"] == 0
+    # a schedule without a flow (flow unknown to the plan) cannot sample
+    from mentpy_amd import _native as nat
+    p3 = nat.NativePlan(n_nodes=3, edges=[(0, 1), (1, 2)], schedule=[0, 1, 2], n_measure=2,
+                        window=2, input_nodes=[0], output_nodes=[2], angle_index=[0, 1, -1],
+                        fixed_angle=[0, 0, 0], n_angles=2)
+    assert p3.info["can_sample"] == 0
+    p4 = nat.NativePlan(n_nodes=3, edges=[(0, 1), (1, 2)], schedule=[0, 1, 2], n_measure=2,
+                        window=2, input_nodes=[0], output_nodes=[2], angle_index=[0, 1, -1],
+                        fixed_angle=[0, 0, 0], n_angles=2, flow=[1, 2, -1])
+    assert p4.info["can_sample"] == 1
+    # a flow successor that is not a neighbour is rejected
+    p5 = nat.NativePlan(n_nodes=3, edges=[(0, 1), (1, 2)], schedule=[0, 1, 2], n_measure=2,
+                        window=2, input_nodes=[0], output_nodes=[2], angle_index=[0, 1, -1],
+                        fixed_angle=[0, 0, 0], n_angles=2, flow=[2, 2, -1])
+    assert p5.info["can_sample"] == 0

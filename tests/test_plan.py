@@ -144,7 +144,7 @@ def test_validation_errors_map_to_reference_exceptions(native):
     with pytest.raises(TypeError):
         mp.PatternSimulator(g, input_state=[1, 0, 0, 0], backend="hip-sv")
     with pytest.raises(NotImplementedError):
-        mp.PatternSimulator(g, backend="hip-sv", force0=False)
+        mp.PatternSimulator(g, backend="hip-dm", force0=False)
     g2 = mp.templates.grid_cluster(2, 3)
     g2[1] = mp.Ment("XZ")
     with pytest.raises(ValueError, match="only XY plane"):
