@@ -77,12 +77,19 @@ typedef struct mbqc_step {
   float fixed_angle;
 } mbqc_step;
 
-/* Noise of the density-matrix path: single-qubit depolarizing `p_depol` on
- * every qubit after all of its CZs, and a readout bit flip `p_readout` on
- * every measured qubit (post-selected on reported outcome 0). */
+/* Noise of the density-matrix path.  The channels of the PennyLane backend's
+ * circuit_noise (mentpy/simulators/pennylane_simulator.py:123-136) act on
+ * every qubit after all CZs; nonzero ones are composed in this field order.
+ * p_readout flips the reported outcome of every measured qubit (the run is
+ * post-selected on reported outcome 0).  All values lie in [0, 1]. */
 typedef struct mbqc_noise {
-  float p_depol;
-  float p_readout;
+  float p_depol;        /* DepolarizingChannel(p) */
+  float p_readout;      /* classical readout flip */
+  float amp_damping;    /* AmplitudeDamping(gamma) */
+  float phase_damping;  /* PhaseDamping(gamma) */
+  float phase_flip;     /* PhaseFlip(p) */
+  float gad_gamma;      /* GeneralizedAmplitudeDamping(gamma, p) */
+  float gad_p;
 } mbqc_noise;
 
 /* Compile a pattern into per-step tables; replaces the set-up of

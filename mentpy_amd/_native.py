@@ -59,7 +59,9 @@ class Step(C.Structure):
 
 
 class Noise(C.Structure):
-    _fields_ = [("p_depol", C.c_float), ("p_readout", C.c_float)]
+    _fields_ = [("p_depol", C.c_float), ("p_readout", C.c_float), ("amp_damping", C.c_float),
+                ("phase_damping", C.c_float), ("phase_flip", C.c_float), ("gad_gamma", C.c_float),
+                ("gad_p", C.c_float)]
 
 
 class BatchJob(C.Structure):

@@ -348,12 +348,23 @@ __device__ __forceinline__ void dm_step(float2 (&r)[1 << NS][1 << NS], float2 gb
   }
 }
 
-// single-qubit depolarizing on slot M of a density matrix
+// Single-qubit noise of the density-matrix path.  Every supported channel is
+// phase covariant: on a qubit's 2x2 blocks it maps the diagonal pair
+// (r00, r11) -> (d00 r00 + d01 r11, d10 r00 + d11 r11) and scales both
+// off-diagonal blocks by the real factor coh.  A measured qubit only shows
+// the trace (preserved) and coh to its XY projection, so its noise (and the
+// readout flip) folds into gamma = coh (1 - 2 p_readout); output qubits get the
+// full map at the end.
+struct DmNoise {
+  float gamma;
+  float d00, d01, d10, d11, coh;
+  int32_t out_noise;  // 0: the output map is the identity
+};
+
 template <int NS, int M>
-__device__ __forceinline__ void dm_depolarize(float2 (&r)[1 << NS][1 << NS], float p) {
+__device__ __forceinline__ void dm_channel(float2 (&r)[1 << NS][1 << NS], const DmNoise& nz) {
   constexpr int H = 1 << NS;
   constexpr int MB = 1 << M;
-  const float keep = 1.0f - 2.0f * p / 3.0f, mix = 2.0f * p / 3.0f, coh = 1.0f - 4.0f * p / 3.0f;
 #pragma unroll
   for (int j = 0; j < H; ++j) {
     if (j & MB) continue;
@@ -361,10 +372,10 @@ __device__ __forceinline__ void dm_depolarize(float2 (&r)[1 << NS][1 << NS], flo
     for (int k = 0; k < H; ++k) {
       if (k & MB) continue;
       const float2 a = r[j][k], d = r[j | MB][k | MB];
-      r[j][k] = make_float2(keep * a.x + mix * d.x, keep * a.y + mix * d.y);
-      r[j | MB][k | MB] = make_float2(keep * d.x + mix * a.x, keep * d.y + mix * a.y);
-      r[j][k | MB] = cscale(r[j][k | MB], coh);
-      r[j | MB][k] = cscale(r[j | MB][k], coh);
+      r[j][k] = make_float2(fmaf(nz.d00, a.x, nz.d01 * d.x), fmaf(nz.d00, a.y, nz.d01 * d.y));
+      r[j | MB][k | MB] = make_float2(fmaf(nz.d10, a.x, nz.d11 * d.x), fmaf(nz.d10, a.y, nz.d11 * d.y));
+      r[j][k | MB] = cscale(r[j][k | MB], nz.coh);
+      r[j | MB][k] = cscale(r[j | MB][k], nz.coh);
     }
   }
 }
@@ -388,13 +399,13 @@ __device__ __forceinline__ void dm_block(float2 (&r)[1 << NS][1 << NS], const mb
 }
 
 template <int NS, int... MS>
-__device__ __forceinline__ void dm_depolarize_outputs(float2 (&r)[1 << NS][1 << NS], uint32_t live,
-                                                      float p, std::integer_sequence<int, MS...>) {
-  ((live & (1u << MS) ? dm_depolarize<NS, MS>(r, p) : void()), ...);
+__device__ __forceinline__ void dm_output_noise(float2 (&r)[1 << NS][1 << NS], uint32_t live,
+                                                const DmNoise& nz, std::integer_sequence<int, MS...>) {
+  ((live & (1u << MS) ? dm_channel<NS, MS>(r, nz) : void()), ...);
 }
 
 template <int NS>
-__global__ __launch_bounds__(256) void k_dm_forward(Job J, float gamma, float p_out, uint32_t flags) {
+__global__ __launch_bounds__(256) void k_dm_forward(Job J, DmNoise nz, uint32_t flags) {
   constexpr int H = 1 << NS;
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const bool active = g < J.total;
@@ -421,9 +432,10 @@ __global__ __launch_bounds__(256) void k_dm_forward(Job J, float gamma, float p_
     }
     float th[NS];
     load_block_angles<NS>(steps, blk * NS, n, ang, J.angle_stride_a, th);
-    dm_block<NS>(r, steps, blk * NS, n, th, gamma, std::make_integer_sequence<int, NS>{});
+    dm_block<NS>(r, steps, blk * NS, n, th, nz.gamma, std::make_integer_sequence<int, NS>{});
   }
-  dm_depolarize_outputs<NS>(r, uint32_t(J.live_mask), p_out, std::make_integer_sequence<int, NS>{});
+  if (nz.out_noise)
+    dm_output_noise<NS>(r, uint32_t(J.live_mask), nz, std::make_integer_sequence<int, NS>{});
   const int32_t* out_pos = J.tab + J.off_out;
   float scale = ldexpf(1.0f, -J.e_dm);
   if (flags & MBQC_FLAG_RENORMALIZE) {
@@ -646,7 +658,7 @@ __global__ __launch_bounds__(T) void k_sv_forward_lds(Job J, uint32_t flags) {
 }
 
 template <int NS, int T>
-__global__ __launch_bounds__(T) void k_dm_forward_lds(Job J, float gamma, float p_out, uint32_t flags) {
+__global__ __launch_bounds__(T) void k_dm_forward_lds(Job J, DmNoise nz, uint32_t flags) {
   constexpr int H = 1 << NS;
   constexpr int HQ = H / 2;
   __shared__ float2 r[H * H];
@@ -676,7 +688,7 @@ __global__ __launch_bounds__(T) void k_dm_forward_lds(Job J, float gamma, float 
     const float th = step.angle_index >= 0 ? ang[int64_t(step.angle_index) * J.angle_stride_a]
                                            : step.fixed_angle;
     float2 ph = make_float2(1.f, 0.f);
-    const float2 gb = cscale(step_coeff<false>(th, &ph), gamma);
+    const float2 gb = cscale(step_coeff<false>(th, &ph), nz.gamma);
     const float2 gbc = cconj(gb);
     const int q = int((step.flags >> 8) & 0xffu);
     const int QB = 1 << q;
@@ -711,19 +723,19 @@ __global__ __launch_bounds__(T) void k_dm_forward_lds(Job J, float gamma, float 
       }
     }
   }
-  // depolarizing of every output qubit
-  const float keep = 1.0f - 2.0f * p_out / 3.0f, mix = 2.0f * p_out / 3.0f, coh = 1.0f - 4.0f * p_out / 3.0f;
+  // noise of every output qubit
   for (int m = 0; m < NS; ++m) {
-    if (!(uint32_t(J.live_mask) & (1u << m)) || p_out == 0.0f) continue;
+    if (!(uint32_t(J.live_mask) & (1u << m)) || !nz.out_noise) continue;
     const int MB = 1 << m;
     __syncthreads();
     for (int i = tid; i < HQ * HQ; i += T) {
       const int j = insert_zero(i / HQ, m), kk = insert_zero(i % HQ, m);
       const float2 a = r[j * H + kk], d = r[(j | MB) * H + (kk | MB)];
-      r[j * H + kk] = make_float2(keep * a.x + mix * d.x, keep * a.y + mix * d.y);
-      r[(j | MB) * H + (kk | MB)] = make_float2(keep * d.x + mix * a.x, keep * d.y + mix * a.y);
-      r[j * H + (kk | MB)] = cscale(r[j * H + (kk | MB)], coh);
-      r[(j | MB) * H + kk] = cscale(r[(j | MB) * H + kk], coh);
+      r[j * H + kk] = make_float2(fmaf(nz.d00, a.x, nz.d01 * d.x), fmaf(nz.d00, a.y, nz.d01 * d.y));
+      r[(j | MB) * H + (kk | MB)] =
+          make_float2(fmaf(nz.d10, a.x, nz.d11 * d.x), fmaf(nz.d10, a.y, nz.d11 * d.y));
+      r[j * H + (kk | MB)] = cscale(r[j * H + (kk | MB)], nz.coh);
+      r[(j | MB) * H + kk] = cscale(r[(j | MB) * H + kk], nz.coh);
     }
   }
   __syncthreads();
@@ -1078,11 +1090,49 @@ hipError_t launch_sv(const Job& J, uint32_t flags, hipStream_t st) {
   return hipGetLastError();
 }
 
+// Phase-covariant single-qubit channel as (coh, a, c): off-diagonal blocks
+// times coh, r00 - r11 -> a (r00 - r11) + c (r00 + r11), trace kept.
+struct Covariant {
+  double coh = 1.0, a = 1.0, c = 0.0;
+};
+// apply `second` after `first`
+Covariant then(const Covariant& first, const Covariant& second) {
+  return {first.coh * second.coh, first.a * second.a, second.a * first.c + second.c};
+}
+
+// Compose the channels of mbqc_noise in the order depolarizing, amplitude
+// damping, phase damping, phase flip, generalized amplitude damping, each with
+// its Kraus operators' effect on (coherence, population difference):
+//   DepolarizingChannel(p): coh = a = 1 - 4p/3
+//   AmplitudeDamping(g):    coh = sqrt(1-g), a = 1-g, c = g
+//   PhaseDamping(g):        coh = sqrt(1-g)
+//   PhaseFlip(p):           coh = 1 - 2p
+//   GeneralizedAmplitudeDamping(g, p): coh = sqrt(1-g), a = 1-g, c = g (2p-1)
+DmNoise compose_noise(const mbqc_noise& n) {
+  Covariant ch;
+  ch = then(ch, {1.0 - 4.0 * n.p_depol / 3.0, 1.0 - 4.0 * n.p_depol / 3.0, 0.0});
+  ch = then(ch, {std::sqrt(1.0 - n.amp_damping), 1.0 - n.amp_damping, double(n.amp_damping)});
+  ch = then(ch, {std::sqrt(1.0 - n.phase_damping), 1.0, 0.0});
+  ch = then(ch, {1.0 - 2.0 * n.phase_flip, 1.0, 0.0});
+  ch = then(ch, {std::sqrt(1.0 - n.gad_gamma), 1.0 - n.gad_gamma,
+                 double(n.gad_gamma) * (2.0 * n.gad_p - 1.0)});
+  DmNoise nz;
+  nz.gamma = float(ch.coh * (1.0 - 2.0 * n.p_readout));
+  // r00' = (P' + M')/2, r11' = (P' - M')/2 with P' = P, M' = a M + c P
+  nz.d00 = float((1.0 + ch.c + ch.a) / 2.0);
+  nz.d01 = float((1.0 + ch.c - ch.a) / 2.0);
+  nz.d10 = float((1.0 - ch.c - ch.a) / 2.0);
+  nz.d11 = float((1.0 - ch.c + ch.a) / 2.0);
+  nz.coh = float(ch.coh);
+  nz.out_noise = (ch.coh != 1.0 || ch.a != 1.0 || ch.c != 0.0) ? 1 : 0;
+  return nz;
+}
+
 template <int NS>
-hipError_t launch_dm(const Job& J, float gamma, float p_out, uint32_t flags, hipStream_t st) {
+hipError_t launch_dm(const Job& J, const DmNoise& nz, uint32_t flags, hipStream_t st) {
   const int threads = 256;
   const int64_t blocks = (J.total + threads - 1) / threads;
-  hipLaunchKernelGGL((k_dm_forward<NS>), dim3(blocks), dim3(threads), 0, st, J, gamma, p_out, flags);
+  hipLaunchKernelGGL((k_dm_forward<NS>), dim3(blocks), dim3(threads), 0, st, J, nz, flags);
   return hipGetLastError();
 }
 
@@ -1134,10 +1184,10 @@ hipError_t launch_sv_lds(const Job& J, uint32_t flags, hipStream_t st) {
 }
 
 template <int NS>
-hipError_t launch_dm_lds(const Job& J, float gamma, float p_out, uint32_t flags, hipStream_t st) {
+hipError_t launch_dm_lds(const Job& J, const DmNoise& nz, uint32_t flags, hipStream_t st) {
   constexpr int Q = 1 << (2 * (NS - 1));
   constexpr int T = Q < 256 ? Q : 256;
-  hipLaunchKernelGGL((k_dm_forward_lds<NS, T>), dim3(J.total), dim3(T), 0, st, J, gamma, p_out, flags);
+  hipLaunchKernelGGL((k_dm_forward_lds<NS, T>), dim3(J.total), dim3(T), 0, st, J, nz, flags);
   return hipGetLastError();
 }
 
@@ -1233,14 +1283,17 @@ int mbqc_dm_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
   if (p->ns > kMaxDmLdsSlots)
     return fail(MBQC_ERR_NOT_IMPLEMENTED, "density-matrix kernels support live windows of at most " +
                                               std::to_string(kMaxDmLdsSlots) + " qubits");
-  float pd = 0.f, pr = 0.f;
-  if (noise) {
-    pd = noise->p_depol;
-    pr = noise->p_readout;
+  DmNoise nz;
+  {
+    const mbqc_noise zero{};
+    const mbqc_noise& n = noise ? *noise : zero;
+    const float vals[7] = {n.p_depol, n.p_readout, n.amp_damping, n.phase_damping,
+                           n.phase_flip, n.gad_gamma, n.gad_p};
+    for (float v : vals)
+      if (!(v >= 0.f && v <= 1.f))
+        return fail(MBQC_ERR_VALUE, "noise parameters must lie in [0, 1]");
+    nz = compose_noise(n);
   }
-  if (!(pd >= 0.f && pd <= 1.f) || !(pr >= 0.f && pr <= 1.f))
-    return fail(MBQC_ERR_VALUE, "noise probabilities must lie in [0, 1]");
-  const float gamma = (1.0f - 4.0f * pd / 3.0f) * (1.0f - 2.0f * pr);
   Job J;
   int rc = make_job(p, angles, angle_stride, angle_stride_a, psi_in, in_stride, out, B, S, &J);
   if (rc != MBQC_OK) return rc;
@@ -1248,12 +1301,12 @@ int mbqc_dm_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
   hipStream_t st = static_cast<hipStream_t>(stream);
   hipError_t e = hipSuccess;
   switch (p->ns) {
-    case 1: e = launch_dm<1>(J, gamma, pd, flags, st); break;
-    case 2: e = launch_dm<2>(J, gamma, pd, flags, st); break;
-    case 3: e = launch_dm<3>(J, gamma, pd, flags, st); break;
-    case 4: e = launch_dm_lds<4>(J, gamma, pd, flags, st); break;
-    case 5: e = launch_dm_lds<5>(J, gamma, pd, flags, st); break;
-    case 6: e = launch_dm_lds<6>(J, gamma, pd, flags, st); break;
+    case 1: e = launch_dm<1>(J, nz, flags, st); break;
+    case 2: e = launch_dm<2>(J, nz, flags, st); break;
+    case 3: e = launch_dm<3>(J, nz, flags, st); break;
+    case 4: e = launch_dm_lds<4>(J, nz, flags, st); break;
+    case 5: e = launch_dm_lds<5>(J, nz, flags, st); break;
+    case 6: e = launch_dm_lds<6>(J, nz, flags, st); break;
   }
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "dm_forward launch");
 }

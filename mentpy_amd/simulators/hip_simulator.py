@@ -211,8 +211,9 @@ def decode_outcomes(rows, n_steps: int):
 
 
 def dm_forward(plan: nat.NativePlan, angles, states, *, p_depol: float = 0.0,
-               p_readout: float = 0.0, renormalize: bool = True, out=None):
-    """Batched density-matrix forward -> ``[B, S, 2^|O|, 2^|O|]`` complex64."""
+               p_readout: float = 0.0, noise=None, renormalize: bool = True, out=None):
+    """Batched density-matrix forward -> ``[B, S, 2^|O|, 2^|O|]`` complex64.
+    ``noise`` (a ``NoiseModel``) replaces ``p_depol`` / ``p_readout``."""
     torch = _torch()
     device = states.device
     _require_gpu(device)
@@ -221,7 +222,10 @@ def dm_forward(plan: nat.NativePlan, angles, states, *, p_depol: float = 0.0,
     d = 1 << info["n_outputs"]
     if out is None:
         out = torch.empty((B, S, d, d), dtype=torch.complex64, device=device)
-    noise = nat.Noise(p_depol=float(p_depol), p_readout=float(p_readout))
+    if noise is None:
+        noise = nat.Noise(p_depol=float(p_depol), p_readout=float(p_readout))
+    elif not isinstance(noise, nat.Noise):
+        noise = noise.native()
     flags = nat.FLAG_RENORMALIZE if renormalize else 0
     sb, sa = _angle_strides(angles)
     with torch.cuda.device(device):
@@ -414,9 +418,11 @@ class HipSimulatorSV(_HipBackend):
 class HipSimulatorDM(_HipBackend):
     """Density-matrix backend with optional noise.
 
-    Extra keywords: ``device``, ``noise`` (a ``mentpy_amd.noise.NoiseModel``)
-    or ``p_depol`` / ``p_readout`` floats, ``renormalize`` (default True: the
-    reference divides by the outcome probability each step).
+    Extra keywords: ``device``, ``noise`` (a ``mentpy_amd.noise.NoiseModel``),
+    or ``p_depol`` / ``p_readout`` floats, or the PennyLane backend's
+    ``circuit_noise`` name with ``p`` / ``gamma`` (pennylane_simulator.py:40-46,
+    123-136); ``renormalize`` (default True: the reference divides by the
+    outcome probability each step).
     """
 
     _exclude = "quantum_output"
@@ -427,9 +433,19 @@ class HipSimulatorDM(_HipBackend):
         noise = kwargs.pop("noise", None)
         pd = kwargs.pop("p_depol", None)
         pr = kwargs.pop("p_readout", None)
+        cn = kwargs.pop("circuit_noise", None)
+        cp = kwargs.pop("p", None)
+        cg = kwargs.pop("gamma", None)
+        given = sum(x is not None for x in (noise, cn)) + (pd is not None)
+        if given > 1:
+            raise ValueError("pass one of noise=, p_depol= or circuit_noise=")
         if noise is None:
-            noise = NoiseModel(depolarizing=pd or 0.0, readout_flip=pr or 0.0)
-        elif pd is not None or pr is not None:
+            if cn is not None:
+                noise = NoiseModel.from_circuit_noise(cn, p=cp or 0.0, gamma=cg or 0.0,
+                                                      readout_flip=pr or 0.0)
+            else:
+                noise = NoiseModel(depolarizing=pd or 0.0, readout_flip=pr or 0.0)
+        elif pr is not None:
             raise ValueError("pass either noise= or p_depol/p_readout, not both")
         self.noise = noise
         self.renormalize = kwargs.pop("renormalize", True)
@@ -437,8 +453,7 @@ class HipSimulatorDM(_HipBackend):
     def run(self, angles, mode: str = "sample", input_state=None):
         self._check_run(angles, input_state)
         a, s = self._device_inputs(angles)
-        rho = dm_forward(self.plan, a, s, p_depol=self.noise.depolarizing,
-                         p_readout=self.noise.readout_flip, renormalize=self.renormalize)
+        rho = dm_forward(self.plan, a, s, noise=self.noise, renormalize=self.renormalize)
         out = rho[0, 0].cpu().numpy().astype(np.complex128)
         self._finish_run()
         self.qstate = out
@@ -452,5 +467,4 @@ class HipSimulatorDM(_HipBackend):
         st = self._input_np if input_states is None else input_states
         s = _as_states(st, self.plan.info["n_inputs"], dev)
         rn = self.renormalize if renormalize is None else renormalize
-        return dm_forward(self.plan, a, s, p_depol=self.noise.depolarizing,
-                          p_readout=self.noise.readout_flip, renormalize=rn)
+        return dm_forward(self.plan, a, s, noise=self.noise, renormalize=rn)

@@ -99,14 +99,18 @@ def brute_sv_sampled(circ, angles, input_state, outcomes):
     return np.transpose(full, out_perm).reshape(-1), prob, fx, fz
 
 
-def brute_dm(circ, angles, input_state, p_depol=0.0, p_readout=0.0):
-    """Output density matrix (quantum_output_nodes order) with noise."""
+def brute_dm(circ, angles, input_state, p_depol=0.0, p_readout=0.0, kraus=None):
+    """Output density matrix (quantum_output_nodes order) with noise: the
+    depolarizing channel ``p_depol`` and then the Kraus channel ``kraus`` (a list of
+    2x2 operators) on every qubit after all CZs, readout flips at the measurements."""
     nodes = list(circ.graph.nodes())
     N = len(nodes)
     psi = brute_graph_state(circ, input_state)
     rho = np.multiply.outer(psi, psi.conj())          # axes: N row, N col
     for k in range(N):
         rho = _depol(rho, k, N, p_depol)
+        if kraus is not None:
+            rho = _kraus(rho, k, N, kraus)
     measured = [v for v in circ.measurement_order if v not in circ.quantum_output_nodes]
     live = list(nodes)
     for v in measured:
@@ -146,6 +150,15 @@ def brute_graph_state(circ, input_state):
         idx[axis[b]] = 1
         full[tuple(idx)] *= -1
     return full
+
+
+def _kraus(rho, k, N, kraus):
+    out = np.zeros_like(rho)
+    for K in kraus:
+        t = np.moveaxis(np.tensordot(K, rho, axes=([1], [k])), 0, k)
+        t = np.moveaxis(np.tensordot(t, K.conj(), axes=([N + k], [1])), -1, N + k)
+        out = out + t
+    return out
 
 
 def _depol(rho, k, N, p):

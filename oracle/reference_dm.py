@@ -20,6 +20,9 @@ the end for output qubits.  A readout flip ``p_m`` makes the post-selected
 update ``((1-p_m) P0 rho P0 + p_m P1 rho P1) / ((1-p_m) prob0 + p_m prob1)``.
 Dense Kraus/projector operators are kron-built as in the reference.
 
+``kraus`` (a list of 2x2 Kraus operators, ``oracle.channels``) adds any other
+single-qubit channel at the same place, after the depolarizing one.
+
 Pinned by ``tests/test_oracle.py``: p = 0 equals the SV oracle's density
 matrix; with noise it equals the brute-force Kraus simulation in gate order
 (``oracle.brute.brute_dm``) on small grids; teleportation known-answer test.
@@ -43,13 +46,25 @@ def depolarize(rho, i, n, p):
     return out
 
 
+def apply_channel(rho, i, n, kraus):
+    """sum_K K_i rho K_i^dag on qubit i of n (dense kron-embedded operators)."""
+    if kraus is None:
+        return rho
+    out = np.zeros_like(rho)
+    for K in kraus:
+        Ke = embed(K, i, n)
+        out = out + Ke @ rho @ Ke.conj().T
+    return out
+
+
 class ReferenceDM:
     """Faithful complex128 restatement of ``NumpySimulatorDM`` (force0 only)
     with optional depolarizing / readout-flip noise."""
 
     def __init__(self, circ, input_state=None, window_size=1, schedule=None,
-                 p_depol=0.0, p_readout=0.0):
+                 p_depol=0.0, p_readout=0.0, kraus=None):
         self.circ = circ
+        self.kraus = kraus
         self.window_size = window_size
         self.schedule = schedule
         self.p_depol = float(p_depol)
@@ -104,6 +119,7 @@ class ReferenceDM:
     def measure_ment(self, plane, stored, angle, i):
         n = self.current_number_simulated_nodes()
         rho = depolarize(self.qstate, i, n, self.p_depol)
+        rho = apply_channel(rho, i, n, self.kraus)
         p0, p1 = povm(plane, stored, angle)
         p0e, p1e = embed(p0, i, n), embed(p1, i, n)
         prob0 = np.real(np.trace(rho @ p0e))
@@ -147,6 +163,7 @@ class ReferenceDM:
         n = len(current)
         for i in range(n):
             self.qstate = depolarize(self.qstate, i, n, self.p_depol)
+            self.qstate = apply_channel(self.qstate, i, n, self.kraus)
         qout = list(self.circ.quantum_output_nodes)
         if qout != current:
             self.qstate = reorder_qubits(self.qstate, current, qout)

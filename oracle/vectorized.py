@@ -112,8 +112,21 @@ def sv_batch(circ, angles, states, window_size=1, schedule=None, phase="referenc
     return psi
 
 
-def dm_batch(circ, angles, states, window_size=1, schedule=None, p_depol=0.0, p_readout=0.0):
-    """``angles [B, n]``, pure ``states [S, 2^|I|]`` -> ``[B, S, d, d]`` with noise."""
+def _kraus_axis(x, kraus, a, b):
+    """sum_K K x K^dag on the (row axis a, column axis b) pair of ``x``."""
+    out = np.zeros_like(x)
+    for K in kraus:
+        t = np.moveaxis(np.tensordot(K, x, axes=([1], [a])), 0, a)
+        t = np.moveaxis(np.tensordot(t, K.conj(), axes=([b], [1])), -1, b)
+        out = out + t
+    return out
+
+
+def dm_batch(circ, angles, states, window_size=1, schedule=None, p_depol=0.0, p_readout=0.0,
+             kraus=None):
+    """``angles [B, n]``, pure ``states [S, 2^|I|]`` -> ``[B, S, d, d]`` with noise
+    (depolarizing ``p_depol``, then the Kraus channel ``kraus`` if given, on
+    every qubit after its CZs; readout flip ``p_readout``)."""
     su = _Setup(circ, window_size, schedule, "quantum_output")
     angles = np.atleast_2d(np.asarray(angles, dtype=np.float64))
     states = np.atleast_2d(np.asarray(states, dtype=complex))
@@ -137,6 +150,10 @@ def dm_batch(circ, angles, states, window_size=1, schedule=None, p_depol=0.0, p_
             # depolarize qubit 0: populations mix by 2p/3, coherences shrink by 1-4p/3
             B00, B11 = (1 - 2 * p / 3) * B00 + (2 * p / 3) * B11, (1 - 2 * p / 3) * B11 + (2 * p / 3) * B00
             B01, B10 = (1 - 4 * p / 3) * B01, (1 - 4 * p / 3) * B10
+        if kraus is not None:
+            blk = np.stack([np.stack([B00, B01], 1), np.stack([B10, B11], 1)], 1)  # [N,2,2,h,h]
+            blk = _kraus_axis(blk, kraus, 1, 2)
+            B00, B01, B10, B11 = blk[:, 0, 0], blk[:, 0, 1], blk[:, 1, 0], blk[:, 1, 1]
         e = np.exp(1j * th)[:, None, None]
         plus_part = 0.5 * (B00 + B11 + e * B01 + np.conj(e) * B10)
         minus_part = 0.5 * (B00 + B11 - e * B01 - np.conj(e) * B10)
@@ -160,6 +177,9 @@ def dm_batch(circ, angles, states, window_size=1, schedule=None, p_depol=0.0, p_
             dmat[:, 0, 1] *= (1 - 4 * p / 3)
             dmat[:, 1, 0] *= (1 - 4 * p / 3)
             dmat = np.moveaxis(dmat, (1, 2), (1 + q, 1 + n + q))
+    if kraus is not None:
+        for q in range(n):
+            dmat = _kraus_axis(dmat, kraus, 1 + q, 1 + n + q)
     tgt = list(circ.quantum_output_nodes)
     perm = [cur.index(v) for v in tgt]
     dmat = np.transpose(dmat, [0] + [1 + q for q in perm] + [1 + n + q for q in perm])

@@ -272,3 +272,53 @@ def test_plan_reports_sampling_capability():
                         window=2, input_nodes=[0], output_nodes=[2], angle_index=[0, 1, -1],
                         fixed_angle=[0, 0, 0], n_angles=2, flow=[2, 2, -1])
     assert p5.info["can_sample"] == 0
+
+
+def test_noise_channels_known_answers(rng):
+    """Kraus sets of the PennyLane channel names: completeness, fixed points and
+    closed-form actions (Nielsen & Chuang 8.3)."""
+    from oracle import channels as ch
+    one = np.array([[0, 0], [0, 1]], dtype=complex)
+    plus = np.full((2, 2), 0.5, dtype=complex)
+    for ks in (ch.depolarizing(0.3), ch.amplitude_damping(0.4), ch.phase_damping(0.25),
+               ch.phase_flip(0.1), ch.generalized_amplitude_damping(0.35, 0.2),
+               ch.noise_kraus(0.1, 0.2, 0.3, 0.05, 0.15, 0.7)):
+        assert np.abs(sum(K.conj().T @ K for K in ks) - np.eye(2)).max() < 1e-14
+    assert np.allclose(ch.apply_1q(ch.amplitude_damping(0.4), one), np.diag([0.4, 0.6]))
+    assert np.allclose(ch.apply_1q(ch.generalized_amplitude_damping(0.3, 0.2), np.diag([0.2, 0.8])),
+                       np.diag([0.2, 0.8]))
+    assert np.allclose(ch.apply_1q(ch.generalized_amplitude_damping(0.3, 1.0), one),
+                       ch.apply_1q(ch.amplitude_damping(0.3), one))
+    assert np.allclose(ch.apply_1q(ch.phase_damping(0.36), plus), [[0.5, 0.4], [0.4, 0.5]])
+    assert np.allclose(ch.apply_1q(ch.phase_flip(0.1), plus), [[0.5, 0.4], [0.4, 0.5]])
+    r = rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2))
+    r = r @ r.conj().T
+    r /= np.trace(r)
+    dep = ch.apply_1q(ch.depolarizing(0.3), r)
+    assert np.allclose(dep, 0.6 * r + 0.4 * np.eye(2) / 2)
+    # composition order: the first listed channel acts first
+    both = ch.apply_1q(ch.noise_kraus(amp_damping=0.3, phase_flip_=0.2), r)
+    seq = ch.apply_1q(ch.phase_flip(0.2), ch.apply_1q(ch.amplitude_damping(0.3), r))
+    assert np.allclose(both, seq)
+
+
+@pytest.mark.parametrize("rows,cols", [(2, 3), (2, 4), (3, 3)])
+@pytest.mark.parametrize("channel", ["amp", "phase_damp", "gad", "mixed"])
+def test_noise_channels_three_oracles_agree(rows, cols, channel, rng):
+    """The windowed DM restatement, its vectorised form and the brute-force
+    gate-order simulation agree for every channel (readout flip included)."""
+    from oracle import channels as ch
+    ks = {"amp": ch.amplitude_damping(0.2), "phase_damp": ch.phase_damping(0.3),
+          "gad": ch.generalized_amplitude_damping(0.25, 0.3),
+          "mixed": ch.noise_kraus(0.05, 0.1, 0.0, 0.07, 0.1, 0.6)}[channel]
+    g = mp.templates.grid_cluster(rows, cols)
+    th = rng.uniform(0, 2 * np.pi, len(g.trainable_nodes))
+    psi = haar_state(rows, rng)
+    want = brute_dm(g, th, psi, p_readout=0.02, kraus=ks)
+    got = ReferenceDM(g, psi, p_readout=0.02, kraus=ks).run(th)
+    vec = dm_batch(g, th[None], psi[None], p_readout=0.02, kraus=ks)[0, 0]
+    assert np.abs(got - want).max() < 1e-12
+    assert np.abs(vec - want).max() < 1e-12
+    # damping is not unital: the output differs from the depolarizing-only one
+    if channel in ("amp", "gad"):
+        assert np.abs(want - brute_dm(g, th, psi, p_readout=0.02)).max() > 1e-3

@@ -168,3 +168,22 @@ def test_kernel_limits_are_reported(native):
     assert rc == native.MBQC_ERR_NOT_IMPLEMENTED
     with pytest.raises(NotImplementedError):
         compile_pattern(mp.templates.grid_cluster(13, 14))
+
+
+def test_circuit_noise_names_map_to_noise_model():
+    """The PennyLane backend's circuit_noise names (pennylane_simulator.py:123-136)."""
+    nm = mp.NoiseModel.from_circuit_noise("generalized_amplitude_damping", p=0.3, gamma=0.2)
+    assert (nm.gad_gamma, nm.gad_p) == (0.2, 0.3)
+    assert mp.NoiseModel.from_circuit_noise("phase_flip", p=0.1).phase_flip == 0.1
+    assert mp.NoiseModel.from_circuit_noise("amplitude_damping", gamma=0.4).amplitude_damping == 0.4
+    with pytest.raises(ValueError, match="Unrecognized circuit noise"):
+        mp.NoiseModel.from_circuit_noise("bit_flip", p=0.1)
+    with pytest.raises(ValueError):
+        mp.NoiseModel(amplitude_damping=1.5)
+    g = mp.templates.grid_cluster(2, 3)
+    with pytest.raises(ValueError, match="Unrecognized circuit noise"):
+        mp.PatternSimulator(g, backend="hip-dm", circuit_noise="bit_flip", p=0.1)
+    with pytest.raises(ValueError):
+        mp.PatternSimulator(g, backend="hip-dm", circuit_noise="phase_flip", p=0.1, p_depol=0.1)
+    sim = mp.PatternSimulator(g, backend="hip-dm", circuit_noise="phase_damping", gamma=0.3)
+    assert sim.simulator.noise.phase_damping == 0.3
