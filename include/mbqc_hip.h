@@ -92,10 +92,26 @@ typedef struct mbqc_noise {
   float gad_p;
 } mbqc_noise;
 
+/* Causal flow of an open graph (nodes 0..n_nodes-1): replaces find_cflow /
+ * causal_flow_aux (mentpy/mbqc/flow.py:115-185) and the measurement order of
+ * MBQCircuit.calculate_order (mentpy/mbqc/mbqcircuit.py:390-422).  Writes
+ * flow[v] (flow successor, -1 for outputs), layer[v] (0 for outputs) and
+ * order[0..n_nodes) (inputs first, then decreasing layer, ties in node order);
+ * *found = 0 when the graph has no causal flow.  |I| != |O| is
+ * MBQC_ERR_VALUE with the reference's message.  Any output pointer may be
+ * NULL.  Host only. */
+int mbqc_causal_flow(int32_t n_nodes, int32_t n_edges, const int32_t* edges, int32_t n_inputs,
+                     const int32_t* input_nodes, int32_t n_outputs, const int32_t* output_nodes,
+                     int32_t* flow, int32_t* layer, int32_t* order, int32_t* found);
+
 /* Compile a pattern into per-step tables; replaces the set-up of
  * NumpySimulatorSV.__init__ / NumpySimulatorDM.__init__
  * (mentpy/simulators/np_simulator_sv.py:38-128, np_simulator_dm.py:33-115):
- * window, input reordering, |+> padding, initial CZ block.  Host only. */
+ * window, input reordering, |+> padding, initial CZ block.  Host only.
+ * With schedule == NULL the schedule is the causal-flow order of
+ * mbqc_causal_flow with the output nodes (the unmeasured ones) last,
+ * n_measure = n_nodes - n_outputs, a window <= 1 becomes n_inputs + 1 and a
+ * NULL flow is the causal flow (a pattern without one is MBQC_ERR_VALUE). */
 int mbqc_plan_create(const mbqc_pattern_desc* desc, mbqc_plan** out);
 void mbqc_plan_destroy(mbqc_plan* plan);
 int mbqc_plan_get_info(const mbqc_plan* plan, mbqc_plan_info* info);

@@ -27,7 +27,7 @@ FLAG_RENORMALIZE = 2
 FLAG_WRITE_PHASE = 4
 
 EXPORTED_SYMBOLS = (
-    "mbqc_plan_create", "mbqc_plan_destroy", "mbqc_plan_get_info", "mbqc_plan_get_tables",
+    "mbqc_causal_flow", "mbqc_plan_create", "mbqc_plan_destroy", "mbqc_plan_get_info", "mbqc_plan_get_tables",
     "mbqc_plan_get_rescale", "mbqc_sv_forward_c64", "mbqc_sv_sample_c64", "mbqc_dm_forward_c64",
     "mbqc_sv_adjoint_workspace_bytes", "mbqc_sv_adjoint_forward_c64",
     "mbqc_sv_adjoint_partials", "mbqc_sv_adjoint_backward_c64", "mbqc_batch_create",
@@ -103,6 +103,8 @@ def lib() -> C.CDLL:
     L.mbqc_plan_get_rescale.restype = C.c_int
     L.mbqc_sv_forward_c64.argtypes = [vp, vp, i64, i64, vp, i64, vp, vp, i64, i64, u32, vp]
     L.mbqc_sv_forward_c64.restype = C.c_int
+    L.mbqc_causal_flow.argtypes = [i32, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp]
+    L.mbqc_causal_flow.restype = C.c_int
     L.mbqc_sv_sample_c64.argtypes = [vp, vp, i64, i64, vp, i64, vp, vp, i64, i64, C.c_uint64, vp]
     L.mbqc_sv_sample_c64.restype = C.c_int
     L.mbqc_dm_forward_c64.argtypes = [vp, vp, i64, i64, vp, i64, vp, i64, i64, C.POINTER(Noise), u32, vp]
@@ -163,7 +165,8 @@ class NativePlan:
         L = lib()
         self._keep = dict(
             edges=_i32(np.asarray(edges, dtype=np.int32).reshape(-1, 2) if len(edges) else np.zeros((0, 2))),
-            schedule=_i32(schedule), inputs=_i32(input_nodes), outputs=_i32(output_nodes),
+            schedule=None if schedule is None else _i32(schedule), inputs=_i32(input_nodes),
+            outputs=_i32(output_nodes),
             angle_index=_i32(angle_index),
             fixed=np.ascontiguousarray(np.asarray(fixed_angle, dtype=np.float32)),
             flow=None if flow is None else _i32(flow),
@@ -172,7 +175,8 @@ class NativePlan:
         p32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
         desc = PatternDesc(
             n_nodes=n_nodes, n_edges=len(k["edges"]), edges=p32(k["edges"]),
-            schedule=p32(k["schedule"]), n_measure=n_measure, window=window,
+            schedule=None if k["schedule"] is None else p32(k["schedule"]), n_measure=n_measure,
+            window=window,
             n_inputs=len(k["inputs"]), input_nodes=p32(k["inputs"]),
             n_outputs=len(k["outputs"]), output_nodes=p32(k["outputs"]),
             angle_index=p32(k["angle_index"]),

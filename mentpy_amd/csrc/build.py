@@ -15,7 +15,7 @@ HERE = Path(__file__).resolve().parent
 PKG = HERE.parent
 LIB_DIR = PKG / "lib"
 LIB = LIB_DIR / "libmbqc_hip.so"
-SOURCES = [HERE / "kernels.hip", HERE / "plan.cpp"]
+SOURCES = [HERE / "kernels.hip", HERE / "plan.cpp", HERE / "flow.cpp"]
 ARCH = os.environ.get("MBQC_OFFLOAD_ARCH", "gfx950")
 
 

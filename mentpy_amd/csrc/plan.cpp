@@ -346,8 +346,33 @@ extern "C" {
 int mbqc_plan_create(const mbqc_pattern_desc* desc, mbqc_plan** out) {
   if (!out) return fail(MBQC_ERR_VALUE, "out is null");
   *out = nullptr;
+  if (!desc) return fail(MBQC_ERR_VALUE, "null pattern description");
+  mbqc_pattern_desc d = *desc;
+  std::vector<int> flow, layer, order;
+  if (!d.schedule) {
+    // schedule from the causal flow, as a flow-carrying MBQCircuit gets it
+    if (d.n_nodes <= 0) return fail(MBQC_ERR_VALUE, "pattern has no nodes");
+    if ((d.n_edges > 0 && !d.edges) || (d.n_inputs > 0 && !d.input_nodes) ||
+        (d.n_outputs > 0 && !d.output_nodes))
+      return fail(MBQC_ERR_VALUE, "null node or edge list");
+    bool found = false;
+    int rc = causal_flow(d.n_nodes, d.n_edges, d.edges, d.n_inputs, d.input_nodes, d.n_outputs,
+                         d.output_nodes, &flow, &layer, &order, &found);
+    if (rc != MBQC_OK) return rc;
+    if (!found)
+      return fail(MBQC_ERR_VALUE, "Schedule must be provided for numpy simulator as the MBQCircuit "
+                                  "does not have a flow.");
+    // unmeasured (output) nodes go last, in their order of appearance
+    std::vector<char> is_out(d.n_nodes, 0);
+    for (int m = 0; m < d.n_outputs; ++m) is_out[d.output_nodes[m]] = 1;
+    std::stable_partition(order.begin(), order.end(), [&](int v) { return !is_out[v]; });
+    d.schedule = order.data();
+    d.n_measure = d.n_nodes - d.n_outputs;
+    if (d.window <= 1) d.window = d.n_inputs + 1;
+    if (!d.flow) d.flow = flow.data();
+  }
   auto* p = new Plan();
-  int rc = compile_plan(desc, p);
+  int rc = compile_plan(&d, p);
   if (rc != MBQC_OK) {
     delete p;
     return rc;

@@ -58,6 +58,12 @@ std::vector<int32_t> pack_device_tables(const Plan& p);
 
 extern thread_local std::string g_last_error;
 int fail(int code, const std::string& msg);
+
+// Causal flow (flow[u] = successor, -1 for outputs), layers and measurement
+// order of an open graph; *found is false when the graph has no causal flow.
+int causal_flow(int n, int n_edges, const int32_t* edges, int n_in, const int32_t* inputs,
+                int n_out, const int32_t* outputs, std::vector<int>* flow, std::vector<int>* layer,
+                std::vector<int>* order, bool* found);
 int compile_plan(const mbqc_pattern_desc* d, Plan* p);
 
 }  // namespace mbqc

@@ -14,7 +14,7 @@ from __future__ import annotations
 
 from typing import Callable, Dict, Optional, Tuple
 
-__all__ = ["find_cflow", "Flow", "check_if_flow"]
+__all__ = ["find_cflow", "find_cflow_native", "Flow", "check_if_flow"]
 
 
 def find_cflow(graph, input_nodes, output_nodes) -> Tuple[Optional[Callable], Optional[Callable], Optional[int], Optional[Dict]]:
@@ -73,6 +73,40 @@ def find_cflow(graph, input_nodes, output_nodes) -> Tuple[Optional[Callable], Op
         return None, None, None, None
     depth = max(succ.values()) if succ else 0
     return (lambda x: succ[x]), (lambda u, v: layer[u] > layer[v]), depth, layer
+
+
+def find_cflow_native(graph, input_nodes, output_nodes):
+    """``find_cflow`` computed by the native library (``mbqc_causal_flow``).
+
+    Returns ``(flow, partial_order, depth, layers, measurement_order)`` with the
+    same meaning as ``find_cflow`` plus the ``calculate_order`` schedule, or
+    five ``None`` when the graph has no causal flow.  The frontier of each round
+    is visited in node order.
+    """
+    import ctypes as C
+    import numpy as np
+    from .. import _native as nat
+    nodes = list(graph.nodes())
+    idx = {v: k for k, v in enumerate(nodes)}
+    n = len(nodes)
+    edges = np.array([(idx[a], idx[b]) for a, b in graph.edges()], np.int32).reshape(-1, 2)
+    ins = np.array([idx[v] for v in input_nodes], np.int32)
+    outs = np.array([idx[v] for v in output_nodes], np.int32)
+    flow = np.zeros(n, np.int32)
+    layer = np.zeros(n, np.int32)
+    order = np.zeros(n, np.int32)
+    found = C.c_int32(0)
+    ptr = lambda a: C.c_void_p(a.ctypes.data)
+    nat.check(nat.lib().mbqc_causal_flow(n, len(edges), ptr(edges), len(ins), ptr(ins), len(outs),
+                                         ptr(outs), ptr(flow), ptr(layer), ptr(order),
+                                         C.byref(found)))
+    if not found.value:
+        return None, None, None, None, None
+    succ = {nodes[u]: nodes[f] for u, f in enumerate(flow) if f >= 0}
+    layers = {nodes[u]: int(l) for u, l in enumerate(layer)}
+    depth = max(succ.values()) if succ else 0
+    return ((lambda x: succ[x]), (lambda u, v: layers[u] > layers[v]), depth, layers,
+            [nodes[k] for k in order])
 
 
 class Flow:
