@@ -322,3 +322,21 @@ def test_noise_channels_three_oracles_agree(rows, cols, channel, rng):
     # damping is not unital: the output differs from the depolarizing-only one
     if channel in ("amp", "gad"):
         assert np.abs(want - brute_dm(g, th, psi, p_readout=0.02)).max() > 1e-3
+
+
+def test_haar_fidelity_density():
+    """P(F) = (N-1)(1-F)^(N-2) integrates to 1 and has mean 1/N; Haar samples
+    of the oracle's data generator follow it (KS test)."""
+    from scipy import integrate, stats
+    from mentpy_amd.utils import generate_haar_random_states, haar_probability_density_of_fidelities
+    for n in (1, 2, 3, 4):
+        N = 2 ** n
+        total, _ = integrate.quad(lambda f: haar_probability_density_of_fidelities(f, n), 0, 1)
+        mean, _ = integrate.quad(lambda f: f * haar_probability_density_of_fidelities(f, n), 0, 1)
+        assert abs(total - 1) < 1e-9 and abs(mean - 1 / N) < 1e-9
+    rng = np.random.default_rng(4)
+    a = generate_haar_random_states(2, 4000, rng)
+    b = generate_haar_random_states(2, 4000, rng)
+    f = np.abs(np.sum(np.conj(a) * b, -1)) ** 2
+    cdf = lambda x: 1 - (1 - x) ** 3          # integral of 3 (1-F)^2
+    assert stats.kstest(f, cdf).pvalue > 1e-3
