@@ -56,19 +56,74 @@ def time_launches(fn, steps, warmup, stream):
 
 def cpu_baseline(circ, psi, seconds=12.0, seed=7):
     """Reference-faithful numpy restatement, single thread, bounded sample."""
+    from threadpoolctl import threadpool_limits
     from oracle.reference_sv import ReferenceSV
     rng = np.random.default_rng(seed)
     sim = ReferenceSV(circ, psi)
     n = len(circ.trainable_nodes)
-    sim.run(rng.uniform(0, 2 * np.pi, n), output_form="sv", input_state=psi)  # warm-up
+    with threadpool_limits(limits=1):
+        sim.run(rng.uniform(0, 2 * np.pi, n), output_form="sv", input_state=psi)  # warm-up
+        count, t0 = 0, time.perf_counter()
+        while True:
+            sim.run(rng.uniform(0, 2 * np.pi, n), output_form="sv", input_state=psi)
+            count += 1
+            el = time.perf_counter() - t0
+            if el >= seconds:
+                break
+    return count / el, count, el
+
+
+def _pool_worker(args):
+    """One process of the multi-core CPU baseline: the same restatement on
+    grid_cluster(4,20) for ``seconds``; returns the simulations it finished."""
+    seconds, seed = args
+    import mentpy_amd as mp
+    from oracle.reference_sv import ReferenceSV
+    rng = np.random.default_rng(seed)
+    circ = mp.templates.grid_cluster(4, 20)
+    psi = mp.utils.generate_haar_random_states(4, rng=rng)
+    sim = ReferenceSV(circ, psi)
+    n = len(circ.trainable_nodes)
+    sim.run(rng.uniform(0, 2 * np.pi, n), output_form="sv", input_state=psi)
     count, t0 = 0, time.perf_counter()
-    while True:
+    while time.perf_counter() - t0 < seconds:
         sim.run(rng.uniform(0, 2 * np.pi, n), output_form="sv", input_state=psi)
         count += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
-    return count / el, count, el
+    return count, time.perf_counter() - t0
+
+
+def cpu_pool_baseline(seconds=8.0, max_workers=16):
+    """The restatement mapped over a process pool of the host's cores, one
+    numpy thread each (as mentpy's parallel tutorial maps simulations over a
+    pool, intro-to-mbqml-parallel.rst:43-47).  Workers are spawned fresh, so no
+    GPU state is inherited."""
+    import multiprocessing as mpr
+    workers = max(1, min(max_workers, len(os.sched_getaffinity(0))))
+    ctx = mpr.get_context("spawn")
+    names = ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")
+    saved = {k: os.environ.get(k) for k in names}
+    os.environ.update({k: "1" for k in names})         # inherited by the spawned workers
+    try:
+        with ctx.Pool(workers) as pool:
+            res = pool.map(_pool_worker, [(seconds, 100 + k) for k in range(workers)])
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    return sum(c / t for c, t in res), workers, sum(c for c, _ in res)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def adjoint_section(circ, sim, dev, steps, stream, sets=(1024, 16384), S=256, seed=99):
@@ -143,9 +198,11 @@ def cpu_gradient_baseline(circ, X, Y, t0, seconds_per_eval_cap=30.0):
         return acc / len(X)
     e = np.zeros(n)
     e[0] = 1e-5
-    tic = time.perf_counter()
-    (cost(t0 + e) - cost(t0 - e)) / 2e-5
-    el = time.perf_counter() - tic
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(limits=1):
+        tic = time.perf_counter()
+        (cost(t0 + e) - cost(t0 - e)) / 2e-5
+        el = time.perf_counter() - tic
     return 1.0 / (el * n), el
 
 
@@ -300,6 +357,7 @@ def main():
     ap.add_argument("--batch", type=int, default=1 << 21)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-pool-seconds", type=float, default=6.0)
     ap.add_argument("--grad-sets", type=int, nargs="+", default=[1024, 16384])
     ap.add_argument("--grad-inputs", type=int, default=256)
     ap.add_argument("--cpu-grad-inputs", type=int, default=32)
@@ -385,7 +443,14 @@ def main():
         rate, count, el = cpu_baseline(circ, psi, args.cpu_seconds)
         line["cpu_baseline"] = {"value": rate, "unit": "sims/s", "cores": 1, "kind": "port",
                                 "sample": f"{count} sims of grid_cluster(4,20) in {el:.1f} s, "
-                                          "step-by-step numpy complex128 restatement, 1 thread"}
+                                          "step-by-step numpy complex128 restatement, 1 thread",
+                                "cpu": cpu_model()}
+        if args.cpu_pool_seconds > 0:
+            prate, workers, pcount = cpu_pool_baseline(args.cpu_pool_seconds)
+            line["cpu_baseline_pool"] = {
+                "value": prate, "unit": "sims/s", "cores": workers, "kind": "port",
+                "sample": f"{pcount} sims of grid_cluster(4,20), {workers} spawned processes x "
+                          f"{args.cpu_pool_seconds:.0f} s, 1 numpy thread each"}
         if X is not None:
             grate, gel = cpu_gradient_baseline(circ, X[:args.cpu_grad_inputs], Y[:args.cpu_grad_inputs], t0)
             grate = grate * args.cpu_grad_inputs / len(X)

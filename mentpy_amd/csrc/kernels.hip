@@ -961,23 +961,12 @@ struct BatchItem {
   int64_t first;   // first simulation index within the job
 };
 
-template <bool REF, int MAXNS>
-__device__ __forceinline__ void run_item(const Job& J, int64_t g, bool active, uint32_t flags) {
-  switch (J.ns) {
-    case 1: sv_sim<1, REF>(J, g, active, flags); break;
-    case 2: sv_sim<2, REF>(J, g, active, flags); break;
-    case 3: sv_sim<3, REF>(J, g, active, flags); break;
-    case 4: if constexpr (MAXNS >= 4) sv_sim<4, REF>(J, g, active, flags); break;
-    case 5: if constexpr (MAXNS >= 5) sv_sim<5, REF>(J, g, active, flags); break;
-    case 6: if constexpr (MAXNS >= 6) sv_sim<6, REF>(J, g, active, flags); break;
-    default: break;
-  }
-}
-
-// Persistent kernel: every wave pulls work items (runs of simulations of one
-// job) from a device counter until none are left.  Items are sorted by
-// decreasing cost on the host, so long patterns start first.
-template <bool REF, int MAXNS>
+// Persistent kernel over the work items of one live-window size NS: every
+// wave pulls items (runs of simulations of one job) from its group's device
+// counter until none are left.  Items are sorted by decreasing cost on the
+// host, so long patterns start first.  One kernel per NS keeps each group at
+// the register footprint (and so the occupancy) of its own window size.
+template <bool REF, int NS>
 __global__ __launch_bounds__(256) void k_batch(const Job* __restrict__ jobs,
                                                const BatchItem* __restrict__ items, int64_t n_items,
                                                int* counter, uint32_t flags) {
@@ -991,13 +980,17 @@ __global__ __launch_bounds__(256) void k_batch(const Job* __restrict__ jobs,
     const Job J = jobs[I.job];
     for (int off = 0; off < I.count; off += 64) {
       const int64_t g = I.first + off + lane;
-      run_item<REF, MAXNS>(J, g, (off + lane) < I.count && g < J.total, flags);
+      sv_sim<NS, REF>(J, g, (off + lane) < I.count && g < J.total, flags);
     }
   }
 }
 
-__global__ void k_reset_counter(int* counter) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+// group counters live kCounterStride ints apart (separate cache lines)
+constexpr int kCounterStride = 64;
+
+__global__ void k_reset_counters(int* counter, int n) {
+  const int i = threadIdx.x;
+  if (blockIdx.x == 0 && i < n) counter[i * kCounterStride] = 0;
 }
 
 }  // namespace mbqc
@@ -1195,12 +1188,20 @@ hipError_t launch_dm_lds(const Job& J, const DmNoise& nz, uint32_t flags, hipStr
 
 struct mbqc_batch {
   int max_ns = 1;
+  int64_t group_begin[kMaxSvSlots + 1] = {};  // items of live-window size ns
+  int64_t group_count[kMaxSvSlots + 1] = {};
   std::vector<Job> jobs;
   std::vector<BatchItem> items;
   Job* d_jobs = nullptr;
   BatchItem* d_items = nullptr;
   int* d_counter = nullptr;
   int device = -1;
+  // side streams and events that run the window-size groups concurrently
+  // (fork from and join back into the caller's stream, so the pattern is
+  // also valid under graph capture)
+  hipStream_t side[kMaxSvSlots + 1] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kMaxSvSlots + 1] = {};
 };
 
 extern "C" {
@@ -1398,7 +1399,12 @@ int mbqc_batch_create(const mbqc_batch_job* jobs, int32_t n_jobs, int32_t sims_p
   if (sims_per_item <= 0 || sims_per_item % 64 != 0)
     return fail(MBQC_ERR_VALUE, "sims_per_item must be a positive multiple of 64");
   auto* bt = new mbqc_batch();
-  std::vector<std::pair<int64_t, size_t>> order;
+  struct Key {
+    int ns;
+    int64_t cost;
+    size_t index;
+  };
+  std::vector<Key> order;
   for (int i = 0; i < n_jobs; ++i) {
     const mbqc_batch_job& bj = jobs[i];
     if (!bj.plan) {
@@ -1426,24 +1432,39 @@ int mbqc_batch_create(const mbqc_batch_job* jobs, int32_t n_jobs, int32_t sims_p
       it.job = i;
       it.first = f;
       it.count = int32_t(std::min<int64_t>(sims_per_item, J.total - f));
-      order.push_back({cost, bt->items.size()});
+      order.push_back({p->ns, cost, bt->items.size()});
       bt->items.push_back(it);
     }
   }
-  // longest patterns first (stable within a job)
-  std::stable_sort(order.begin(), order.end(),
-                   [](const std::pair<int64_t, size_t>& a, const std::pair<int64_t, size_t>& b) {
-                     return a.first > b.first;
-                   });
+  // grouped by live-window size; longest patterns first within a group
+  // (stable within a job)
+  std::stable_sort(order.begin(), order.end(), [](const Key& a, const Key& b) {
+    return a.ns != b.ns ? a.ns > b.ns : a.cost > b.cost;
+  });
   std::vector<BatchItem> sorted;
   sorted.reserve(order.size());
-  for (auto& o : order) sorted.push_back(bt->items[o.second]);
+  for (size_t k = 0; k < order.size(); ++k) {
+    const int ns = order[k].ns;
+    if (bt->group_count[ns] == 0) bt->group_begin[ns] = int64_t(k);
+    ++bt->group_count[ns];
+    sorted.push_back(bt->items[order[k].index]);
+  }
   bt->items.swap(sorted);
   hipError_t e = hipGetDevice(&bt->device);
   if (e == hipSuccess) e = hipMalloc(&bt->d_jobs, sizeof(Job) * bt->jobs.size());
   if (e == hipSuccess && !bt->items.empty())
     e = hipMalloc(&bt->d_items, sizeof(BatchItem) * bt->items.size());
-  if (e == hipSuccess) e = hipMalloc(&bt->d_counter, sizeof(int) * 64);
+  if (e == hipSuccess) e = hipMalloc(&bt->d_counter, sizeof(int) * kCounterStride * (kMaxSvSlots + 1));
+  int n_groups = 0;
+  for (int ns = 1; ns <= kMaxSvSlots; ++ns) n_groups += bt->group_count[ns] > 0;
+  if (e == hipSuccess && n_groups > 1) {
+    e = hipEventCreateWithFlags(&bt->fork, hipEventDisableTiming);
+    for (int ns = 1; ns <= kMaxSvSlots && e == hipSuccess; ++ns) {
+      if (bt->group_count[ns] == 0) continue;
+      e = hipStreamCreateWithFlags(&bt->side[ns], hipStreamNonBlocking);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&bt->join[ns], hipEventDisableTiming);
+    }
+  }
   if (e == hipSuccess)
     e = hipMemcpy(bt->d_jobs, bt->jobs.data(), sizeof(Job) * bt->jobs.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess && !bt->items.empty())
@@ -1462,6 +1483,11 @@ void mbqc_batch_destroy(mbqc_batch* bt) {
   if (bt->d_jobs) (void)hipFree(bt->d_jobs);
   if (bt->d_items) (void)hipFree(bt->d_items);
   if (bt->d_counter) (void)hipFree(bt->d_counter);
+  for (int ns = 0; ns <= kMaxSvSlots; ++ns) {
+    if (bt->side[ns]) (void)hipStreamDestroy(bt->side[ns]);
+    if (bt->join[ns]) (void)hipEventDestroy(bt->join[ns]);
+  }
+  if (bt->fork) (void)hipEventDestroy(bt->fork);
   delete bt;
 }
 
@@ -1499,28 +1525,52 @@ int mbqc_batch_launch(mbqc_batch* bt, int32_t waves, int32_t wg_waves, uint32_t 
   if (waves < wg_waves) return fail(MBQC_ERR_VALUE, "waves must be at least wg_waves");
   if (bt->items.empty()) return MBQC_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, bt->d_counter);
-  const int blocks = waves / wg_waves;
+  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, bt->d_counter, kMaxSvSlots + 1);
   const int threads = 64 * wg_waves;
-  const int64_t n = int64_t(bt->items.size());
   const bool ref = flags & MBQC_FLAG_REF_PHASE;
-  const int mx = bt->max_ns <= 3 ? 3 : bt->max_ns;
-#define MBQC_BATCH_LAUNCH(R, M)                                                                  \
-  hipLaunchKernelGGL((k_batch<R, M>), dim3(blocks), dim3(threads), 0, st, bt->d_jobs, bt->d_items, \
-                     n, bt->d_counter, flags)
-  switch (mx * 2 + (ref ? 1 : 0)) {
-    case 6: MBQC_BATCH_LAUNCH(false, 3); break;
-    case 7: MBQC_BATCH_LAUNCH(true, 3); break;
-    case 8: MBQC_BATCH_LAUNCH(false, 4); break;
-    case 9: MBQC_BATCH_LAUNCH(true, 4); break;
-    case 10: MBQC_BATCH_LAUNCH(false, 5); break;
-    case 11: MBQC_BATCH_LAUNCH(true, 5); break;
-    default:
-      if (ref) MBQC_BATCH_LAUNCH(true, 6); else MBQC_BATCH_LAUNCH(false, 6);
-      break;
-  }
+  const bool concurrent = bt->fork != nullptr;
+  hipError_t e = hipSuccess;
+  if (concurrent) e = hipEventRecord(bt->fork, st);
+  // one persistent launch per live-window size, largest windows first; with
+  // several sizes each runs on its own stream, forked from and joined into st
+  for (int ns = kMaxSvSlots; ns >= 1 && e == hipSuccess; --ns) {
+    const int64_t n = bt->group_count[ns];
+    if (n == 0) continue;
+    hipStream_t gs = st;
+    if (concurrent) {
+      gs = bt->side[ns];
+      e = hipStreamWaitEvent(gs, bt->fork, 0);
+      if (e != hipSuccess) break;
+    }
+    // no more resident waves than the group has items
+    const int64_t wv = std::min<int64_t>(waves, ((n + wg_waves - 1) / wg_waves) * wg_waves);
+    const int blocks = int(wv / wg_waves);
+    const BatchItem* items = bt->d_items + bt->group_begin[ns];
+    int* counter = bt->d_counter + ns * kCounterStride;
+#define MBQC_BATCH_LAUNCH(R, N)                                                                  \
+  hipLaunchKernelGGL((k_batch<R, N>), dim3(blocks), dim3(threads), 0, gs, bt->d_jobs, items, n, \
+                     counter, flags)
+    switch (ns * 2 + (ref ? 1 : 0)) {
+      case 2: MBQC_BATCH_LAUNCH(false, 1); break;
+      case 3: MBQC_BATCH_LAUNCH(true, 1); break;
+      case 4: MBQC_BATCH_LAUNCH(false, 2); break;
+      case 5: MBQC_BATCH_LAUNCH(true, 2); break;
+      case 6: MBQC_BATCH_LAUNCH(false, 3); break;
+      case 7: MBQC_BATCH_LAUNCH(true, 3); break;
+      case 8: MBQC_BATCH_LAUNCH(false, 4); break;
+      case 9: MBQC_BATCH_LAUNCH(true, 4); break;
+      case 10: MBQC_BATCH_LAUNCH(false, 5); break;
+      case 11: MBQC_BATCH_LAUNCH(true, 5); break;
+      case 12: MBQC_BATCH_LAUNCH(false, 6); break;
+      default: MBQC_BATCH_LAUNCH(true, 6); break;
+    }
 #undef MBQC_BATCH_LAUNCH
-  hipError_t e = hipGetLastError();
+    e = hipGetLastError();
+    if (e == hipSuccess && concurrent) {
+      e = hipEventRecord(bt->join[ns], gs);
+      if (e == hipSuccess) e = hipStreamWaitEvent(st, bt->join[ns], 0);
+    }
+  }
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "batch launch");
 }
 
