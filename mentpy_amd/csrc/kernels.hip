@@ -504,9 +504,11 @@ struct SampleMod {
 };
 
 template <int NS>
-__global__ __launch_bounds__(256) void k_sv_sample(Job J, const uint64_t* xm, const uint64_t* zm,
-                                                   const int32_t* out_off, int32_t* outcomes,
-                                                   int32_t words, uint64_t seed) {
+__global__ __launch_bounds__(256) void k_sv_sample(Job J, const uint64_t* __restrict__ xm,
+                                                   const uint64_t* __restrict__ zm,
+                                                   const int32_t* __restrict__ out_off,
+                                                   int32_t* __restrict__ outcomes, int32_t words,
+                                                   uint64_t seed) {
   constexpr int H = 1 << NS;
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const bool active = g < J.total;
@@ -532,42 +534,32 @@ __global__ __launch_bounds__(256) void k_sv_sample(Job J, const uint64_t* xm, co
     xo |= uint32_t((mod.fx >> off) & 1u) << (J.n_out - 1 - m);
     zo |= uint32_t((mod.fz >> off) & 1u) << (J.n_out - 1 - m);
   }
+  // all output positions first: the outcome stores above keep the compiler
+  // from moving these table loads across the output stores below
   const int32_t* out_pos = J.tab + J.off_out;
+  int pos[H];
+#pragma unroll
+  for (int j = 0; j < H; ++j) pos[j] = out_pos[j];
   float n2 = 0.0f;
 #pragma unroll
   for (int j = 0; j < H; ++j)
-    if (out_pos[j] >= 0) n2 = fmaf(t[j].x, t[j].x, fmaf(t[j].y, t[j].y, n2));
+    if (pos[j] >= 0) n2 = fmaf(t[j].x, t[j].x, fmaf(t[j].y, t[j].y, n2));
   const float scale = 1.0f / sqrtf(n2);
   if (active) {
+    // the output frame Z^z X^x applied while storing:
+    // new[o] = (-1)^{popcount(o & z)} old[o ^ x], i.e. old[p] lands at p ^ x
     float2* o = J.out + g * (int64_t(1) << J.n_out);
 #pragma unroll
     for (int j = 0; j < H; ++j) {
-      const int p = out_pos[j];
-      if (p >= 0) o[p] = cscale(t[j], scale);
+      const int p = pos[j];
+      if (p >= 0) {
+        const uint32_t q = uint32_t(p) ^ xo;
+        o[q] = flip(cscale(t[j], scale), czbit(q, zo));
+      }
     }
     row[words] = int32_t(xo);
     row[words + 1] = int32_t(zo);
   }
-}
-
-// Apply the output frame Z^z X^x of every simulation in place:
-// new[o] = (-1)^{popcount(o & z)} old[o ^ x]; each index pair is handled once.
-__global__ void k_apply_frame(float2* out, const int32_t* outcomes, int32_t words, int64_t total,
-                              int32_t n_out) {
-  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int64_t d = int64_t(1) << n_out;
-  if (e >= total * d) return;
-  const int64_t g = e >> n_out;
-  const int o = int(e & (d - 1));
-  const int32_t* row = outcomes + g * (words + 2);
-  const int x = row[words], z = row[words + 1];
-  const int o2 = o ^ x;
-  if (o2 < o) return;
-  float2* r = out + g * d;
-  const float2 a = r[o], c = r[o2];
-  const uint32_t s1 = czbit(uint32_t(o), uint32_t(z)), s2 = czbit(uint32_t(o2), uint32_t(z));
-  r[o] = flip(c, s1);
-  if (o2 != o) r[o2] = flip(a, s2);
 }
 
 // ------------------------------------------------------- LDS-resident kernels
@@ -579,9 +571,12 @@ __device__ __forceinline__ int insert_zero(int v, int q) {
   return ((v >> q) << (q + 1)) | (v & ((1 << q) - 1));
 }
 
+// Sum over the T threads of a workgroup (T a power of two); a workgroup of
+// fewer than 64 threads only exchanges among its own lanes.
 template <int T>
 __device__ __forceinline__ float block_sum(float v, float* red) {
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  constexpr int first = T >= 64 ? 32 : T / 2;
+  for (int m = first; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
   if constexpr (T > 64) {
     const int w = threadIdx.x >> 6;
     __syncthreads();
@@ -1147,6 +1142,8 @@ hipError_t launch_adj_bwd(const Job& J, const float2* ckpt, const float2* seed, 
 }
 
 constexpr int kMaxSvSlots = 6;       // register-resident SV kernels
+// below this many simulations the DM path uses one workgroup per simulation
+constexpr int64_t kDmLaneKernelMinSims = 16384;
 constexpr int kMaxSvLdsSlots = 12;   // LDS-resident SV kernel
 constexpr int kMaxDmSlots = 3;       // register-resident DM kernel
 constexpr int kMaxDmLdsSlots = 6;    // LDS-resident DM kernel
@@ -1161,11 +1158,6 @@ hipError_t launch_sample(const Job& J, const int32_t* tab, const TableOffsets& o
   const uint64_t* zm = reinterpret_cast<const uint64_t*>(tab + o.zmask);
   hipLaunchKernelGGL((k_sv_sample<NS>), dim3(blocks), dim3(threads), 0, st, J, xm, zm,
                      tab + o.out_offset, outcomes, words, seed);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  const int64_t elems = J.total << J.n_out;
-  hipLaunchKernelGGL(k_apply_frame, dim3((elems + threads - 1) / threads), dim3(threads), 0, st,
-                     J.out, outcomes, words, J.total, J.n_out);
   return hipGetLastError();
 }
 
@@ -1301,10 +1293,14 @@ int mbqc_dm_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
   if (J.total == 0) return MBQC_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
   hipError_t e = hipSuccess;
+  // Small batches of small windows leave the one-simulation-per-lane kernel
+  // with a few waves on a few CUs; there one workgroup per simulation (one
+  // quartet per thread, state in LDS) spreads the work over the whole chip.
+  const bool small = J.total < kDmLaneKernelMinSims;
   switch (p->ns) {
-    case 1: e = launch_dm<1>(J, nz, flags, st); break;
-    case 2: e = launch_dm<2>(J, nz, flags, st); break;
-    case 3: e = launch_dm<3>(J, nz, flags, st); break;
+    case 1: e = small ? launch_dm_lds<1>(J, nz, flags, st) : launch_dm<1>(J, nz, flags, st); break;
+    case 2: e = small ? launch_dm_lds<2>(J, nz, flags, st) : launch_dm<2>(J, nz, flags, st); break;
+    case 3: e = small ? launch_dm_lds<3>(J, nz, flags, st) : launch_dm<3>(J, nz, flags, st); break;
     case 4: e = launch_dm_lds<4>(J, nz, flags, st); break;
     case 5: e = launch_dm_lds<5>(J, nz, flags, st); break;
     case 6: e = launch_dm_lds<6>(J, nz, flags, st); break;

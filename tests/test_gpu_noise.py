@@ -75,3 +75,31 @@ def test_dm_composed_channels(rows, cols, rng):
     td = 0.5 * np.abs(np.linalg.eigvalsh(out - ref)).sum(-1).max()
     print(f"composed {rows}x{cols}: trace distance {td:.2e}", flush=True)
     assert td < 1e-5
+
+
+@pytest.mark.parametrize("rows,cols", [(2, 6), (3, 6)])
+def test_dm_lane_and_workgroup_kernels_agree(rows, cols, rng):
+    """Batches of >= 16384 simulations run one simulation per lane; smaller ones
+    one simulation per workgroup.  Both paths on the same inputs agree, and a
+    sample of the large batch matches the oracle."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from oracle import channels as ch
+    from oracle.vectorized import dm_batch
+    g = mp.templates.grid_cluster(rows, cols)
+    sim = mp.PatternSimulator(g, backend="hip-dm", circuit_noise="amplitude_damping", gamma=0.1,
+                              p_readout=0.01)
+    B = 65536
+    th = torch.rand((B, len(g.trainable_nodes)), generator=torch.Generator().manual_seed(4))
+    th = (th * 2 * np.pi).cuda()
+    psi = haar_state(rows, rng)
+    big = sim.run_batch(th, psi)                 # lane kernel
+    small = sim.run_batch(th[:300], psi)         # workgroup kernel
+    diff = (big[:300] - small).abs().max().item()
+    idx = rng.choice(B, 64, replace=False)
+    ref = dm_batch(g, th[idx].double().cpu().numpy(), psi[None], p_readout=0.01,
+                   kraus=ch.amplitude_damping(0.1))
+    td = 0.5 * np.abs(np.linalg.eigvalsh(big[idx].cpu().numpy() - ref)).sum(-1).max()
+    print(f"{rows}x{cols}: lane vs workgroup {diff:.2e}, trace distance {td:.2e}", flush=True)
+    assert diff < 2e-6
+    assert td < 1e-5
