@@ -155,6 +155,24 @@ int mbqc_sv_sample_c64(const mbqc_plan* plan, const float* angles, int64_t angle
                        int64_t angle_stride_a, const void* psi_in, int64_t in_stride, void* out,
                        int32_t* outcomes, int64_t B, int64_t S, uint64_t seed, void* stream);
 
+/* Step-by-step API: the reference's window state after its first k
+ * measurements, as NumpySimulatorSV.measure / NumpySimulatorDM.measure leave
+ * it in qstate (np_simulator_sv.py:164-225, np_simulator_dm.py:151-216).
+ * The window holds nodes schedule[k .. k+W-1], W = mbqc_plan_window_qubits,
+ * oldest first (most significant bit), the next node already appended as |+>
+ * with its CZs.  Measured node j (schedule order) uses
+ * step_angles[b*step_stride + j], so any angle can be given per step; inputs as
+ * in the forward calls.  The state is normalised (SV: MBQC_FLAG_REF_PHASE
+ * selects the reference's global phase).  out: [B*S][2^W] (SV) or
+ * [B*S][2^W][2^W] (DM; noise of the measured qubits as in the full run). */
+int32_t mbqc_plan_window_qubits(const mbqc_plan* plan, int32_t k);
+int mbqc_sv_window_c64(const mbqc_plan* plan, const float* step_angles, int64_t step_stride,
+                       const void* psi_in, int64_t in_stride, int32_t k, void* out, int64_t B,
+                       int64_t S, uint32_t flags, void* stream);
+int mbqc_dm_window_c64(const mbqc_plan* plan, const float* step_angles, int64_t step_stride,
+                       const void* psi_in, int64_t in_stride, int32_t k, void* out, int64_t B,
+                       int64_t S, const mbqc_noise* noise, void* stream);
+
 /* Density-matrix forward, complex64: replaces NumpySimulatorDM.run / measure /
  * measure_ment (np_simulator_dm.py:151-283, 307-346) and
  * partial_trace_mixed_state (state_ops.py:77-119); the noise channels follow

@@ -589,6 +589,68 @@ __device__ __forceinline__ float block_sum(float v, float* red) {
   return v;
 }
 
+// One step on an LDS-resident statevector of H amplitudes, the pairs split
+// over the T threads of the workgroup (caller synchronises before and after).
+template <int T>
+__device__ __forceinline__ void sv_lds_step(float2* st, int H, const mbqc_step& step, float2 b,
+                                            int tid) {
+  const int q = int((step.flags >> 8) & 0xffu);
+  const int QB = 1 << q;
+  const float s = (step.flags & 2u) ? -1.0f : 1.0f;
+  const float app = (step.flags & 1u) ? 1.0f : 0.0f;
+  for (int i = tid; i < H / 2; i += T) {
+    const int j = insert_zero(i, q);
+    const float2 x = st[j], y = st[j | QB];
+    const float c = (__builtin_popcount(unsigned(j) & step.nmask) & 1) ? -app : app;
+    const float al = c * s, be = c * (1.0f - s);
+    float2 u0;
+    u0.x = fmaf(-b.y, y.y, fmaf(b.x, y.x, x.x));
+    u0.y = fmaf(b.x, y.y, fmaf(b.y, y.x, x.y));
+    st[j] = u0;
+    st[j | QB] = make_float2(fmaf(al, u0.x, be * x.x), fmaf(al, u0.y, be * x.y));
+  }
+}
+
+// One step on an LDS-resident density matrix (H x H), one quartet of the
+// measured qubit per thread iteration; gb = gamma e^{-i theta}.
+template <int T>
+__device__ __forceinline__ void dm_lds_step(float2* r, int H, const mbqc_step& step, float2 gb,
+                                            int tid) {
+  const int HQ = H / 2;
+  const float2 gbc = cconj(gb);
+  const int q = int((step.flags >> 8) & 0xffu);
+  const int QB = 1 << q;
+  const bool app = step.flags & 1u, sneg = step.flags & 2u;
+  for (int i = tid; i < HQ * HQ; i += T) {
+    const int j = insert_zero(i / HQ, q), kk = insert_zero(i % HQ, q);
+    const float2 B00 = r[j * H + kk], B01 = r[j * H + (kk | QB)];
+    const float2 B10 = r[(j | QB) * H + kk], B11 = r[(j | QB) * H + (kk | QB)];
+    const float2 U = cmul(gbc, B01), V = cmul(gb, B10);
+    const float2 P = cadd(B00, B11);
+    const float2 r00 = cadd(P, cadd(U, V));
+    if (app) {
+      const uint32_t sj = czbit(j, step.nmask), sk = czbit(kk, step.nmask);
+      if (sneg) {
+        const float2 M = csub(B00, B11);
+        r[j * H + kk] = r00;
+        r[j * H + (kk | QB)] = flip(cadd(M, csub(V, U)), sk);
+        r[(j | QB) * H + kk] = flip(cadd(M, csub(U, V)), sj);
+        r[(j | QB) * H + (kk | QB)] = flip(csub(P, cadd(U, V)), sj ^ sk);
+      } else {
+        r[j * H + kk] = r00;
+        r[j * H + (kk | QB)] = flip(r00, sk);
+        r[(j | QB) * H + kk] = flip(r00, sj);
+        r[(j | QB) * H + (kk | QB)] = flip(r00, sj ^ sk);
+      }
+    } else {
+      r[j * H + kk] = r00;
+      r[j * H + (kk | QB)] = make_float2(0.f, 0.f);
+      r[(j | QB) * H + kk] = make_float2(0.f, 0.f);
+      r[(j | QB) * H + (kk | QB)] = make_float2(0.f, 0.f);
+    }
+  }
+}
+
 template <int NS, int T>
 __global__ __launch_bounds__(T) void k_sv_forward_lds(Job J, uint32_t flags) {
   constexpr int H = 1 << NS;
@@ -616,22 +678,8 @@ __global__ __launch_bounds__(T) void k_sv_forward_lds(Job J, uint32_t flags) {
     const float th = step.angle_index >= 0 ? ang[int64_t(step.angle_index) * J.angle_stride_a]
                                            : step.fixed_angle;
     const float2 b = ref ? step_coeff<true>(th, &ph) : step_coeff<false>(th, &ph);
-    const int q = int((step.flags >> 8) & 0xffu);
-    const int QB = 1 << q;
-    const float s = (step.flags & 2u) ? -1.0f : 1.0f;
-    const float app = (step.flags & 1u) ? 1.0f : 0.0f;
     __syncthreads();
-    for (int i = tid; i < H / 2; i += T) {
-      const int j = insert_zero(i, q);
-      const float2 x = st[j], y = st[j | QB];
-      const float c = (__builtin_popcount(unsigned(j) & step.nmask) & 1) ? -app : app;
-      const float al = c * s, be = c * (1.0f - s);
-      float2 u0;
-      u0.x = fmaf(-b.y, y.y, fmaf(b.x, y.x, x.x));
-      u0.y = fmaf(b.x, y.y, fmaf(b.y, y.x, x.y));
-      st[j] = u0;
-      st[j | QB] = make_float2(fmaf(al, u0.x, be * x.x), fmaf(al, u0.y, be * x.y));
-    }
+    sv_lds_step<T>(st, H, step, b, tid);
   }
   __syncthreads();
   ph = cscale(ph, 1.0f / sqrtf(fmaf(ph.x, ph.x, ph.y * ph.y)));
@@ -684,39 +732,8 @@ __global__ __launch_bounds__(T) void k_dm_forward_lds(Job J, DmNoise nz, uint32_
                                            : step.fixed_angle;
     float2 ph = make_float2(1.f, 0.f);
     const float2 gb = cscale(step_coeff<false>(th, &ph), nz.gamma);
-    const float2 gbc = cconj(gb);
-    const int q = int((step.flags >> 8) & 0xffu);
-    const int QB = 1 << q;
-    const bool app = step.flags & 1u, sneg = step.flags & 2u;
     __syncthreads();
-    for (int i = tid; i < HQ * HQ; i += T) {
-      const int j = insert_zero(i / HQ, q), kk = insert_zero(i % HQ, q);
-      const float2 B00 = r[j * H + kk], B01 = r[j * H + (kk | QB)];
-      const float2 B10 = r[(j | QB) * H + kk], B11 = r[(j | QB) * H + (kk | QB)];
-      const float2 U = cmul(gbc, B01), V = cmul(gb, B10);
-      const float2 P = cadd(B00, B11);
-      const float2 r00 = cadd(P, cadd(U, V));
-      if (app) {
-        const uint32_t sj = czbit(j, step.nmask), sk = czbit(kk, step.nmask);
-        if (sneg) {
-          const float2 M = csub(B00, B11);
-          r[j * H + kk] = r00;
-          r[j * H + (kk | QB)] = flip(cadd(M, csub(V, U)), sk);
-          r[(j | QB) * H + kk] = flip(cadd(M, csub(U, V)), sj);
-          r[(j | QB) * H + (kk | QB)] = flip(csub(P, cadd(U, V)), sj ^ sk);
-        } else {
-          r[j * H + kk] = r00;
-          r[j * H + (kk | QB)] = flip(r00, sk);
-          r[(j | QB) * H + kk] = flip(r00, sj);
-          r[(j | QB) * H + (kk | QB)] = flip(r00, sj ^ sk);
-        }
-      } else {
-        r[j * H + kk] = r00;
-        r[j * H + (kk | QB)] = make_float2(0.f, 0.f);
-        r[(j | QB) * H + kk] = make_float2(0.f, 0.f);
-        r[(j | QB) * H + (kk | QB)] = make_float2(0.f, 0.f);
-      }
-    }
+    dm_lds_step<T>(r, H, step, gb, tid);
   }
   // noise of every output qubit
   for (int m = 0; m < NS; ++m) {
@@ -950,6 +967,124 @@ __global__ void k_sgd(float* __restrict__ x, const float* __restrict__ g, float*
 }
 
 // ---------------------------------------------------------------- batched
+// ------------------------------------------------------- step-by-step windows
+// Window index r -> live register index j, the appended qubit's bit a and the
+// sign of its CZs (see WindowMap).
+__device__ __forceinline__ int window_source(const WindowMap& m, int r, uint32_t* sbit) {
+  int j = 0, a = 0;
+  for (int i = 0; i < m.W; ++i) {
+    const int bit = (r >> (m.W - 1 - i)) & 1;
+    const int s = m.slot[i];
+    if (s >= 0)
+      j |= bit << s;
+    else
+      a = bit;
+  }
+  *sbit = a ? czbit(uint32_t(r), m.nb) : 0u;
+  return j;
+}
+
+// Reference window after k_steps measurements, statevector: one workgroup per
+// simulation, live state in dynamic LDS (2^n_slots amplitudes), step j at the
+// caller's angle step_angles[b * step_stride + j].
+__global__ __launch_bounds__(64) void k_sv_window(Job J, const float* __restrict__ step_angles,
+                                                  int64_t step_stride, int32_t k_steps,
+                                                  WindowMap map, uint32_t flags) {
+  constexpr int T = 64;
+  extern __shared__ float2 st[];
+  __shared__ float red[4];
+  const int ns = J.ns, H = 1 << ns;
+  const int64_t g = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int64_t bi = g / J.S, si = g - bi * J.S;
+  const float* ang = step_angles + bi * step_stride;
+  const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
+  const int32_t* in_index = J.tab + J.off_in;
+  const int32_t* sign = J.tab + J.off_sign;
+  const int32_t* resc = J.tab + J.off_svres;
+  const float2* psi = J.psi + si * J.in_stride;
+  for (int j = tid; j < H; j += T) st[j] = flip(psi[in_index[j]], sign[j] ? 0x80000000u : 0u);
+  float2 ph = make_float2(1.0f, 0.0f);
+  const bool ref = flags & MBQC_FLAG_REF_PHASE;
+  for (int k = 0; k < k_steps; ++k) {
+    if (k % ns == 0 && resc[k / ns]) {
+      __syncthreads();
+      for (int j = tid; j < H; j += T) st[j] = cscale(st[j], 0x1p-32f);
+    }
+    const mbqc_step step = steps[k];
+    const float2 b = ref ? step_coeff<true>(ang[k], &ph) : step_coeff<false>(ang[k], &ph);
+    __syncthreads();
+    sv_lds_step<T>(st, H, step, b, tid);
+  }
+  __syncthreads();
+  ph = cscale(ph, 1.0f / sqrtf(fmaf(ph.x, ph.x, ph.y * ph.y)));
+  float n2 = 0.0f;
+  for (int j = tid; j < H; j += T) n2 = fmaf(st[j].x, st[j].x, fmaf(st[j].y, st[j].y, n2));
+  n2 = block_sum<T>(n2, red);
+  bool appended = false;
+  for (int i = 0; i < map.W; ++i) appended |= map.slot[i] < 0;
+  const float2 f = cscale(ph, (appended ? 0.70710678118654752440f : 1.0f) / sqrtf(n2));
+  const int D = 1 << map.W;
+  float2* o = J.out + g * int64_t(D);
+  for (int r = tid; r < D; r += T) {
+    uint32_t sb;
+    const int j = window_source(map, r, &sb);
+    o[r] = flip(cmul(st[j], f), sb);
+  }
+}
+
+// The same for the density matrix (state in dynamic LDS, 4^n_slots entries);
+// the measured qubits' noise is applied as in the full run, output noise is not.
+__global__ __launch_bounds__(64) void k_dm_window(Job J, const float* __restrict__ step_angles,
+                                                  int64_t step_stride, int32_t k_steps,
+                                                  WindowMap map, DmNoise nz) {
+  constexpr int T = 64;
+  extern __shared__ float2 r[];
+  __shared__ float red[4];
+  const int ns = J.ns, H = 1 << ns;
+  const int64_t g = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int64_t bi = g / J.S, si = g - bi * J.S;
+  const float* ang = step_angles + bi * step_stride;
+  const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
+  const int32_t* in_index = J.tab + J.off_in;
+  const int32_t* sign = J.tab + J.off_sign;
+  const int32_t* resc = J.tab + J.off_dmres;
+  const float2* psi = J.psi + si * J.in_stride;
+  for (int e = tid; e < H * H; e += T) {
+    const int j = e / H, k = e % H;
+    const float2 vj = flip(psi[in_index[j]], sign[j] ? 0x80000000u : 0u);
+    const float2 vk = flip(psi[in_index[k]], sign[k] ? 0x80000000u : 0u);
+    r[e] = cmul(vj, cconj(vk));
+  }
+  for (int k = 0; k < k_steps; ++k) {
+    if (k % ns == 0 && resc[k / ns]) {
+      __syncthreads();
+      for (int e = tid; e < H * H; e += T) r[e] = cscale(r[e], 0x1p-32f);
+    }
+    const mbqc_step step = steps[k];
+    float2 ph = make_float2(1.f, 0.f);
+    const float2 gb = cscale(step_coeff<false>(ang[k], &ph), nz.gamma);
+    __syncthreads();
+    dm_lds_step<T>(r, H, step, gb, tid);
+  }
+  __syncthreads();
+  float tr = 0.0f;
+  for (int j = tid; j < H; j += T) tr += r[j * H + j].x;
+  tr = block_sum<T>(tr, red);
+  bool appended = false;
+  for (int i = 0; i < map.W; ++i) appended |= map.slot[i] < 0;
+  const float scale = (appended ? 0.5f : 1.0f) / tr;
+  const int D = 1 << map.W;
+  float2* o = J.out + g * int64_t(D) * D;
+  for (int e = tid; e < D * D; e += T) {
+    const int row = e / D, col = e % D;
+    uint32_t sr, sc;
+    const int jr = window_source(map, row, &sr), jc = window_source(map, col, &sc);
+    o[e] = flip(cscale(r[jr * H + jc], scale), sr ^ sc);
+  }
+}
+
 struct BatchItem {
   int32_t job;
   int32_t count;   // simulations in this item
@@ -1266,6 +1401,63 @@ int mbqc_sv_sample_c64(const mbqc_plan* plan, const float* angles, int64_t angle
     default: return fail(MBQC_ERR_NOT_IMPLEMENTED, "sampled outcomes need n_slots <= 6");
   }
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "sv_sample launch");
+}
+
+// Shared set-up of the window entry points.
+static int window_job(const mbqc_plan* plan, const float* step_angles, int64_t step_stride,
+                      const void* psi_in, int64_t in_stride, int32_t k, void* out, int64_t B,
+                      int64_t S, int max_slots, Job* J, WindowMap* map, Plan** pp) {
+  if (!plan) return fail(MBQC_ERR_VALUE, "null plan");
+  Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(plan));
+  if (p->ns > max_slots)
+    return fail(MBQC_ERR_NOT_IMPLEMENTED, "window snapshots support live windows of at most " +
+                                              std::to_string(max_slots) + " qubits");
+  if (k > 0 && !step_angles) return fail(MBQC_ERR_VALUE, "step angles pointer is null");
+  if (step_stride < 0) return fail(MBQC_ERR_VALUE, "negative stride");
+  int rc = window_map(*p, k, map);
+  if (rc != MBQC_OK) return rc;
+  // with k == 0 no angle is read; the input pointer stands in for the
+  // (unused) angle pointer that make_job requires
+  rc = make_job(p, step_angles ? step_angles : reinterpret_cast<const float*>(psi_in), 0, 1, psi_in,
+                in_stride, out, B, S, J);
+  *pp = p;
+  return rc;
+}
+
+int mbqc_sv_window_c64(const mbqc_plan* plan, const float* step_angles, int64_t step_stride,
+                       const void* psi_in, int64_t in_stride, int32_t k, void* out, int64_t B,
+                       int64_t S, uint32_t flags, void* stream) {
+  Job J;
+  WindowMap map;
+  Plan* p = nullptr;
+  int rc = window_job(plan, step_angles, step_stride, psi_in, in_stride, k, out, B, S,
+                      kMaxSvLdsSlots, &J, &map, &p);
+  if (rc != MBQC_OK) return rc;
+  if (J.total == 0) return MBQC_OK;
+  const size_t lds = sizeof(float2) << p->ns;
+  hipLaunchKernelGGL(k_sv_window, dim3(J.total), dim3(64), lds, static_cast<hipStream_t>(stream), J,
+                     step_angles, step_stride, k, map, flags);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? MBQC_OK : hip_fail(e, "sv_window launch");
+}
+
+int mbqc_dm_window_c64(const mbqc_plan* plan, const float* step_angles, int64_t step_stride,
+                       const void* psi_in, int64_t in_stride, int32_t k, void* out, int64_t B,
+                       int64_t S, const mbqc_noise* noise, void* stream) {
+  Job J;
+  WindowMap map;
+  Plan* p = nullptr;
+  int rc = window_job(plan, step_angles, step_stride, psi_in, in_stride, k, out, B, S,
+                      kMaxDmLdsSlots, &J, &map, &p);
+  if (rc != MBQC_OK) return rc;
+  const mbqc_noise zero{};
+  const DmNoise nz = compose_noise(noise ? *noise : zero);
+  if (J.total == 0) return MBQC_OK;
+  const size_t lds = sizeof(float2) << (2 * p->ns);
+  hipLaunchKernelGGL(k_dm_window, dim3(J.total), dim3(64), lds, static_cast<hipStream_t>(stream), J,
+                     step_angles, step_stride, k, map, nz);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? MBQC_OK : hip_fail(e, "dm_window launch");
 }
 
 int mbqc_dm_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,

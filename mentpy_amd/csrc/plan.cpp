@@ -120,6 +120,8 @@ int compile_plan(const mbqc_pattern_desc* d, Plan* p) {
   }
   auto has_edge = [&](int a, int b) { return edges.count(edge_key(a, b)) > 0; };
 
+  p->schedule.assign(d->schedule, d->schedule + N);
+  p->nbr = nbr;
   const int ns = (w >= 2 && nin <= w - 1 && nout <= w - 1) ? w - 1 : w;
   if (ns > kMaxSlots)
     return fail(MBQC_ERR_NOT_IMPLEMENTED, "live window of " + std::to_string(ns) +
@@ -337,6 +339,31 @@ std::vector<int32_t> pack_device_tables(const Plan& p) {
   return t;
 }
 
+int window_map(const Plan& p, int k, WindowMap* m) {
+  if (k < 0 || k > p.n_steps) return fail(MBQC_ERR_VALUE, "measurement count out of range");
+  const int N = p.n_nodes, ns = p.ns;
+  const int W = std::min(p.w, N - k);
+  if (W > kMaxSlots + 1) return fail(MBQC_ERR_NOT_IMPLEMENTED, "window too large");
+  m->W = W;
+  m->nb = 0;
+  // live nodes are schedule[k .. k+ns-1] (fewer at the end of the schedule);
+  // with lag 1 the reference window holds one more node, already appended
+  const int n_live = std::min(ns, N - k);
+  for (int i = 0; i < W; ++i) {
+    if (i < n_live) {
+      m->slot[i] = ((ns - 1 - (k + i)) % ns + ns) % ns;
+    } else {
+      m->slot[i] = -1;
+      const int v = p.schedule[k + i];
+      for (int x : p.nbr[v])
+        for (int t = 0; t < n_live; ++t)
+          if (p.schedule[k + t] == x) m->nb |= 1u << (W - 1 - t);
+    }
+  }
+  for (int i = W; i < kMaxSlots + 1; ++i) m->slot[i] = -1;
+  return MBQC_OK;
+}
+
 }  // namespace mbqc
 
 using namespace mbqc;
@@ -379,6 +406,13 @@ int mbqc_plan_create(const mbqc_pattern_desc* desc, mbqc_plan** out) {
   }
   *out = reinterpret_cast<mbqc_plan*>(p);
   return MBQC_OK;
+}
+
+int32_t mbqc_plan_window_qubits(const mbqc_plan* plan, int32_t k) {
+  if (!plan) return fail(MBQC_ERR_VALUE, "null plan");
+  const Plan* p = reinterpret_cast<const Plan*>(plan);
+  if (k < 0 || k > p->n_steps) return fail(MBQC_ERR_VALUE, "measurement count out of range");
+  return std::min(p->w, p->n_nodes - k);
 }
 
 int mbqc_plan_get_info(const mbqc_plan* plan, mbqc_plan_info* info) {

@@ -25,6 +25,8 @@ struct Plan {
   int e_sv = 0, e_dm = 0;   // final growth exponents after the rescale schedule
   std::vector<mbqc_step> steps;
   std::vector<int32_t> in_index, init_sign, out_pos, out_slot;
+  std::vector<int32_t> schedule;          // node order, measured nodes first
+  std::vector<std::vector<int>> nbr;      // adjacency (applied edges and dropped ones)
   // Per-step pair coefficients (statevector kernels, n_slots <= kMaxCoefSlots):
   // for step k and pair i (the i-th slot index with the measured bit clear),
   // new[bit=1] = alpha * u0 + beta * x, stored as coef[(k * 2^ns) + 2i + {0,1}].
@@ -55,6 +57,18 @@ struct TableOffsets {
 };
 TableOffsets table_offsets(const Plan& p);
 std::vector<int32_t> pack_device_tables(const Plan& p);
+
+// Reference window after k measurements, in terms of the live slots: window
+// qubit i (0 = oldest, the most significant bit) is node schedule[k + i]; it
+// lives in register slot slot[i], or is the node the reference has already
+// appended as |+> (slot -1, the last qubit) whose CZ neighbours among the
+// window are the window-index bits nb.
+struct WindowMap {
+  int32_t W;
+  int32_t slot[kMaxSlots + 1];
+  uint32_t nb;
+};
+int window_map(const Plan& p, int k, WindowMap* m);
 
 extern thread_local std::string g_last_error;
 int fail(int code, const std::string& msg);

@@ -269,6 +269,7 @@ class _HipBackend(BaseSimulator):
                 "live window of at most 6 qubits")
         self._rng = np.random.default_rng(seed)
         self.last_outcomes = None
+        self._step_angles = []
         n_in = len(mbqcircuit.input_nodes)
         if input_state is None:
             input_state = np.ones(1)
@@ -305,13 +306,49 @@ class _HipBackend(BaseSimulator):
 
     def reset(self, input_state=None):
         self.current_measurement = 0
+        self._step_angles = []
         if input_state is not None:
             self._set_input(input_state)
         self.outcomes = {}
 
     def measure(self, angle: float, **kwargs):
-        raise NotImplementedError(
-            "step-by-step measure() is not provided by the hip backends; use run()")
+        """Measure the next scheduled node at ``angle`` (post-selected outcome 0)
+        and return ``(qstate, outcome)``, the reference's window state after the
+        step (`np_simulator_sv.py:164-225`, `np_simulator_dm.py:151-216`).
+
+        The window is recomputed on the GPU from the input through all steps so
+        far (one launch per call).
+        """
+        if self.current_measurement >= len(self.schedule_measure):
+            raise ValueError("No more measurements to be done.")
+        if not self.force0:
+            raise NotImplementedError("step-by-step measure() draws no outcomes; use run()")
+        node = self.schedule_measure[self.current_measurement]
+        ment = self.mbqcircuit[node]
+        ment.matrix(angle)                 # the reference's angle checks (ment.py:218-229)
+        if ment.plane == "X":
+            theta = 0.0
+        elif ment.plane == "Y":
+            theta = np.pi / 2
+        else:
+            theta = float(angle if angle is not None else ment.angle)
+        self._step_angles.append(theta)
+        self.current_measurement += 1
+        self.qstate = self._window(len(self._step_angles))
+        self.outcomes[node] = 0
+        return self.qstate, 0
+
+    def _window_args(self, k):
+        torch = _torch()
+        dev = self.device
+        _require_gpu(dev)
+        a = torch.tensor(self._step_angles if self._step_angles else [0.0], dtype=torch.float32,
+                         device=dev)
+        s = _as_states(self._input_np, self.plan.info["n_inputs"], dev)
+        W = int(nat.lib().mbqc_plan_window_qubits(self.plan.handle, k))
+        if W < 0:
+            nat.check(W)
+        return a, s, W
 
     def _check_run(self, angles, input_state):
         if input_state is not None:
@@ -319,7 +356,9 @@ class _HipBackend(BaseSimulator):
         if len(angles) != len(self.mbqcircuit.trainable_nodes):
             raise ValueError(f"Number of angles ({len(angles)}) does not match number of "
                              f"trainable nodes ({len(self.mbqcircuit.trainable_nodes)}).")
-        if self.current_measurement >= len(self.schedule_measure) and self.schedule_measure:
+        # mentpy's run() measures every scheduled node through measure(), so
+        # after any earlier measurement it runs out of nodes and raises
+        if self.current_measurement > 0 and self.schedule_measure:
             raise ValueError("No more measurements to be done.")
 
     def _finish_run(self, bits=None):
@@ -392,6 +431,17 @@ class HipSimulatorSV(_HipBackend):
             return np.outer(psi, np.conj(psi))
         return psi
 
+    def _window(self, k):
+        torch = _torch()
+        a, s, W = self._window_args(k)
+        out = torch.empty((1, 1, 1 << W), dtype=torch.complex64, device=s.device)
+        flags = nat.FLAG_REF_PHASE if self.phase == "reference" else 0
+        with torch.cuda.device(s.device):
+            nat.check(nat.lib().mbqc_sv_window_c64(
+                self.plan.handle, C.c_void_p(a.data_ptr()), 0, C.c_void_p(s.data_ptr()), 0, k,
+                C.c_void_p(out.data_ptr()), 1, 1, flags, _stream_ptr(s.device)))
+        return out[0, 0].cpu().numpy().astype(np.complex128)
+
     def run_batch(self, angles, input_states=None, *, phase: Optional[str] = None,
                   renormalize: Optional[bool] = None, want_phase: bool = False):
         """All-pairs batch: ``angles [B, n]`` x ``input_states [S, 2^|I|]`` ->
@@ -458,6 +508,17 @@ class HipSimulatorDM(_HipBackend):
         self._finish_run()
         self.qstate = out
         return out
+
+    def _window(self, k):
+        torch = _torch()
+        a, s, W = self._window_args(k)
+        out = torch.empty((1, 1, 1 << W, 1 << W), dtype=torch.complex64, device=s.device)
+        noise = self.noise.native()
+        with torch.cuda.device(s.device):
+            nat.check(nat.lib().mbqc_dm_window_c64(
+                self.plan.handle, C.c_void_p(a.data_ptr()), 0, C.c_void_p(s.data_ptr()), 0, k,
+                C.c_void_p(out.data_ptr()), 1, 1, C.byref(noise), _stream_ptr(s.device)))
+        return out[0, 0].cpu().numpy().astype(np.complex128)
 
     def run_batch(self, angles, input_states=None, renormalize: Optional[bool] = None):
         """``angles [B, n]`` x pure ``input_states [S, 2^|I|]`` -> ``[B, S, d, d]``."""
