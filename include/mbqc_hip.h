@@ -229,6 +229,18 @@ int64_t mbqc_batch_items(const mbqc_batch* batch);
 int mbqc_batch_launch(mbqc_batch* batch, int32_t waves, int32_t wg_waves, uint32_t flags,
                       void* stream);
 
+/* MB-QML infidelity loss and adjoint seed from the transfer matrices, fused
+ * (the Gram form of 1 - mean_s |<phi_s|T psi_s>|^2; the reference evaluates
+ * calculator.fidelity per pair, mentpy/calculator/borrows.py:4-6, inside the
+ * tutorial loss, docs/tutorials/intro-to-mbqml.rst:35-54): for each parameter
+ * set b with t_b = vec(T_b) ([B][K] complex64, K = 2^(|I|+|O|)),
+ * z = M t_b, loss[b] = 1 - Re(t_b^H z)/S, seed[b] = -2 z/S, where MT is the
+ * transposed Hermitian data Gram matrix ([K][K]).  workspace:
+ * mbqc_gram_workspace_bytes(B, K) bytes of device memory.  K <= 512. */
+size_t mbqc_gram_workspace_bytes(int64_t B, int32_t K);
+int mbqc_gram_loss_seed_c64(const void* t, const void* MT, int64_t B, int32_t K, int32_t S,
+                            float* loss, void* seed, float* workspace, void* stream);
+
 /* Device-resident optimizer steps on n float32 parameters (all pointers device
  * memory, updated in place), replacing the host updates of
  * AdamOptimizer.step (mentpy/optimizers/adam.py:54-66) and SGDOptimizer.step

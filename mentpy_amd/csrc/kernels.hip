@@ -861,20 +861,38 @@ __device__ __forceinline__ void adj_recompute(float2 (&t)[1 << NS], float2 (&ys)
   }
 }
 
+// Sum of v over aligned groups of `red` lanes (red a power of two <= 64),
+// every lane of a group ending with the group's sum.  Groups up to 16 lanes
+// use DPP moves (xor 1 and xor 2 quad permutations, then the 8- and 16-lane
+// mirrors, each pairing one half of the group with the other); wider groups
+// finish with cross-lane shuffles.
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ float lane_group_sum(float v, int red) {
+  if (red >= 2) v += dpp_move<0xB1>(v);     // quad_perm [1,0,3,2]
+  if (red >= 4) v += dpp_move<0x4E>(v);     // quad_perm [2,3,0,1]
+  if (red >= 8) v += dpp_move<0x141>(v);    // row_half_mirror
+  if (red >= 16) v += dpp_move<0x140>(v);   // row_mirror
+  if (red >= 32) v += __shfl_xor(v, 16, 64);
+  if (red >= 64) v += __shfl_xor(v, 32, 64);
+  return v;
+}
+
 template <int NS, int SUB>
 __device__ __forceinline__ void adj_back(float2 (&lam)[1 << NS], const float2 (&ys)[NS][1 << (NS - 1)],
                                          const float2 (&bs)[NS], const mbqc_step* steps,
                                          const float* coef, int k0, int n, const AdjCtx& A,
-                                         int64_t bi, int64_t ci, bool active) {
+                                         float* prow, bool writer) {
   const int k = k0 + SUB;
   if (k < n) {
     float d = sv_adj_step<NS, NS - 1 - SUB>(lam, ys[SUB], bs[SUB], coef + (k << NS));
     const int ai = steps[k].angle_index;
     if (ai >= 0) {
-      for (int m = 1; m < A.red; m <<= 1) d += __shfl_xor(d, m, 64);
-      const int64_t p = A.P == 1 ? 0 : (A.red == 64 ? ci / 64 : ci);
-      const bool writer = active && (A.red == 1 || (ci % A.red) == 0);
-      if (writer) A.gpart[(bi * A.P + p) * A.n_angles + ai] = d;
+      d = lane_group_sum(d, A.red);
+      if (writer) prow[ai] = d;
     }
   }
 }
@@ -882,12 +900,12 @@ __device__ __forceinline__ void adj_back(float2 (&lam)[1 << NS], const float2 (&
 template <int NS, int... SUBS>
 __device__ __forceinline__ void adj_block(float2 (&t)[1 << NS], float2 (&lam)[1 << NS],
                                           const mbqc_step* steps, const float* coef, int k0, int n,
-                                          const float (&th)[NS], const AdjCtx& A, int64_t bi,
-                                          int64_t ci, bool active, std::integer_sequence<int, SUBS...>) {
+                                          const float (&th)[NS], const AdjCtx& A, float* prow,
+                                          bool writer, std::integer_sequence<int, SUBS...>) {
   float2 ys[NS][1 << (NS - 1)];
   float2 bs[NS];
   (adj_recompute<NS, SUBS>(t, ys, bs, coef, k0, n, th), ...);
-  (adj_back<NS, NS - 1 - SUBS>(lam, ys, bs, steps, coef, k0, n, A, bi, ci, active), ...);
+  (adj_back<NS, NS - 1 - SUBS>(lam, ys, bs, steps, coef, k0, n, A, prow, writer), ...);
 }
 
 template <int NS>
@@ -913,6 +931,11 @@ __global__ __launch_bounds__(64) void k_sv_adj_backward(Job J, const float2* ckp
   }
   const int n = J.n_steps;
   const int nblk = (n + NS - 1) / NS;
+  // this lane's partial-sum row and whether it writes it (first lane of each
+  // group of A.red columns), fixed for the whole sweep
+  const int64_t prow_i = A.P == 1 ? 0 : (A.red == 64 ? ci / 64 : ci);
+  float* prow = A.gpart + (bi * A.P + prow_i) * A.n_angles;
+  const bool writer = active && (A.red == 1 || (ci & (A.red - 1)) == 0);
   for (int blk = nblk - 1; blk >= 0; --blk) {
     float2 t[H];
 #pragma unroll
@@ -920,13 +943,151 @@ __global__ __launch_bounds__(64) void k_sv_adj_backward(Job J, const float2* ckp
       t[j] = active ? ckpt[(int64_t(blk) * H + j) * J.total + g] : make_float2(0.f, 0.f);
     float th[NS];
     load_block_angles<NS>(steps, blk * NS, n, ang, J.angle_stride_a, th);
-    adj_block<NS>(t, lam, steps, coef, blk * NS, n, th, A, bi, ci, active,
+    adj_block<NS>(t, lam, steps, coef, blk * NS, n, th, A, prow, writer,
                   std::make_integer_sequence<int, NS>{});
     if (resc[blk]) {
 #pragma unroll
       for (int j = 0; j < H; ++j) lam[j] = cscale(lam[j], 0x1p-32f);
     }
   }
+}
+
+// ---------------------------------------------------------------- MB-QML loss
+// Infidelity loss and adjoint seed of the MB-QML cost from the transfer
+// matrices, one fused pass (the Gram form of 1 - mean_s |<phi_s|T psi_s>|^2,
+// mentpy/calculator/borrows.py:4-6 with the tutorial loss,
+// docs/tutorials/intro-to-mbqml.rst:35-54):
+//   z_b = M t_b,  loss_b = 1 - Re(t_b^H z_b) / S,  seed_b = -2 z_b / S,
+// with t_b = vec(T_b) (K = 2^(|I|+|O|) entries) and the data Gram matrix M
+// given transposed (MT[k][n] = M[n][k]).  A workgroup covers kGramSets
+// parameter sets x 64 output indices n; its 256 threads are 64 n x 4 groups
+// of kGramSets/4 sets; the sets' t rows are staged in LDS, MT is read
+// coalesced along n.  Loss partial sums go through LDS and one atomic-free
+// second pass (k_gram_loss) adds them per set.
+constexpr int kGramSets = 16;
+
+__global__ __launch_bounds__(256) void k_gram_seed(const float2* __restrict__ t,
+                                                   const float2* __restrict__ MT, int64_t B, int K,
+                                                   float inv_S, float2* __restrict__ seed,
+                                                   float* __restrict__ lpart) {
+  extern __shared__ float2 ts[];                       // [kGramSets][K]
+  constexpr int G = kGramSets / 4;
+  const int tid = threadIdx.x;
+  const int nl = tid & 63, grp = tid >> 6;
+  const int n = blockIdx.x * 64 + nl;
+  const int64_t b0 = int64_t(blockIdx.y) * kGramSets;
+  for (int e = tid; e < kGramSets * K; e += 256) {
+    const int sb = e / K, k = e - sb * K;
+    ts[e] = (b0 + sb < B) ? t[(b0 + sb) * K + k] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  // lanes past K (K < 64) stay for the wave-wide loss reduction below
+  const bool live = n < K;
+  const int nn = live ? n : 0;
+  float2 acc[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) acc[g] = make_float2(0.f, 0.f);
+  const float2* tg = ts + grp * G * K;
+  // K is a power of two >= 4: chunks of up to 16 loads of MT are issued
+  // together before their FMAs, so the loop does not wait once per k
+  constexpr int U = 16;
+  const int u = K < U ? K : U;
+  for (int k0 = 0; k0 < K; k0 += u) {
+    float2 m[U];
+#pragma unroll
+    for (int i = 0; i < U; ++i)
+      if (i < u) m[i] = MT[int64_t(k0 + i) * K + nn];
+#pragma unroll
+    for (int i = 0; i < U; ++i) {
+      if (i < u) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const float2 x = tg[g * K + k0 + i];
+          acc[g].x = fmaf(x.x, m[i].x, fmaf(-x.y, m[i].y, acc[g].x));
+          acc[g].y = fmaf(x.x, m[i].y, fmaf(x.y, m[i].x, acc[g].y));
+        }
+      }
+    }
+  }
+  const float sc = -2.0f * inv_S;
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int64_t b = b0 + grp * G + g;
+    if (b < B) {
+      if (live) seed[b * K + n] = cscale(acc[g], sc);
+      // Re(conj(t[b,n]) z[b,n]), one partial per (set, 64-wide n block)
+      const float2 x = tg[g * K + nn];
+      float part = live ? fmaf(x.x, acc[g].x, x.y * acc[g].y) : 0.0f;
+      for (int m2 = 32; m2 >= 1; m2 >>= 1) part += __shfl_xor(part, m2, 64);
+      if (nl == 0) lpart[b * gridDim.x + blockIdx.x] = part;
+    }
+  }
+}
+
+// The same on the matrix cores, for K a multiple of 16: one wave per 16 x 16
+// tile of Z (16 parameter sets x 16 indices n), complex products as four real
+// v_mfma_f32_16x16x4f32 per k-chunk of 4 (Zr += Tr Mr' - Ti Mi', Zi += Tr Mi' +
+// Ti Mr').  Operand layout of the 16x16x4 f32 MFMA: lane l holds A[l%16][l/16]
+// and B[l/16][l%16]; the accumulator holds D[4(l/16)+r][l%16], r = 0..3.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(64) void k_gram_seed_mfma(const float2* __restrict__ t,
+                                                       const float2* __restrict__ MT, int64_t B,
+                                                       int K, float inv_S,
+                                                       float2* __restrict__ seed,
+                                                       float* __restrict__ lpart) {
+  const int l = threadIdx.x;
+  const int li = l & 15, lk = l >> 4;
+  const int n0 = blockIdx.x * 16;
+  const int64_t b0 = int64_t(blockIdx.y) * 16;
+  const int64_t ba = b0 + li;                        // A row of this lane
+  const bool arow = ba < B;
+  const float2* ta = t + (arow ? ba : 0) * K + lk;
+  const float2* mb = MT + int64_t(lk) * K + n0 + li;
+  f32x4 zr = {0.f, 0.f, 0.f, 0.f}, zi = {0.f, 0.f, 0.f, 0.f};
+  constexpr int U = 8;                               // k-chunks of 4 loaded together
+  for (int k0 = 0; k0 < K; k0 += 4 * U) {
+    float2 a[U], b[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + 4 * u;
+      const bool in = k < K;
+      a[u] = (in && arow) ? ta[k] : make_float2(0.f, 0.f);
+      b[u] = in ? mb[int64_t(k) * K] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      zr = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, b[u].x, zr, 0, 0, 0);
+      zr = __builtin_amdgcn_mfma_f32_16x16x4f32(-a[u].y, b[u].y, zr, 0, 0, 0);
+      zi = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, b[u].y, zi, 0, 0, 0);
+      zi = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].y, b[u].x, zi, 0, 0, 0);
+    }
+  }
+  const float sc = -2.0f * inv_S;
+  const int n = n0 + li;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int64_t b = b0 + 4 * lk + r;              // uniform within each 16-lane group
+    float part = 0.0f;
+    if (b < B) {
+      const float2 z = make_float2(zr[r], zi[r]);
+      seed[b * K + n] = cscale(z, sc);
+      const float2 x = t[b * K + n];
+      part = fmaf(x.x, z.x, x.y * z.y);
+    }
+    // sum over the 16 lanes (indices n) of this row
+    for (int m = 8; m >= 1; m >>= 1) part += __shfl_xor(part, m, 64);
+    if (li == 0 && b < B) lpart[b * gridDim.x + blockIdx.x] = part;
+  }
+}
+
+__global__ void k_gram_loss(const float* __restrict__ lpart, int64_t B, int nblk, float inv_S,
+                            float* __restrict__ loss) {
+  const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  float s = 0.0f;
+  for (int i = 0; i < nblk; ++i) s += lpart[b * nblk + i];
+  loss[b] = 1.0f - s * inv_S;
 }
 
 // ---------------------------------------------------------------- optimizer
@@ -966,7 +1127,6 @@ __global__ void k_sgd(float* __restrict__ x, const float* __restrict__ g, float*
   x[i] += nesterov ? fmaf(momentum, vi, -lr * gi) : vi;
 }
 
-// ---------------------------------------------------------------- batched
 // ------------------------------------------------------- step-by-step windows
 // Window index r -> live register index j, the appended qubit's bit a and the
 // sign of its CZs (see WindowMap).
@@ -1085,6 +1245,7 @@ __global__ __launch_bounds__(64) void k_dm_window(Job J, const float* __restrict
   }
 }
 
+// ---------------------------------------------------------------- batched
 struct BatchItem {
   int32_t job;
   int32_t count;   // simulations in this item
@@ -1261,7 +1422,9 @@ hipError_t launch_dm(const Job& J, const DmNoise& nz, uint32_t flags, hipStream_
 
 template <int NS>
 hipError_t launch_adj_fwd(const Job& J, float2* ckpt, hipStream_t st) {
-  const int threads = 256;
+  // one wave per workgroup: at small batches the waves spread over all CUs
+  // instead of sharing a few CUs' scalar units and instruction issue
+  const int threads = 64;
   const int64_t blocks = (J.total + threads - 1) / threads;
   hipLaunchKernelGGL((k_sv_adj_forward<NS>), dim3(blocks), dim3(threads), 0, st, J, ckpt);
   return hipGetLastError();
@@ -1680,6 +1843,41 @@ void mbqc_batch_destroy(mbqc_batch* bt) {
 }
 
 int64_t mbqc_batch_items(const mbqc_batch* bt) { return bt ? int64_t(bt->items.size()) : -1; }
+
+size_t mbqc_gram_workspace_bytes(int64_t B, int32_t K) {
+  if (B < 0 || K <= 0) return 0;
+  return size_t(B) * size_t((K + 15) / 16) * sizeof(float);
+}
+
+int mbqc_gram_loss_seed_c64(const void* t, const void* MT, int64_t B, int32_t K, int32_t S,
+                            float* loss, void* seed, float* workspace, void* stream) {
+  if (!t || !MT || !loss || !seed || !workspace) return fail(MBQC_ERR_VALUE, "null pointer");
+  if (B < 0 || K <= 0 || S <= 0) return fail(MBQC_ERR_VALUE, "sizes must be B >= 0, K >= 1, S >= 1");
+  if (int64_t(kGramSets) * K * int64_t(sizeof(float2)) > 64 * 1024)
+    return fail(MBQC_ERR_NOT_IMPLEMENTED, "Gram loss supports K <= 512");
+  if (B == 0) return MBQC_OK;
+  if ((B + 15) / 16 > 65535) return fail(MBQC_ERR_VALUE, "too many parameter sets");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const float inv_S = 1.0f / float(S);
+  int nblk;
+  if (K % 16 == 0) {
+    nblk = K / 16;
+    hipLaunchKernelGGL(k_gram_seed_mfma, dim3(nblk, unsigned((B + 15) / 16)), dim3(64), 0, st,
+                       static_cast<const float2*>(t), static_cast<const float2*>(MT), B, K, inv_S,
+                       static_cast<float2*>(seed), workspace);
+  } else {
+    nblk = (K + 63) / 64;
+    const size_t lds = size_t(kGramSets) * K * sizeof(float2);
+    hipLaunchKernelGGL(k_gram_seed, dim3(nblk, unsigned((B + kGramSets - 1) / kGramSets)),
+                       dim3(256), lds, st, static_cast<const float2*>(t),
+                       static_cast<const float2*>(MT), B, K, inv_S, static_cast<float2*>(seed),
+                       workspace);
+  }
+  hipLaunchKernelGGL(k_gram_loss, dim3(unsigned((B + 255) / 256)), dim3(256), 0, st, workspace, B,
+                     nblk, inv_S, loss);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? MBQC_OK : hip_fail(e, "gram loss launch");
+}
 
 int mbqc_adam_step_f32(float* x, const float* grad, float* m, float* v, int64_t n, float lr, float b1,
                        float b2, float eps, int32_t* step, void* stream) {

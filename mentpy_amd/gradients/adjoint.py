@@ -141,6 +141,11 @@ def _basis(n_in, device):
     return torch.eye(1 << n_in, dtype=torch.complex64, device=device)
 
 
+# Parameter-set counts up to which the fused VALU Gram kernel is used (above
+# it the library GEMM has enough tiles to fill the chip).
+_FUSED_GRAM_MAX_SETS = 8192
+
+
 class MBQMLStep:
     """Fixed-shape MB-QML loss-and-gradient step for ``B`` parameter sets.
 
@@ -161,7 +166,8 @@ class MBQMLStep:
     (``capture``); write new angles into ``self.angles`` before each replay.
     """
 
-    def __init__(self, sim, inputs, targets, n_sets: int, mode: str = "auto"):
+    def __init__(self, sim, inputs, targets, n_sets: int, mode: str = "auto",
+                 gram_kernel: str = "auto"):
         torch = _torch()
         self.engine = _engine_for(sim)
         eng = self.engine
@@ -192,6 +198,15 @@ class MBQMLStep:
             M = torch.matmul(D.conj().transpose(0, 1).contiguous(), D)
             self.MT = M.transpose(0, 1).contiguous()               # Z = t @ M^T = (M t)^T
             self.Z = torch.empty((self.B, K), dtype=torch.complex64, device=dev)
+            if gram_kernel == "auto":
+                gram_kernel = "fused" if (K <= 512 and self.B <= _FUSED_GRAM_MAX_SETS) else "blas"
+            if gram_kernel not in ("fused", "blas"):
+                raise ValueError("gram_kernel must be 'auto', 'fused' or 'blas'")
+            if gram_kernel == "fused" and K > 512:
+                raise NotImplementedError("the fused Gram kernel supports K <= 512")
+            self.gram_kernel = gram_kernel
+            nbytes = int(nat.lib().mbqc_gram_workspace_bytes(self.B, K))
+            self._gws = torch.empty(max(nbytes // 4, 1), dtype=torch.float32, device=dev)
         self._ws, self._need = eng._workspace(self.B * C_)
 
     def _launch_forward(self):
@@ -224,7 +239,13 @@ class MBQMLStep:
         with torch.cuda.device(self.engine.device):
             self._launch_forward()
             S = self.S
-            if self.mode == "gram":
+            if self.mode == "gram" and self.gram_kernel == "fused":
+                stream = C.c_void_p(torch.cuda.current_stream(self.engine.device).cuda_stream)
+                nat.check(nat.lib().mbqc_gram_loss_seed_c64(
+                    C.c_void_p(self.out.data_ptr()), C.c_void_p(self.MT.data_ptr()), self.B,
+                    self.K, S, C.c_void_p(self.loss.data_ptr()), C.c_void_p(self.seed.data_ptr()),
+                    C.c_void_p(self._gws.data_ptr()), stream))
+            elif self.mode == "gram":
                 t = self.out.reshape(self.B, self.K)
                 torch.matmul(t, self.MT, out=self.Z)
                 self.loss.copy_(1.0 - (t.conj() * self.Z).sum(1).real / S)
