@@ -64,23 +64,38 @@ __device__ __forceinline__ void sincos_poly(float x, float* sn, float* cs) {
   *cs = __uint_as_float(__float_as_uint(c0) ^ (uint32_t((q + 1) & 2) << 30));
 }
 
-// e^{-i theta}; with the reference phase also the per-step global phase
-// sgn(cos(theta/2)) e^{i theta/2} (the factor the reference's amplitude-sum
-// partial trace introduces, mentpy/calculator/state_ops.py:68-73).
-template <bool REF>
-__device__ __forceinline__ float2 step_coeff(float th, float2* ph) {
-  if constexpr (REF) {
-    float s2, c2;
-    sincos_poly(0.5f * th, &s2, &c2);
-    const float2 b = make_float2(fmaf(c2, c2, -s2 * s2), -2.0f * s2 * c2);
-    const float sg = c2 < 0.0f ? -1.0f : 1.0f;
-    *ph = cmul(*ph, make_float2(fabsf(c2), sg * s2));
-    return b;
-  } else {
-    float s, c;
-    sincos_poly(th, &s, &c);
-    return make_float2(c, -s);
+// The reference's global phase, prod_k sgn(cos(theta_k/2)) e^{i theta_k/2}
+// (the factor its amplitude-sum partial trace introduces,
+// mentpy/calculator/state_ops.py:68-73).  With m_k = rint(theta_k / 2 pi),
+// theta_k/2 - pi m_k lies in [-pi/2, pi/2], so sgn(cos(theta_k/2)) = (-1)^m_k
+// (sgn(0) = +1: rint rounds half to even) and the product is
+// exp(i (sum_k theta_k / 2 - pi sum_k m_k)).  The angles are summed in double
+// (full-rate on this GPU) and the integers m_k in float (exact), and the phase
+// is formed once at the end.
+struct RefPhase {
+  double th_sum = 0.0;
+  float m_sum = 0.0f;
+  __device__ __forceinline__ void add(float th) {
+    m_sum += rintf(th * 0.15915494309189533577f);
+    th_sum += double(th);
   }
+  __device__ __forceinline__ float2 value() const {
+    constexpr double kPi = 3.14159265358979323846;
+    double a = 0.5 * th_sum - kPi * double(m_sum);
+    a -= 2.0 * kPi * rint(a * (0.5 / kPi));
+    double sd, cd;
+    sincos(a, &sd, &cd);
+    return make_float2(float(cd), float(sd));
+  }
+};
+
+// b = e^{-i theta}; with REF the step's reference phase is accumulated too.
+template <bool REF>
+__device__ __forceinline__ float2 step_coeff(float th, RefPhase* ph) {
+  if constexpr (REF) ph->add(th);
+  float s, c;
+  sincos_poly(th, &s, &c);
+  return make_float2(c, -s);
 }
 
 // ---------------------------------------------------------------- SV step
@@ -133,7 +148,7 @@ struct NoModify {
 
 template <int NS, bool REF, bool FULL, int SUB, typename Mod>
 __device__ __forceinline__ void sv_sub(float2 (&t)[1 << NS], const float* coef, int k0,
-                                       int n_steps, const float (&th)[NS], float2* ph, Mod& mod) {
+                                       int n_steps, const float (&th)[NS], RefPhase* ph, Mod& mod) {
   const int k = k0 + SUB;
   if (FULL || k < n_steps) {
     const float2 b = mod(k, step_coeff<REF>(th[SUB], ph));
@@ -143,7 +158,7 @@ __device__ __forceinline__ void sv_sub(float2 (&t)[1 << NS], const float* coef, 
 
 template <int NS, bool REF, bool FULL, typename Mod, int... SUBS>
 __device__ __forceinline__ void sv_block(float2 (&t)[1 << NS], const float* coef, int k0,
-                                         int n_steps, const float (&th)[NS], float2* ph, Mod& mod,
+                                         int n_steps, const float (&th)[NS], RefPhase* ph, Mod& mod,
                                          std::integer_sequence<int, SUBS...>) {
   (sv_sub<NS, REF, FULL, SUBS>(t, coef, k0, n_steps, th, ph, mod), ...);
 }
@@ -172,7 +187,7 @@ __device__ __forceinline__ void fetch_block_angles(const mbqc_step* steps, int k
 // the start of every block (after its rescale).
 template <int NS, bool REF, typename OnBlock, typename Mod = NoModify>
 __device__ __forceinline__ void sv_run_steps(float2 (&t)[1 << NS], const Job& J, const float* ang,
-                                             float2* ph, OnBlock on_block, Mod&& mod = Mod()) {
+                                             RefPhase* ph, OnBlock on_block, Mod&& mod = Mod()) {
   constexpr int H = 1 << NS;
   const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
   const int32_t* resc = J.tab + J.off_svres;
@@ -244,10 +259,9 @@ __device__ __forceinline__ void sv_sim(const Job& J, int64_t g, bool active, uin
   const float* ang = J.angles + bi * J.angle_stride;
   float2 t[H];
   sv_load_input<NS>(J, J.psi + si * J.in_stride, t);
-  float2 ph = make_float2(1.0f, 0.0f);
-  sv_run_steps<NS, REF>(t, J, ang, &ph, [](int, const float2 (&)[H]) {});
-  // the accumulated phase drifts off unit modulus in float; renormalise it
-  ph = cscale(ph, 1.0f / sqrtf(fmaf(ph.x, ph.x, ph.y * ph.y)));
+  RefPhase rp;
+  sv_run_steps<NS, REF>(t, J, ang, &rp, [](int, const float2 (&)[H]) {});
+  const float2 ph = REF ? rp.value() : make_float2(1.0f, 0.0f);
   // output gather: slot-index j -> output index out_pos[j] (dead slots skipped)
   const int32_t* out_pos = J.tab + J.off_out;
   float scale = sv_final_scale(J.e_sv);
@@ -385,8 +399,7 @@ __device__ __forceinline__ void dm_sub(float2 (&r)[1 << NS][1 << NS], const mbqc
                                        int n_steps, const float (&th)[NS], float gamma) {
   const int k = k0 + SUB;
   if (k < n_steps) {
-    float2 ph = make_float2(1.f, 0.f);
-    const float2 b = step_coeff<false>(th[SUB], &ph);
+    const float2 b = step_coeff<false>(th[SUB], nullptr);
     dm_step<NS, NS - 1 - SUB>(r, cscale(b, gamma), steps[k].flags, steps[k].nmask);
   }
 }
@@ -517,7 +530,7 @@ __global__ __launch_bounds__(256) void k_sv_sample(Job J, const uint64_t* __rest
   const float* ang = J.angles + bi * J.angle_stride;
   float2 t[H];
   sv_load_input<NS>(J, J.psi + si * J.in_stride, t);
-  float2 ph = make_float2(1.0f, 0.0f);
+  RefPhase ph;
   SampleMod mod;
   mod.seed = seed;
   mod.sim = uint64_t(gg);
@@ -667,7 +680,7 @@ __global__ __launch_bounds__(T) void k_sv_forward_lds(Job J, uint32_t flags) {
   const int32_t* out_pos = J.tab + J.off_out;
   const float2* psi = J.psi + si * J.in_stride;
   for (int j = tid; j < H; j += T) st[j] = flip(psi[in_index[j]], sign[j] ? 0x80000000u : 0u);
-  float2 ph = make_float2(1.0f, 0.0f);
+  RefPhase rp;
   const bool ref = flags & MBQC_FLAG_REF_PHASE;
   for (int k = 0; k < J.n_steps; ++k) {
     if (k % NS == 0 && resc[k / NS]) {
@@ -677,12 +690,12 @@ __global__ __launch_bounds__(T) void k_sv_forward_lds(Job J, uint32_t flags) {
     const mbqc_step step = steps[k];
     const float th = step.angle_index >= 0 ? ang[int64_t(step.angle_index) * J.angle_stride_a]
                                            : step.fixed_angle;
-    const float2 b = ref ? step_coeff<true>(th, &ph) : step_coeff<false>(th, &ph);
+    const float2 b = ref ? step_coeff<true>(th, &rp) : step_coeff<false>(th, nullptr);
     __syncthreads();
     sv_lds_step<T>(st, H, step, b, tid);
   }
   __syncthreads();
-  ph = cscale(ph, 1.0f / sqrtf(fmaf(ph.x, ph.x, ph.y * ph.y)));
+  const float2 ph = ref ? rp.value() : make_float2(1.0f, 0.0f);
   float scale = sv_final_scale(J.e_sv);
   if (flags & MBQC_FLAG_RENORMALIZE) {
     float n2 = 0.0f;
@@ -730,8 +743,7 @@ __global__ __launch_bounds__(T) void k_dm_forward_lds(Job J, DmNoise nz, uint32_
     const mbqc_step step = steps[k];
     const float th = step.angle_index >= 0 ? ang[int64_t(step.angle_index) * J.angle_stride_a]
                                            : step.fixed_angle;
-    float2 ph = make_float2(1.f, 0.f);
-    const float2 gb = cscale(step_coeff<false>(th, &ph), nz.gamma);
+    const float2 gb = cscale(step_coeff<false>(th, nullptr), nz.gamma);
     __syncthreads();
     dm_lds_step<T>(r, H, step, gb, tid);
   }
@@ -781,7 +793,7 @@ __global__ __launch_bounds__(256) void k_sv_adj_forward(Job J, float2* ckpt) {
   const float* ang = J.angles + bi * J.angle_stride;
   float2 t[H];
   sv_load_input<NS>(J, J.psi + si * J.in_stride, t);
-  float2 ph = make_float2(1.f, 0.f);
+  RefPhase ph;
   const int64_t total = J.total;
   sv_run_steps<NS, false>(t, J, ang, &ph, [&](int blk, const float2 (&tt)[H]) {
     if (active) {
@@ -854,8 +866,7 @@ __device__ __forceinline__ void adj_recompute(float2 (&t)[1 << NS], float2 (&ys)
                                               const float (&th)[NS]) {
   const int k = k0 + SUB;
   if (k < n) {
-    float2 ph = make_float2(1.f, 0.f);
-    bs[SUB] = step_coeff<false>(th[SUB], &ph);
+    bs[SUB] = step_coeff<false>(th[SUB], nullptr);
     take_y<NS, NS - 1 - SUB>(t, ys[SUB]);
     if (SUB + 1 < NS) sv_step<NS, NS - 1 - SUB>(t, bs[SUB], coef + (k << NS));
   }
@@ -1164,7 +1175,7 @@ __global__ __launch_bounds__(64) void k_sv_window(Job J, const float* __restrict
   const int32_t* resc = J.tab + J.off_svres;
   const float2* psi = J.psi + si * J.in_stride;
   for (int j = tid; j < H; j += T) st[j] = flip(psi[in_index[j]], sign[j] ? 0x80000000u : 0u);
-  float2 ph = make_float2(1.0f, 0.0f);
+  RefPhase rp;
   const bool ref = flags & MBQC_FLAG_REF_PHASE;
   for (int k = 0; k < k_steps; ++k) {
     if (k % ns == 0 && resc[k / ns]) {
@@ -1172,12 +1183,12 @@ __global__ __launch_bounds__(64) void k_sv_window(Job J, const float* __restrict
       for (int j = tid; j < H; j += T) st[j] = cscale(st[j], 0x1p-32f);
     }
     const mbqc_step step = steps[k];
-    const float2 b = ref ? step_coeff<true>(ang[k], &ph) : step_coeff<false>(ang[k], &ph);
+    const float2 b = ref ? step_coeff<true>(ang[k], &rp) : step_coeff<false>(ang[k], nullptr);
     __syncthreads();
     sv_lds_step<T>(st, H, step, b, tid);
   }
   __syncthreads();
-  ph = cscale(ph, 1.0f / sqrtf(fmaf(ph.x, ph.x, ph.y * ph.y)));
+  const float2 ph = ref ? rp.value() : make_float2(1.0f, 0.0f);
   float n2 = 0.0f;
   for (int j = tid; j < H; j += T) n2 = fmaf(st[j].x, st[j].x, fmaf(st[j].y, st[j].y, n2));
   n2 = block_sum<T>(n2, red);
@@ -1223,8 +1234,7 @@ __global__ __launch_bounds__(64) void k_dm_window(Job J, const float* __restrict
       for (int e = tid; e < H * H; e += T) r[e] = cscale(r[e], 0x1p-32f);
     }
     const mbqc_step step = steps[k];
-    float2 ph = make_float2(1.f, 0.f);
-    const float2 gb = cscale(step_coeff<false>(ang[k], &ph), nz.gamma);
+    const float2 gb = cscale(step_coeff<false>(ang[k], nullptr), nz.gamma);
     __syncthreads();
     dm_lds_step<T>(r, H, step, gb, tid);
   }
