@@ -317,6 +317,39 @@ def _vstack2(a: MBQCircuit, b: MBQCircuit) -> MBQCircuit:
                       a.output_nodes + [i + n for i in b.output_nodes], measurements=meas)
 
 
+def merge(state1: MBQCircuit, state2: MBQCircuit, along=()) -> MBQCircuit:
+    """Glue ``state2`` onto ``state1`` by identifying output ``i`` of the first
+    with input ``j`` of the second for each ``(i, j)`` in ``along``
+    (`mentpy/mbqc/mbqcircuit.py:459-497`): the merged node keeps the second
+    pattern's label (shifted by ``len(state1.graph)``) and its measurement; the
+    remaining inputs of the second pattern become inputs, and each merged
+    output of the first is replaced by the matching output of the second."""
+    along = list(along)
+    for i, j in along:
+        if i not in state1.output_nodes or j not in state2.input_nodes:
+            raise ValueError(f"Cannot merge states at indices {i} and {j}")
+    n1 = len(state1.graph)
+    graph = nx.disjoint_union(state1.graph, state2.graph)
+    along1, along2 = zip(*along) if along else ((), ())
+    inputs = list(state1.input_nodes) + [i + n1 for i in state2.input_nodes if i not in along2]
+    outputs, used = [], []
+    for j in state1.output_nodes:
+        if j in along1:
+            k = list(state2.input_nodes).index(along2[along1.index(j)])
+            used.append(k)
+            outputs.append(state2.output_nodes[k] + n1)
+        else:
+            outputs.append(j)
+    outputs += [i + n1 for k, i in enumerate(state2.output_nodes) if k not in used]
+    meas = dict(state1.measurements)
+    meas.update({i + n1: m for i, m in state2.measurements.items()})
+    for i, j in along:
+        graph.add_edge(i, j + n1)
+        graph = nx.contracted_edge(graph, (j + n1, i), self_loops=False)
+        del meas[i]
+    return MBQCircuit(GraphState(graph), inputs, outputs, measurements=meas)
+
+
 def hstack(states):
     from functools import reduce
     if len(states) == 0:

@@ -1,18 +1,19 @@
 """Resource-graph templates used by the simulate path.
 
-``linear_cluster``, ``many_wires`` and ``grid_cluster`` build the same graphs,
-node numbering, edge insertion order and I/O nodes as mentpy
-(`mentpy/mbqc/templates.py:16-147`); insertion order matters because it fixes
-``trainable_nodes`` (the angle layout) and the tie order of the schedule.
+``linear_cluster``, ``many_wires``, ``grid_cluster``, ``muta`` and ``spturb``
+build the same graphs, node numbering, edge insertion order and I/O nodes as
+mentpy (`mentpy/mbqc/templates.py:16-294`); insertion order matters because it
+fixes ``trainable_nodes`` (the angle layout) and the tie order of the schedule.
 """
 from __future__ import annotations
 
 from typing import List
 
+from ..operators.ment import Ment
 from .graphstate import GraphState
-from .mbqcircuit import MBQCircuit
+from .mbqcircuit import MBQCircuit, hstack, merge
 
-__all__ = ["linear_cluster", "many_wires", "grid_cluster"]
+__all__ = ["linear_cluster", "many_wires", "grid_cluster", "muta", "spturb"]
 
 
 def linear_cluster(n: int, **kwargs) -> MBQCircuit:
@@ -52,3 +53,64 @@ def grid_cluster(n: int, m: int, periodic: bool = False, **kwargs) -> MBQCircuit
             g.add_edge(j, (n - 1) * m + j)
     return MBQCircuit(g, input_nodes=[i * m for i in range(n)],
                       output_nodes=[i * m + m - 1 for i in range(n)], **kwargs)
+
+
+def muta(n_wires: int, n_layers: int, **kwargs) -> MBQCircuit:
+    """Multiple Triangle Ansatz (`mentpy/mbqc/templates.py:150-216`): per wire a
+    block of ``n_wires`` 5-node wires whose second node is joined to the first
+    and third node of every other wire; blocks are stacked horizontally over
+    wires, then over layers.  As in mentpy, each ``add_edge`` recompiles the
+    pattern, which resets the measurements (so ``restrict_trainable`` has no
+    lasting effect)."""
+    options = {"restrict_trainable": True, "one_column": False}
+    options.update(kwargs)
+    size = 5
+    big = None
+    for wire in range(n_wires):
+        if options["one_column"] and wire != 0:
+            break
+        g = many_wires([size] * n_wires)
+        if options["restrict_trainable"]:
+            g.trainable_nodes = list(set(g.trainable_nodes) - set(i - 1 for i in g.output_nodes))
+        for connect in range(n_wires):
+            if connect != wire:
+                g.add_edge(size * wire + 1, size * connect)
+                g.add_edge(size * wire + 1, size * connect + 2)
+        big = g if big is None else hstack((big, g))
+    out = None
+    for _ in range(n_layers):
+        out = big if out is None else hstack((out, big))
+    return out
+
+
+def spturb(n_qubits: int, n_layers: int, periodic: bool = False, **kwargs) -> MBQCircuit:
+    """Symmetry Protected Topological Perturbator Ansatz
+    (`mentpy/mbqc/templates.py:219-294`): 3-node wires with a trainable middle
+    node, and per layer two rounds of 12-node symmetry blocks merged onto
+    output pairs (i, i+2)."""
+    if n_qubits < 4:
+        raise ValueError("n_qubits must be greater than 4")
+    gr = many_wires([5, 2, 5]).graph
+    gr.add_edge(2, 6)
+    gr.add_edge(9, 6)
+    block1 = MBQCircuit(gr, input_nodes=[0, 7], output_nodes=[4, 11],
+                        measurements={5: Ment(plane="XY")}, default_measurement=Ment(plane="X"))
+    block2 = MBQCircuit(gr, input_nodes=[0, 7], output_nodes=[4, 11],
+                        measurements={5: Ment(plane="XY"), 1: Ment(plane="Y"), 3: Ment(plane="Y"),
+                                      8: Ment(plane="Y"), 10: Ment(plane="Y")},
+                        default_measurement=Ment(plane="X"))
+
+    def layer():
+        return many_wires([3] * n_qubits, measurements={3 * i + 1: Ment() for i in range(n_qubits)},
+                          default_measurement=Ment(plane="X"))
+    spt = layer()
+    n_blocks = n_qubits if periodic else n_qubits - 2
+    for lay in range(n_layers):
+        if lay != 0:
+            spt = hstack((spt, layer()))
+        for m in range(2):
+            for i in range(n_blocks):
+                n1 = spt.output_nodes[i]
+                n2 = spt.output_nodes[(i + 2) % n_qubits]
+                spt = merge(spt, block1 if m % 2 == 0 else block2, [(n1, 0), (n2, 7)])
+    return spt

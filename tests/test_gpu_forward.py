@@ -78,14 +78,18 @@ def test_dm_large_windows_lds_kernel(rows, cols, rng):
     assert err < AMP_TOL
 
 
-@pytest.mark.parametrize("rows,cols", [(2, 4), (2, 5)])
-def test_random_measurements_equal_across_backends(rows, cols, rng):
-    """mentpy tests/test_simulators.py:33-64: window_size=min(5, #trainable),
-    two nodes fixed to X, hip-sv and hip-dm agree with the oracle."""
+@pytest.mark.parametrize("template", ["grid-2-4", "grid-2-5", "linear-4", "linear-5", "muta-2-1"])
+def test_random_measurements_equal_across_backends(template, rng):
+    """mentpy tests/test_simulators.py:33-64 (grid_cluster(2, 3+i),
+    linear_cluster(3+i), muta(2, 1)): window_size=min(5, #trainable), two
+    nodes fixed to X, hip-sv and hip-dm agree with the oracle."""
     _torch()
     import mentpy_amd as mp
     from oracle.reference_sv import ReferenceSV
-    g = mp.templates.grid_cluster(rows, cols)
+    kind, *args = template.split("-")
+    args = [int(a) for a in args]
+    g = {"grid": mp.templates.grid_cluster, "linear": mp.templates.linear_cluster,
+         "muta": mp.templates.muta}[kind](*args)
     idx = rng.choice(g.outputc, 2, replace=False)
     g[int(idx[0])] = mp.Ment("X")
     g[int(idx[1])] = mp.Ment("X")

@@ -340,3 +340,25 @@ def test_haar_fidelity_density():
     f = np.abs(np.sum(np.conj(a) * b, -1)) ** 2
     cdf = lambda x: 1 - (1 - x) ** 3          # integral of 3 (1-F)^2
     assert stats.kstest(f, cdf).pvalue > 1e-3
+
+
+@pytest.mark.parametrize("n_layer", [1, 2])
+@pytest.mark.parametrize("n_qubits", [4, 5])
+@pytest.mark.parametrize("periodic", [True, False])
+def test_spturb_trainable_nodes(n_layer, n_qubits, periodic):
+    """mentpy tests/mbqc/test_mbqc_templates.py:13-24."""
+    spt = mp.templates.spturb(n_qubits, n_layer, periodic=periodic)
+    periodic_params = n_qubits if periodic else n_qubits - 2
+    assert len(spt.trainable_nodes) == n_qubits * n_layer + 2 * n_layer * periodic_params
+
+
+def test_muta_oracle_matches_brute_force(rng):
+    """muta(2, 1) (the third template of mentpy's cross-backend test): the
+    windowed restatement equals the gate-order brute force at window 5."""
+    g = mp.templates.muta(2, 1)
+    assert g.flow is not None and len(g.trainable_nodes) == 16
+    th = rng.uniform(0, 2 * np.pi, len(g.trainable_nodes))
+    psi = haar_state(2, rng)
+    out = ReferenceSV(g, psi, window_size=5).run(th, output_form="sv")
+    want = brute_sv(g, th, psi)
+    assert infidelity(out, want) < 1e-12
