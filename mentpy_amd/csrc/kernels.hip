@@ -784,7 +784,9 @@ __global__ __launch_bounds__(T) void k_dm_forward_lds(Job J, DmNoise nz, uint32_
 // of the live state at the start of every block of NS steps (after the
 // block's rescale), stored column-fastest for coalesced access.
 template <int NS>
-__global__ __launch_bounds__(256) void k_sv_adj_forward(Job J, float2* ckpt) {
+// ckpt is __restrict__ so the checkpoint stores inside the step loop do not
+// count as possible writes to the plan tables (which keeps table reads scalar).
+__global__ __launch_bounds__(256) void k_sv_adj_forward(Job J, float2* __restrict__ ckpt) {
   constexpr int H = 1 << NS;
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const bool active = g < J.total;
@@ -920,8 +922,12 @@ __device__ __forceinline__ void adj_block(float2 (&t)[1 << NS], float2 (&lam)[1 
 }
 
 template <int NS>
-__global__ __launch_bounds__(64) void k_sv_adj_backward(Job J, const float2* ckpt, const float2* seed,
-                                                        AdjCtx A) {
+// The pointers are __restrict__ (and the partial sums come in as their own
+// argument): the per-step partial-sum stores would otherwise count as possible
+// writes to the plan tables, turning every table read into a vector load.
+__global__ __launch_bounds__(64) void k_sv_adj_backward(Job J, const float2* __restrict__ ckpt,
+                                                        const float2* __restrict__ seed, AdjCtx A,
+                                                        float* __restrict__ gpart) {
   constexpr int H = 1 << NS;
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const bool active = g < J.total;
@@ -945,7 +951,7 @@ __global__ __launch_bounds__(64) void k_sv_adj_backward(Job J, const float2* ckp
   // this lane's partial-sum row and whether it writes it (first lane of each
   // group of A.red columns), fixed for the whole sweep
   const int64_t prow_i = A.P == 1 ? 0 : (A.red == 64 ? ci / 64 : ci);
-  float* prow = A.gpart + (bi * A.P + prow_i) * A.n_angles;
+  float* prow = gpart + (bi * A.P + prow_i) * A.n_angles;
   const bool writer = active && (A.red == 1 || (ci & (A.red - 1)) == 0);
   for (int blk = nblk - 1; blk >= 0; --blk) {
     float2 t[H];
@@ -1445,7 +1451,8 @@ hipError_t launch_adj_bwd(const Job& J, const float2* ckpt, const float2* seed, 
                           hipStream_t st) {
   const int threads = 64;
   const int64_t blocks = (J.total + threads - 1) / threads;
-  hipLaunchKernelGGL((k_sv_adj_backward<NS>), dim3(blocks), dim3(threads), 0, st, J, ckpt, seed, A);
+  hipLaunchKernelGGL((k_sv_adj_backward<NS>), dim3(blocks), dim3(threads), 0, st, J, ckpt, seed, A,
+                     A.gpart);
   return hipGetLastError();
 }
 
