@@ -408,8 +408,9 @@ def main():
     ap.add_argument("--dm-batch", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(multi-GPU sharding over angle vectors is described in DESIGN.md)"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 (simulations "
+                          "are independent, so several GPUs would each take a share of the "
+                          "angle vectors)"}))
         sys.exit(2)
 
     import torch

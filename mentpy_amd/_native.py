@@ -89,8 +89,7 @@ def lib() -> C.CDLL:
     import torch  # noqa: F401
     if not LIB_PATH.exists():
         raise RuntimeError(
-            f"native library {LIB_PATH} is not built; run `python mentpy_amd/csrc/build.py` "
-            "(or __graft_entry__.build()) first")
+            f"native library {LIB_PATH} is not built; run `python mentpy_amd/csrc/build.py` first")
     L = C.CDLL(str(LIB_PATH))
     vp, i32, i64, u32, sz = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_size_t
     L.mbqc_plan_create.argtypes = [C.POINTER(PatternDesc), C.POINTER(vp)]

@@ -6,8 +6,9 @@
 partial order ``layer[u] > layer[v]`` (`flow.py:147`); ``trainable_nodes``
 follows the default-measurement rule (`mbqcircuit.py:110-113, 325-353`).
 Written as a recursive restatement (like the reference) so it shares no code
-with ``mentpy_amd.mbqc``.  Pinned by the grid closed form: layers are columns,
-order is column-major with rows ascending (proved in the survey of the path).
+with ``mentpy_amd.mbqc``.  Pinned by the grid closed form: layers are columns
+(the peeling removes one column per round, outputs first), so the order is
+column-major with rows ascending.
 """
 from __future__ import annotations
 

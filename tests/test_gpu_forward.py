@@ -1,8 +1,8 @@
 """GPU parity of the hip-sv / hip-dm forward kernels against the complex128 oracle.
 
 Tolerances (complex64 arithmetic, float32 angles): max amplitude error 2e-5
-in the reference phase convention, output infidelity 1e-5 (the north-star
-bar), DM max entry error 2e-5 and trace distance 1e-5.
+in the reference phase convention, output infidelity 1e-5 (the fidelity
+target against complex128), DM max entry error 2e-5 and trace distance 1e-5.
 """
 import numpy as np
 import pytest
