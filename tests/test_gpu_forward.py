@@ -198,3 +198,22 @@ def test_dm_grid_matches_oracle(rows, cols, noise, rng):
     print(f"dm {rows}x{cols} noise {noise}: max err {err:.3e} trace dist {td:.3e}", flush=True)
     assert err < AMP_TOL
     assert td < 1e-5
+
+
+def test_golden_teleportation_cases(rng):
+    """mentpy tests/test_simulators.py:15-30 cases from tests/golden/known_answers.json
+    on both GPU backends."""
+    _torch()
+    import json
+    from pathlib import Path
+    import mentpy_amd as mp
+    data = json.loads((Path(__file__).parent / "golden" / "known_answers.json").read_text())
+    for case in data["teleportation"]:
+        g = mp.templates.linear_cluster(case["n_nodes"])
+        for backend in ("hip-sv", "hip-dm"):
+            ps = mp.PatternSimulator(g, backend=backend)
+            for _ in range(3):
+                st = haar_state(1, rng)
+                ps.reset(input_state=st)
+                out = ps(case["angles"])
+                assert np.allclose(out, np.outer(st, st.conj()), atol=case["atol"]), backend

@@ -362,3 +362,31 @@ def test_muta_oracle_matches_brute_force(rng):
     out = ReferenceSV(g, psi, window_size=5).run(th, output_form="sv")
     want = brute_sv(g, th, psi)
     assert infidelity(out, want) < 1e-12
+
+
+def _load_golden():
+    import json
+    from pathlib import Path
+    data = json.loads((Path(__file__).parent / "golden" / "known_answers.json").read_text())
+
+    def arr(d):
+        return (np.asarray(d["re"]) + 1j * np.asarray(d["im"])).reshape(d["shape"])
+    return data, arr
+
+
+def test_golden_known_answers_fixture(rng):
+    """The literal cases of mentpy's known-answer tests (tests/golden/known_answers.json)
+    against the oracle and the pattern model."""
+    data, arr = _load_golden()
+    for case in data["pure2density"]:
+        assert np.allclose(pure2density(arr(case["psi"])), arr(case["rho"]))
+    for case in data["partial_trace"]:
+        assert np.allclose(partial_trace(arr(case["state"]), case["indices"]), arr(case["expected"]))
+    for case in data["ment_matrix"]:
+        assert np.allclose(mp.Ment(case["angle"], case["plane"]).matrix(), arr(case["matrix"]))
+    for case in data["teleportation"]:
+        g = mp.templates.linear_cluster(case["n_nodes"])
+        st = haar_state(1, rng)
+        for cls in (ReferenceSV, ReferenceDM):
+            out = cls(g, st).run(case["angles"])
+            assert np.allclose(out, np.outer(st, st.conj()), atol=case["atol"])
