@@ -602,6 +602,23 @@ __device__ __forceinline__ float block_sum(float v, float* red) {
   return v;
 }
 
+// Angles of the workgroup's simulation for steps [k0, k0 + kAngleChunk), read
+// by all threads into LDS at once, so a step does not wait on its own global
+// load (the caller synchronises before and after).
+constexpr int kAngleChunk = 32;
+
+template <int T>
+__device__ __forceinline__ void stage_angles(float* th_s, const mbqc_step* steps, const float* ang,
+                                             int64_t sa, int k0, int n_steps, int tid) {
+  for (int i = tid; i < kAngleChunk; i += T) {
+    const int k = k0 + i;
+    if (k < n_steps) {
+      const mbqc_step st = steps[k];
+      th_s[i] = st.angle_index >= 0 ? ang[int64_t(st.angle_index) * sa] : st.fixed_angle;
+    }
+  }
+}
+
 // One step on an LDS-resident statevector of H amplitudes, the pairs split
 // over the T threads of the workgroup (caller synchronises before and after).
 template <int T>
@@ -682,14 +699,19 @@ __global__ __launch_bounds__(T) void k_sv_forward_lds(Job J, uint32_t flags) {
   for (int j = tid; j < H; j += T) st[j] = flip(psi[in_index[j]], sign[j] ? 0x80000000u : 0u);
   RefPhase rp;
   const bool ref = flags & MBQC_FLAG_REF_PHASE;
+  __shared__ float th_s[kAngleChunk];
   for (int k = 0; k < J.n_steps; ++k) {
     if (k % NS == 0 && resc[k / NS]) {
       __syncthreads();
       for (int j = tid; j < H; j += T) st[j] = cscale(st[j], 0x1p-32f);
     }
+    if (k % kAngleChunk == 0) {
+      __syncthreads();
+      stage_angles<T>(th_s, steps, ang, J.angle_stride_a, k, J.n_steps, tid);
+      __syncthreads();
+    }
     const mbqc_step step = steps[k];
-    const float th = step.angle_index >= 0 ? ang[int64_t(step.angle_index) * J.angle_stride_a]
-                                           : step.fixed_angle;
+    const float th = th_s[k % kAngleChunk];
     const float2 b = ref ? step_coeff<true>(th, &rp) : step_coeff<false>(th, nullptr);
     __syncthreads();
     sv_lds_step<T>(st, H, step, b, tid);
@@ -713,15 +735,23 @@ __global__ __launch_bounds__(T) void k_sv_forward_lds(Job J, uint32_t flags) {
   if (tid == 0 && (flags & MBQC_FLAG_WRITE_PHASE) && J.phase_out) J.phase_out[g] = ph;
 }
 
-template <int NS, int T>
-__global__ __launch_bounds__(T) void k_dm_forward_lds(Job J, DmNoise nz, uint32_t flags) {
+// Density matrix in LDS: T threads per simulation (one quartet each per step)
+// and G simulations per workgroup, so that small windows (T < 64) still fill
+// whole waves.
+template <int NS, int T, int G>
+__global__ __launch_bounds__(T * G) void k_dm_forward_lds(Job J, DmNoise nz, uint32_t flags) {
   constexpr int H = 1 << NS;
   constexpr int HQ = H / 2;
-  __shared__ float2 r[H * H];
+  __shared__ float2 rs[G][H * H];
+  __shared__ float th_all[G][kAngleChunk];
   __shared__ float red[4];
-  const int64_t g = blockIdx.x;
-  const int tid = threadIdx.x;
-  const int64_t bi = g / J.S, si = g - bi * J.S;
+  const int gs = threadIdx.x / T, tid = threadIdx.x % T;
+  const int64_t g = int64_t(blockIdx.x) * G + gs;
+  const bool active = g < J.total;
+  const int64_t gg = active ? g : J.total - 1;
+  float2* r = rs[gs];
+  float* th_s = th_all[gs];
+  const int64_t bi = gg / J.S, si = gg - bi * J.S;
   const float* ang = J.angles + bi * J.angle_stride;
   const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
   const int32_t* in_index = J.tab + J.off_in;
@@ -740,9 +770,13 @@ __global__ __launch_bounds__(T) void k_dm_forward_lds(Job J, DmNoise nz, uint32_
       __syncthreads();
       for (int e = tid; e < H * H; e += T) r[e] = cscale(r[e], 0x1p-32f);
     }
+    if (k % kAngleChunk == 0) {
+      __syncthreads();
+      stage_angles<T>(th_s, steps, ang, J.angle_stride_a, k, J.n_steps, tid);
+      __syncthreads();
+    }
     const mbqc_step step = steps[k];
-    const float th = step.angle_index >= 0 ? ang[int64_t(step.angle_index) * J.angle_stride_a]
-                                           : step.fixed_angle;
+    const float th = th_s[k % kAngleChunk];
     const float2 gb = cscale(step_coeff<false>(th, nullptr), nz.gamma);
     __syncthreads();
     dm_lds_step<T>(r, H, step, gb, tid);
@@ -768,9 +802,10 @@ __global__ __launch_bounds__(T) void k_dm_forward_lds(Job J, DmNoise nz, uint32_
     float tr = 0.0f;
     for (int j = tid; j < H; j += T)
       if (out_pos[j] >= 0) tr += r[j * H + j].x;
-    tr = block_sum<T>(tr, red);
+    tr = block_sum<T>(tr, red);      // T < 64: sums within this simulation's lanes
     scale = 1.0f / tr;
   }
+  if (!active) return;
   const int64_t d = int64_t(1) << J.n_out;
   float2* o = J.out + g * d * d;
   for (int e = tid; e < H * H; e += T) {
@@ -1457,8 +1492,9 @@ hipError_t launch_adj_bwd(const Job& J, const float2* ckpt, const float2* seed, 
 }
 
 constexpr int kMaxSvSlots = 6;       // register-resident SV kernels
-// below this many simulations the DM path uses one workgroup per simulation
-constexpr int64_t kDmLaneKernelMinSims = 16384;
+// below this many simulations the DM path keeps each density matrix in LDS
+// (one quartet per thread, several simulations per workgroup)
+constexpr int64_t kDmLaneKernelMinSims = 65536;
 constexpr int kMaxSvLdsSlots = 12;   // LDS-resident SV kernel
 constexpr int kMaxDmSlots = 3;       // register-resident DM kernel
 constexpr int kMaxDmLdsSlots = 6;    // LDS-resident DM kernel
@@ -1485,9 +1521,11 @@ hipError_t launch_sv_lds(const Job& J, uint32_t flags, hipStream_t st) {
 
 template <int NS>
 hipError_t launch_dm_lds(const Job& J, const DmNoise& nz, uint32_t flags, hipStream_t st) {
-  constexpr int Q = 1 << (2 * (NS - 1));
-  constexpr int T = Q < 256 ? Q : 256;
-  hipLaunchKernelGGL((k_dm_forward_lds<NS, T>), dim3(J.total), dim3(T), 0, st, J, nz, flags);
+  constexpr int Q = 1 << (2 * (NS - 1));          // quartets per step
+  constexpr int T = Q < 256 ? Q : 256;            // threads per simulation
+  constexpr int G = T < 64 ? 64 / T : 1;          // simulations per workgroup
+  const int64_t blocks = (J.total + G - 1) / G;
+  hipLaunchKernelGGL((k_dm_forward_lds<NS, T, G>), dim3(blocks), dim3(T * G), 0, st, J, nz, flags);
   return hipGetLastError();
 }
 
@@ -1666,8 +1704,9 @@ int mbqc_dm_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
   hipStream_t st = static_cast<hipStream_t>(stream);
   hipError_t e = hipSuccess;
   // Small batches of small windows leave the one-simulation-per-lane kernel
-  // with a few waves on a few CUs; there one workgroup per simulation (one
-  // quartet per thread, state in LDS) spreads the work over the whole chip.
+  // with a few waves on a few CUs; there the LDS kernel (one quartet per
+  // thread, several small simulations per 64-lane workgroup) spreads the work
+  // over the whole chip.
   const bool small = J.total < kDmLaneKernelMinSims;
   switch (p->ns) {
     case 1: e = small ? launch_dm_lds<1>(J, nz, flags, st) : launch_dm<1>(J, nz, flags, st); break;

@@ -79,7 +79,7 @@ def test_dm_composed_channels(rows, cols, rng):
 
 @pytest.mark.parametrize("rows,cols", [(2, 6), (3, 6)])
 def test_dm_lane_and_workgroup_kernels_agree(rows, cols, rng):
-    """Batches of >= 16384 simulations run one simulation per lane; smaller ones
+    """Batches of >= 65536 simulations run one simulation per lane; smaller ones
     one simulation per workgroup.  Both paths on the same inputs agree, and a
     sample of the large batch matches the oracle."""
     torch = _torch()
