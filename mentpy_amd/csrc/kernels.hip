@@ -321,8 +321,31 @@ __global__ __launch_bounds__(256) void k_sv_forward(Job J, uint32_t flags) {
 //   r_cd = P+U+V                                       (no CZ to the measured qubit)
 // new[(c,j),(d,k)] = sign_j^c sign_k^d r_cd.  The factor 1/2 of |+><+| and
 // the division by the outcome probability (1/2) are tracked as an exponent.
+// Hermitian H x H matrix in registers, stored as its upper triangle
+// (row-major, a <= b); an entry below the diagonal is the conjugate of its
+// mirror.  Every index is a compile-time constant in the unrolled loops, so
+// the choice between the stored entry and its conjugate costs nothing.
+template <int H>
+struct HermReg {
+  static constexpr int N = H * (H + 1) / 2;
+  float2 u[N];
+  static constexpr int idx(int a, int b) { return a * H - (a * (a - 1)) / 2 + (b - a); }
+  __device__ __forceinline__ float2 get(int a, int b) const {
+    return a <= b ? u[idx(a, b)] : cconj(u[idx(b, a)]);
+  }
+  __device__ __forceinline__ void set(int a, int b, float2 v) {
+    if (a <= b)
+      u[idx(a, b)] = v;
+    else
+      u[idx(b, a)] = cconj(v);
+  }
+};
+
+// One DM step on slot Q (the quartet update of the measured qubit with the
+// appended |+><+| and its CZ signs), on the quartets (j, k) with j <= k only:
+// the others are their conjugate transposes.
 template <int NS, int Q>
-__device__ __forceinline__ void dm_step(float2 (&r)[1 << NS][1 << NS], float2 gb, uint32_t flags,
+__device__ __forceinline__ void dm_step(HermReg<1 << NS>& r, float2 gb, uint32_t flags,
                                         uint32_t nmask) {
   constexpr int H = 1 << NS;
   constexpr int QB = 1 << Q;
@@ -332,32 +355,33 @@ __device__ __forceinline__ void dm_step(float2 (&r)[1 << NS][1 << NS], float2 gb
   for (int j = 0; j < H; ++j) {
     if (j & QB) continue;
 #pragma unroll
-    for (int k = 0; k < H; ++k) {
+    for (int k = j; k < H; ++k) {
       if (k & QB) continue;
-      const float2 B00 = r[j][k], B01 = r[j][k | QB], B10 = r[j | QB][k], B11 = r[j | QB][k | QB];
+      const float2 B00 = r.get(j, k), B01 = r.get(j, k | QB);
+      const float2 B10 = r.get(j | QB, k), B11 = r.get(j | QB, k | QB);
       const float2 U = cmul(gbc, B01), V = cmul(gb, B10);
       const float2 P = cadd(B00, B11);
       const float2 r00 = cadd(P, cadd(U, V));
+      float2 n01, n10, n11;
       if (app) {
         const uint32_t sj = czbit(j, nmask), sk = czbit(k, nmask);
         if (sneg) {
           const float2 M = csub(B00, B11);
-          r[j][k] = r00;
-          r[j][k | QB] = flip(cadd(M, csub(V, U)), sk);
-          r[j | QB][k] = flip(cadd(M, csub(U, V)), sj);
-          r[j | QB][k | QB] = flip(csub(P, cadd(U, V)), sj ^ sk);
+          n01 = flip(cadd(M, csub(V, U)), sk);
+          n10 = flip(cadd(M, csub(U, V)), sj);
+          n11 = flip(csub(P, cadd(U, V)), sj ^ sk);
         } else {
-          r[j][k] = r00;
-          r[j][k | QB] = flip(r00, sk);
-          r[j | QB][k] = flip(r00, sj);
-          r[j | QB][k | QB] = flip(r00, sj ^ sk);
+          n01 = flip(r00, sk);
+          n10 = flip(r00, sj);
+          n11 = flip(r00, sj ^ sk);
         }
       } else {
-        r[j][k] = r00;
-        r[j][k | QB] = make_float2(0.f, 0.f);
-        r[j | QB][k] = make_float2(0.f, 0.f);
-        r[j | QB][k | QB] = make_float2(0.f, 0.f);
+        n01 = n10 = n11 = make_float2(0.f, 0.f);
       }
+      r.set(j, k, r00);
+      r.set(j, k | QB, n01);
+      if (j != k) r.set(j | QB, k, n10);   // for j == k it is the conjugate of n01
+      r.set(j | QB, k | QB, n11);
     }
   }
 }
@@ -376,26 +400,27 @@ struct DmNoise {
 };
 
 template <int NS, int M>
-__device__ __forceinline__ void dm_channel(float2 (&r)[1 << NS][1 << NS], const DmNoise& nz) {
+__device__ __forceinline__ void dm_channel(HermReg<1 << NS>& r, const DmNoise& nz) {
   constexpr int H = 1 << NS;
   constexpr int MB = 1 << M;
 #pragma unroll
   for (int j = 0; j < H; ++j) {
     if (j & MB) continue;
 #pragma unroll
-    for (int k = 0; k < H; ++k) {
+    for (int k = j; k < H; ++k) {
       if (k & MB) continue;
-      const float2 a = r[j][k], d = r[j | MB][k | MB];
-      r[j][k] = make_float2(fmaf(nz.d00, a.x, nz.d01 * d.x), fmaf(nz.d00, a.y, nz.d01 * d.y));
-      r[j | MB][k | MB] = make_float2(fmaf(nz.d10, a.x, nz.d11 * d.x), fmaf(nz.d10, a.y, nz.d11 * d.y));
-      r[j][k | MB] = cscale(r[j][k | MB], nz.coh);
-      r[j | MB][k] = cscale(r[j | MB][k], nz.coh);
+      const float2 a = r.get(j, k), d = r.get(j | MB, k | MB);
+      r.set(j, k, make_float2(fmaf(nz.d00, a.x, nz.d01 * d.x), fmaf(nz.d00, a.y, nz.d01 * d.y)));
+      r.set(j | MB, k | MB,
+            make_float2(fmaf(nz.d10, a.x, nz.d11 * d.x), fmaf(nz.d10, a.y, nz.d11 * d.y)));
+      r.set(j, k | MB, cscale(r.get(j, k | MB), nz.coh));
+      if (j != k) r.set(j | MB, k, cscale(r.get(j | MB, k), nz.coh));
     }
   }
 }
 
 template <int NS, bool REF, int SUB>
-__device__ __forceinline__ void dm_sub(float2 (&r)[1 << NS][1 << NS], const mbqc_step* steps, int k0,
+__device__ __forceinline__ void dm_sub(HermReg<1 << NS>& r, const mbqc_step* steps, int k0,
                                        int n_steps, const float (&th)[NS], float gamma) {
   const int k = k0 + SUB;
   if (k < n_steps) {
@@ -405,14 +430,14 @@ __device__ __forceinline__ void dm_sub(float2 (&r)[1 << NS][1 << NS], const mbqc
 }
 
 template <int NS, int... SUBS>
-__device__ __forceinline__ void dm_block(float2 (&r)[1 << NS][1 << NS], const mbqc_step* steps, int k0,
+__device__ __forceinline__ void dm_block(HermReg<1 << NS>& r, const mbqc_step* steps, int k0,
                                          int n_steps, const float (&th)[NS], float gamma,
                                          std::integer_sequence<int, SUBS...>) {
   (dm_sub<NS, false, SUBS>(r, steps, k0, n_steps, th, gamma), ...);
 }
 
 template <int NS, int... MS>
-__device__ __forceinline__ void dm_output_noise(float2 (&r)[1 << NS][1 << NS], uint32_t live,
+__device__ __forceinline__ void dm_output_noise(HermReg<1 << NS>& r, uint32_t live,
                                                 const DmNoise& nz, std::integer_sequence<int, MS...>) {
   ((live & (1u << MS) ? dm_channel<NS, MS>(r, nz) : void()), ...);
 }
@@ -429,19 +454,17 @@ __global__ __launch_bounds__(256) void k_dm_forward(Job J, DmNoise nz, uint32_t 
   const int32_t* resc = J.tab + J.off_dmres;
   float2 v[H];
   sv_load_input<NS>(J, J.psi + si * J.in_stride, v);
-  float2 r[H][H];
+  HermReg<H> r;
 #pragma unroll
   for (int j = 0; j < H; ++j)
 #pragma unroll
-    for (int k = 0; k < H; ++k) r[j][k] = cmul(v[j], cconj(v[k]));
+    for (int k = j; k < H; ++k) r.set(j, k, cmul(v[j], cconj(v[k])));
   const int n = J.n_steps;
   const int nblk = (n + NS - 1) / NS;
   for (int blk = 0; blk < nblk; ++blk) {
     if (resc[blk]) {
 #pragma unroll
-      for (int j = 0; j < H; ++j)
-#pragma unroll
-        for (int k = 0; k < H; ++k) r[j][k] = cscale(r[j][k], 0x1p-32f);
+      for (int e = 0; e < HermReg<H>::N; ++e) r.u[e] = cscale(r.u[e], 0x1p-32f);
     }
     float th[NS];
     load_block_angles<NS>(steps, blk * NS, n, ang, J.angle_stride_a, th);
@@ -455,7 +478,7 @@ __global__ __launch_bounds__(256) void k_dm_forward(Job J, DmNoise nz, uint32_t 
     float tr = 0.0f;
 #pragma unroll
     for (int j = 0; j < H; ++j)
-      if (out_pos[j] >= 0) tr += r[j][j].x;
+      if (out_pos[j] >= 0) tr += r.get(j, j).x;
     scale = 1.0f / tr;
   }
   if (active) {
@@ -468,7 +491,7 @@ __global__ __launch_bounds__(256) void k_dm_forward(Job J, DmNoise nz, uint32_t 
 #pragma unroll
       for (int k = 0; k < H; ++k) {
         const int pk = out_pos[k];
-        if (pk >= 0) o[int64_t(pj) * d + pk] = cscale(r[j][k], scale);
+        if (pk >= 0) o[int64_t(pj) * d + pk] = cscale(r.get(j, k), scale);
       }
     }
   }
@@ -1494,7 +1517,7 @@ hipError_t launch_adj_bwd(const Job& J, const float2* ckpt, const float2* seed, 
 constexpr int kMaxSvSlots = 6;       // register-resident SV kernels
 // below this many simulations the DM path keeps each density matrix in LDS
 // (one quartet per thread, several simulations per workgroup)
-constexpr int64_t kDmLaneKernelMinSims = 65536;
+constexpr int64_t kDmLaneKernelMinSims = 49152;
 constexpr int kMaxSvLdsSlots = 12;   // LDS-resident SV kernel
 constexpr int kMaxDmSlots = 3;       // register-resident DM kernel
 constexpr int kMaxDmLdsSlots = 6;    // LDS-resident DM kernel

@@ -77,18 +77,27 @@ def test_dm_composed_channels(rows, cols, rng):
     assert td < 1e-5
 
 
-@pytest.mark.parametrize("rows,cols", [(2, 6), (3, 6)])
-def test_dm_lane_and_workgroup_kernels_agree(rows, cols, rng):
-    """Batches of >= 65536 simulations run one simulation per lane; smaller ones
-    one simulation per workgroup.  Both paths on the same inputs agree, and a
-    sample of the large batch matches the oracle."""
+@pytest.mark.parametrize("rows,cols,composed", [(2, 6, False), (3, 6, False), (3, 5, True),
+                                                (1, 7, True)])
+def test_dm_lane_and_workgroup_kernels_agree(rows, cols, composed, rng):
+    """Batches of >= 49152 simulations run one simulation per lane (Hermitian
+    upper-triangle registers); smaller ones keep the density matrix in LDS.
+    Both paths on the same inputs agree, and a sample of the large batch
+    matches the oracle."""
     torch = _torch()
     import mentpy_amd as mp
     from oracle import channels as ch
     from oracle.vectorized import dm_batch
-    g = mp.templates.grid_cluster(rows, cols)
-    sim = mp.PatternSimulator(g, backend="hip-dm", circuit_noise="amplitude_damping", gamma=0.1,
-                              p_readout=0.01)
+    g = mp.templates.grid_cluster(rows, cols) if rows > 1 else mp.templates.linear_cluster(cols)
+    if composed:
+        nm = mp.NoiseModel(depolarizing=0.02, readout_flip=0.01, amplitude_damping=0.05,
+                           phase_damping=0.03, phase_flip=0.02, gad_gamma=0.1, gad_p=0.3)
+        sim = mp.PatternSimulator(g, backend="hip-dm", noise=nm)
+        kraus = ch.noise_kraus(0.02, 0.05, 0.03, 0.02, 0.1, 0.3)
+    else:
+        sim = mp.PatternSimulator(g, backend="hip-dm", circuit_noise="amplitude_damping",
+                                  gamma=0.1, p_readout=0.01)
+        kraus = ch.amplitude_damping(0.1)
     B = 65536
     th = torch.rand((B, len(g.trainable_nodes)), generator=torch.Generator().manual_seed(4))
     th = (th * 2 * np.pi).cuda()
@@ -97,8 +106,7 @@ def test_dm_lane_and_workgroup_kernels_agree(rows, cols, rng):
     small = sim.run_batch(th[:300], psi)         # workgroup kernel
     diff = (big[:300] - small).abs().max().item()
     idx = rng.choice(B, 64, replace=False)
-    ref = dm_batch(g, th[idx].double().cpu().numpy(), psi[None], p_readout=0.01,
-                   kraus=ch.amplitude_damping(0.1))
+    ref = dm_batch(g, th[idx].double().cpu().numpy(), psi[None], p_readout=0.01, kraus=kraus)
     td = 0.5 * np.abs(np.linalg.eigvalsh(big[idx].cpu().numpy() - ref)).sum(-1).max()
     print(f"{rows}x{cols}: lane vs workgroup {diff:.2e}, trace distance {td:.2e}", flush=True)
     assert diff < 2e-6
