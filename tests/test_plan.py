@@ -187,3 +187,36 @@ def test_circuit_noise_names_map_to_noise_model():
         mp.PatternSimulator(g, backend="hip-dm", circuit_noise="phase_flip", p=0.1, p_depol=0.1)
     sim = mp.PatternSimulator(g, backend="hip-dm", circuit_noise="phase_damping", gamma=0.3)
     assert sim.simulator.noise.phase_damping == 0.3
+
+
+def test_host_only_abi_functions(native):
+    """Host-side entry points that need no GPU: window sizes of the step API,
+    the Gram workspace size and the causal-flow error codes."""
+    import ctypes as C
+    from mentpy_amd.simulators import compile_pattern
+    L = native.lib()
+    plan, schedule, meas, w = compile_pattern(mp.templates.grid_cluster(3, 5))
+    N, n = len(schedule), len(meas)
+    for k in range(n + 1):
+        assert L.mbqc_plan_window_qubits(plan.handle, k) == min(w, N - k)
+    assert L.mbqc_plan_window_qubits(plan.handle, n + 1) == native.MBQC_ERR_VALUE
+    assert L.mbqc_plan_window_qubits(plan.handle, -1) == native.MBQC_ERR_VALUE
+    assert L.mbqc_gram_workspace_bytes(1024, 256) == 1024 * 16 * 4
+    assert L.mbqc_gram_workspace_bytes(-1, 256) == 0
+    e = np.array([[0, 1], [1, 2]], np.int32)
+    ins, outs = np.array([0], np.int32), np.array([1, 2], np.int32)
+    found = C.c_int32(-1)
+    rc = L.mbqc_causal_flow(3, 2, C.c_void_p(e.ctypes.data), 1, C.c_void_p(ins.ctypes.data), 2,
+                            C.c_void_p(outs.ctypes.data), None, None, None, C.byref(found))
+    assert rc == native.MBQC_ERR_VALUE
+    assert b"different size" in L.mbqc_last_error()
+    # a graph without causal flow: found = 0, no error
+    e2 = np.array([[0, 1], [1, 2], [2, 0], [0, 3], [3, 2]], np.int32)
+    ins2, outs2 = np.array([0], np.int32), np.array([3], np.int32)
+    rc = L.mbqc_causal_flow(4, 5, C.c_void_p(e2.ctypes.data), 1, C.c_void_p(ins2.ctypes.data), 1,
+                            C.c_void_p(outs2.ctypes.data), None, None, None, C.byref(found))
+    assert rc == native.MBQC_OK
+    from mentpy_amd.mbqc.flow import find_cflow
+    import networkx as nx
+    G = nx.Graph([tuple(x) for x in e2.tolist()])
+    assert (found.value == 1) == (find_cflow(G, [0], [3])[0] is not None)
