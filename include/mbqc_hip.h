@@ -205,6 +205,33 @@ int mbqc_sv_adjoint_backward_c64(const mbqc_plan* plan, const float* angles, int
                                  const void* ckpt, size_t ckpt_bytes, float* grad_partial,
                                  void* stream);
 
+/* Time-segmented transfer-matrix adjoint (no reference counterpart: the
+ * reference differentiates by finite differences, mentpy/gradients/
+ * _finite_difference.py:9-24).  The columns are the 2^n_inputs input basis
+ * states, as in mbqc_sv_adjoint_forward_c64 with psi_in = identity and
+ * C = 2^n_inputs.  The pattern's blocks of n_slots steps are cut into
+ * n_segments segments whose transfer matrices are computed concurrently and
+ * chained per parameter set, so that one parameter set occupies n_segments
+ * times as many lanes for 1/n_segments of the steps each (small batches fill
+ * the chip).  2 <= n_segments <= min(16, number of blocks).
+ *
+ * mbqc_sv_segments_auto: the segment count the library picks for B sets
+ * (1 means: use the unsegmented entry points).
+ * Forward: out[B][2^n_inputs][2^n_outputs] = T_b (column c = image of input
+ * basis state c).  Backward: seed in the same layout (dL = Re sum conj(seed)
+ * d(out)); writes grad[B][n_angles] (entries of non-trainable angles are left
+ * untouched).  The backward needs the workspace of the forward of the same
+ * angles. */
+int32_t mbqc_sv_segments_auto(const mbqc_plan* plan, int64_t B);
+size_t mbqc_sv_segmented_workspace_bytes(const mbqc_plan* plan, int64_t B, int32_t n_segments);
+int mbqc_sv_segmented_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
+                                  int64_t angle_stride_a, void* out, int64_t B, int32_t n_segments,
+                                  void* workspace, size_t workspace_bytes, void* stream);
+int mbqc_sv_segmented_backward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
+                                   int64_t angle_stride_a, const void* seed, int64_t B,
+                                   int32_t n_segments, void* workspace, size_t workspace_bytes,
+                                   float* grad, void* stream);
+
 /* Batched forward of many plans in one persistent launch (no reference
  * counterpart: the reference runs one simulation per Python call).  Plans are
  * first bound into a batch, which packs their tables on the device. */

@@ -32,7 +32,9 @@ EXPORTED_SYMBOLS = (
     "mbqc_plan_window_qubits", "mbqc_sv_window_c64", "mbqc_dm_window_c64",
     "mbqc_gram_workspace_bytes", "mbqc_gram_loss_seed_c64",
     "mbqc_sv_adjoint_workspace_bytes", "mbqc_sv_adjoint_forward_c64",
-    "mbqc_sv_adjoint_partials", "mbqc_sv_adjoint_backward_c64", "mbqc_batch_create",
+    "mbqc_sv_adjoint_partials", "mbqc_sv_adjoint_backward_c64", "mbqc_sv_segments_auto",
+    "mbqc_sv_segmented_workspace_bytes", "mbqc_sv_segmented_forward_c64",
+    "mbqc_sv_segmented_backward_c64", "mbqc_batch_create",
     "mbqc_batch_destroy", "mbqc_batch_items", "mbqc_batch_launch", "mbqc_adam_step_f32",
     "mbqc_sgd_step_f32", "mbqc_last_error", "mbqc_abi_version",
 )
@@ -128,6 +130,14 @@ def lib() -> C.CDLL:
     L.mbqc_sv_adjoint_partials.restype = i32
     L.mbqc_sv_adjoint_backward_c64.argtypes = [vp, vp, i64, i64, vp, i64, i64, vp, sz, vp, vp]
     L.mbqc_sv_adjoint_backward_c64.restype = C.c_int
+    L.mbqc_sv_segments_auto.argtypes = [vp, i64]
+    L.mbqc_sv_segments_auto.restype = i32
+    L.mbqc_sv_segmented_workspace_bytes.argtypes = [vp, i64, i32]
+    L.mbqc_sv_segmented_workspace_bytes.restype = sz
+    L.mbqc_sv_segmented_forward_c64.argtypes = [vp, vp, i64, i64, vp, i64, i32, vp, sz, vp]
+    L.mbqc_sv_segmented_forward_c64.restype = C.c_int
+    L.mbqc_sv_segmented_backward_c64.argtypes = [vp, vp, i64, i64, vp, i64, i32, vp, sz, vp, vp]
+    L.mbqc_sv_segmented_backward_c64.restype = C.c_int
     L.mbqc_batch_create.argtypes = [C.POINTER(BatchJob), i32, i32, C.POINTER(vp)]
     L.mbqc_batch_create.restype = C.c_int
     L.mbqc_batch_destroy.argtypes = [vp]

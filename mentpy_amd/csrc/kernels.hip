@@ -12,6 +12,7 @@
 // traffic.  HBM sees only the angles, the input states and the outputs.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <utility>
@@ -182,27 +183,27 @@ __device__ __forceinline__ void fetch_block_angles(const mbqc_step* steps, int k
   }
 }
 
-// Run all steps: full blocks of NS unrolled steps with the next block's
-// angles in flight, then the partial tail block.  `on_block(blk, t)` runs at
-// the start of every block (after its rescale).
+// Run blocks [blk0, blk1) of NS steps: full blocks of NS unrolled steps with
+// the next block's angles in flight, then the partial tail block.
+// `on_block(blk, t)` runs at the start of every block (after its rescale).
 template <int NS, bool REF, typename OnBlock, typename Mod = NoModify>
-__device__ __forceinline__ void sv_run_steps(float2 (&t)[1 << NS], const Job& J, const float* ang,
-                                             RefPhase* ph, OnBlock on_block, Mod&& mod = Mod()) {
+__device__ __forceinline__ void sv_run_blocks(float2 (&t)[1 << NS], const Job& J, const float* ang,
+                                              RefPhase* ph, int blk0, int blk1, OnBlock on_block,
+                                              Mod&& mod = Mod()) {
   constexpr int H = 1 << NS;
   const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
   const int32_t* resc = J.tab + J.off_svres;
   const float* coef = reinterpret_cast<const float*>(J.tab + J.off_coef);
   const int n = J.n_steps;
   const int nfull = n / NS;
-  const int nblk = (n + NS - 1) / NS;
   float th_next[NS];
   const int64_t sa = J.angle_stride_a;
-  fetch_block_angles<NS>(steps, 0, n, ang, sa, th_next);
-  for (int blk = 0; blk < nblk; ++blk) {
+  fetch_block_angles<NS>(steps, blk0 * NS, n, ang, sa, th_next);
+  for (int blk = blk0; blk < blk1; ++blk) {
     float th[NS];
 #pragma unroll
     for (int s2 = 0; s2 < NS; ++s2) th[s2] = th_next[s2];
-    if (blk + 1 < nblk) fetch_block_angles<NS>(steps, (blk + 1) * NS, n, ang, sa, th_next);
+    if (blk + 1 < blk1) fetch_block_angles<NS>(steps, (blk + 1) * NS, n, ang, sa, th_next);
     if (resc[blk]) {
 #pragma unroll
       for (int j = 0; j < H; ++j) t[j] = cscale(t[j], 0x1p-32f);
@@ -213,6 +214,14 @@ __device__ __forceinline__ void sv_run_steps(float2 (&t)[1 << NS], const Job& J,
     else
       sv_block<NS, REF, false>(t, coef, blk * NS, n, th, ph, mod, std::make_integer_sequence<int, NS>{});
   }
+}
+
+// All steps of the pattern.
+template <int NS, bool REF, typename OnBlock, typename Mod = NoModify>
+__device__ __forceinline__ void sv_run_steps(float2 (&t)[1 << NS], const Job& J, const float* ang,
+                                             RefPhase* ph, OnBlock on_block, Mod&& mod = Mod()) {
+  sv_run_blocks<NS, REF>(t, J, ang, ph, 0, (J.n_steps + NS - 1) / NS, on_block,
+                         static_cast<Mod&&>(mod));
 }
 
 template <int NS>
@@ -1027,6 +1036,294 @@ __global__ __launch_bounds__(64) void k_sv_adj_backward(Job J, const float2* __r
   }
 }
 
+// ---------------------------------------------------------------- segmented adjoint
+// Transfer-matrix adjoint cut in time.  The blocks of the pattern are split
+// into segments p = 0..n-1 at block boundaries; segment p's transfer matrix
+// T_p (from the window state at its first block, before that block's rescale,
+// to the window state at the next segment's first block) is computed for all
+// segments at once, one window basis column per lane (segment 0 starts from
+// the input basis states instead, and the last segment ends with the output
+// gather and the final scale).  A small per-set kernel chains them:
+//   S_1 = T_0,  S_{p+1} = T_p S_p,  T = T_{n-1} S_{n-1},
+// and, given the seed G = dL/dconj(T), propagates the cotangents back:
+//   L_n = G,  G_p = L_{p+1} S_p^H,  L_p = T_p^H L_{p+1},  G_0 = L_1,
+// so that dL = Re sum_p tr(G_p^H dT_p).  The backward sweep of segment p then
+// runs from its checkpoints with the columns of G_p as seeds.  With n segments
+// a parameter set occupies n times as many lanes, each for 1/n of the steps.
+constexpr int kMaxSegments = 16;
+
+struct SegInfo {
+  int32_t n;                        // segments
+  int32_t cols0, cols;              // columns of segment 0 (2^n_in) and of the others (2^NS)
+  int32_t rows_last;                // rows of the last segment's matrix (2^n_out)
+  int64_t B;                        // parameter sets
+  int32_t blk[kMaxSegments + 1];    // first block of each segment; blk[n] = number of blocks
+  int64_t wave[kMaxSegments + 1];   // first 64-lane wave of each segment; wave[n] = total
+  int64_t ckpt[kMaxSegments];       // float2 offsets into the workspace: checkpoints,
+  int64_t tmat[kMaxSegments];       //   T_p (columns of rows_p entries),
+  int64_t smat[kMaxSegments];       //   S_p for p >= 2 (columns of 2^NS entries),
+  int64_t gmat[kMaxSegments];       //   G_p (columns of rows_p entries)
+};
+
+__device__ __forceinline__ int seg_of_wave(const SegInfo& G, int64_t w) {
+  int p = 0;
+  for (int q = 1; q < G.n; ++q) p = (w >= G.wave[q]) ? q : p;
+  return p;
+}
+
+__device__ __forceinline__ float2 cfma(float2 a, float2 b, float2 acc) {
+  return make_float2(fmaf(a.x, b.x, fmaf(-a.y, b.y, acc.x)), fmaf(a.x, b.y, fmaf(a.y, b.x, acc.y)));
+}
+// acc + conj(a) b
+__device__ __forceinline__ float2 cfma_conj(float2 a, float2 b, float2 acc) {
+  return make_float2(fmaf(a.x, b.x, fmaf(a.y, b.y, acc.x)), fmaf(a.x, b.y, fmaf(-a.y, b.x, acc.y)));
+}
+
+// One lane per (segment, set, column); one wave per workgroup, every wave
+// inside one segment.
+template <int NS>
+__global__ __launch_bounds__(64) void k_sv_seg_forward(Job J, SegInfo G, float2* __restrict__ ws) {
+  constexpr int H = 1 << NS;
+  const int p = seg_of_wave(G, blockIdx.x);
+  const int C = p == 0 ? G.cols0 : G.cols;
+  const int64_t Np = (G.wave[p + 1] - G.wave[p]) * 64;
+  const int64_t l = (int64_t(blockIdx.x) - G.wave[p]) * 64 + threadIdx.x;
+  const bool active = l < G.B * C;
+  const int64_t ll = active ? l : 0;
+  const int64_t bi = ll / C;
+  const int c = int(ll - bi * C);
+  const float* ang = J.angles + bi * J.angle_stride;
+  float2 t[H];
+  if (p == 0) {
+    const int32_t* in_index = J.tab + J.off_in;
+    const int32_t* sign = J.tab + J.off_sign;
+#pragma unroll
+    for (int j = 0; j < H; ++j)
+      t[j] = make_float2(in_index[j] == c ? (sign[j] ? -1.0f : 1.0f) : 0.0f, 0.0f);
+  } else {
+#pragma unroll
+    for (int j = 0; j < H; ++j) t[j] = make_float2(j == c ? 1.0f : 0.0f, 0.0f);
+  }
+  RefPhase ph;
+  float2* ck = ws + G.ckpt[p];
+  const int b0 = G.blk[p];
+  sv_run_blocks<NS, false>(t, J, ang, &ph, b0, G.blk[p + 1], [&](int blk, const float2 (&tt)[H]) {
+    if (active) {
+#pragma unroll
+      for (int j = 0; j < H; ++j) ck[(int64_t(blk - b0) * H + j) * Np + l] = tt[j];
+    }
+  });
+  if (!active) return;
+  if (p + 1 < G.n) {
+    float2* o = ws + G.tmat[p] + ll * H;
+#pragma unroll
+    for (int j = 0; j < H; ++j) o[j] = t[j];
+  } else {
+    const int32_t* out_pos = J.tab + J.off_out;
+    const float scale = sv_final_scale(J.e_sv);
+    float2* o = ws + G.tmat[p] + ll * G.rows_last;
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      const int q = out_pos[j];
+      if (q >= 0) o[q] = cscale(t[j], scale);
+    }
+  }
+}
+
+// The chain kernels hold every matrix in LDS as 2^NS x 2^NS, column-major
+// with a column stride of 2^NS + 1 (lanes that walk a row hit distinct
+// banks), zero-padded past its rows and columns, so all products run over the
+// full, unrolled dimension.  Each thread owns kPer entries e = tid + 256 i; the
+// next step's matrices are fetched into registers while the current product
+// runs, so the chain waits on global memory only once.
+template <int NS>
+struct SegTile {
+  static constexpr int H = 1 << NS;
+  static constexpr int LD = H + 1;
+  static constexpr int kPer = (H * H + 255) / 256;
+  float2 v[kPer];
+  __device__ __forceinline__ void fetch(const float2* src, int rows, int cols) {
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int e = int(threadIdx.x) + 256 * i;
+      const int c = e >> NS, r = e & (H - 1);
+      v[i] = (e < H * H && r < rows && c < cols) ? src[c * rows + r] : make_float2(0.f, 0.f);
+    }
+  }
+  __device__ __forceinline__ void put(float2* lds) const {
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int e = int(threadIdx.x) + 256 * i;
+      if (e < H * H) lds[(e >> NS) * LD + (e & (H - 1))] = v[i];
+    }
+  }
+};
+
+// Per parameter set (one 256-thread workgroup): S_{p+1} = T_p S_p and
+// T = T_{n-1} S_{n-1}.
+template <int NS>
+__global__ __launch_bounds__(256) void k_seg_chain_forward(SegInfo G, float2* __restrict__ ws,
+                                                           float2* __restrict__ out) {
+  constexpr int H = 1 << NS;
+  constexpr int kPer = SegTile<NS>::kPer;
+  constexpr int LD = SegTile<NS>::LD;
+  __shared__ float2 Sm[H * LD], Tm[H * LD];
+  const int64_t b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int C0 = G.cols0;
+  auto rows_of = [&](int p) { return p + 1 == G.n ? G.rows_last : H; };
+  SegTile<NS> s0, tn;
+  s0.fetch(ws + G.tmat[0] + b * C0 * H, H, C0);
+  tn.fetch(ws + G.tmat[1] + b * H * rows_of(1), rows_of(1), H);
+  s0.put(Sm);
+  for (int p = 1; p < G.n; ++p) {
+    const bool last = p + 1 == G.n;
+    const int R = rows_of(p);
+    __syncthreads();
+    tn.put(Tm);
+    __syncthreads();
+    if (!last) tn.fetch(ws + G.tmat[p + 1] + b * H * rows_of(p + 1), rows_of(p + 1), H);
+    float2 acc[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int e = tid + 256 * i;
+      const int c = (e >> NS) & (H - 1), r = e & (H - 1);
+      float2 a = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < H; ++k) a = cfma(Tm[k * LD + r], Sm[c * LD + k], a);
+      acc[i] = a;
+    }
+    __syncthreads();
+    float2* dst = last ? out + b * C0 * R : ws + G.smat[p + 1] + b * C0 * H;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int e = tid + 256 * i;
+      if (e < H * H) {
+        const int c = e >> NS, r = e & (H - 1);
+        if (!last) Sm[c * LD + r] = acc[i];
+        if (r < R && c < C0) dst[c * R + r] = acc[i];
+      }
+    }
+  }
+}
+
+// Per parameter set: the segment seeds G_p from the output seed G.
+template <int NS>
+__global__ __launch_bounds__(256) void k_seg_chain_backward(SegInfo G, float2* __restrict__ ws,
+                                                            const float2* __restrict__ seed) {
+  constexpr int H = 1 << NS;
+  constexpr int kPer = SegTile<NS>::kPer;
+  constexpr int LD = SegTile<NS>::LD;
+  __shared__ float2 Lm[H * LD], Sm[H * LD], Tm[H * LD];
+  const int64_t b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int C0 = G.cols0;
+  auto rows_of = [&](int p) { return p + 1 == G.n ? G.rows_last : H; };
+  // S_1 = T_0
+  auto s_src = [&](int p) { return (p == 1 ? ws + G.tmat[0] : ws + G.smat[p]) + b * C0 * H; };
+  SegTile<NS> l0, sn, tn;
+  l0.fetch(seed + b * C0 * G.rows_last, G.rows_last, C0);
+  sn.fetch(s_src(G.n - 1), H, C0);
+  tn.fetch(ws + G.tmat[G.n - 1] + b * H * G.rows_last, G.rows_last, H);
+  l0.put(Lm);
+  for (int p = G.n - 1; p >= 1; --p) {
+    const int R = rows_of(p);                         // rows of L and of T_p
+    __syncthreads();
+    sn.put(Sm);
+    tn.put(Tm);
+    __syncthreads();
+    if (p > 1) {
+      sn.fetch(s_src(p - 1), H, C0);
+      tn.fetch(ws + G.tmat[p - 1] + b * H * H, H, H);
+    }
+    // G_p[r][j] = sum_c L[r][c] conj(S_p[j][c])
+    float2* gp = ws + G.gmat[p] + b * H * R;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int e = tid + 256 * i;
+      const int j = (e >> NS) & (H - 1), r = e & (H - 1);
+      float2 a = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int c = 0; c < H; ++c) a = cfma_conj(Sm[c * LD + j], Lm[c * LD + r], a);
+      if (e < H * H && r < R) gp[j * R + r] = a;
+    }
+    // L_p[k][c] = sum_r conj(T_p[r][k]) L[r][c]
+    float2 acc[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int e = tid + 256 * i;
+      const int c = (e >> NS) & (H - 1), k = e & (H - 1);
+      float2 a = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int r = 0; r < H; ++r) a = cfma_conj(Tm[k * LD + r], Lm[c * LD + r], a);
+      acc[i] = a;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int e = tid + 256 * i;
+      if (e < H * H) Lm[(e >> NS) * LD + (e & (H - 1))] = acc[i];
+    }
+  }
+  __syncthreads();
+  float2* g0 = ws + G.gmat[0] + b * C0 * H;
+  for (int e = tid; e < C0 * H; e += 256) g0[e] = Lm[(e >> NS) * LD + (e & (H - 1))];
+}
+
+template <int NS>
+__global__ __launch_bounds__(64) void k_sv_seg_backward(Job J, SegInfo G, const float2* __restrict__ ws,
+                                                        int32_t n_angles, float* __restrict__ grad) {
+  constexpr int H = 1 << NS;
+  const int p = seg_of_wave(G, blockIdx.x);
+  const int C = p == 0 ? G.cols0 : G.cols;
+  const int64_t Np = (G.wave[p + 1] - G.wave[p]) * 64;
+  const int64_t l = (int64_t(blockIdx.x) - G.wave[p]) * 64 + threadIdx.x;
+  const bool active = l < G.B * C;
+  const int64_t ll = active ? l : 0;
+  const int64_t bi = ll / C;
+  const int c = int(ll - bi * C);
+  const float* ang = J.angles + bi * J.angle_stride;
+  const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
+  const int32_t* resc = J.tab + J.off_svres;
+  const float* coef = reinterpret_cast<const float*>(J.tab + J.off_coef);
+  float2 lam[H];
+  if (p + 1 == G.n) {
+    const int32_t* out_pos = J.tab + J.off_out;
+    const float scale = sv_final_scale(J.e_sv);
+    const float2* sd = ws + G.gmat[p] + ll * G.rows_last;
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      const int q = out_pos[j];
+      lam[j] = (active && q >= 0) ? cscale(sd[q], scale) : make_float2(0.f, 0.f);
+    }
+  } else {
+    const float2* sd = ws + G.gmat[p] + ll * H;
+#pragma unroll
+    for (int j = 0; j < H; ++j) lam[j] = active ? sd[j] : make_float2(0.f, 0.f);
+  }
+  AdjCtx A{};
+  A.red = C;
+  float* prow = grad + bi * n_angles;
+  const bool writer = active && c == 0;
+  const float2* ck = ws + G.ckpt[p];
+  const int n = J.n_steps;
+  const int b0 = G.blk[p];
+  for (int blk = G.blk[p + 1] - 1; blk >= b0; --blk) {
+    float2 t[H];
+#pragma unroll
+    for (int j = 0; j < H; ++j)
+      t[j] = active ? ck[(int64_t(blk - b0) * H + j) * Np + l] : make_float2(0.f, 0.f);
+    float th[NS];
+    load_block_angles<NS>(steps, blk * NS, n, ang, J.angle_stride_a, th);
+    adj_block<NS>(t, lam, steps, coef, blk * NS, n, th, A, prow, writer,
+                  std::make_integer_sequence<int, NS>{});
+    if (resc[blk]) {
+#pragma unroll
+      for (int j = 0; j < H; ++j) lam[j] = cscale(lam[j], 0x1p-32f);
+    }
+  }
+}
+
 // ---------------------------------------------------------------- MB-QML loss
 // Infidelity loss and adjoint seed of the MB-QML cost from the transfer
 // matrices, one fused pass (the Gram form of 1 - mean_s |<phi_s|T psi_s>|^2,
@@ -1514,6 +1811,63 @@ hipError_t launch_adj_bwd(const Job& J, const float2* ckpt, const float2* seed, 
   return hipGetLastError();
 }
 
+// Segment layout of the time-cut adjoint for B parameter sets and n segments;
+// returns the workspace size in bytes (0 when n is out of range).
+size_t seg_layout(const Plan& p, int64_t B, int n, SegInfo* G) {
+  const int nblk = p.n_blocks();
+  if (n < 2 || n > kMaxSegments || n > nblk || B < 0) return 0;
+  SegInfo g{};
+  const int64_t H = int64_t(1) << p.ns;
+  g.n = n;
+  g.cols0 = 1 << p.n_in;
+  g.cols = int32_t(H);
+  g.rows_last = 1 << p.n_out;
+  g.B = B;
+  auto align = [](int64_t x) { return (x + 63) & ~int64_t(63); };
+  int64_t off = 0, wave = 0;
+  for (int q = 0; q <= n; ++q) g.blk[q] = int32_t((int64_t(q) * nblk) / n);
+  for (int q = 0; q < n; ++q) {
+    const int64_t C = q == 0 ? g.cols0 : H;
+    const int64_t rows = q + 1 == n ? g.rows_last : H;
+    const int64_t waves = (B * C + 63) / 64;
+    g.wave[q] = wave;
+    wave += waves;
+    g.ckpt[q] = off;
+    off = align(off + int64_t(g.blk[q + 1] - g.blk[q]) * H * waves * 64);
+    g.tmat[q] = off;
+    off = align(off + B * C * rows);
+    g.gmat[q] = off;
+    off = align(off + B * C * rows);
+    g.smat[q] = off;
+    if (q >= 2) off = align(off + B * g.cols0 * H);
+  }
+  g.wave[n] = wave;
+  if (G) *G = g;
+  return size_t(off) * sizeof(float2);
+}
+
+template <int NS>
+hipError_t launch_seg_forward(const Job& J, const SegInfo& G, float2* ws, float2* out, hipStream_t st) {
+  if (G.wave[G.n] > 0)
+    hipLaunchKernelGGL((k_sv_seg_forward<NS>), dim3(G.wave[G.n]), dim3(64), 0, st, J, G, ws);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && G.B > 0)
+    hipLaunchKernelGGL((k_seg_chain_forward<NS>), dim3(G.B), dim3(256), 0, st, G, ws, out);
+  return e == hipSuccess ? hipGetLastError() : e;
+}
+
+template <int NS>
+hipError_t launch_seg_backward(const Job& J, const SegInfo& G, float2* ws, const float2* seed,
+                               int32_t n_angles, float* grad, hipStream_t st) {
+  if (G.B > 0)
+    hipLaunchKernelGGL((k_seg_chain_backward<NS>), dim3(G.B), dim3(256), 0, st, G, ws, seed);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && G.wave[G.n] > 0)
+    hipLaunchKernelGGL((k_sv_seg_backward<NS>), dim3(G.wave[G.n]), dim3(64), 0, st, J, G, ws,
+                       n_angles, grad);
+  return e == hipSuccess ? hipGetLastError() : e;
+}
+
 constexpr int kMaxSvSlots = 6;       // register-resident SV kernels
 // below this many simulations the DM path keeps each density matrix in LDS
 // (one quartet per thread, several simulations per workgroup)
@@ -1550,6 +1904,26 @@ hipError_t launch_dm_lds(const Job& J, const DmNoise& nz, uint32_t flags, hipStr
   const int64_t blocks = (J.total + G - 1) / G;
   hipLaunchKernelGGL((k_dm_forward_lds<NS, T, G>), dim3(blocks), dim3(T * G), 0, st, J, nz, flags);
   return hipGetLastError();
+}
+
+int seg_setup(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
+              int64_t angle_stride_a, const void* buf, int64_t B, int32_t n_segments,
+              void* workspace, size_t workspace_bytes, Plan** pp, Job* J, SegInfo* G) {
+  if (!plan) return fail(MBQC_ERR_VALUE, "null plan");
+  Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(plan));
+  if (p->ns > kMaxAdjSlots)
+    return fail(MBQC_ERR_NOT_IMPLEMENTED, "adjoint kernels support live windows of at most " +
+                                              std::to_string(kMaxAdjSlots) + " qubits");
+  if (n_segments < 2 || n_segments > std::min(kMaxSegments, p->n_blocks()))
+    return fail(MBQC_ERR_VALUE, "n_segments must be in 2.." +
+                                    std::to_string(std::min(kMaxSegments, p->n_blocks())));
+  const size_t need = seg_layout(*p, B, n_segments, G);
+  if (!workspace || workspace_bytes < need) return fail(MBQC_ERR_VALUE, "segmented workspace too small");
+  // the input pointer is unused (columns are basis states); pass buf as a valid pointer
+  int rc = make_job(p, angles, angle_stride, angle_stride_a, buf, 0, const_cast<void*>(buf), B, 1, J);
+  if (rc != MBQC_OK) return rc;
+  *pp = p;
+  return MBQC_OK;
 }
 
 }  // namespace
@@ -1746,6 +2120,74 @@ size_t mbqc_sv_adjoint_workspace_bytes(const mbqc_plan* plan, int64_t n_columns)
   if (!plan || n_columns < 0) return 0;
   const Plan* p = reinterpret_cast<const Plan*>(plan);
   return size_t(p->n_blocks()) * (size_t(1) << p->ns) * size_t(n_columns) * 8u;
+}
+
+int32_t mbqc_sv_segments_auto(const mbqc_plan* plan, int64_t B) {
+  if (!plan || B <= 0) return 1;
+  const Plan* p = reinterpret_cast<const Plan*>(plan);
+  // Measured on the 4x16 and 4x20 grids (16 lanes per set): the best count
+  // falls from 4 segments at 16,384 lanes to 1 at 131,072 and more, about as
+  // 4 sqrt(16384 / lanes); more segments add chain work per set.
+  const double lanes = double(B) * double(int64_t(1) << p->ns);
+  int64_t n = std::llround(4.0 * std::sqrt(16384.0 / lanes));
+  n = std::min<int64_t>(n, std::min(kMaxSegments, p->n_blocks()));
+  return int32_t(std::max<int64_t>(n, 1));
+}
+
+size_t mbqc_sv_segmented_workspace_bytes(const mbqc_plan* plan, int64_t B, int32_t n_segments) {
+  if (!plan) return 0;
+  return seg_layout(*reinterpret_cast<const Plan*>(plan), B, n_segments, nullptr);
+}
+
+
+int mbqc_sv_segmented_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
+                                  int64_t angle_stride_a, void* out, int64_t B, int32_t n_segments,
+                                  void* workspace, size_t workspace_bytes, void* stream) {
+  Plan* p = nullptr;
+  Job J;
+  SegInfo G;
+  int rc = seg_setup(plan, angles, angle_stride, angle_stride_a, out, B, n_segments, workspace,
+                     workspace_bytes, &p, &J, &G);
+  if (rc != MBQC_OK) return rc;
+  if (B == 0) return MBQC_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  float2* ws = static_cast<float2*>(workspace);
+  float2* o = static_cast<float2*>(out);
+  hipError_t e = hipSuccess;
+  switch (p->ns) {
+    case 1: e = launch_seg_forward<1>(J, G, ws, o, st); break;
+    case 2: e = launch_seg_forward<2>(J, G, ws, o, st); break;
+    case 3: e = launch_seg_forward<3>(J, G, ws, o, st); break;
+    case 4: e = launch_seg_forward<4>(J, G, ws, o, st); break;
+    case 5: e = launch_seg_forward<5>(J, G, ws, o, st); break;
+  }
+  return e == hipSuccess ? MBQC_OK : hip_fail(e, "segmented forward launch");
+}
+
+int mbqc_sv_segmented_backward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
+                                   int64_t angle_stride_a, const void* seed, int64_t B,
+                                   int32_t n_segments, void* workspace, size_t workspace_bytes,
+                                   float* grad, void* stream) {
+  if (!seed || !grad) return fail(MBQC_ERR_VALUE, "null seed or gradient pointer");
+  Plan* p = nullptr;
+  Job J;
+  SegInfo G;
+  int rc = seg_setup(plan, angles, angle_stride, angle_stride_a, seed, B, n_segments, workspace,
+                     workspace_bytes, &p, &J, &G);
+  if (rc != MBQC_OK) return rc;
+  if (B == 0) return MBQC_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  float2* ws = static_cast<float2*>(workspace);
+  const float2* sd = static_cast<const float2*>(seed);
+  hipError_t e = hipSuccess;
+  switch (p->ns) {
+    case 1: e = launch_seg_backward<1>(J, G, ws, sd, p->n_angles, grad, st); break;
+    case 2: e = launch_seg_backward<2>(J, G, ws, sd, p->n_angles, grad, st); break;
+    case 3: e = launch_seg_backward<3>(J, G, ws, sd, p->n_angles, grad, st); break;
+    case 4: e = launch_seg_backward<4>(J, G, ws, sd, p->n_angles, grad, st); break;
+    case 5: e = launch_seg_backward<5>(J, G, ws, sd, p->n_angles, grad, st); break;
+  }
+  return e == hipSuccess ? MBQC_OK : hip_fail(e, "segmented backward launch");
 }
 
 int32_t mbqc_sv_adjoint_partials(int64_t C) {

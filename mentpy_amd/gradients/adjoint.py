@@ -162,12 +162,17 @@ class MBQMLStep:
     * ``"transfer"``: transfer matrix, then ``T X`` and ``(Phi f) X^H`` GEMMs.
     * ``"direct"``: the inputs themselves are the propagated columns.
 
+    In the transfer-matrix modes the steps can be cut in time
+    (``segments``): the segments' transfer matrices are computed
+    concurrently and chained per parameter set, which fills the chip when
+    ``B`` is small.  ``"auto"`` lets the library pick (1 = no cut).
+
     All buffers are allocated once, so a step can be captured in a HIP graph
     (``capture``); write new angles into ``self.angles`` before each replay.
     """
 
     def __init__(self, sim, inputs, targets, n_sets: int, mode: str = "auto",
-                 gram_kernel: str = "auto"):
+                 gram_kernel: str = "auto", segments="auto"):
         torch = _torch()
         self.engine = _engine_for(sim)
         eng = self.engine
@@ -207,13 +212,31 @@ class MBQMLStep:
             self.gram_kernel = gram_kernel
             nbytes = int(nat.lib().mbqc_gram_workspace_bytes(self.B, K))
             self._gws = torch.empty(max(nbytes // 4, 1), dtype=torch.float32, device=dev)
-        self._ws, self._need = eng._workspace(self.B * C_)
+        L = nat.lib()
+        if mode == "direct":
+            segments = 1
+        elif segments == "auto":
+            segments = int(L.mbqc_sv_segments_auto(eng.plan.handle, self.B))
+        self.segments = int(segments)
+        if self.segments > 1:
+            self._need = int(L.mbqc_sv_segmented_workspace_bytes(eng.plan.handle, self.B, self.segments))
+            if self._need == 0:
+                raise ValueError(f"segments={self.segments} is out of range for this pattern")
+            self._ws = torch.empty(self._need, dtype=torch.uint8, device=dev)
+        else:
+            self._ws, self._need = eng._workspace(self.B * C_)
 
     def _launch_forward(self):
         torch = _torch()
         eng, info = self.engine, self.engine.plan.info
         stream = C.c_void_p(torch.cuda.current_stream(eng.device).cuda_stream)
         sb, sa = _angle_strides(self.angles)
+        if self.segments > 1:
+            nat.check(nat.lib().mbqc_sv_segmented_forward_c64(
+                eng.plan.handle, C.c_void_p(self.angles.data_ptr()), sb, sa,
+                C.c_void_p(self.out.data_ptr()), self.B, self.segments,
+                C.c_void_p(self._ws.data_ptr()), self._need, stream))
+            return
         nat.check(nat.lib().mbqc_sv_adjoint_forward_c64(
             eng.plan.handle, C.c_void_p(self.angles.data_ptr()), sb, sa,
             C.c_void_p(self.cols.data_ptr()), self.cols.shape[1], C.c_void_p(self.out.data_ptr()),
@@ -224,6 +247,12 @@ class MBQMLStep:
         eng = self.engine
         stream = C.c_void_p(torch.cuda.current_stream(eng.device).cuda_stream)
         sb, sa = _angle_strides(self.angles)
+        if self.segments > 1:
+            nat.check(nat.lib().mbqc_sv_segmented_backward_c64(
+                eng.plan.handle, C.c_void_p(self.angles.data_ptr()), sb, sa,
+                C.c_void_p(self.seed.data_ptr()), self.B, self.segments,
+                C.c_void_p(self._ws.data_ptr()), self._need, C.c_void_p(self.part.data_ptr()), stream))
+            return
         nat.check(nat.lib().mbqc_sv_adjoint_backward_c64(
             eng.plan.handle, C.c_void_p(self.angles.data_ptr()), sb, sa,
             C.c_void_p(self.seed.data_ptr()), self.B, self.cols.shape[0],

@@ -249,3 +249,41 @@ def test_fused_gram_loss_matches_library_gemm(rows, cols, S, B, rng):
           f"grad diff {(ga - gb).abs().max().item():.2e}", flush=True)
     assert (la - lb).abs().max().item() < 2e-6
     assert (ga - gb).abs().max().item() < 2e-5
+
+
+@pytest.mark.parametrize("rows,cols,S,segs,mode", [
+    (4, 16, 256, 8, "gram"), (4, 16, 40, 2, "transfer"), (2, 7, 9, 3, "gram"), (3, 6, 20, 4, "gram"),
+    (5, 5, 40, 2, "transfer"), (1, 9, 5, 4, "gram"), (4, 6, 30, 5, "gram"), (3, 11, 24, 10, "gram")])
+def test_segmented_adjoint_matches_unsegmented(rows, cols, S, segs, mode, rng):
+    """The time-segmented transfer-matrix adjoint (segments' transfer matrices
+    computed concurrently, chained per parameter set) gives the transfer
+    matrix, loss and gradient of the single sweep, and the exact shift rule;
+    segment counts up to one block per segment, set counts that leave
+    partial waves."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from mentpy_amd.gradients import MBQMLStep
+    from oracle.gradients import exact_shift_gradient
+    g = mp.templates.grid_cluster(rows, cols) if rows > 1 else mp.templates.linear_cluster(cols)
+    sim = mp.PatternSimulator(g, backend="hip-sv")
+    X, Y = _dataset(rows, S, rng)
+    B = 7
+    th = rng.uniform(0, 2 * np.pi, (B, len(g.trainable_nodes))).astype(np.float32)
+    one = MBQMLStep(sim, X, Y, n_sets=B, mode=mode, segments=1)
+    cut = MBQMLStep(sim, X, Y, n_sets=B, mode=mode, segments=segs)
+    assert cut.segments == segs
+    l1, g1 = one.run(th)
+    l1, g1, t1 = l1.clone(), g1.clone(), one.out.clone()
+    l2, g2 = cut.run(th)
+    tscale = t1.abs().max().item()
+    terr = (cut.out - t1).abs().max().item() / tscale
+    lerr = (l1 - l2).abs().max().item()
+    gerr = (g1 - g2).abs().max().item()
+    print(f"{rows}x{cols} {mode} segments={segs}: T rel err {terr:.2e} loss diff {lerr:.2e} "
+          f"grad diff {gerr:.2e}", flush=True)
+    assert terr < 2e-6
+    assert lerr < 2e-6
+    assert gerr < 2e-5
+    cost = _oracle_cost(g, X, Y)
+    ref_g = exact_shift_gradient(cost, th[0].astype(np.float64))
+    assert np.abs(g2[0].cpu().numpy() - ref_g).max() < 2e-4

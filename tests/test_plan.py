@@ -220,3 +220,24 @@ def test_host_only_abi_functions(native):
     import networkx as nx
     G = nx.Graph([tuple(x) for x in e2.tolist()])
     assert (found.value == 1) == (find_cflow(G, [0], [3])[0] is not None)
+
+
+def test_segmented_adjoint_host_sizes(native):
+    """Segment count choice and workspace sizes of the time-segmented adjoint
+    (host-only): 4x16 grid = 15 blocks of 4 steps, 16 columns per segment."""
+    from mentpy_amd.simulators import compile_pattern
+    L = native.lib()
+    plan = compile_pattern(mp.templates.grid_cluster(4, 16))[0]
+    assert L.mbqc_sv_segments_auto(plan.handle, 1024) == 4
+    assert L.mbqc_sv_segments_auto(plan.handle, 4096) == 2
+    assert L.mbqc_sv_segments_auto(plan.handle, 16384) == 1
+    assert L.mbqc_sv_segments_auto(plan.handle, 1) == 15          # capped by the block count
+    assert L.mbqc_sv_segmented_workspace_bytes(plan.handle, 1024, 1) == 0
+    assert L.mbqc_sv_segmented_workspace_bytes(plan.handle, 1024, 16) == 0
+    ws = L.mbqc_sv_segmented_workspace_bytes(plan.handle, 1024, 8)
+    # checkpoints (15 blocks x 16 amplitudes x 16 columns per set) dominate
+    ckpt = 15 * 16 * 1024 * 16 * 8
+    assert ckpt < ws < ckpt + 8 * 4 * 1024 * 16 * 16 * 8
+    assert L.mbqc_sv_segmented_forward_c64(plan.handle, None, 0, 1, None, 4, 1, None, 0, None) \
+        == native.MBQC_ERR_VALUE
+    assert b"n_segments" in L.mbqc_last_error()
