@@ -952,12 +952,21 @@ __device__ __forceinline__ float dpp_move(float x) {
 }
 
 __device__ __forceinline__ float lane_group_sum(float v, int red) {
-  if (red >= 2) v += dpp_move<0xB1>(v);     // quad_perm [1,0,3,2]
-  if (red >= 4) v += dpp_move<0x4E>(v);     // quad_perm [2,3,0,1]
-  if (red >= 8) v += dpp_move<0x141>(v);    // row_half_mirror
-  if (red >= 16) v += dpp_move<0x140>(v);   // row_mirror
-  if (red >= 32) v += __shfl_xor(v, 16, 64);
-  if (red >= 64) v += __shfl_xor(v, 32, 64);
+  // one uniform branch for the usual groups of 16 or more lanes
+  if (red >= 16) {
+    v += dpp_move<0xB1>(v);                 // quad_perm [1,0,3,2]
+    v += dpp_move<0x4E>(v);                 // quad_perm [2,3,0,1]
+    v += dpp_move<0x141>(v);                // row_half_mirror
+    v += dpp_move<0x140>(v);                // row_mirror
+    if (red >= 32) {
+      v += __shfl_xor(v, 16, 64);
+      if (red >= 64) v += __shfl_xor(v, 32, 64);
+    }
+  } else {
+    if (red >= 2) v += dpp_move<0xB1>(v);
+    if (red >= 4) v += dpp_move<0x4E>(v);
+    if (red >= 8) v += dpp_move<0x141>(v);
+  }
   return v;
 }
 
@@ -1020,11 +1029,13 @@ __global__ __launch_bounds__(64) void k_sv_adj_backward(Job J, const float2* __r
   const int64_t prow_i = A.P == 1 ? 0 : (A.red == 64 ? ci / 64 : ci);
   float* prow = gpart + (bi * A.P + prow_i) * A.n_angles;
   const bool writer = active && (A.red == 1 || (ci & (A.red - 1)) == 0);
+  // inactive lanes recompute simulation 0 (their sums stay in their own,
+  // inactive lane groups and are never written)
   for (int blk = nblk - 1; blk >= 0; --blk) {
     float2 t[H];
+    const float2* cb = ckpt + int64_t(blk) * H * J.total + gg;
 #pragma unroll
-    for (int j = 0; j < H; ++j)
-      t[j] = active ? ckpt[(int64_t(blk) * H + j) * J.total + g] : make_float2(0.f, 0.f);
+    for (int j = 0; j < H; ++j) t[j] = cb[int64_t(j) * J.total];
     float th[NS];
     load_block_angles<NS>(steps, blk * NS, n, ang, J.angle_stride_a, th);
     adj_block<NS>(t, lam, steps, coef, blk * NS, n, th, A, prow, writer,
@@ -1080,9 +1091,12 @@ __device__ __forceinline__ float2 cfma_conj(float2 a, float2 b, float2 acc) {
 }
 
 // One lane per (segment, set, column); one wave per workgroup, every wave
-// inside one segment.
+// inside one segment.  `ckpt` and `ws` are the same workspace: the checkpoint
+// stores go through their own restrict pointer, so the compiler can prove
+// they never touch the plan tables and keeps the table reads on scalar loads.
 template <int NS>
-__global__ __launch_bounds__(64) void k_sv_seg_forward(Job J, SegInfo G, float2* __restrict__ ws) {
+__global__ __launch_bounds__(64) void k_sv_seg_forward(Job J, SegInfo G, float2* __restrict__ ckpt,
+                                                       float2* __restrict__ ws) {
   constexpr int H = 1 << NS;
   const int p = seg_of_wave(G, blockIdx.x);
   const int C = p == 0 ? G.cols0 : G.cols;
@@ -1105,7 +1119,7 @@ __global__ __launch_bounds__(64) void k_sv_seg_forward(Job J, SegInfo G, float2*
     for (int j = 0; j < H; ++j) t[j] = make_float2(j == c ? 1.0f : 0.0f, 0.0f);
   }
   RefPhase ph;
-  float2* ck = ws + G.ckpt[p];
+  float2* ck = ckpt + G.ckpt[p];
   const int b0 = G.blk[p];
   sv_run_blocks<NS, false>(t, J, ang, &ph, b0, G.blk[p + 1], [&](int blk, const float2 (&tt)[H]) {
     if (active) {
@@ -1308,11 +1322,12 @@ __global__ __launch_bounds__(64) void k_sv_seg_backward(Job J, SegInfo G, const 
   const float2* ck = ws + G.ckpt[p];
   const int n = J.n_steps;
   const int b0 = G.blk[p];
+  // inactive lanes recompute lane 0 of the segment (see k_sv_adj_backward)
   for (int blk = G.blk[p + 1] - 1; blk >= b0; --blk) {
     float2 t[H];
+    const float2* cb = ck + int64_t(blk - b0) * H * Np + ll;
 #pragma unroll
-    for (int j = 0; j < H; ++j)
-      t[j] = active ? ck[(int64_t(blk - b0) * H + j) * Np + l] : make_float2(0.f, 0.f);
+    for (int j = 0; j < H; ++j) t[j] = cb[int64_t(j) * Np];
     float th[NS];
     load_block_angles<NS>(steps, blk * NS, n, ang, J.angle_stride_a, th);
     adj_block<NS>(t, lam, steps, coef, blk * NS, n, th, A, prow, writer,
@@ -1849,7 +1864,7 @@ size_t seg_layout(const Plan& p, int64_t B, int n, SegInfo* G) {
 template <int NS>
 hipError_t launch_seg_forward(const Job& J, const SegInfo& G, float2* ws, float2* out, hipStream_t st) {
   if (G.wave[G.n] > 0)
-    hipLaunchKernelGGL((k_sv_seg_forward<NS>), dim3(G.wave[G.n]), dim3(64), 0, st, J, G, ws);
+    hipLaunchKernelGGL((k_sv_seg_forward<NS>), dim3(G.wave[G.n]), dim3(64), 0, st, J, G, ws, ws);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess && G.B > 0)
     hipLaunchKernelGGL((k_seg_chain_forward<NS>), dim3(G.B), dim3(256), 0, st, G, ws, out);
