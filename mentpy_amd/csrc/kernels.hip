@@ -254,14 +254,64 @@ __device__ __forceinline__ float sv_final_scale(int e) {
   return (e & 1) ? s * 0.70710678118654752440f : s;
 }
 
-// One SV simulation on this lane (g = b*S + s).  Inactive lanes run on
-// simulation 0 and write nothing.
-// One SV simulation on this lane (g = b*S + s).  Inactive lanes run on
-// simulation 0 and write nothing.  With STAGE (kernels whose waves cover 64
-// consecutive simulations and outputs of at most 2^NS <= 16 amplitudes) the
-// wave's output rows go through LDS and leave as contiguous 512-byte stores.
-template <int NS, bool REF, bool STAGE = false>
-__device__ __forceinline__ void sv_sim(const Job& J, int64_t g, bool active, uint32_t flags) {
+// Makes this wave's LDS writes visible to its own later LDS reads: waits for
+// them to complete and keeps the compiler from moving LDS accesses across.
+// (Waves only exchange data through their own LDS tile, so no workgroup
+// barrier is needed, and persistent waves with different trip counts can use
+// it.)
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0), other counters unchanged
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Output rows of a wave of 64 consecutive simulations (row r = simulation
+// g0 + r, 2^n_out complex entries, slot index j -> column out_pos[j], times f)
+// through a per-wave LDS tile of 16 columns: each pass writes columns
+// [c0, c0 + 16) of all 64 rows, so a store instruction covers four 128-byte
+// row segments instead of one entry in each of 64 rows.  Used for outputs of
+// 32 amplitudes (five live slots); rows r >= rows_valid are not written.
+template <int NS>
+__device__ __forceinline__ void wave_store_rows(const float2 (&t)[1 << NS], const int32_t* out_pos,
+                                                float2 f, float2* obase, int n_out, int rows_valid) {
+  constexpr int H = 1 << NS;
+  constexpr int CW = H < 16 ? H : 16;
+  constexpr int LOG_CW = NS < 4 ? NS : 4;
+  __shared__ float2 stage[4][64 * (CW + 1)];
+  const int lane = threadIdx.x & 63;
+  float2* sw = stage[threadIdx.x >> 6];
+  const int dout = 1 << n_out;
+  const int cs = n_out < LOG_CW ? n_out : LOG_CW;   // log2 of the columns per pass
+  const int cw = 1 << cs;
+  for (int c0 = 0; c0 < dout; c0 += cw) {
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      const int p = out_pos[j] - c0;
+      if (p >= 0 && p < cw) sw[lane * (CW + 1) + p] = cmul(t[j], f);
+    }
+    wave_lds_sync();
+    for (int e = lane; e < 64 * cw; e += 64) {
+      const int r = e >> cs, c = e & (cw - 1);
+      if (r < rows_valid) obase[int64_t(r) * dout + c0 + c] = sw[r * (CW + 1) + c];
+    }
+    wave_lds_sync();
+  }
+}
+
+// How a wave's output rows are stored.
+enum StoreMode {
+  kStoreDirect = 0,   // each lane writes its own row
+  kStoreBlock = 1,    // all rows of the workgroup's waves staged at once (outputs of <= 16
+                      // amplitudes; every wave of the workgroup must take this path)
+  kStoreColumns = 2,  // wave_store_rows (any wave on its own)
+};
+
+// One SV simulation on this lane (g = b*S + s); the wave covers the 64
+// consecutive simulations g - lane .. g - lane + 63, of which the first
+// rows_valid are written (kStoreDirect: active lanes write).  Inactive lanes
+// run on simulation 0.
+template <int NS, bool REF, int STORE>
+__device__ __forceinline__ void sv_sim(const Job& J, int64_t g, bool active, int rows_valid,
+                                       uint32_t flags) {
   constexpr int H = 1 << NS;
   const int64_t gg = active ? g : 0;
   const int64_t bi = gg / J.S, si = gg - bi * J.S;
@@ -282,26 +332,27 @@ __device__ __forceinline__ void sv_sim(const Job& J, int64_t g, bool active, uin
     scale = 1.0f / sqrtf(n2);
   }
   const float2 f = cscale(ph, scale);
-  if constexpr (STAGE) {
-    static_assert(H <= 16, "staged output supports at most 16 amplitudes");
+  const int lane = threadIdx.x & 63;
+  const int64_t g0 = g - lane;
+  if constexpr (STORE == kStoreBlock) {
+    static_assert(H <= 16, "block staging supports at most 16 amplitudes");
     __shared__ float2 stage[4][64 * (H + 1)];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int dout = 1 << J.n_out;
-    float2* sw = stage[w];
+    float2* sw = stage[threadIdx.x >> 6];
 #pragma unroll
     for (int j = 0; j < H; ++j) {
       const int p = out_pos[j];
       if (p >= 0) sw[lane * (dout + 1) + p] = cmul(t[j], f);
     }
     __syncthreads();
-    const int64_t g0 = g - lane;
     float2* obase = J.out + g0 * dout;
     const int shift = J.n_out;
     for (int e = lane; e < 64 * dout; e += 64) {
       const int r = e >> shift, c = e & (dout - 1);
-      if (g0 + r < J.total) obase[e] = sw[r * (dout + 1) + c];
+      if (r < rows_valid) obase[e] = sw[r * (dout + 1) + c];
     }
-    if (active && (flags & MBQC_FLAG_WRITE_PHASE) && J.phase_out) J.phase_out[g] = ph;
+  } else if constexpr (STORE == kStoreColumns) {
+    wave_store_rows<NS>(t, out_pos, f, J.out + g0 * (int64_t(1) << J.n_out), J.n_out, rows_valid);
   } else {
     if (active) {
       float2* o = J.out + g * (int64_t(1) << J.n_out);
@@ -310,15 +361,24 @@ __device__ __forceinline__ void sv_sim(const Job& J, int64_t g, bool active, uin
         const int p = out_pos[j];
         if (p >= 0) o[p] = cmul(t[j], f);
       }
-      if ((flags & MBQC_FLAG_WRITE_PHASE) && J.phase_out) J.phase_out[g] = ph;
     }
   }
+  if (active && (flags & MBQC_FLAG_WRITE_PHASE) && J.phase_out) J.phase_out[g] = ph;
 }
+
+// Output store per live-window size (measured on 1M-simulation batches):
+// block staging up to 16 amplitudes; column passes for 32 (5x6 grid:
+// 0.44 -> 0.20 ms, 5x30: 1.07 -> 0.75 ms); direct stores for 64 (staging
+// was slower there).
+template <int NS>
+constexpr int kForwardStore = NS <= 4 ? kStoreBlock : (NS == 5 ? kStoreColumns : kStoreDirect);
 
 template <int NS, bool REF>
 __global__ __launch_bounds__(256) void k_sv_forward(Job J, uint32_t flags) {
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  sv_sim<NS, REF, (NS <= 4)>(J, g, g < J.total, flags);
+  const int64_t g0 = g - (threadIdx.x & 63);
+  const int rows = int(J.total - g0 < 64 ? J.total - g0 : 64);
+  sv_sim<NS, REF, kForwardStore<NS>>(J, g, g < J.total, rows, flags);
 }
 
 // ---------------------------------------------------------------- DM step
@@ -1657,7 +1717,11 @@ __global__ __launch_bounds__(256) void k_batch(const Job* __restrict__ jobs,
     const Job J = jobs[I.job];
     for (int off = 0; off < I.count; off += 64) {
       const int64_t g = I.first + off + lane;
-      sv_sim<NS, REF>(J, g, (off + lane) < I.count && g < J.total, flags);
+      const int64_t left = J.total - (I.first + off);
+      const int rows = int(std::min<int64_t>(std::min<int64_t>(I.count - off, left), 64));
+      // direct stores: staging tiles would cost the concurrently running
+      // window-size groups LDS occupancy (measured slower)
+      sv_sim<NS, REF, kStoreDirect>(J, g, (off + lane) < I.count && g < J.total, rows, flags);
     }
   }
 }
