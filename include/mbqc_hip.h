@@ -72,7 +72,8 @@ typedef struct mbqc_plan_info {
 typedef struct mbqc_step {
   int32_t angle_index;    /* >= 0: angle vector entry; < 0: use fixed_angle */
   uint32_t flags;         /* bit0 append, bit1 measured qubit is a CZ neighbour of
-                             the appended one, bits 8..15 measured slot */
+                             the appended one, bits 8..15 measured slot, bits
+                             16..31 row of the step's pair coefficients */
   uint32_t nmask;         /* slots that are CZ neighbours of the appended qubit */
   float fixed_angle;
 } mbqc_step;

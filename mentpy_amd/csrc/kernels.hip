@@ -147,21 +147,34 @@ struct NoModify {
   __device__ __forceinline__ float2 operator()(int, float2 b) const { return b; }
 };
 
-template <int NS, bool REF, bool FULL, int SUB, typename Mod>
-__device__ __forceinline__ void sv_sub(float2 (&t)[1 << NS], const float* coef, int k0,
-                                       int n_steps, const float (&th)[NS], RefPhase* ph, Mod& mod) {
+// Pair coefficients of step k: row (steps[k].flags >> 16) of the plan's
+// coefficient table.  Long patterns store each distinct row once (a grid has
+// one per slot), so their rows stay within the scalar cache; short ones keep
+// one row per step (row = k), and kernels built with PER_STEP address them
+// without reading the step flags first.
+template <int NS, bool PER_STEP = false>
+__device__ __forceinline__ const float* coef_row(const float* coef, const mbqc_step* steps, int k) {
+  if constexpr (PER_STEP) return coef + (k << NS);
+  return coef + (int(steps[k].flags >> 16) << NS);
+}
+
+template <int NS, bool REF, bool FULL, bool PER_STEP, int SUB, typename Mod>
+__device__ __forceinline__ void sv_sub(float2 (&t)[1 << NS], const mbqc_step* steps,
+                                       const float* coef, int k0, int n_steps, const float (&th)[NS],
+                                       RefPhase* ph, Mod& mod) {
   const int k = k0 + SUB;
   if (FULL || k < n_steps) {
     const float2 b = mod(k, step_coeff<REF>(th[SUB], ph));
-    sv_step<NS, NS - 1 - SUB>(t, b, coef + (k << NS));
+    sv_step<NS, NS - 1 - SUB>(t, b, coef_row<NS, PER_STEP>(coef, steps, k));
   }
 }
 
-template <int NS, bool REF, bool FULL, typename Mod, int... SUBS>
-__device__ __forceinline__ void sv_block(float2 (&t)[1 << NS], const float* coef, int k0,
-                                         int n_steps, const float (&th)[NS], RefPhase* ph, Mod& mod,
+template <int NS, bool REF, bool FULL, bool PER_STEP, typename Mod, int... SUBS>
+__device__ __forceinline__ void sv_block(float2 (&t)[1 << NS], const mbqc_step* steps,
+                                         const float* coef, int k0, int n_steps,
+                                         const float (&th)[NS], RefPhase* ph, Mod& mod,
                                          std::integer_sequence<int, SUBS...>) {
-  (sv_sub<NS, REF, FULL, SUBS>(t, coef, k0, n_steps, th, ph, mod), ...);
+  (sv_sub<NS, REF, FULL, PER_STEP, SUBS>(t, steps, coef, k0, n_steps, th, ph, mod), ...);
 }
 
 // Angles of the NS steps of a block (steps past the end get 0).  The index is
@@ -186,7 +199,7 @@ __device__ __forceinline__ void fetch_block_angles(const mbqc_step* steps, int k
 // Run blocks [blk0, blk1) of NS steps: full blocks of NS unrolled steps with
 // the next block's angles in flight, then the partial tail block.
 // `on_block(blk, t)` runs at the start of every block (after its rescale).
-template <int NS, bool REF, typename OnBlock, typename Mod = NoModify>
+template <int NS, bool REF, bool PER_STEP, typename OnBlock, typename Mod = NoModify>
 __device__ __forceinline__ void sv_run_blocks(float2 (&t)[1 << NS], const Job& J, const float* ang,
                                               RefPhase* ph, int blk0, int blk1, OnBlock on_block,
                                               Mod&& mod = Mod()) {
@@ -210,9 +223,10 @@ __device__ __forceinline__ void sv_run_blocks(float2 (&t)[1 << NS], const Job& J
     }
     on_block(blk, t);
     if (blk < nfull)
-      sv_block<NS, REF, true>(t, coef, blk * NS, n, th, ph, mod, std::make_integer_sequence<int, NS>{});
+      sv_block<NS, REF, true, PER_STEP>(t, steps, coef, blk * NS, n, th, ph, mod, std::make_integer_sequence<int, NS>{});
     else
-      sv_block<NS, REF, false>(t, coef, blk * NS, n, th, ph, mod, std::make_integer_sequence<int, NS>{});
+      sv_block<NS, REF, false, PER_STEP>(t, steps, coef, blk * NS, n, th, ph, mod,
+                               std::make_integer_sequence<int, NS>{});
   }
 }
 
@@ -220,8 +234,8 @@ __device__ __forceinline__ void sv_run_blocks(float2 (&t)[1 << NS], const Job& J
 template <int NS, bool REF, typename OnBlock, typename Mod = NoModify>
 __device__ __forceinline__ void sv_run_steps(float2 (&t)[1 << NS], const Job& J, const float* ang,
                                              RefPhase* ph, OnBlock on_block, Mod&& mod = Mod()) {
-  sv_run_blocks<NS, REF>(t, J, ang, ph, 0, (J.n_steps + NS - 1) / NS, on_block,
-                         static_cast<Mod&&>(mod));
+  sv_run_blocks<NS, REF, false>(t, J, ang, ph, 0, (J.n_steps + NS - 1) / NS, on_block,
+                                static_cast<Mod&&>(mod));
 }
 
 template <int NS>
@@ -309,7 +323,7 @@ enum StoreMode {
 // consecutive simulations g - lane .. g - lane + 63, of which the first
 // rows_valid are written (kStoreDirect: active lanes write).  Inactive lanes
 // run on simulation 0.
-template <int NS, bool REF, int STORE>
+template <int NS, bool REF, int STORE, bool PER_STEP = false>
 __device__ __forceinline__ void sv_sim(const Job& J, int64_t g, bool active, int rows_valid,
                                        uint32_t flags) {
   constexpr int H = 1 << NS;
@@ -319,7 +333,8 @@ __device__ __forceinline__ void sv_sim(const Job& J, int64_t g, bool active, int
   float2 t[H];
   sv_load_input<NS>(J, J.psi + si * J.in_stride, t);
   RefPhase rp;
-  sv_run_steps<NS, REF>(t, J, ang, &rp, [](int, const float2 (&)[H]) {});
+  sv_run_blocks<NS, REF, PER_STEP>(t, J, ang, &rp, 0, (J.n_steps + NS - 1) / NS,
+                                   [](int, const float2 (&)[H]) {});
   const float2 ph = REF ? rp.value() : make_float2(1.0f, 0.0f);
   // output gather: slot-index j -> output index out_pos[j] (dead slots skipped)
   const int32_t* out_pos = J.tab + J.off_out;
@@ -373,12 +388,12 @@ __device__ __forceinline__ void sv_sim(const Job& J, int64_t g, bool active, int
 template <int NS>
 constexpr int kForwardStore = NS <= 4 ? kStoreBlock : (NS == 5 ? kStoreColumns : kStoreDirect);
 
-template <int NS, bool REF>
+template <int NS, bool REF, bool PER_STEP>
 __global__ __launch_bounds__(256) void k_sv_forward(Job J, uint32_t flags) {
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t g0 = g - (threadIdx.x & 63);
   const int rows = int(J.total - g0 < 64 ? J.total - g0 : 64);
-  sv_sim<NS, REF, kForwardStore<NS>>(J, g, g < J.total, rows, flags);
+  sv_sim<NS, REF, kForwardStore<NS>, PER_STEP>(J, g, g < J.total, rows, flags);
 }
 
 // ---------------------------------------------------------------- DM step
@@ -991,13 +1006,14 @@ struct AdjCtx {
 
 template <int NS, int SUB>
 __device__ __forceinline__ void adj_recompute(float2 (&t)[1 << NS], float2 (&ys)[NS][1 << (NS - 1)],
-                                              float2 (&bs)[NS], const float* coef, int k0, int n,
+                                              float2 (&bs)[NS], const mbqc_step* steps,
+                                              const float* coef, int k0, int n,
                                               const float (&th)[NS]) {
   const int k = k0 + SUB;
   if (k < n) {
     bs[SUB] = step_coeff<false>(th[SUB], nullptr);
     take_y<NS, NS - 1 - SUB>(t, ys[SUB]);
-    if (SUB + 1 < NS) sv_step<NS, NS - 1 - SUB>(t, bs[SUB], coef + (k << NS));
+    if (SUB + 1 < NS) sv_step<NS, NS - 1 - SUB>(t, bs[SUB], coef_row<NS>(coef, steps, k));
   }
 }
 
@@ -1037,7 +1053,7 @@ __device__ __forceinline__ void adj_back(float2 (&lam)[1 << NS], const float2 (&
                                          float* prow, bool writer) {
   const int k = k0 + SUB;
   if (k < n) {
-    float d = sv_adj_step<NS, NS - 1 - SUB>(lam, ys[SUB], bs[SUB], coef + (k << NS));
+    float d = sv_adj_step<NS, NS - 1 - SUB>(lam, ys[SUB], bs[SUB], coef_row<NS>(coef, steps, k));
     const int ai = steps[k].angle_index;
     if (ai >= 0) {
       d = lane_group_sum(d, A.red);
@@ -1053,7 +1069,7 @@ __device__ __forceinline__ void adj_block(float2 (&t)[1 << NS], float2 (&lam)[1 
                                           bool writer, std::integer_sequence<int, SUBS...>) {
   float2 ys[NS][1 << (NS - 1)];
   float2 bs[NS];
-  (adj_recompute<NS, SUBS>(t, ys, bs, coef, k0, n, th), ...);
+  (adj_recompute<NS, SUBS>(t, ys, bs, steps, coef, k0, n, th), ...);
   (adj_back<NS, NS - 1 - SUBS>(lam, ys, bs, steps, coef, k0, n, A, prow, writer), ...);
 }
 
@@ -1181,7 +1197,7 @@ __global__ __launch_bounds__(64) void k_sv_seg_forward(Job J, SegInfo G, float2*
   RefPhase ph;
   float2* ck = ckpt + G.ckpt[p];
   const int b0 = G.blk[p];
-  sv_run_blocks<NS, false>(t, J, ang, &ph, b0, G.blk[p + 1], [&](int blk, const float2 (&tt)[H]) {
+  sv_run_blocks<NS, false, false>(t, J, ang, &ph, b0, G.blk[p + 1], [&](int blk, const float2 (&tt)[H]) {
     if (active) {
 #pragma unroll
       for (int j = 0; j < H; ++j) ck[(int64_t(blk - b0) * H + j) * Np + l] = tt[j];
@@ -1703,6 +1719,56 @@ struct BatchItem {
 // counter until none are left.  Items are sorted by decreasing cost on the
 // host, so long patterns start first.  One kernel per NS keeps each group at
 // the register footprint (and so the occupancy) of its own window size.
+// A Job read from device memory (rather than passed as a kernel argument)
+// holds generic pointers: loads and stores through them compile to flat
+// instructions, and the plan-table loads could not be proven free of the
+// kernel's own stores, so they would all be per-lane vector loads.  The same
+// layout with the pointers typed as global memory, and the read-only plan
+// tables as constant memory, restores global vector instructions and scalar
+// table loads.
+struct GlobalJob {
+  const __attribute__((address_space(4))) int32_t* tab;
+  int32_t off_in, off_sign, off_out, off_svres, off_dmres, e_sv, e_dm;
+  int32_t ns, n_steps, n_out, live_mask, off_coef;
+  const __attribute__((address_space(1))) float* angles;
+  int64_t angle_stride;
+  int64_t angle_stride_a;
+  const __attribute__((address_space(1))) float2* psi;
+  int64_t in_stride;
+  __attribute__((address_space(1))) float2* out;
+  __attribute__((address_space(1))) float2* phase_out;
+  int64_t S, total;
+};
+static_assert(sizeof(GlobalJob) == sizeof(Job), "GlobalJob mirrors Job");
+
+__device__ __forceinline__ Job load_job(const Job* jobs, int i) {
+  const GlobalJob g = reinterpret_cast<const GlobalJob*>(jobs)[i];
+  Job j;
+  j.tab = (const int32_t*)g.tab;
+  j.off_in = g.off_in;
+  j.off_sign = g.off_sign;
+  j.off_out = g.off_out;
+  j.off_svres = g.off_svres;
+  j.off_dmres = g.off_dmres;
+  j.e_sv = g.e_sv;
+  j.e_dm = g.e_dm;
+  j.ns = g.ns;
+  j.n_steps = g.n_steps;
+  j.n_out = g.n_out;
+  j.live_mask = g.live_mask;
+  j.off_coef = g.off_coef;
+  j.angles = (const float*)g.angles;
+  j.angle_stride = g.angle_stride;
+  j.angle_stride_a = g.angle_stride_a;
+  j.psi = (const float2*)g.psi;
+  j.in_stride = g.in_stride;
+  j.out = (float2*)g.out;
+  j.phase_out = (float2*)g.phase_out;
+  j.S = g.S;
+  j.total = g.total;
+  return j;
+}
+
 template <bool REF, int NS>
 __global__ __launch_bounds__(256) void k_batch(const Job* __restrict__ jobs,
                                                const BatchItem* __restrict__ items, int64_t n_items,
@@ -1714,7 +1780,7 @@ __global__ __launch_bounds__(256) void k_batch(const Job* __restrict__ jobs,
     it = __builtin_amdgcn_readfirstlane(it);
     if (it >= n_items) break;
     const BatchItem I = items[it];
-    const Job J = jobs[I.job];
+    const Job J = load_job(jobs, I.job);
     for (int off = 0; off < I.count; off += 64) {
       const int64_t g = I.first + off + lane;
       const int64_t left = J.total - (I.first + off);
@@ -1814,13 +1880,21 @@ int make_job(Plan* p, const float* angles, int64_t angle_stride, int64_t angle_s
 }
 
 template <int NS>
-hipError_t launch_sv(const Job& J, uint32_t flags, hipStream_t st) {
+hipError_t launch_sv(const Job& J, bool per_step, uint32_t flags, hipStream_t st) {
   const int threads = 256;
   const int64_t blocks = (J.total + threads - 1) / threads;
-  if (flags & MBQC_FLAG_REF_PHASE)
-    hipLaunchKernelGGL((k_sv_forward<NS, true>), dim3(blocks), dim3(threads), 0, st, J, flags);
-  else
-    hipLaunchKernelGGL((k_sv_forward<NS, false>), dim3(blocks), dim3(threads), 0, st, J, flags);
+  const bool ref = flags & MBQC_FLAG_REF_PHASE;
+  if (per_step) {
+    if (ref)
+      hipLaunchKernelGGL((k_sv_forward<NS, true, true>), dim3(blocks), dim3(threads), 0, st, J, flags);
+    else
+      hipLaunchKernelGGL((k_sv_forward<NS, false, true>), dim3(blocks), dim3(threads), 0, st, J, flags);
+  } else {
+    if (ref)
+      hipLaunchKernelGGL((k_sv_forward<NS, true, false>), dim3(blocks), dim3(threads), 0, st, J, flags);
+    else
+      hipLaunchKernelGGL((k_sv_forward<NS, false, false>), dim3(blocks), dim3(threads), 0, st, J, flags);
+  }
   return hipGetLastError();
 }
 
@@ -2052,12 +2126,12 @@ int mbqc_sv_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
   hipStream_t st = static_cast<hipStream_t>(stream);
   hipError_t e = hipSuccess;
   switch (p->ns) {
-    case 1: e = launch_sv<1>(J, flags, st); break;
-    case 2: e = launch_sv<2>(J, flags, st); break;
-    case 3: e = launch_sv<3>(J, flags, st); break;
-    case 4: e = launch_sv<4>(J, flags, st); break;
-    case 5: e = launch_sv<5>(J, flags, st); break;
-    case 6: e = launch_sv<6>(J, flags, st); break;
+    case 1: e = launch_sv<1>(J, p->coef_per_step, flags, st); break;
+    case 2: e = launch_sv<2>(J, p->coef_per_step, flags, st); break;
+    case 3: e = launch_sv<3>(J, p->coef_per_step, flags, st); break;
+    case 4: e = launch_sv<4>(J, p->coef_per_step, flags, st); break;
+    case 5: e = launch_sv<5>(J, p->coef_per_step, flags, st); break;
+    case 6: e = launch_sv<6>(J, p->coef_per_step, flags, st); break;
     case 7: e = launch_sv_lds<7>(J, flags, st); break;
     case 8: e = launch_sv_lds<8>(J, flags, st); break;
     case 9: e = launch_sv_lds<9>(J, flags, st); break;

@@ -17,6 +17,7 @@
 // after the trace, as in the reference.  CZs to nodes no longer in the window
 // are dropped exactly as the reference drops them.
 #include <algorithm>
+#include <map>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -224,10 +225,18 @@ int compile_plan(const mbqc_pattern_desc* d, Plan* p) {
   // c (x + s b y) with s = -1 when the measured qubit is a CZ neighbour of the
   // appended one, c = CZ sign of the pair (0 without an append).  Written as
   // alpha u0 + beta x:  alpha = c s,  beta = c (1 - s).
+  // Step k keeps its row index in flags bits 16..31.  A table of at most
+  // kCoefPerStepBytes keeps one row per step (row k, addressed without the
+  // flags by the forward kernel); a longer one stores each distinct row once
+  // (a grid repeats its rows column after column), so it stays within the
+  // scalar cache.
   if (ns <= kMaxCoefSlots) {
-    p->coef.assign(size_t(nm) << ns, 0.0f);
+    p->coef.clear();
+    p->coef_per_step = (size_t(nm) << ns) * sizeof(float) <= kCoefPerStepBytes;
+    std::map<std::vector<float>, uint32_t> row_index;
+    std::vector<float> row(size_t(1) << ns);
     for (int k = 0; k < nm; ++k) {
-      const mbqc_step& st = p->steps[k];
+      mbqc_step& st = p->steps[k];
       const int q = int((st.flags >> 8) & 0xffu);
       const bool app = st.flags & 1u, sneg = st.flags & 2u;
       int i = 0;
@@ -236,10 +245,18 @@ int compile_plan(const mbqc_pattern_desc* d, Plan* p) {
         const float sgn = (__builtin_popcount(unsigned(j) & st.nmask) & 1) ? -1.0f : 1.0f;
         const float c = app ? sgn : 0.0f;
         const float s = sneg ? -1.0f : 1.0f;
-        p->coef[(size_t(k) << ns) + 2 * i] = c * s;
-        p->coef[(size_t(k) << ns) + 2 * i + 1] = c * (1.0f - s);
+        row[2 * i] = c * s;
+        row[2 * i + 1] = c * (1.0f - s);
         ++i;
       }
+      uint32_t r = uint32_t(k);
+      if (!p->coef_per_step) {
+        auto it = row_index.find(row);
+        if (it == row_index.end()) it = row_index.emplace(row, uint32_t(row_index.size())).first;
+        r = it->second;
+      }
+      if (size_t(r) * row.size() == p->coef.size()) p->coef.insert(p->coef.end(), row.begin(), row.end());
+      st.flags = (st.flags & 0xffffu) | (r << 16);
     }
   }
   // Pauli-frame tables for sampled outcomes (PennyLane backend order,

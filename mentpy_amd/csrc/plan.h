@@ -11,6 +11,8 @@ namespace mbqc {
 constexpr int kMaxSlots = 12;   // host tables; kernels support fewer (see launchers)
 constexpr int kMaxDevices = 16;
 constexpr int kMaxCoefSlots = 6;
+// largest pair-coefficient table kept with one row per step
+constexpr size_t kCoefPerStepBytes = 8192;
 
 // Growth control: the kernels skip the 1/sqrt(2) of every appended |+>, so the
 // SV state grows by sqrt(2) (the DM by 2) per append; at the start of a block
@@ -27,10 +29,12 @@ struct Plan {
   std::vector<int32_t> in_index, init_sign, out_pos, out_slot;
   std::vector<int32_t> schedule;          // node order, measured nodes first
   std::vector<std::vector<int>> nbr;      // adjacency (applied edges and dropped ones)
-  // Per-step pair coefficients (statevector kernels, n_slots <= kMaxCoefSlots):
-  // for step k and pair i (the i-th slot index with the measured bit clear),
-  // new[bit=1] = alpha * u0 + beta * x, stored as coef[(k * 2^ns) + 2i + {0,1}].
+  // Pair coefficients (statevector kernels, n_slots <= kMaxCoefSlots): for
+  // step k and pair i (the i-th slot index with the measured bit clear),
+  // new[bit=1] = alpha * u0 + beta * x, stored as coef[(r * 2^ns) + 2i + {0,1}]
+  // with r = steps[k].flags >> 16 (distinct rows stored once).
   std::vector<float> coef;
+  bool coef_per_step = false;   // row of step k is k (table small enough to keep per step)
   // Pauli-frame tables of the sampled mode (empty when not available): per step
   // the 64-bit masks, relative to the step's schedule position, of the nodes
   // whose X (flow successor) / Z (its later neighbours) frame bit an outcome 1
@@ -49,7 +53,7 @@ struct Plan {
 // Device table layout (int32 words), built by pack_device_tables():
 //   [steps: 4*n_steps][in_index: H][init_sign: H][out_pos: H]
 //   [sv_rescale: n_blocks][dm_rescale: n_blocks][sv_e_final, dm_e_final]
-//   [coef: n_steps * 2^ns floats, only when ns <= kMaxCoefSlots]
+//   [coef: n_rows * 2^ns floats (distinct rows), only when ns <= kMaxCoefSlots]
 //   [xmask, zmask: n_steps uint64 each, out_offset: n_out, only when sampling is possible]
 struct TableOffsets {
   int steps, in_index, init_sign, out_pos, sv_rescale, dm_rescale, e_final, coef, xmask, zmask,
