@@ -287,7 +287,9 @@ def config_dm(args, dev, stream):
             "max_trace_distance_vs_c128": float(td.max()),
             "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": FP32_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
-                         "traffic": None, "kernel": "k_dm_forward<3>", "flops_per_sim": f_sim}}
+                         "traffic": None,
+                         "kernel": "k_dm_forward<3>" if B >= 49152 else "k_dm_forward_lds<3,16,4>",
+                         "flops_per_sim": f_sim}}
 
 
 def config_sample(args, dev, stream):
@@ -464,7 +466,7 @@ def main():
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                 "traffic": int(TRAFFIC_BYTES_PER_SIM_4X20 * B),
-                "kernel": "k_sv_forward<4,true>", "flops_per_sim": f_sim}
+                "kernel": "k_sv_forward<4,true,true>", "flops_per_sim": f_sim}
 
     line = {
         "metric": "pattern sims/s + adjoint angle-grads/s, 4x20 grid, c64 vs c128 error",
