@@ -1507,24 +1507,48 @@ __global__ __launch_bounds__(64) void k_gram_seed_mfma(const float2* __restrict_
   const bool arow = ba < B;
   const float2* ta = t + (arow ? ba : 0) * K + lk;
   const float2* mb = MT + int64_t(lk) * K + n0 + li;
-  f32x4 zr = {0.f, 0.f, 0.f, 0.f}, zi = {0.f, 0.f, 0.f, 0.f};
+  // Two accumulator pairs (even / odd k-chunks) keep consecutive MFMAs
+  // independent; the next U chunks are loaded while the current ones run
+  // (one wave per SIMD at the usual sizes, so nothing else hides the loads).
+  f32x4 zr0 = {0.f, 0.f, 0.f, 0.f}, zi0 = {0.f, 0.f, 0.f, 0.f};
+  f32x4 zr1 = {0.f, 0.f, 0.f, 0.f}, zi1 = {0.f, 0.f, 0.f, 0.f};
   constexpr int U = 8;                               // k-chunks of 4 loaded together
-  for (int k0 = 0; k0 < K; k0 += 4 * U) {
-    float2 a[U], b[U];
+  float2 an[U], bn[U];
+  auto fetch = [&](int k0) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int k = k0 + 4 * u;
       const bool in = k < K;
-      a[u] = (in && arow) ? ta[k] : make_float2(0.f, 0.f);
-      b[u] = in ? mb[int64_t(k) * K] : make_float2(0.f, 0.f);
+      an[u] = (in && arow) ? ta[k] : make_float2(0.f, 0.f);
+      bn[u] = in ? mb[int64_t(k) * K] : make_float2(0.f, 0.f);
     }
+  };
+  fetch(0);
+  for (int k0 = 0; k0 < K; k0 += 4 * U) {
+    float2 a[U], b[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      zr = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, b[u].x, zr, 0, 0, 0);
-      zr = __builtin_amdgcn_mfma_f32_16x16x4f32(-a[u].y, b[u].y, zr, 0, 0, 0);
-      zi = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, b[u].y, zi, 0, 0, 0);
-      zi = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].y, b[u].x, zi, 0, 0, 0);
+      a[u] = an[u];
+      b[u] = bn[u];
     }
+    if (k0 + 4 * U < K) fetch(k0 + 4 * U);
+#pragma unroll
+    for (int u = 0; u < U; u += 2) {
+      zr0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, b[u].x, zr0, 0, 0, 0);
+      zi0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, b[u].y, zi0, 0, 0, 0);
+      zr1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u + 1].x, b[u + 1].x, zr1, 0, 0, 0);
+      zi1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u + 1].x, b[u + 1].y, zi1, 0, 0, 0);
+      zr0 = __builtin_amdgcn_mfma_f32_16x16x4f32(-a[u].y, b[u].y, zr0, 0, 0, 0);
+      zi0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].y, b[u].x, zi0, 0, 0, 0);
+      zr1 = __builtin_amdgcn_mfma_f32_16x16x4f32(-a[u + 1].y, b[u + 1].y, zr1, 0, 0, 0);
+      zi1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u + 1].y, b[u + 1].x, zi1, 0, 0, 0);
+    }
+  }
+  f32x4 zr, zi;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    zr[r] = zr0[r] + zr1[r];
+    zi[r] = zi0[r] + zi1[r];
   }
   const float sc = -2.0f * inv_S;
   const int n = n0 + li;
@@ -1551,6 +1575,125 @@ __global__ void k_gram_loss(const float* __restrict__ lpart, int64_t B, int nblk
   float s = 0.0f;
   for (int i = 0; i < nblk; ++i) s += lpart[b * nblk + i];
   loss[b] = 1.0f - s * inv_S;
+}
+
+// ---------------------------------------------------------------- segment chains on MFMA
+// The chain products of the segmented adjoint for 16-amplitude windows as
+// 16x16x16 complex products on the matrix cores, one wave per parameter set:
+// four real v_mfma_f32_16x16x4f32 per k-chunk of 4 (re*re - im*im, re*im +
+// im*re).  Operands come from LDS tiles (column-major, stride 17, zero-padded
+// past their rows/columns): lane l reads A[l%16][k0 + l/16] and
+// B[k0 + l/16][l%16], and holds D[4(l/16) + i][l%16] of the result.  Every
+// tile access walks 16 consecutive rows or a stride-17 row, so the reads are
+// free of bank conflicts.
+constexpr int kChainLd = 17;
+
+// acc += A * B for one k-chunk; a = A[row][kk], b = B[kk][col] of this lane
+__device__ __forceinline__ void cmfma_chunk(f32x4& dr, f32x4& di, float2 a, float2 b) {
+  dr = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, dr, 0, 0, 0);
+  di = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.y, di, 0, 0, 0);
+  dr = __builtin_amdgcn_mfma_f32_16x16x4f32(-a.y, b.y, dr, 0, 0, 0);
+  di = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.x, di, 0, 0, 0);
+}
+
+// 16x16 tile from global (column-major, `rows` x `cols` entries, ld = rows)
+// into LDS (stride 17, zero-padded), by the 64 lanes of the wave.
+__device__ __forceinline__ void chain_tile_load(float2* lds, const float2* src, int rows, int cols) {
+  const int l = threadIdx.x & 63;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int e = l + 64 * i;
+    const int c = e >> 4, r = e & 15;
+    lds[c * kChainLd + r] = (r < rows && c < cols) ? src[c * rows + r] : make_float2(0.f, 0.f);
+  }
+}
+
+// D tile of this lane into LDS (column-major) and, for rows < rows and
+// columns < cols, into global memory (ld = rows).
+__device__ __forceinline__ void chain_tile_store(float2* lds, float2* dst, const f32x4& dr,
+                                                 const f32x4& di, int rows, int cols) {
+  const int l = threadIdx.x & 63;
+  const int c = l & 15, g = l >> 4;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = 4 * g + i;
+    const float2 v = make_float2(dr[i], di[i]);
+    if (lds) lds[c * kChainLd + r] = v;
+    if (dst && r < rows && c < cols) dst[c * rows + r] = v;
+  }
+}
+
+// Per parameter set (one wave): S_{p+1} = T_p S_p and T = T_{n-1} S_{n-1}
+// (16-amplitude windows).  (Fetching every tile up front measured slower than
+// one step's tiles at a time.)
+__global__ __launch_bounds__(64) void k_seg_chain_forward_mfma(SegInfo G, float2* __restrict__ ws,
+                                                               float2* __restrict__ out) {
+  __shared__ float2 Sm[16 * kChainLd], Tm[16 * kChainLd];
+  const int64_t b = blockIdx.x;
+  const int l = threadIdx.x;
+  const int C0 = G.cols0;
+  chain_tile_load(Sm, ws + G.tmat[0] + b * C0 * 16, 16, C0);
+  for (int p = 1; p < G.n; ++p) {
+    const bool last = p + 1 == G.n;
+    const int R = last ? G.rows_last : 16;
+    chain_tile_load(Tm, ws + G.tmat[p] + b * 16 * R, R, 16);
+    wave_lds_sync();
+    f32x4 dr = {0.f, 0.f, 0.f, 0.f}, di = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k0 = 0; k0 < 16; k0 += 4) {
+      const int kk = k0 + (l >> 4), rc = l & 15;
+      cmfma_chunk(dr, di, Tm[kk * kChainLd + rc], Sm[rc * kChainLd + kk]);
+    }
+    wave_lds_sync();
+    if (last)
+      chain_tile_store(nullptr, out + b * C0 * R, dr, di, R, C0);
+    else
+      chain_tile_store(Sm, ws + G.smat[p + 1] + b * C0 * 16, dr, di, 16, C0);
+  }
+}
+
+// Per parameter set (one wave): the segment seeds G_p = L_{p+1} S_p^H with
+// L_p = T_p^H L_{p+1}, L_n = G (16-amplitude windows).
+__global__ __launch_bounds__(64) void k_seg_chain_backward_mfma(SegInfo G, float2* __restrict__ ws,
+                                                                const float2* __restrict__ seed) {
+  __shared__ float2 Lm[16 * kChainLd], Sm[16 * kChainLd], Tm[16 * kChainLd];
+  const int64_t b = blockIdx.x;
+  const int l = threadIdx.x;
+  const int C0 = G.cols0;
+  int R = G.rows_last;                                   // rows of L
+  chain_tile_load(Lm, seed + b * C0 * R, R, C0);
+  const int kk0 = l >> 4, rc = l & 15;
+  for (int p = G.n - 1; p >= 1; --p) {
+    chain_tile_load(Sm, (p == 1 ? ws + G.tmat[0] : ws + G.smat[p]) + b * C0 * 16, 16, C0);
+    chain_tile_load(Tm, ws + G.tmat[p] + b * 16 * R, R, 16);
+    wave_lds_sync();
+    // G_p[r][j] = sum_c L[r][c] conj(S_p[j][c]):  A = L, B[c][j] = conj(S_p[j][c])
+    f32x4 gr = {0.f, 0.f, 0.f, 0.f}, gi = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k0 = 0; k0 < 16; k0 += 4) {
+      const int c = k0 + kk0;
+      cmfma_chunk(gr, gi, Lm[c * kChainLd + rc], cconj(Sm[c * kChainLd + rc]));
+    }
+    // L_p[k][c] = sum_r conj(T_p[r][k]) L[r][c]:  A[k][r] = conj(T_p[r][k]), B = L
+    f32x4 lr = {0.f, 0.f, 0.f, 0.f}, li = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k0 = 0; k0 < 16; k0 += 4) {
+      const int r = k0 + kk0;
+      cmfma_chunk(lr, li, cconj(Tm[rc * kChainLd + r]), Lm[rc * kChainLd + r]);
+    }
+    wave_lds_sync();
+    chain_tile_store(nullptr, ws + G.gmat[p] + b * 16 * R, gr, gi, R, 16);
+    chain_tile_store(Lm, nullptr, lr, li, 16, 16);
+    wave_lds_sync();
+    R = 16;
+  }
+  float2* g0 = ws + G.gmat[0] + b * C0 * 16;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int e = l + 64 * i;
+    const int c = e >> 4, r = e & 15;
+    if (c < C0) g0[c * 16 + r] = Lm[c * kChainLd + r];
+  }
 }
 
 // ---------------------------------------------------------------- optimizer
@@ -2004,16 +2147,24 @@ hipError_t launch_seg_forward(const Job& J, const SegInfo& G, float2* ws, float2
   if (G.wave[G.n] > 0)
     hipLaunchKernelGGL((k_sv_seg_forward<NS>), dim3(G.wave[G.n]), dim3(64), 0, st, J, G, ws, ws);
   hipError_t e = hipGetLastError();
-  if (e == hipSuccess && G.B > 0)
-    hipLaunchKernelGGL((k_seg_chain_forward<NS>), dim3(G.B), dim3(256), 0, st, G, ws, out);
+  if (e == hipSuccess && G.B > 0) {
+    if constexpr (NS == 4)
+      hipLaunchKernelGGL(k_seg_chain_forward_mfma, dim3(G.B), dim3(64), 0, st, G, ws, out);
+    else
+      hipLaunchKernelGGL((k_seg_chain_forward<NS>), dim3(G.B), dim3(256), 0, st, G, ws, out);
+  }
   return e == hipSuccess ? hipGetLastError() : e;
 }
 
 template <int NS>
 hipError_t launch_seg_backward(const Job& J, const SegInfo& G, float2* ws, const float2* seed,
                                int32_t n_angles, float* grad, hipStream_t st) {
-  if (G.B > 0)
-    hipLaunchKernelGGL((k_seg_chain_backward<NS>), dim3(G.B), dim3(256), 0, st, G, ws, seed);
+  if (G.B > 0) {
+    if constexpr (NS == 4)
+      hipLaunchKernelGGL(k_seg_chain_backward_mfma, dim3(G.B), dim3(64), 0, st, G, ws, seed);
+    else
+      hipLaunchKernelGGL((k_seg_chain_backward<NS>), dim3(G.B), dim3(256), 0, st, G, ws, seed);
+  }
   hipError_t e = hipGetLastError();
   if (e == hipSuccess && G.wave[G.n] > 0)
     hipLaunchKernelGGL((k_sv_seg_backward<NS>), dim3(G.wave[G.n]), dim3(64), 0, st, J, G, ws,

@@ -253,13 +253,14 @@ def test_fused_gram_loss_matches_library_gemm(rows, cols, S, B, rng):
 
 @pytest.mark.parametrize("rows,cols,S,segs,mode", [
     (4, 16, 256, 8, "gram"), (4, 16, 40, 2, "transfer"), (2, 7, 9, 3, "gram"), (3, 6, 20, 4, "gram"),
-    (5, 5, 40, 2, "transfer"), (1, 9, 5, 4, "gram"), (4, 6, 30, 5, "gram"), (3, 11, 24, 10, "gram")])
+    (5, 5, 40, 2, "transfer"), (1, 9, 5, 4, "gram"), (4, 6, 30, 5, "gram"), (3, 11, 24, 10, "gram"),
+    (4, 20, 40, 15, "gram"), (4, 20, 40, 16, "transfer")])
 def test_segmented_adjoint_matches_unsegmented(rows, cols, S, segs, mode, rng):
     """The time-segmented transfer-matrix adjoint (segments' transfer matrices
     computed concurrently, chained per parameter set) gives the transfer
     matrix, loss and gradient of the single sweep, and the exact shift rule;
     segment counts up to one block per segment, set counts that leave
-    partial waves."""
+    partial waves; 16-amplitude windows chain on the matrix cores."""
     torch = _torch()
     import mentpy_amd as mp
     from mentpy_amd.gradients import MBQMLStep
