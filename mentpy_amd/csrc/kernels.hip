@@ -158,23 +158,69 @@ __device__ __forceinline__ const float* coef_row(const float* coef, const mbqc_s
   return coef + (int(steps[k].flags >> 16) << NS);
 }
 
-template <int NS, bool REF, bool FULL, bool PER_STEP, int SUB, typename Mod>
-__device__ __forceinline__ void sv_sub(float2 (&t)[1 << NS], const mbqc_step* steps,
-                                       const float* coef, int k0, int n_steps, const float (&th)[NS],
-                                       RefPhase* ph, Mod& mod) {
-  const int k = k0 + SUB;
-  if (FULL || k < n_steps) {
-    const float2 b = mod(k, step_coeff<REF>(th[SUB], ph));
-    sv_step<NS, NS - 1 - SUB>(t, b, coef_row<NS, PER_STEP>(coef, steps, k));
+// Lanes that share an angle vector in aligned groups of G = kShareGroup<NS>
+// consecutive lanes (S a multiple of G inputs per vector) can share the sine
+// and cosine work of a block: lane j of a group evaluates step j and the
+// values are broadcast inside the group (DPP quad permutations for G <= 4, a
+// ds_swizzle for G = 8; no memory access).
+__host__ __device__ constexpr int share_group(int ns) { return ns <= 2 ? 2 : (ns <= 4 ? 4 : 8); }
+template <int NS>
+constexpr int kShareGroup = share_group(NS);
+
+template <int G, int SRC>
+__device__ __forceinline__ float group_bcast(float v) {
+  if constexpr (G == 2) {
+    constexpr int ctrl = SRC | (SRC << 2) | ((2 + SRC) << 4) | ((2 + SRC) << 6);
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xf, 0xf, false));
+  } else if constexpr (G == 4) {
+    constexpr int ctrl = SRC | (SRC << 2) | (SRC << 4) | (SRC << 6);
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xf, 0xf, false));
+  } else {
+    // bit mode: lane' = (lane & and_mask) | or_mask within each 32-lane half
+    constexpr int pattern = (0x20 - G) | (SRC << 5);
+    return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), pattern));
   }
 }
 
-template <int NS, bool REF, bool FULL, bool PER_STEP, typename Mod, int... SUBS>
+template <int NS, int... SUBS>
+__device__ __forceinline__ void shared_block_coeffs(const float (&th)[NS], float2 (&bb)[NS],
+                                                    std::integer_sequence<int, SUBS...>) {
+  constexpr int G = kShareGroup<NS>;
+  static_assert(NS >= 2 && NS <= G, "one step per lane of the group");
+  const int j = int(threadIdx.x) & (G - 1);
+  float mine = th[0];
+#pragma unroll
+  for (int sub = 1; sub < NS; ++sub) mine = (j == sub) ? th[sub] : mine;
+  float sn, cs;
+  sincos_poly(mine, &sn, &cs);
+  ((bb[SUBS] = make_float2(group_bcast<G, SUBS>(cs), -group_bcast<G, SUBS>(sn))), ...);
+}
+
+template <int NS, bool REF, bool FULL, bool PER_STEP, bool SH, int SUB, typename Mod>
+__device__ __forceinline__ void sv_sub(float2 (&t)[1 << NS], const mbqc_step* steps,
+                                       const float* coef, int k0, int n_steps, const float (&th)[NS],
+                                       const float2 (&bb)[NS], RefPhase* ph, Mod& mod) {
+  const int k = k0 + SUB;
+  if (FULL || k < n_steps) {
+    float2 b0;
+    if constexpr (SH) {
+      if constexpr (REF) ph->add(th[SUB]);
+      b0 = bb[SUB];
+    } else {
+      b0 = step_coeff<REF>(th[SUB], ph);
+    }
+    sv_step<NS, NS - 1 - SUB>(t, mod(k, b0), coef_row<NS, PER_STEP>(coef, steps, k));
+  }
+}
+
+template <int NS, bool REF, bool FULL, bool PER_STEP, bool SH, typename Mod, int... SUBS>
 __device__ __forceinline__ void sv_block(float2 (&t)[1 << NS], const mbqc_step* steps,
                                          const float* coef, int k0, int n_steps,
                                          const float (&th)[NS], RefPhase* ph, Mod& mod,
-                                         std::integer_sequence<int, SUBS...>) {
-  (sv_sub<NS, REF, FULL, PER_STEP, SUBS>(t, steps, coef, k0, n_steps, th, ph, mod), ...);
+                                         std::integer_sequence<int, SUBS...> seq) {
+  float2 bb[NS];
+  if constexpr (SH) shared_block_coeffs<NS>(th, bb, seq);
+  (sv_sub<NS, REF, FULL, PER_STEP, SH, SUBS>(t, steps, coef, k0, n_steps, th, bb, ph, mod), ...);
 }
 
 // Angles of the NS steps of a block (steps past the end get 0).  The index is
@@ -199,7 +245,8 @@ __device__ __forceinline__ void fetch_block_angles(const mbqc_step* steps, int k
 // Run blocks [blk0, blk1) of NS steps: full blocks of NS unrolled steps with
 // the next block's angles in flight, then the partial tail block.
 // `on_block(blk, t)` runs at the start of every block (after its rescale).
-template <int NS, bool REF, bool PER_STEP, typename OnBlock, typename Mod = NoModify>
+template <int NS, bool REF, bool PER_STEP, bool SH = false, typename OnBlock,
+          typename Mod = NoModify>
 __device__ __forceinline__ void sv_run_blocks(float2 (&t)[1 << NS], const Job& J, const float* ang,
                                               RefPhase* ph, int blk0, int blk1, OnBlock on_block,
                                               Mod&& mod = Mod()) {
@@ -223,10 +270,11 @@ __device__ __forceinline__ void sv_run_blocks(float2 (&t)[1 << NS], const Job& J
     }
     on_block(blk, t);
     if (blk < nfull)
-      sv_block<NS, REF, true, PER_STEP>(t, steps, coef, blk * NS, n, th, ph, mod, std::make_integer_sequence<int, NS>{});
+      sv_block<NS, REF, true, PER_STEP, SH>(t, steps, coef, blk * NS, n, th, ph, mod,
+                                            std::make_integer_sequence<int, NS>{});
     else
-      sv_block<NS, REF, false, PER_STEP>(t, steps, coef, blk * NS, n, th, ph, mod,
-                               std::make_integer_sequence<int, NS>{});
+      sv_block<NS, REF, false, PER_STEP, SH>(t, steps, coef, blk * NS, n, th, ph, mod,
+                                             std::make_integer_sequence<int, NS>{});
   }
 }
 
@@ -323,7 +371,7 @@ enum StoreMode {
 // consecutive simulations g - lane .. g - lane + 63, of which the first
 // rows_valid are written (kStoreDirect: active lanes write).  Inactive lanes
 // run on simulation 0.
-template <int NS, bool REF, int STORE, bool PER_STEP = false>
+template <int NS, bool REF, int STORE, bool PER_STEP = false, bool SH = false>
 __device__ __forceinline__ void sv_sim(const Job& J, int64_t g, bool active, int rows_valid,
                                        uint32_t flags) {
   constexpr int H = 1 << NS;
@@ -333,8 +381,8 @@ __device__ __forceinline__ void sv_sim(const Job& J, int64_t g, bool active, int
   float2 t[H];
   sv_load_input<NS>(J, J.psi + si * J.in_stride, t);
   RefPhase rp;
-  sv_run_blocks<NS, REF, PER_STEP>(t, J, ang, &rp, 0, (J.n_steps + NS - 1) / NS,
-                                   [](int, const float2 (&)[H]) {});
+  sv_run_blocks<NS, REF, PER_STEP, SH>(t, J, ang, &rp, 0, (J.n_steps + NS - 1) / NS,
+                                       [](int, const float2 (&)[H]) {});
   const float2 ph = REF ? rp.value() : make_float2(1.0f, 0.0f);
   // output gather: slot-index j -> output index out_pos[j] (dead slots skipped)
   const int32_t* out_pos = J.tab + J.off_out;
@@ -1912,7 +1960,7 @@ __device__ __forceinline__ Job load_job(const Job* jobs, int i) {
   return j;
 }
 
-template <bool REF, int NS>
+template <bool REF, int NS, bool SH>
 __global__ __launch_bounds__(256) void k_batch(const Job* __restrict__ jobs,
                                                const BatchItem* __restrict__ items, int64_t n_items,
                                                int* counter, uint32_t flags) {
@@ -1930,7 +1978,8 @@ __global__ __launch_bounds__(256) void k_batch(const Job* __restrict__ jobs,
       const int rows = int(std::min<int64_t>(std::min<int64_t>(I.count - off, left), 64));
       // direct stores: staging tiles would cost the concurrently running
       // window-size groups LDS occupancy (measured slower)
-      sv_sim<NS, REF, kStoreDirect>(J, g, (off + lane) < I.count && g < J.total, rows, flags);
+      sv_sim<NS, REF, kStoreDirect, false, SH>(J, g, (off + lane) < I.count && g < J.total, rows,
+                                               flags);
     }
   }
 }
@@ -2172,6 +2221,36 @@ hipError_t launch_seg_backward(const Job& J, const SegInfo& G, float2* ws, const
   return e == hipSuccess ? hipGetLastError() : e;
 }
 
+struct BatchGroupLaunch {
+  dim3 grid, block;
+  hipStream_t stream;
+  const Job* jobs;
+  const BatchItem* items;
+  int64_t n_items;
+  int* counter;
+  uint32_t flags;
+};
+
+// Lanes g = b*S + s of one angle vector form aligned groups of
+// share_group(NS) lanes when every job's S is a multiple of it (work items
+// start at multiples of 64 and B*S is then a multiple of the group too).
+template <int NS>
+void launch_batch_group(const BatchGroupLaunch& L, bool ref, bool share) {
+#define MBQC_BATCH_LAUNCH(R, SH)                                                              \
+  hipLaunchKernelGGL((k_batch<R, NS, SH>), L.grid, L.block, 0, L.stream, L.jobs, L.items, \
+                     L.n_items, L.counter, L.flags)
+  if constexpr (NS >= 2) {
+    if (share) {
+      if (ref) MBQC_BATCH_LAUNCH(true, true);
+      else MBQC_BATCH_LAUNCH(false, true);
+      return;
+    }
+  }
+  if (ref) MBQC_BATCH_LAUNCH(true, false);
+  else MBQC_BATCH_LAUNCH(false, false);
+#undef MBQC_BATCH_LAUNCH
+}
+
 constexpr int kMaxSvSlots = 6;       // register-resident SV kernels
 // below this many simulations the DM path keeps each density matrix in LDS
 // (one quartet per thread, several simulations per workgroup)
@@ -2248,6 +2327,8 @@ struct mbqc_batch {
   hipStream_t side[kMaxSvSlots + 1] = {};
   hipEvent_t fork = nullptr;
   hipEvent_t join[kMaxSvSlots + 1] = {};
+  // every job of the group has S a multiple of share_group(ns)
+  bool share[kMaxSvSlots + 1] = {};
 };
 
 extern "C" {
@@ -2575,6 +2656,7 @@ int mbqc_batch_create(const mbqc_batch_job* jobs, int32_t n_jobs, int32_t sims_p
   if (sims_per_item <= 0 || sims_per_item % 64 != 0)
     return fail(MBQC_ERR_VALUE, "sims_per_item must be a positive multiple of 64");
   auto* bt = new mbqc_batch();
+  for (bool& sh : bt->share) sh = true;
   struct Key {
     int ns;
     int64_t cost;
@@ -2602,6 +2684,7 @@ int mbqc_batch_create(const mbqc_batch_job* jobs, int32_t n_jobs, int32_t sims_p
     }
     bt->jobs.push_back(J);
     bt->max_ns = std::max(bt->max_ns, p->ns);
+    if (p->ns < 2 || J.S % share_group(p->ns) != 0) bt->share[p->ns] = false;
     const int64_t cost = int64_t(p->n_steps) << p->ns;
     for (int64_t f = 0; f < J.total; f += sims_per_item) {
       BatchItem it{};
@@ -2758,24 +2841,16 @@ int mbqc_batch_launch(mbqc_batch* bt, int32_t waves, int32_t wg_waves, uint32_t 
     const int blocks = int(wv / wg_waves);
     const BatchItem* items = bt->d_items + bt->group_begin[ns];
     int* counter = bt->d_counter + ns * kCounterStride;
-#define MBQC_BATCH_LAUNCH(R, N)                                                                  \
-  hipLaunchKernelGGL((k_batch<R, N>), dim3(blocks), dim3(threads), 0, gs, bt->d_jobs, items, n, \
-                     counter, flags)
-    switch (ns * 2 + (ref ? 1 : 0)) {
-      case 2: MBQC_BATCH_LAUNCH(false, 1); break;
-      case 3: MBQC_BATCH_LAUNCH(true, 1); break;
-      case 4: MBQC_BATCH_LAUNCH(false, 2); break;
-      case 5: MBQC_BATCH_LAUNCH(true, 2); break;
-      case 6: MBQC_BATCH_LAUNCH(false, 3); break;
-      case 7: MBQC_BATCH_LAUNCH(true, 3); break;
-      case 8: MBQC_BATCH_LAUNCH(false, 4); break;
-      case 9: MBQC_BATCH_LAUNCH(true, 4); break;
-      case 10: MBQC_BATCH_LAUNCH(false, 5); break;
-      case 11: MBQC_BATCH_LAUNCH(true, 5); break;
-      case 12: MBQC_BATCH_LAUNCH(false, 6); break;
-      default: MBQC_BATCH_LAUNCH(true, 6); break;
+const BatchGroupLaunch L{dim3(blocks), dim3(threads), gs, bt->d_jobs, items, n, counter, flags};
+    const bool sh = bt->share[ns];
+    switch (ns) {
+      case 1: launch_batch_group<1>(L, ref, false); break;
+      case 2: launch_batch_group<2>(L, ref, sh); break;
+      case 3: launch_batch_group<3>(L, ref, sh); break;
+      case 4: launch_batch_group<4>(L, ref, sh); break;
+      case 5: launch_batch_group<5>(L, ref, sh); break;
+      default: launch_batch_group<6>(L, ref, sh); break;
     }
-#undef MBQC_BATCH_LAUNCH
     e = hipGetLastError();
     if (e == hipSuccess && concurrent) {
       e = hipEventRecord(bt->join[ns], gs);

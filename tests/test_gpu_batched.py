@@ -57,3 +57,32 @@ def test_batched_matches_single_launches(rng, tmp_path):
         for o, want in zip(bf.outputs, outs):
             assert torch.equal(o, want), f"graph replay {rep} differs"
     print(f"autotune chose {cfg}", flush=True)
+
+
+@pytest.mark.parametrize("S_list", [(8, 8, 16, 8, 16, 8), (8, 3, 16, 8, 4, 8)])
+def test_batched_shared_coefficient_groups(S_list, rng):
+    """Window-size groups whose jobs all have S a multiple of the lane-group
+    size (2 lanes for 2-slot windows, 4 for 3-4, 8 for 5-6) evaluate each
+    block's sines/cosines once per group of lanes; a group with one job that
+    cannot share (S = 3, or S = 4 with 6 slots) runs unshared.  Either way
+    every job equals its single launch and the oracle."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from mentpy_amd.batched import BatchedForward
+    from oracle.vectorized import sv_batch
+    jobs = []
+    for i, S in enumerate(S_list):
+        r = 2 + i % 5
+        g = mp.templates.grid_cluster(r, 5 + i)
+        sim = mp.PatternSimulator(g, backend="hip-sv")
+        th = rng.uniform(0, 2 * np.pi, (24, len(g.trainable_nodes))).astype(np.float32)
+        st = np.stack([haar_state(r, rng) for _ in range(S)])
+        jobs.append((sim, th, st, g))
+    bf = BatchedForward([(s, th, st) for s, th, st, _ in jobs], sims_per_item=128)
+    outs = [o.clone() for o in bf.launch(waves=512, wg_waves=2)]
+    torch.cuda.synchronize()
+    for (sim, th, st, g), o in zip(jobs, outs):
+        single = sim.run_batch(th, st)
+        assert (single - o).abs().max().item() < 1e-6
+        ref = sv_batch(g, th[:3].astype(np.float64), st)
+        assert np.abs(o[:3].cpu().numpy() - ref).max() < 2e-5
