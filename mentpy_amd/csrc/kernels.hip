@@ -332,15 +332,20 @@ __device__ __forceinline__ void wave_lds_sync() {
 // [c0, c0 + 16) of all 64 rows, so a store instruction covers four 128-byte
 // row segments instead of one entry in each of 64 rows.  Used for outputs of
 // 32 amplitudes (five live slots); rows r >= rows_valid are not written.
+// The tiles are dynamic LDS (kColumnTile entries per wave of the workgroup),
+// so a one-wave workgroup reserves one tile.
+constexpr int kColumnTile = 64 * 17;
+
 template <int NS>
 __device__ __forceinline__ void wave_store_rows(const float2 (&t)[1 << NS], const int32_t* out_pos,
                                                 float2 f, float2* obase, int n_out, int rows_valid) {
   constexpr int H = 1 << NS;
   constexpr int CW = H < 16 ? H : 16;
   constexpr int LOG_CW = NS < 4 ? NS : 4;
-  __shared__ float2 stage[4][64 * (CW + 1)];
+  static_assert(64 * (CW + 1) <= kColumnTile, "tile size");
+  extern __shared__ float2 column_tiles[];
   const int lane = threadIdx.x & 63;
-  float2* sw = stage[threadIdx.x >> 6];
+  float2* sw = column_tiles + (threadIdx.x >> 6) * kColumnTile;
   const int dout = 1 << n_out;
   const int cs = n_out < LOG_CW ? n_out : LOG_CW;   // log2 of the columns per pass
   const int cw = 1 << cs;
@@ -1960,6 +1965,12 @@ __device__ __forceinline__ Job load_job(const Job* jobs, int i) {
   return j;
 }
 
+// Output stores of the batched kernel: direct (staging the 32-amplitude rows
+// through LDS tiles measured slower here: 3.54e9 -> 3.27e9 sims/s on the
+// 64-grid batch, whose window-size groups run concurrently).
+template <int NS>
+constexpr int kBatchStore = kStoreDirect;
+
 template <bool REF, int NS, bool SH>
 __global__ __launch_bounds__(256) void k_batch(const Job* __restrict__ jobs,
                                                const BatchItem* __restrict__ items, int64_t n_items,
@@ -1978,8 +1989,8 @@ __global__ __launch_bounds__(256) void k_batch(const Job* __restrict__ jobs,
       const int rows = int(std::min<int64_t>(std::min<int64_t>(I.count - off, left), 64));
       // direct stores: staging tiles would cost the concurrently running
       // window-size groups LDS occupancy (measured slower)
-      sv_sim<NS, REF, kStoreDirect, false, SH>(J, g, (off + lane) < I.count && g < J.total, rows,
-                                               flags);
+      sv_sim<NS, REF, kBatchStore<NS>, false, SH>(J, g, (off + lane) < I.count && g < J.total, rows,
+                                                  flags);
     }
   }
 }
@@ -2075,17 +2086,18 @@ template <int NS>
 hipError_t launch_sv(const Job& J, bool per_step, uint32_t flags, hipStream_t st) {
   const int threads = 256;
   const int64_t blocks = (J.total + threads - 1) / threads;
+  const size_t lds = kForwardStore<NS> == kStoreColumns ? (threads / 64) * kColumnTile * sizeof(float2) : 0;
   const bool ref = flags & MBQC_FLAG_REF_PHASE;
   if (per_step) {
     if (ref)
-      hipLaunchKernelGGL((k_sv_forward<NS, true, true>), dim3(blocks), dim3(threads), 0, st, J, flags);
+      hipLaunchKernelGGL((k_sv_forward<NS, true, true>), dim3(blocks), dim3(threads), lds, st, J, flags);
     else
-      hipLaunchKernelGGL((k_sv_forward<NS, false, true>), dim3(blocks), dim3(threads), 0, st, J, flags);
+      hipLaunchKernelGGL((k_sv_forward<NS, false, true>), dim3(blocks), dim3(threads), lds, st, J, flags);
   } else {
     if (ref)
-      hipLaunchKernelGGL((k_sv_forward<NS, true, false>), dim3(blocks), dim3(threads), 0, st, J, flags);
+      hipLaunchKernelGGL((k_sv_forward<NS, true, false>), dim3(blocks), dim3(threads), lds, st, J, flags);
     else
-      hipLaunchKernelGGL((k_sv_forward<NS, false, false>), dim3(blocks), dim3(threads), 0, st, J, flags);
+      hipLaunchKernelGGL((k_sv_forward<NS, false, false>), dim3(blocks), dim3(threads), lds, st, J, flags);
   }
   return hipGetLastError();
 }
@@ -2236,8 +2248,10 @@ struct BatchGroupLaunch {
 // start at multiples of 64 and B*S is then a multiple of the group too).
 template <int NS>
 void launch_batch_group(const BatchGroupLaunch& L, bool ref, bool share) {
-#define MBQC_BATCH_LAUNCH(R, SH)                                                              \
-  hipLaunchKernelGGL((k_batch<R, NS, SH>), L.grid, L.block, 0, L.stream, L.jobs, L.items, \
+  const size_t lds =
+      kBatchStore<NS> == kStoreColumns ? (L.block.x / 64) * kColumnTile * sizeof(float2) : 0;
+#define MBQC_BATCH_LAUNCH(R, SH)                                                                \
+  hipLaunchKernelGGL((k_batch<R, NS, SH>), L.grid, L.block, lds, L.stream, L.jobs, L.items, \
                      L.n_items, L.counter, L.flags)
   if constexpr (NS >= 2) {
     if (share) {
