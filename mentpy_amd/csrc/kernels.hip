@@ -981,7 +981,10 @@ __global__ __launch_bounds__(T * G) void k_dm_forward_lds(Job J, DmNoise nz, uin
 template <int NS>
 // ckpt is __restrict__ so the checkpoint stores inside the step loop do not
 // count as possible writes to the plan tables (which keeps table reads scalar).
-__global__ __launch_bounds__(256) void k_sv_adj_forward(Job J, float2* __restrict__ ckpt) {
+// A checkpoint is kept every `ck` blocks (1 or 2; 2 halves the checkpoint
+// traffic, which bounds large batches, for one extra block of recompute in
+// the backward sweep).
+__global__ __launch_bounds__(256) void k_sv_adj_forward(Job J, float2* __restrict__ ckpt, int ck) {
   constexpr int H = 1 << NS;
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const bool active = g < J.total;
@@ -993,9 +996,10 @@ __global__ __launch_bounds__(256) void k_sv_adj_forward(Job J, float2* __restric
   RefPhase ph;
   const int64_t total = J.total;
   sv_run_steps<NS, false>(t, J, ang, &ph, [&](int blk, const float2 (&tt)[H]) {
-    if (active) {
+    if (active && (blk & (ck - 1)) == 0) {
+      const int64_t c = blk >> (ck - 1);
 #pragma unroll
-      for (int j = 0; j < H; ++j) ckpt[(int64_t(blk) * H + j) * total + g] = tt[j];
+      for (int j = 0; j < H; ++j) ckpt[(c * H + j) * total + g] = tt[j];
     }
   });
   const int32_t* out_pos = J.tab + J.off_out;
@@ -1051,6 +1055,7 @@ __device__ __forceinline__ void take_y(const float2 (&t)[1 << NS], float2 (&y)[1
 
 struct AdjCtx {
   float* gpart;     // [B][P][n_angles]
+  int32_t ck;       // blocks per checkpoint (1 or 2)
   int64_t C;        // columns per parameter set
   int32_t P;        // partial sums per parameter set
   int32_t red;      // lanes reduced together (1, pow2 <= 64)
@@ -1160,16 +1165,42 @@ __global__ __launch_bounds__(64) void k_sv_adj_backward(Job J, const float2* __r
   const bool writer = active && (A.red == 1 || (ci & (A.red - 1)) == 0);
   // inactive lanes recompute simulation 0 (their sums stay in their own,
   // inactive lane groups and are never written)
-  for (int blk = nblk - 1; blk >= 0; --blk) {
-    float2 t[H];
-    const float2* cb = ckpt + int64_t(blk) * H * J.total + gg;
+  const int ck = A.ck;
+  const int ngrp = (nblk + ck - 1) / ck;
+  for (int grp = ngrp - 1; grp >= 0; --grp) {
+    float2 s0[H];
+    const float2* cb = ckpt + int64_t(grp) * H * J.total + gg;
 #pragma unroll
-    for (int j = 0; j < H; ++j) t[j] = cb[int64_t(j) * J.total];
-    float th[NS];
-    load_block_angles<NS>(steps, blk * NS, n, ang, J.angle_stride_a, th);
-    adj_block<NS>(t, lam, steps, coef, blk * NS, n, th, A, prow, writer,
+    for (int j = 0; j < H; ++j) s0[j] = cb[int64_t(j) * J.total];
+    const int b0 = grp * ck;
+    float th0[NS];
+    load_block_angles<NS>(steps, b0 * NS, n, ang, J.angle_stride_a, th0);
+    if (ck == 2 && b0 + 1 < nblk) {
+      // the state at block b0 + 1: block b0 (a full block) from the
+      // checkpoint, then b0 + 1's rescale
+      float2 t[H];
+#pragma unroll
+      for (int j = 0; j < H; ++j) t[j] = s0[j];
+      RefPhase ph;
+      NoModify nomod;
+      sv_block<NS, false, true, false, false>(t, steps, coef, b0 * NS, n, th0, &ph, nomod,
+                                              std::make_integer_sequence<int, NS>{});
+      if (resc[b0 + 1]) {
+#pragma unroll
+        for (int j = 0; j < H; ++j) t[j] = cscale(t[j], 0x1p-32f);
+      }
+      float th1[NS];
+      load_block_angles<NS>(steps, (b0 + 1) * NS, n, ang, J.angle_stride_a, th1);
+      adj_block<NS>(t, lam, steps, coef, (b0 + 1) * NS, n, th1, A, prow, writer,
+                    std::make_integer_sequence<int, NS>{});
+      if (resc[b0 + 1]) {
+#pragma unroll
+        for (int j = 0; j < H; ++j) lam[j] = cscale(lam[j], 0x1p-32f);
+      }
+    }
+    adj_block<NS>(s0, lam, steps, coef, b0 * NS, n, th0, A, prow, writer,
                   std::make_integer_sequence<int, NS>{});
-    if (resc[blk]) {
+    if (resc[b0]) {
 #pragma unroll
       for (int j = 0; j < H; ++j) lam[j] = cscale(lam[j], 0x1p-32f);
     }
@@ -2148,13 +2179,19 @@ hipError_t launch_dm(const Job& J, const DmNoise& nz, uint32_t flags, hipStream_
   return hipGetLastError();
 }
 
+// Blocks per checkpoint of the unsegmented adjoint: from 65,536 columns the
+// sweeps are bound by the checkpoint traffic (one 2^NS-amplitude state per
+// column and block, written once and read once), so every second block is
+// kept and the backward sweep recomputes the other from it.
+inline int adj_ckpt_blocks(int64_t n_columns) { return n_columns >= 65536 ? 2 : 1; }
+
 template <int NS>
-hipError_t launch_adj_fwd(const Job& J, float2* ckpt, hipStream_t st) {
+hipError_t launch_adj_fwd(const Job& J, float2* ckpt, int ck, hipStream_t st) {
   // one wave per workgroup: at small batches the waves spread over all CUs
   // instead of sharing a few CUs' scalar units and instruction issue
   const int threads = 64;
   const int64_t blocks = (J.total + threads - 1) / threads;
-  hipLaunchKernelGGL((k_sv_adj_forward<NS>), dim3(blocks), dim3(threads), 0, st, J, ckpt);
+  hipLaunchKernelGGL((k_sv_adj_forward<NS>), dim3(blocks), dim3(threads), 0, st, J, ckpt, ck);
   return hipGetLastError();
 }
 
@@ -2518,7 +2555,8 @@ int mbqc_dm_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
 size_t mbqc_sv_adjoint_workspace_bytes(const mbqc_plan* plan, int64_t n_columns) {
   if (!plan || n_columns < 0) return 0;
   const Plan* p = reinterpret_cast<const Plan*>(plan);
-  return size_t(p->n_blocks()) * (size_t(1) << p->ns) * size_t(n_columns) * 8u;
+  const int ck = adj_ckpt_blocks(n_columns);
+  return size_t((p->n_blocks() + ck - 1) / ck) * (size_t(1) << p->ns) * size_t(n_columns) * 8u;
 }
 
 int32_t mbqc_sv_segments_auto(const mbqc_plan* plan, int64_t B) {
@@ -2615,11 +2653,11 @@ int mbqc_sv_adjoint_forward_c64(const mbqc_plan* plan, const float* angles, int6
   float2* ck = static_cast<float2*>(ckpt);
   hipError_t e = hipSuccess;
   switch (p->ns) {
-    case 1: e = launch_adj_fwd<1>(J, ck, st); break;
-    case 2: e = launch_adj_fwd<2>(J, ck, st); break;
-    case 3: e = launch_adj_fwd<3>(J, ck, st); break;
-    case 4: e = launch_adj_fwd<4>(J, ck, st); break;
-    case 5: e = launch_adj_fwd<5>(J, ck, st); break;
+    case 1: e = launch_adj_fwd<1>(J, ck, adj_ckpt_blocks(B * C), st); break;
+    case 2: e = launch_adj_fwd<2>(J, ck, adj_ckpt_blocks(B * C), st); break;
+    case 3: e = launch_adj_fwd<3>(J, ck, adj_ckpt_blocks(B * C), st); break;
+    case 4: e = launch_adj_fwd<4>(J, ck, adj_ckpt_blocks(B * C), st); break;
+    case 5: e = launch_adj_fwd<5>(J, ck, adj_ckpt_blocks(B * C), st); break;
   }
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "adjoint forward launch");
 }
@@ -2648,6 +2686,7 @@ int mbqc_sv_adjoint_backward_c64(const mbqc_plan* plan, const float* angles, int
   A.P = mbqc_sv_adjoint_partials(C);
   A.red = (C <= 64 && (C & (C - 1)) == 0) ? int32_t(C) : (C % 64 == 0 ? 64 : 1);
   A.n_angles = p->n_angles;
+  A.ck = adj_ckpt_blocks(B * C);
   hipStream_t st = static_cast<hipStream_t>(stream);
   const float2* ck = static_cast<const float2*>(ckpt);
   const float2* sd = static_cast<const float2*>(seed);

@@ -288,3 +288,35 @@ def test_segmented_adjoint_matches_unsegmented(rows, cols, S, segs, mode, rng):
     cost = _oracle_cost(g, X, Y)
     ref_g = exact_shift_gradient(cost, th[0].astype(np.float64))
     assert np.abs(g2[0].cpu().numpy() - ref_g).max() < 2e-4
+
+
+@pytest.mark.parametrize("rows,cols,mode,S,B", [(4, 16, "gram", 64, 4096), (2, 7, "direct", 256, 256),
+                                                (3, 6, "direct", 128, 512)])
+def test_adjoint_checkpoint_pairs(rows, cols, mode, S, B, rng):
+    """From 65,536 columns the unsegmented adjoint keeps one checkpoint per
+    two blocks and recomputes the other block in the backward sweep (odd and
+    even block counts): it matches the exact shift rule of the oracle and,
+    for the transfer matrix, the time-segmented path."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from mentpy_amd.gradients import MBQMLStep
+    from oracle.gradients import exact_shift_gradient
+    g = mp.templates.grid_cluster(rows, cols)
+    sim = mp.PatternSimulator(g, backend="hip-sv")
+    X, Y = _dataset(rows, S, rng)
+    th = rng.uniform(0, 2 * np.pi, (B, len(g.trainable_nodes))).astype(np.float32)
+    one = MBQMLStep(sim, X, Y, n_sets=B, mode=mode, segments=1)
+    l1, g1 = one.run(th)
+    l1, g1 = l1.clone(), g1.clone()
+    if mode == "gram":
+        cut = MBQMLStep(sim, X, Y, n_sets=B, mode=mode, segments=2)
+        l2, g2 = cut.run(th)
+        assert (l1 - l2).abs().max().item() < 2e-6
+        assert (g1 - g2).abs().max().item() < 2e-5
+    cost = _oracle_cost(g, X, Y)
+    for b in (0, B - 1):
+        ref = exact_shift_gradient(cost, th[b].astype(np.float64))
+        err = np.abs(g1[b].cpu().numpy() - ref).max()
+        print(f"{rows}x{cols} {mode} B={B} set {b}: grad err {err:.2e}", flush=True)
+        assert err < 2e-4
+        assert abs(l1[b].item() - cost(th[b].astype(np.float64))) < 2e-5
