@@ -266,6 +266,11 @@ int mbqc_batch_launch(mbqc_batch* batch, int32_t waves, int32_t wg_waves, uint32
  * transposed Hermitian data Gram matrix ([K][K]).  workspace:
  * mbqc_gram_workspace_bytes(B, K) bytes of device memory.  K <= 512. */
 size_t mbqc_gram_workspace_bytes(int64_t B, int32_t K);
+/* The same loss and seed when z = M t ([B][K]) was computed elsewhere (a
+ * library GEMM, faster than the fused kernel for many thousands of sets):
+ * loss[b] = 1 - Re(t_b^H z_b)/S, seed[b] = -2 z_b/S.  Any K. */
+int mbqc_gram_finish_c64(const void* t, const void* z, int64_t B, int32_t K, int32_t S, float* loss,
+                         void* seed, void* stream);
 int mbqc_gram_loss_seed_c64(const void* t, const void* MT, int64_t B, int32_t K, int32_t S,
                             float* loss, void* seed, float* workspace, void* stream);
 

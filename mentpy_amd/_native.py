@@ -30,7 +30,7 @@ EXPORTED_SYMBOLS = (
     "mbqc_causal_flow", "mbqc_plan_create", "mbqc_plan_destroy", "mbqc_plan_get_info", "mbqc_plan_get_tables",
     "mbqc_plan_get_rescale", "mbqc_sv_forward_c64", "mbqc_sv_sample_c64", "mbqc_dm_forward_c64",
     "mbqc_plan_window_qubits", "mbqc_sv_window_c64", "mbqc_dm_window_c64",
-    "mbqc_gram_workspace_bytes", "mbqc_gram_loss_seed_c64",
+    "mbqc_gram_workspace_bytes", "mbqc_gram_loss_seed_c64", "mbqc_gram_finish_c64",
     "mbqc_sv_adjoint_workspace_bytes", "mbqc_sv_adjoint_forward_c64",
     "mbqc_sv_adjoint_partials", "mbqc_sv_adjoint_backward_c64", "mbqc_sv_segments_auto",
     "mbqc_sv_segmented_workspace_bytes", "mbqc_sv_segmented_forward_c64",
@@ -130,6 +130,8 @@ def lib() -> C.CDLL:
     L.mbqc_sv_adjoint_partials.restype = i32
     L.mbqc_sv_adjoint_backward_c64.argtypes = [vp, vp, i64, i64, vp, i64, i64, vp, sz, vp, vp]
     L.mbqc_sv_adjoint_backward_c64.restype = C.c_int
+    L.mbqc_gram_finish_c64.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp]
+    L.mbqc_gram_finish_c64.restype = C.c_int
     L.mbqc_sv_segments_auto.argtypes = [vp, i64]
     L.mbqc_sv_segments_auto.restype = i32
     L.mbqc_sv_segmented_workspace_bytes.argtypes = [vp, i64, i32]

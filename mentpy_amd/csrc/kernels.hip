@@ -1652,6 +1652,28 @@ __global__ __launch_bounds__(64) void k_gram_seed_mfma(const float2* __restrict_
   }
 }
 
+// Loss and seed from t and a precomputed z = M t (library GEMM path): one
+// workgroup per parameter set, loss = 1 - Re(t^H z) / S, seed = -2 z / S.
+__global__ __launch_bounds__(256) void k_gram_finish(const float2* __restrict__ t,
+                                                     const float2* __restrict__ z, int K, float inv_S,
+                                                     float* __restrict__ loss,
+                                                     float2* __restrict__ seed) {
+  __shared__ float red[4];
+  const int64_t b = blockIdx.x;
+  const float2* tb = t + b * K;
+  const float2* zb = z + b * K;
+  float2* sb = seed + b * K;
+  const float sc = -2.0f * inv_S;
+  float part = 0.0f;
+  for (int k = threadIdx.x; k < K; k += 256) {
+    const float2 x = tb[k], y = zb[k];
+    part = fmaf(x.x, y.x, fmaf(x.y, y.y, part));
+    sb[k] = cscale(y, sc);
+  }
+  part = block_sum<256>(part, red);
+  if (threadIdx.x == 0) loss[b] = 1.0f - part * inv_S;
+}
+
 __global__ void k_gram_loss(const float* __restrict__ lpart, int64_t B, int nblk, float inv_S,
                             float* __restrict__ loss) {
   const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -2804,6 +2826,19 @@ void mbqc_batch_destroy(mbqc_batch* bt) {
 }
 
 int64_t mbqc_batch_items(const mbqc_batch* bt) { return bt ? int64_t(bt->items.size()) : -1; }
+
+int mbqc_gram_finish_c64(const void* t, const void* z, int64_t B, int32_t K, int32_t S, float* loss,
+                         void* seed, void* stream) {
+  if (!t || !z || !loss || !seed) return fail(MBQC_ERR_VALUE, "null pointer");
+  if (B < 0 || K <= 0 || S <= 0) return fail(MBQC_ERR_VALUE, "sizes must be B >= 0, K >= 1, S >= 1");
+  if (B == 0) return MBQC_OK;
+  if (B > 0x7fffffff) return fail(MBQC_ERR_VALUE, "too many parameter sets");
+  hipLaunchKernelGGL(k_gram_finish, dim3(unsigned(B)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     static_cast<const float2*>(t), static_cast<const float2*>(z), K, 1.0f / float(S),
+                     loss, static_cast<float2*>(seed));
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? MBQC_OK : hip_fail(e, "gram finish launch");
+}
 
 size_t mbqc_gram_workspace_bytes(int64_t B, int32_t K) {
   if (B < 0 || K <= 0) return 0;

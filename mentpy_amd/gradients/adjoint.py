@@ -277,9 +277,10 @@ class MBQMLStep:
             elif self.mode == "gram":
                 t = self.out.reshape(self.B, self.K)
                 torch.matmul(t, self.MT, out=self.Z)
-                self.loss.copy_(1.0 - (t.conj() * self.Z).sum(1).real / S)
-                if grad:
-                    torch.mul(self.Z.reshape(self.seed.shape), -2.0 / S, out=self.seed)
+                stream = C.c_void_p(torch.cuda.current_stream(self.engine.device).cuda_stream)
+                nat.check(nat.lib().mbqc_gram_finish_c64(
+                    C.c_void_p(t.data_ptr()), C.c_void_p(self.Z.data_ptr()), self.B, self.K, S,
+                    C.c_void_p(self.loss.data_ptr()), C.c_void_p(self.seed.data_ptr()), stream))
             elif self.mode == "transfer":
                 Tm = self.out.transpose(1, 2)                                # [B, dout, din]
                 outs = torch.matmul(Tm, self.X.transpose(0, 1))             # [B, dout, S]
