@@ -718,9 +718,32 @@ __global__ __launch_bounds__(256) void k_sv_sample(Job J, const uint64_t* __rest
   for (int j = 0; j < H; ++j)
     if (pos[j] >= 0) n2 = fmaf(t[j].x, t[j].x, fmaf(t[j].y, t[j].y, n2));
   const float scale = 1.0f / sqrtf(n2);
-  if (active) {
-    // the output frame Z^z X^x applied while storing:
-    // new[o] = (-1)^{popcount(o & z)} old[o ^ x], i.e. old[p] lands at p ^ x
+  // the output frame Z^z X^x applied while storing:
+  // new[o] = (-1)^{popcount(o & z)} old[o ^ x], i.e. old[p] lands at p ^ x
+  if constexpr (H <= 16) {
+    // outputs of up to 16 amplitudes leave through LDS as contiguous rows
+    // (the workgroup's waves cover consecutive simulations)
+    __shared__ float2 stage[4][64 * 17];
+    const int lane = threadIdx.x & 63;
+    const int dout = 1 << J.n_out;
+    float2* sw = stage[threadIdx.x >> 6];
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      const int p = pos[j];
+      if (p >= 0) {
+        const uint32_t q = uint32_t(p) ^ xo;
+        sw[lane * (dout + 1) + int(q)] = flip(cscale(t[j], scale), czbit(q, zo));
+      }
+    }
+    __syncthreads();
+    const int64_t g0 = g - lane;
+    const int rows = int(J.total - g0 < 64 ? J.total - g0 : 64);
+    float2* obase = J.out + g0 * dout;
+    for (int e = lane; e < 64 * dout; e += 64) {
+      const int r = e >> J.n_out, c = e & (dout - 1);
+      if (r < rows) obase[e] = sw[r * (dout + 1) + c];
+    }
+  } else if (active) {
     float2* o = J.out + g * (int64_t(1) << J.n_out);
 #pragma unroll
     for (int j = 0; j < H; ++j) {
@@ -730,6 +753,8 @@ __global__ __launch_bounds__(256) void k_sv_sample(Job J, const uint64_t* __rest
         o[q] = flip(cscale(t[j], scale), czbit(q, zo));
       }
     }
+  }
+  if (active) {
     row[words] = int32_t(xo);
     row[words + 1] = int32_t(zo);
   }
