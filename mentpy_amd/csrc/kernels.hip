@@ -1027,16 +1027,14 @@ __global__ __launch_bounds__(256) void k_sv_adj_forward(Job J, float2* __restric
       for (int j = 0; j < H; ++j) ckpt[(c * H + j) * total + g] = tt[j];
     }
   });
+  // the wave's output rows leave through its LDS column tile (dynamic LDS of
+  // one tile per wave)
   const int32_t* out_pos = J.tab + J.off_out;
   const float scale = sv_final_scale(J.e_sv);
-  if (active) {
-    float2* o = J.out + g * (int64_t(1) << J.n_out);
-#pragma unroll
-    for (int j = 0; j < H; ++j) {
-      const int p = out_pos[j];
-      if (p >= 0) o[p] = cscale(t[j], scale);
-    }
-  }
+  const int64_t g0 = g - (threadIdx.x & 63);
+  const int rows = int(total - g0 < 64 ? total - g0 : 64);
+  wave_store_rows<NS>(t, out_pos, make_float2(scale, 0.0f), J.out + g0 * (int64_t(1) << J.n_out),
+                      J.n_out, rows);
 }
 
 // Adjoint of one SV step (slot Q).  Forward: u0 = x + b y, new0 = u0,
@@ -2238,7 +2236,8 @@ hipError_t launch_adj_fwd(const Job& J, float2* ckpt, int ck, hipStream_t st) {
   // instead of sharing a few CUs' scalar units and instruction issue
   const int threads = 64;
   const int64_t blocks = (J.total + threads - 1) / threads;
-  hipLaunchKernelGGL((k_sv_adj_forward<NS>), dim3(blocks), dim3(threads), 0, st, J, ckpt, ck);
+  hipLaunchKernelGGL((k_sv_adj_forward<NS>), dim3(blocks), dim3(threads),
+                     (threads / 64) * kColumnTile * sizeof(float2), st, J, ckpt, ck);
   return hipGetLastError();
 }
 
