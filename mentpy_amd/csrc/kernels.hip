@@ -336,7 +336,8 @@ __device__ __forceinline__ void wave_lds_sync() {
 // so a one-wave workgroup reserves one tile.
 constexpr int kColumnTile = 64 * 17;
 
-template <int NS>
+// IDENT: slot index j is column j (out_pos unused, f = 1).
+template <int NS, bool IDENT = false>
 __device__ __forceinline__ void wave_store_rows(const float2 (&t)[1 << NS], const int32_t* out_pos,
                                                 float2 f, float2* obase, int n_out, int rows_valid) {
   constexpr int H = 1 << NS;
@@ -352,8 +353,8 @@ __device__ __forceinline__ void wave_store_rows(const float2 (&t)[1 << NS], cons
   for (int c0 = 0; c0 < dout; c0 += cw) {
 #pragma unroll
     for (int j = 0; j < H; ++j) {
-      const int p = out_pos[j] - c0;
-      if (p >= 0 && p < cw) sw[lane * (CW + 1) + p] = cmul(t[j], f);
+      const int p = (IDENT ? j : out_pos[j]) - c0;
+      if (p >= 0 && p < cw) sw[lane * (CW + 1) + p] = IDENT ? t[j] : cmul(t[j], f);
     }
     wave_lds_sync();
     for (int e = lane; e < 64 * cw; e += 64) {
@@ -1310,20 +1311,17 @@ __global__ __launch_bounds__(64) void k_sv_seg_forward(Job J, SegInfo G, float2*
       for (int j = 0; j < H; ++j) ck[(int64_t(blk - b0) * H + j) * Np + l] = tt[j];
     }
   });
-  if (!active) return;
+  // the wave's rows (64 consecutive lanes of the segment) leave through its
+  // LDS column tile
+  const int64_t l0 = l - threadIdx.x;
+  const int rows = int(G.B * C - l0 < 64 ? G.B * C - l0 : 64);
   if (p + 1 < G.n) {
-    float2* o = ws + G.tmat[p] + ll * H;
-#pragma unroll
-    for (int j = 0; j < H; ++j) o[j] = t[j];
+    wave_store_rows<NS, true>(t, nullptr, make_float2(1.f, 0.f), ws + G.tmat[p] + l0 * H, NS, rows);
   } else {
     const int32_t* out_pos = J.tab + J.off_out;
     const float scale = sv_final_scale(J.e_sv);
-    float2* o = ws + G.tmat[p] + ll * G.rows_last;
-#pragma unroll
-    for (int j = 0; j < H; ++j) {
-      const int q = out_pos[j];
-      if (q >= 0) o[q] = cscale(t[j], scale);
-    }
+    wave_store_rows<NS>(t, out_pos, make_float2(scale, 0.f), ws + G.tmat[p] + l0 * G.rows_last,
+                        J.n_out, rows);
   }
 }
 
@@ -2289,7 +2287,8 @@ size_t seg_layout(const Plan& p, int64_t B, int n, SegInfo* G) {
 template <int NS>
 hipError_t launch_seg_forward(const Job& J, const SegInfo& G, float2* ws, float2* out, hipStream_t st) {
   if (G.wave[G.n] > 0)
-    hipLaunchKernelGGL((k_sv_seg_forward<NS>), dim3(G.wave[G.n]), dim3(64), 0, st, J, G, ws, ws);
+    hipLaunchKernelGGL((k_sv_seg_forward<NS>), dim3(G.wave[G.n]), dim3(64),
+                       kColumnTile * sizeof(float2), st, J, G, ws, ws);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess && G.B > 0) {
     if constexpr (NS == 4)
