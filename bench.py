@@ -179,7 +179,8 @@ def adjoint_section(circ, sim, dev, steps, stream, sets=(1024, 16384), S=256, se
            "grad_max_abs_err_vs_c128": err,
            "grad_workload": f"4x20 grid ({n} angles), loss 1-mean_s|<U psi_s|out_s>|^2 over {S} Haar "
                             f"inputs; adjoint = one graph-replayed step (transfer matrix + data Gram "
-                            f"matrix GEMM); {big} parameter sets (and {small})"}
+                            f"matrix GEMM); {big} parameter sets (and {small}); a grad is the full "
+                            f"{n}-angle gradient of one parameter set"}
     return out, (X, Y, t0, cost)
 
 
