@@ -291,7 +291,8 @@ def test_segmented_adjoint_matches_unsegmented(rows, cols, S, segs, mode, rng):
 
 
 @pytest.mark.parametrize("rows,cols,mode,S,B", [(4, 16, "gram", 64, 4096), (2, 7, "direct", 256, 256),
-                                                (3, 6, "direct", 128, 512)])
+                                                (3, 6, "direct", 128, 512), (1, 9, "direct", 64, 1024),
+                                                (5, 4, "direct", 128, 512)])
 def test_adjoint_checkpoint_pairs(rows, cols, mode, S, B, rng):
     """From 65,536 columns the unsegmented adjoint keeps one checkpoint per
     two blocks and recomputes the other block in the backward sweep (odd and
@@ -301,7 +302,7 @@ def test_adjoint_checkpoint_pairs(rows, cols, mode, S, B, rng):
     import mentpy_amd as mp
     from mentpy_amd.gradients import MBQMLStep
     from oracle.gradients import exact_shift_gradient
-    g = mp.templates.grid_cluster(rows, cols)
+    g = mp.templates.grid_cluster(rows, cols) if rows > 1 else mp.templates.linear_cluster(cols)
     sim = mp.PatternSimulator(g, backend="hip-sv")
     X, Y = _dataset(rows, S, rng)
     th = rng.uniform(0, 2 * np.pi, (B, len(g.trainable_nodes))).astype(np.float32)

@@ -217,3 +217,37 @@ def test_golden_teleportation_cases(rng):
                 ps.reset(input_state=st)
                 out = ps(case["angles"])
                 assert np.allclose(out, np.outer(st, st.conj()), atol=case["atol"]), backend
+
+
+@pytest.mark.parametrize("rows,cols", [(4, 40), (5, 30), (6, 12)])
+def test_long_patterns_deduplicated_coefficients(rows, cols, rng):
+    """Patterns whose per-step coefficient table would exceed 8 KB store each
+    distinct row once: forward, sampled-outcome and adjoint kernels read them
+    through the step flags and still match the oracle."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from oracle.vectorized import sv_batch
+    g = mp.templates.grid_cluster(rows, cols)
+    sim = mp.PatternSimulator(g, backend="hip-sv")
+    info = sim.simulator.plan.info
+    assert info["n_steps"] * 4 << info["n_slots"] > 8192
+    th = rng.uniform(0, 2 * np.pi, (5, len(g.trainable_nodes)))
+    st = np.stack([haar_state(rows, rng) for _ in range(2)])
+    out = sim.run_batch(th, st).cpu().numpy()
+    ref = sv_batch(g, th.astype(np.float32).astype(np.float64), st)
+    assert np.abs(out - ref).max() < AMP_TOL
+    if rows <= 5:
+        from mentpy_amd.gradients import mbqml_infidelity
+        from oracle.gradients import exact_shift_gradient
+        from mentpy_amd.operators.gates import haar_unitary
+        U = haar_unitary(2 ** rows, rng)
+        X = np.stack([haar_state(rows, rng) for _ in range(6)])
+        Y = X @ U.T
+        th32 = th[:2].astype(np.float32)
+        loss, grad = mbqml_infidelity(sim, th32, X, Y, mode="direct")
+
+        def cost(t):
+            o = sv_batch(g, np.asarray(t)[None], X)[0]
+            return 1.0 - np.mean(np.abs(np.sum(np.conj(Y) * o, -1)) ** 2)
+        ref_g = exact_shift_gradient(cost, th32[0].astype(np.float64))
+        assert np.abs(grad[0].cpu().numpy() - ref_g).max() < 2e-4

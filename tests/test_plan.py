@@ -241,3 +241,22 @@ def test_segmented_adjoint_host_sizes(native):
     assert L.mbqc_sv_segmented_forward_c64(plan.handle, None, 0, 1, None, 4, 1, None, 0, None) \
         == native.MBQC_ERR_VALUE
     assert b"n_segments" in L.mbqc_last_error()
+
+
+def test_coefficient_rows_per_step_or_deduplicated(native):
+    """Step k's pair coefficients are row flags >> 16 of the plan's table: one
+    row per step (row k) while the table fits 8 KB, each distinct row once
+    beyond that (a grid has one per slot)."""
+    from mentpy_amd.simulators import compile_pattern
+    for rows, cols in [(4, 20), (4, 40), (5, 30), (6, 20), (2, 40)]:
+        plan = compile_pattern(mp.templates.grid_cluster(rows, cols))[0]
+        st = plan.tables()[0]
+        ns, n = plan.info["n_slots"], plan.info["n_steps"]
+        idx = np.array([int(x[1]) >> 16 for x in st])
+        if n * 4 << ns <= 8192:
+            assert np.array_equal(idx, np.arange(n)), (rows, cols)
+        else:
+            assert idx.max() + 1 <= 2 * ns, (rows, cols, idx.max())
+            # a row is shared only by steps with the same measured slot
+            for r in set(idx.tolist()):
+                assert len({(int(x[1]) >> 8) & 0xFF for x, i in zip(st, idx) if i == r}) == 1
