@@ -805,6 +805,23 @@ __device__ __forceinline__ void stage_angles(float* th_s, const mbqc_step* steps
   }
 }
 
+// The projection coefficients gamma e^{-i theta} of steps [k0, k0 + kAngleChunk),
+// computed once per step by the workgroup's threads in parallel and kept in
+// LDS, so the sine and cosine are not on every step's dependent chain (the
+// caller synchronises before and after).
+template <int T>
+__device__ __forceinline__ void stage_coeffs(float2* gb_s, const mbqc_step* steps, const float* ang,
+                                             int64_t sa, int k0, int n_steps, float gamma, int tid) {
+  for (int i = tid; i < kAngleChunk; i += T) {
+    const int k = k0 + i;
+    if (k < n_steps) {
+      const mbqc_step st = steps[k];
+      const float th = st.angle_index >= 0 ? ang[int64_t(st.angle_index) * sa] : st.fixed_angle;
+      gb_s[i] = cscale(step_coeff<false>(th, nullptr), gamma);
+    }
+  }
+}
+
 // One step on an LDS-resident statevector of H amplitudes, the pairs split
 // over the T threads of the workgroup (caller synchronises before and after).
 template <int T>
@@ -929,14 +946,14 @@ __global__ __launch_bounds__(T * G) void k_dm_forward_lds(Job J, DmNoise nz, uin
   constexpr int H = 1 << NS;
   constexpr int HQ = H / 2;
   __shared__ float2 rs[G][H * H];
-  __shared__ float th_all[G][kAngleChunk];
+  __shared__ float2 gb_all[G][kAngleChunk];
   __shared__ float red[4];
   const int gs = threadIdx.x / T, tid = threadIdx.x % T;
   const int64_t g = int64_t(blockIdx.x) * G + gs;
   const bool active = g < J.total;
   const int64_t gg = active ? g : J.total - 1;
   float2* r = rs[gs];
-  float* th_s = th_all[gs];
+  float2* gb_s = gb_all[gs];
   const int64_t bi = gg / J.S, si = gg - bi * J.S;
   const float* ang = J.angles + bi * J.angle_stride;
   const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
@@ -958,12 +975,11 @@ __global__ __launch_bounds__(T * G) void k_dm_forward_lds(Job J, DmNoise nz, uin
     }
     if (k % kAngleChunk == 0) {
       __syncthreads();
-      stage_angles<T>(th_s, steps, ang, J.angle_stride_a, k, J.n_steps, tid);
+      stage_coeffs<T>(gb_s, steps, ang, J.angle_stride_a, k, J.n_steps, nz.gamma, tid);
       __syncthreads();
     }
     const mbqc_step step = steps[k];
-    const float th = th_s[k % kAngleChunk];
-    const float2 gb = cscale(step_coeff<false>(th, nullptr), nz.gamma);
+    const float2 gb = gb_s[k % kAngleChunk];
     __syncthreads();
     dm_lds_step<T>(r, H, step, gb, tid);
   }
