@@ -390,3 +390,29 @@ def test_golden_known_answers_fixture(rng):
         for cls in (ReferenceSV, ReferenceDM):
             out = cls(g, st).run(case["angles"])
             assert np.allclose(out, np.outer(st, st.conj()), atol=case["atol"])
+
+
+def test_haar_sampler_matches_scipy_unitary_group():
+    """mentpy's random_su (mentpy/operators/gates.py:29-34) draws
+    scipy.stats.unitary_group.rvs(2**n) and divides by det(U)^(1/2**n).  With
+    the same seeded generator the build's sampler draws the same Ginibre
+    matrix and returns the same unitary (scipy 1.15's QR-with-phase-fix
+    algorithm); the result is special unitary, and generate_haar_random_states
+    returns its first columns (mentpy/utils/generate_data.py:15-30)."""
+    from scipy.stats import unitary_group
+    from mentpy_amd.operators.gates import haar_unitary, random_su
+    from mentpy_amd.utils import generate_haar_random_states
+    for n in (1, 2, 3, 4):
+        d = 2 ** n
+        for seed in range(3):
+            mine = haar_unitary(d, np.random.default_rng(seed))
+            ref = unitary_group.rvs(d, random_state=np.random.default_rng(seed))
+            assert np.abs(mine - ref).max() < 1e-12
+            su = random_su(n, np.random.default_rng(seed))
+            assert abs(np.linalg.det(su) - 1) < 1e-12
+            assert np.abs(su.conj().T @ su - np.eye(d)).max() < 1e-12
+            assert np.abs(su * np.power(np.linalg.det(ref), 1.0 / d) - ref).max() < 1e-12
+    st = generate_haar_random_states(3, 5, np.random.default_rng(7))
+    g = np.random.default_rng(7)
+    cols = np.stack([random_su(3, g)[:, 0] for _ in range(5)])
+    assert np.array_equal(st, cols)
