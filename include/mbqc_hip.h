@@ -34,6 +34,8 @@ extern "C" {
 #define MBQC_FLAG_REF_PHASE 1u    /* reference global phase: prod_k sgn(cos(t_k/2)) e^{i t_k/2} */
 #define MBQC_FLAG_RENORMALIZE 2u  /* divide the output by its norm (SV) / trace (DM) */
 #define MBQC_FLAG_WRITE_PHASE 4u  /* also write the per-sim global phase (SV forward) */
+#define MBQC_FLAG_TABLE_STEPS 8u  /* SV forward: read the pair coefficients from the
+                                     plan's table also for a periodic plan */
 
 /* Host description of a pattern; all arrays are host memory, copied by
  * mbqc_plan_create.  Node ids are 0..n_nodes-1. */
@@ -66,6 +68,10 @@ typedef struct mbqc_plan_info {
   int32_t n_dropped_edges;/* edges never applied because the window was too small */
   int32_t min_window;     /* smallest window that applies every edge */
   int32_t can_sample;     /* 1: mbqc_sv_sample_c64 accepts this plan */
+  int32_t periodic;       /* 1: every step appends a CZ neighbour of the measured
+                             qubit and the CZ pattern repeats every n_slots steps
+                             (grids in column order, chains): the SV forward
+                             takes its pair coefficients at compile time */
 } mbqc_plan_info;
 
 /* Per-step table entry (16 bytes) as stored on the device. */

@@ -25,6 +25,7 @@ MBQC_ERR_RUNTIME = -4
 FLAG_REF_PHASE = 1
 FLAG_RENORMALIZE = 2
 FLAG_WRITE_PHASE = 4
+FLAG_TABLE_STEPS = 8
 
 EXPORTED_SYMBOLS = (
     "mbqc_causal_flow", "mbqc_plan_create", "mbqc_plan_destroy", "mbqc_plan_get_info", "mbqc_plan_get_tables",
@@ -54,7 +55,7 @@ class PatternDesc(C.Structure):
 class PlanInfo(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "n_slots", "lag", "n_steps", "n_inputs", "n_outputs", "n_angles", "n_nodes", "window",
-        "n_appends", "n_dropped_edges", "min_window", "can_sample")]
+        "n_appends", "n_dropped_edges", "min_window", "can_sample", "periodic")]
 
 
 class Step(C.Structure):

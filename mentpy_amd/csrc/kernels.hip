@@ -125,6 +125,31 @@ __device__ __forceinline__ void sv_step(float2 (&t)[1 << NS], float2 b, const fl
   }
 }
 
+// sv_step of a periodic pattern (every step appends a CZ neighbour of the
+// measured qubit; the appended qubit's other neighbours are
+// periodic_nmask(NS, NS - 1 - Q), a compile-time mask): the new Q=1 half is
+// sign (2 x - u0) with sign the parity of j & NMASK, the same arithmetic as
+// alpha u0 + beta x with alpha = -sign, beta = 2 sign, with the signs folded
+// into the instructions instead of read from the coefficient table.
+template <int NS, int Q, uint32_t NMASK>
+__device__ __forceinline__ void sv_step_periodic(float2 (&t)[1 << NS], float2 b) {
+  constexpr int H = 1 << NS;
+  constexpr int QB = 1 << Q;
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    if (j & QB) continue;
+    const float2 x = t[j], y = t[j | QB];
+    float2 u0;
+    u0.x = fmaf(-b.y, y.y, fmaf(b.x, y.x, x.x));
+    u0.y = fmaf(b.x, y.y, fmaf(b.y, y.x, x.y));
+    t[j] = u0;
+    if (__builtin_popcount(uint32_t(j) & NMASK) & 1)
+      t[j | QB] = make_float2(fmaf(-2.0f, x.x, u0.x), fmaf(-2.0f, x.y, u0.y));
+    else
+      t[j | QB] = make_float2(fmaf(2.0f, x.x, -u0.x), fmaf(2.0f, x.y, -u0.y));
+  }
+}
+
 // Job description shared by the SV / DM / adjoint kernels (passed by value
 // or read from global memory by the persistent kernel).
 struct Job {
@@ -195,7 +220,8 @@ __device__ __forceinline__ void shared_block_coeffs(const float (&th)[NS], float
   ((bb[SUBS] = make_float2(group_bcast<G, SUBS>(cs), -group_bcast<G, SUBS>(sn))), ...);
 }
 
-template <int NS, bool REF, bool FULL, bool PER_STEP, bool SH, int SUB, typename Mod>
+template <int NS, bool REF, bool FULL, bool PER_STEP, bool SH, bool PERIODIC, int SUB,
+          typename Mod>
 __device__ __forceinline__ void sv_sub(float2 (&t)[1 << NS], const mbqc_step* steps,
                                        const float* coef, int k0, int n_steps, const float (&th)[NS],
                                        const float2 (&bb)[NS], RefPhase* ph, Mod& mod) {
@@ -208,18 +234,24 @@ __device__ __forceinline__ void sv_sub(float2 (&t)[1 << NS], const mbqc_step* st
     } else {
       b0 = step_coeff<REF>(th[SUB], ph);
     }
-    sv_step<NS, NS - 1 - SUB>(t, mod(k, b0), coef_row<NS, PER_STEP>(coef, steps, k));
+    if constexpr (PERIODIC)
+      sv_step_periodic<NS, NS - 1 - SUB, periodic_nmask(NS, SUB)>(t, mod(k, b0));
+    else
+      sv_step<NS, NS - 1 - SUB>(t, mod(k, b0), coef_row<NS, PER_STEP>(coef, steps, k));
   }
 }
 
-template <int NS, bool REF, bool FULL, bool PER_STEP, bool SH, typename Mod, int... SUBS>
+template <int NS, bool REF, bool FULL, bool PER_STEP, bool SH, bool PERIODIC, typename Mod,
+          int... SUBS>
 __device__ __forceinline__ void sv_block(float2 (&t)[1 << NS], const mbqc_step* steps,
                                          const float* coef, int k0, int n_steps,
                                          const float (&th)[NS], RefPhase* ph, Mod& mod,
                                          std::integer_sequence<int, SUBS...> seq) {
   float2 bb[NS];
   if constexpr (SH) shared_block_coeffs<NS>(th, bb, seq);
-  (sv_sub<NS, REF, FULL, PER_STEP, SH, SUBS>(t, steps, coef, k0, n_steps, th, bb, ph, mod), ...);
+  (sv_sub<NS, REF, FULL, PER_STEP, SH, PERIODIC, SUBS>(t, steps, coef, k0, n_steps, th, bb, ph,
+                                                       mod),
+   ...);
 }
 
 // Angles of the NS steps of a block (steps past the end get 0).  The index is
@@ -244,8 +276,10 @@ __device__ __forceinline__ void fetch_block_angles(const mbqc_step* steps, int k
 // Run blocks [blk0, blk1) of NS steps: full blocks of NS unrolled steps with
 // the next block's angles in flight, then the partial tail block.
 // `on_block(blk, t)` runs at the start of every block (after its rescale).
-template <int NS, bool REF, bool PER_STEP, bool SH = false, typename OnBlock,
-          typename Mod = NoModify>
+// PERIODIC: the full blocks take the compile-time coefficients of a periodic
+// plan (Plan::periodic), the tail block the table.
+template <int NS, bool REF, bool PER_STEP, bool SH = false, bool PERIODIC = false,
+          typename OnBlock, typename Mod = NoModify>
 __device__ __forceinline__ void sv_run_blocks(float2 (&t)[1 << NS], const Job& J, const float* ang,
                                               RefPhase* ph, int blk0, int blk1, OnBlock on_block,
                                               Mod&& mod = Mod()) {
@@ -269,11 +303,11 @@ __device__ __forceinline__ void sv_run_blocks(float2 (&t)[1 << NS], const Job& J
     }
     on_block(blk, t);
     if (blk < nfull)
-      sv_block<NS, REF, true, PER_STEP, SH>(t, steps, coef, blk * NS, n, th, ph, mod,
-                                            std::make_integer_sequence<int, NS>{});
+      sv_block<NS, REF, true, PER_STEP, SH, PERIODIC>(t, steps, coef, blk * NS, n, th, ph, mod,
+                                                      std::make_integer_sequence<int, NS>{});
     else
-      sv_block<NS, REF, false, PER_STEP, SH>(t, steps, coef, blk * NS, n, th, ph, mod,
-                                             std::make_integer_sequence<int, NS>{});
+      sv_block<NS, REF, false, PER_STEP, SH, false>(t, steps, coef, blk * NS, n, th, ph, mod,
+                                                    std::make_integer_sequence<int, NS>{});
   }
 }
 
@@ -376,7 +410,8 @@ enum StoreMode {
 // consecutive simulations g - lane .. g - lane + 63, of which the first
 // rows_valid are written (kStoreDirect: active lanes write).  Inactive lanes
 // run on simulation 0.
-template <int NS, bool REF, int STORE, bool PER_STEP = false, bool SH = false>
+template <int NS, bool REF, int STORE, bool PER_STEP = false, bool SH = false,
+          bool PERIODIC = false>
 __device__ __forceinline__ void sv_sim(const Job& J, int64_t g, bool active, int rows_valid,
                                        uint32_t flags) {
   constexpr int H = 1 << NS;
@@ -386,7 +421,7 @@ __device__ __forceinline__ void sv_sim(const Job& J, int64_t g, bool active, int
   float2 t[H];
   sv_load_input<NS>(J, J.psi + si * J.in_stride, t);
   RefPhase rp;
-  sv_run_blocks<NS, REF, PER_STEP, SH>(t, J, ang, &rp, 0, (J.n_steps + NS - 1) / NS,
+  sv_run_blocks<NS, REF, PER_STEP, SH, PERIODIC>(t, J, ang, &rp, 0, (J.n_steps + NS - 1) / NS,
                                        [](int, const float2 (&)[H]) {});
   const float2 ph = REF ? rp.value() : make_float2(1.0f, 0.0f);
   // output gather: slot-index j -> output index out_pos[j] (dead slots skipped)
@@ -447,6 +482,15 @@ __global__ __launch_bounds__(256) void k_sv_forward(Job J, uint32_t flags) {
   const int64_t g0 = g - (threadIdx.x & 63);
   const int rows = int(J.total - g0 < 64 ? J.total - g0 : 64);
   sv_sim<NS, REF, kForwardStore<NS>, PER_STEP>(J, g, g < J.total, rows, flags);
+}
+
+// k_sv_forward for periodic plans (grids in column order, chains).
+template <int NS, bool REF>
+__global__ __launch_bounds__(256) void k_sv_forward_periodic(Job J, uint32_t flags) {
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t g0 = g - (threadIdx.x & 63);
+  const int rows = int(J.total - g0 < 64 ? J.total - g0 : 64);
+  sv_sim<NS, REF, kForwardStore<NS>, false, false, true>(J, g, g < J.total, rows, flags);
 }
 
 // ---------------------------------------------------------------- DM step
@@ -1222,7 +1266,7 @@ __global__ __launch_bounds__(64) void k_sv_adj_backward(Job J, const float2* __r
       for (int j = 0; j < H; ++j) t[j] = s0[j];
       RefPhase ph;
       NoModify nomod;
-      sv_block<NS, false, true, false, false>(t, steps, coef, b0 * NS, n, th0, &ph, nomod,
+      sv_block<NS, false, true, false, false, false>(t, steps, coef, b0 * NS, n, th0, &ph, nomod,
                                               std::make_integer_sequence<int, NS>{});
       if (resc[b0 + 1]) {
 #pragma unroll
@@ -2172,12 +2216,19 @@ int make_job(Plan* p, const float* angles, int64_t angle_stride, int64_t angle_s
 }
 
 template <int NS>
-hipError_t launch_sv(const Job& J, bool per_step, uint32_t flags, hipStream_t st) {
+hipError_t launch_sv(const Job& J, bool per_step, bool periodic, uint32_t flags, hipStream_t st) {
   const int threads = 256;
   const int64_t blocks = (J.total + threads - 1) / threads;
   const size_t lds = kForwardStore<NS> == kStoreColumns ? (threads / 64) * kColumnTile * sizeof(float2) : 0;
   const bool ref = flags & MBQC_FLAG_REF_PHASE;
-  if (per_step) {
+  if (periodic) {
+    if (ref)
+      hipLaunchKernelGGL((k_sv_forward_periodic<NS, true>), dim3(blocks), dim3(threads), lds, st, J,
+                         flags);
+    else
+      hipLaunchKernelGGL((k_sv_forward_periodic<NS, false>), dim3(blocks), dim3(threads), lds, st, J,
+                         flags);
+  } else if (per_step) {
     if (ref)
       hipLaunchKernelGGL((k_sv_forward<NS, true, true>), dim3(blocks), dim3(threads), lds, st, J, flags);
     else
@@ -2468,13 +2519,14 @@ int mbqc_sv_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
   if (J.total == 0) return MBQC_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
   hipError_t e = hipSuccess;
+  const bool periodic = p->periodic && !(flags & MBQC_FLAG_TABLE_STEPS);
   switch (p->ns) {
-    case 1: e = launch_sv<1>(J, p->coef_per_step, flags, st); break;
-    case 2: e = launch_sv<2>(J, p->coef_per_step, flags, st); break;
-    case 3: e = launch_sv<3>(J, p->coef_per_step, flags, st); break;
-    case 4: e = launch_sv<4>(J, p->coef_per_step, flags, st); break;
-    case 5: e = launch_sv<5>(J, p->coef_per_step, flags, st); break;
-    case 6: e = launch_sv<6>(J, p->coef_per_step, flags, st); break;
+    case 1: e = launch_sv<1>(J, p->coef_per_step, periodic, flags, st); break;
+    case 2: e = launch_sv<2>(J, p->coef_per_step, periodic, flags, st); break;
+    case 3: e = launch_sv<3>(J, p->coef_per_step, periodic, flags, st); break;
+    case 4: e = launch_sv<4>(J, p->coef_per_step, periodic, flags, st); break;
+    case 5: e = launch_sv<5>(J, p->coef_per_step, periodic, flags, st); break;
+    case 6: e = launch_sv<6>(J, p->coef_per_step, periodic, flags, st); break;
     case 7: e = launch_sv_lds<7>(J, flags, st); break;
     case 8: e = launch_sv_lds<8>(J, flags, st); break;
     case 9: e = launch_sv_lds<9>(J, flags, st); break;

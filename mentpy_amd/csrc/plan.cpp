@@ -259,6 +259,11 @@ int compile_plan(const mbqc_pattern_desc* d, Plan* p) {
       st.flags = (st.flags & 0xffffu) | (r << 16);
     }
   }
+  p->periodic = ns <= kMaxCoefSlots && nm > 0 && nm % ns == 0;
+  for (int k = 0; k < nm && p->periodic; ++k) {
+    const mbqc_step& st = p->steps[k];
+    p->periodic = (st.flags & 3u) == 3u && st.nmask == periodic_nmask(ns, k % ns);
+  }
   // Pauli-frame tables for sampled outcomes (PennyLane backend order,
   // pennylane_simulator.py:146-153): outcome 1 on schedule[k] flips the X
   // frame of its flow successor and the Z frame of the successor's neighbours
@@ -447,6 +452,7 @@ int mbqc_plan_get_info(const mbqc_plan* plan, mbqc_plan_info* info) {
   info->n_dropped_edges = p->n_dropped;
   info->min_window = p->min_window;
   info->can_sample = p->can_sample ? 1 : 0;
+  info->periodic = p->periodic ? 1 : 0;
   return MBQC_OK;
 }
 

@@ -21,6 +21,12 @@ constexpr size_t kCoefPerStepBytes = 8192;
 constexpr int kSvRescaleAppends = 64;   // 2^(64/2) growth -> rescale by 2^-32
 constexpr int kDmRescaleAppends = 32;   // 2^32 growth     -> rescale by 2^-32
 
+// Neighbour mask of sub-step `sub` (step k = blk * ns + sub) of a periodic
+// plan: the appended qubit is a CZ neighbour of the measured one and, for
+// sub > 0, of the qubit appended one step earlier (slot ns - sub).  This is
+// the step structure of a grid of ns rows in column order (a chain for ns = 1).
+constexpr uint32_t periodic_nmask(int ns, int sub) { return sub == 0 ? 0u : (1u << (ns - sub)); }
+
 struct Plan {
   int ns = 0, lag = 0, w = 0, n_nodes = 0, n_steps = 0, n_in = 0, n_out = 0, n_angles = 0;
   int n_appends = 0, n_edges = 0, n_dropped = 0, min_window = 0;
@@ -35,6 +41,10 @@ struct Plan {
   // with r = steps[k].flags >> 16 (distinct rows stored once).
   std::vector<float> coef;
   bool coef_per_step = false;   // row of step k is k (table small enough to keep per step)
+  // every step appends a CZ neighbour of the measured qubit with the other
+  // neighbours periodic_nmask(ns, k % ns), and n_steps % ns == 0: the forward
+  // kernel can take its coefficients at compile time
+  bool periodic = false;
   // Pauli-frame tables of the sampled mode (empty when not available): per step
   // the 64-bit masks, relative to the step's schedule position, of the nodes
   // whose X (flow successor) / Z (its later neighbours) frame bit an outcome 1

@@ -149,10 +149,13 @@ def _require_gpu(device):
 
 
 def sv_forward(plan: nat.NativePlan, angles, states, *, ref_phase: bool = True,
-               renormalize: bool = True, want_phase: bool = False, out=None):
+               renormalize: bool = True, want_phase: bool = False, out=None,
+               table_steps: bool = False):
     """Batched statevector forward: ``angles [B, n]`` x ``states [S, 2^|I|]`` ->
     ``[B, S, 2^|O|]`` complex64 on the device (and the per-simulation global
-    phase ``[B, S]`` if ``want_phase``)."""
+    phase ``[B, S]`` if ``want_phase``).  ``table_steps`` makes a periodic
+    plan (grids, chains) read its pair coefficients from the table like any
+    other plan; both give identical results."""
     torch = _torch()
     device = states.device
     _require_gpu(device)
@@ -162,7 +165,8 @@ def sv_forward(plan: nat.NativePlan, angles, states, *, ref_phase: bool = True,
         out = torch.empty((B, S, 1 << info["n_outputs"]), dtype=torch.complex64, device=device)
     phase = torch.empty((B, S), dtype=torch.complex64, device=device) if want_phase else None
     flags = ((nat.FLAG_REF_PHASE if ref_phase else 0) | (nat.FLAG_RENORMALIZE if renormalize else 0)
-             | (nat.FLAG_WRITE_PHASE if want_phase else 0))
+             | (nat.FLAG_WRITE_PHASE if want_phase else 0)
+             | (nat.FLAG_TABLE_STEPS if table_steps else 0))
     sb, sa = _angle_strides(angles)
     with torch.cuda.device(device):
         nat.check(nat.lib().mbqc_sv_forward_c64(

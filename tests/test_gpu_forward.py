@@ -251,3 +251,33 @@ def test_long_patterns_deduplicated_coefficients(rows, cols, rng):
             return 1.0 - np.mean(np.abs(np.sum(np.conj(Y) * o, -1)) ** 2)
         ref_g = exact_shift_gradient(cost, th32[0].astype(np.float64))
         assert np.abs(grad[0].cpu().numpy() - ref_g).max() < 2e-4
+
+
+@pytest.mark.parametrize("rows,cols", [(1, 9), (2, 7), (3, 5), (4, 20), (5, 6), (6, 5)])
+@pytest.mark.parametrize("ref_phase", [True, False])
+def test_periodic_steps_equal_table_steps(rows, cols, ref_phase, rng):
+    """The compile-time coefficients of periodic plans perform the table's
+    arithmetic exactly: outputs and phases are bit-identical to the table
+    path, and match the oracle."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from mentpy_amd.simulators.hip_simulator import sv_forward
+    from oracle.vectorized import sv_batch
+    g = mp.templates.grid_cluster(rows, cols) if rows > 1 else mp.templates.linear_cluster(cols)
+    sim = mp.PatternSimulator(g, backend="hip-sv")
+    plan = sim.simulator.plan
+    assert plan.info["periodic"] == 1
+    B, S = 301, 2                      # a partial last wave
+    th = rng.uniform(-4 * np.pi, 4 * np.pi, (B, len(g.trainable_nodes)))
+    st = np.stack([haar_state(len(g.input_nodes), rng) for _ in range(S)])
+    a = torch.tensor(th, dtype=torch.float32, device="cuda")
+    s = torch.tensor(st, dtype=torch.complex64, device="cuda")
+    o1, p1 = sv_forward(plan, a, s, ref_phase=ref_phase, want_phase=True)
+    o2, p2 = sv_forward(plan, a, s, ref_phase=ref_phase, want_phase=True, table_steps=True)
+    torch.cuda.synchronize()
+    assert torch.equal(o1, o2) and torch.equal(p1, p2)
+    ref = sv_batch(g, th.astype(np.float32).astype(np.float64), st)
+    if ref_phase:
+        assert np.abs(o1.cpu().numpy() - ref).max() < AMP_TOL
+    else:
+        assert phase_aligned_max_err(o1.cpu().numpy(), ref) < AMP_TOL

@@ -260,3 +260,31 @@ def test_coefficient_rows_per_step_or_deduplicated(native):
             # a row is shared only by steps with the same measured slot
             for r in set(idx.tolist()):
                 assert len({(int(x[1]) >> 8) & 0xFF for x, i in zip(st, idx) if i == r}) == 1
+
+
+def test_periodic_plans(native):
+    """Grids in column order and chains are periodic: every step appends a CZ
+    neighbour of the measured qubit and the appended qubit's other CZ
+    neighbours repeat every n_slots steps, so the forward kernel fixes the pair
+    coefficients at compile time.  Other patterns and wider windows are not."""
+    from mentpy_amd.simulators import compile_pattern
+    cases = [(mp.templates.linear_cluster(9), None, 1)]
+    cases += [(mp.templates.grid_cluster(r, c), None, 1)
+              for r, c in [(2, 5), (3, 4), (4, 20), (4, 16), (5, 7), (6, 8)]]
+    cases += [(mp.templates.grid_cluster(3, 4), 5, 0),      # window above the minimum
+              (mp.templates.grid_cluster(7, 3), None, 0),   # 7 slots: no coefficient table
+              (mp.templates.muta(2, 1), None, 0)]
+    for g, w, expect in cases:
+        plan = compile_pattern(g, window_size=w or 1)[0]
+        info = plan.info
+        assert info["periodic"] == expect, (len(g.graph), w)
+        if expect:
+            st = plan.tables()[0]
+            ns, n = info["n_slots"], info["n_steps"]
+            flags = st[:, 1].astype(np.int64)
+            nmask = st[:, 2].astype(np.int64)
+            assert n % ns == 0
+            assert np.all(flags & 3 == 3)
+            assert np.array_equal(nmask, np.tile(nmask[:ns], n // ns))
+            # the appended qubit's earlier neighbour sits in the slot appended one step before
+            assert nmask[0] == 0 and all(nmask[s] == 1 << (ns - s) for s in range(1, ns))
