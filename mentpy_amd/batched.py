@@ -108,12 +108,16 @@ class BatchedForward:
         return {"waves": cus * 4 * 4, "wg_waves": 4, "sims_per_item": self.sims_per_item}
 
     def launch(self, waves: Optional[int] = None, wg_waves: Optional[int] = None,
-               ref_phase: bool = True, renormalize: bool = True):
+               ref_phase: bool = True, renormalize: bool = True, table_steps: bool = False):
+        """One persistent launch over every job.  ``table_steps`` makes groups
+        of periodic plans read the pair coefficients from the tables (same
+        results)."""
         torch = _torch()
         cfg = self.config or self.default_config()
         waves = int(waves or cfg["waves"])
         wg = int(wg_waves or cfg["wg_waves"])
-        flags = (nat.FLAG_REF_PHASE if ref_phase else 0) | (nat.FLAG_RENORMALIZE if renormalize else 0)
+        flags = ((nat.FLAG_REF_PHASE if ref_phase else 0) | (nat.FLAG_RENORMALIZE if renormalize else 0)
+                 | (nat.FLAG_TABLE_STEPS if table_steps else 0))
         stream = C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
         with torch.cuda.device(self.device):
             nat.check(nat.lib().mbqc_batch_launch(self._handle, waves, wg, flags, stream))

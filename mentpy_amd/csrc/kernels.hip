@@ -312,11 +312,11 @@ __device__ __forceinline__ void sv_run_blocks(float2 (&t)[1 << NS], const Job& J
 }
 
 // All steps of the pattern.
-template <int NS, bool REF, typename OnBlock, typename Mod = NoModify>
+template <int NS, bool REF, bool PERIODIC = false, typename OnBlock, typename Mod = NoModify>
 __device__ __forceinline__ void sv_run_steps(float2 (&t)[1 << NS], const Job& J, const float* ang,
                                              RefPhase* ph, OnBlock on_block, Mod&& mod = Mod()) {
-  sv_run_blocks<NS, REF, false>(t, J, ang, ph, 0, (J.n_steps + NS - 1) / NS, on_block,
-                                static_cast<Mod&&>(mod));
+  sv_run_blocks<NS, REF, false, false, PERIODIC>(t, J, ang, ph, 0, (J.n_steps + NS - 1) / NS,
+                                                 on_block, static_cast<Mod&&>(mod));
 }
 
 template <int NS>
@@ -720,7 +720,7 @@ struct SampleMod {
   }
 };
 
-template <int NS>
+template <int NS, bool PERIODIC>
 __global__ __launch_bounds__(256) void k_sv_sample(Job J, const uint64_t* __restrict__ xm,
                                                    const uint64_t* __restrict__ zm,
                                                    const int32_t* __restrict__ out_off,
@@ -743,7 +743,7 @@ __global__ __launch_bounds__(256) void k_sv_sample(Job J, const uint64_t* __rest
   mod.n = J.n_steps;
   int32_t* row = outcomes + gg * (words + 2);
   mod.out_words = active ? row : nullptr;
-  sv_run_steps<NS, false>(t, J, ang, &ph, [](int, const float2 (&)[H]) {}, mod);
+  sv_run_steps<NS, false, PERIODIC>(t, J, ang, &ph, [](int, const float2 (&)[H]) {}, mod);
   // residual frame of the output qubits, in output-index bit order
   uint32_t xo = 0, zo = 0;
   for (int m = 0; m < J.n_out; ++m) {
@@ -2104,7 +2104,7 @@ __device__ __forceinline__ Job load_job(const Job* jobs, int i) {
 template <int NS>
 constexpr int kBatchStore = kStoreDirect;
 
-template <bool REF, int NS, bool SH>
+template <bool REF, int NS, bool SH, bool PERIODIC>
 __global__ __launch_bounds__(256) void k_batch(const Job* __restrict__ jobs,
                                                const BatchItem* __restrict__ items, int64_t n_items,
                                                int* counter, uint32_t flags) {
@@ -2122,8 +2122,8 @@ __global__ __launch_bounds__(256) void k_batch(const Job* __restrict__ jobs,
       const int rows = int(std::min<int64_t>(std::min<int64_t>(I.count - off, left), 64));
       // direct stores: staging tiles would cost the concurrently running
       // window-size groups LDS occupancy (measured slower)
-      sv_sim<NS, REF, kBatchStore<NS>, false, SH>(J, g, (off + lane) < I.count && g < J.total, rows,
-                                                  flags);
+      sv_sim<NS, REF, kBatchStore<NS>, false, SH, PERIODIC>(
+          J, g, (off + lane) < I.count && g < J.total, rows, flags);
     }
   }
 }
@@ -2394,12 +2394,13 @@ struct BatchGroupLaunch {
 // Lanes g = b*S + s of one angle vector form aligned groups of
 // share_group(NS) lanes when every job's S is a multiple of it (work items
 // start at multiples of 64 and B*S is then a multiple of the group too).
-template <int NS>
-void launch_batch_group(const BatchGroupLaunch& L, bool ref, bool share) {
+// `periodic`: every plan of the group is periodic.
+template <int NS, bool PER>
+void launch_batch_group_p(const BatchGroupLaunch& L, bool ref, bool share) {
   const size_t lds =
       kBatchStore<NS> == kStoreColumns ? (L.block.x / 64) * kColumnTile * sizeof(float2) : 0;
-#define MBQC_BATCH_LAUNCH(R, SH)                                                                \
-  hipLaunchKernelGGL((k_batch<R, NS, SH>), L.grid, L.block, lds, L.stream, L.jobs, L.items, \
+#define MBQC_BATCH_LAUNCH(R, SH)                                                                     \
+  hipLaunchKernelGGL((k_batch<R, NS, SH, PER>), L.grid, L.block, lds, L.stream, L.jobs, L.items, \
                      L.n_items, L.counter, L.flags)
   if constexpr (NS >= 2) {
     if (share) {
@@ -2413,6 +2414,12 @@ void launch_batch_group(const BatchGroupLaunch& L, bool ref, bool share) {
 #undef MBQC_BATCH_LAUNCH
 }
 
+template <int NS>
+void launch_batch_group(const BatchGroupLaunch& L, bool ref, bool share, bool periodic) {
+  if (periodic) launch_batch_group_p<NS, true>(L, ref, share);
+  else launch_batch_group_p<NS, false>(L, ref, share);
+}
+
 constexpr int kMaxSvSlots = 6;       // register-resident SV kernels
 // below this many simulations the DM path keeps each density matrix in LDS
 // (one quartet per thread, several simulations per workgroup)
@@ -2424,13 +2431,17 @@ constexpr int kMaxAdjSlots = 5;
 
 template <int NS>
 hipError_t launch_sample(const Job& J, const int32_t* tab, const TableOffsets& o, int32_t* outcomes,
-                         int32_t words, uint64_t seed, hipStream_t st) {
+                         int32_t words, uint64_t seed, bool periodic, hipStream_t st) {
   const int threads = 256;
   const int64_t blocks = (J.total + threads - 1) / threads;
   const uint64_t* xm = reinterpret_cast<const uint64_t*>(tab + o.xmask);
   const uint64_t* zm = reinterpret_cast<const uint64_t*>(tab + o.zmask);
-  hipLaunchKernelGGL((k_sv_sample<NS>), dim3(blocks), dim3(threads), 0, st, J, xm, zm,
-                     tab + o.out_offset, outcomes, words, seed);
+  if (periodic)
+    hipLaunchKernelGGL((k_sv_sample<NS, true>), dim3(blocks), dim3(threads), 0, st, J, xm, zm,
+                       tab + o.out_offset, outcomes, words, seed);
+  else
+    hipLaunchKernelGGL((k_sv_sample<NS, false>), dim3(blocks), dim3(threads), 0, st, J, xm, zm,
+                       tab + o.out_offset, outcomes, words, seed);
   return hipGetLastError();
 }
 
@@ -2491,6 +2502,8 @@ struct mbqc_batch {
   hipEvent_t join[kMaxSvSlots + 1] = {};
   // every job of the group has S a multiple of share_group(ns)
   bool share[kMaxSvSlots + 1] = {};
+  // every plan of the group is periodic (Plan::periodic)
+  bool periodic[kMaxSvSlots + 1] = {};
 };
 
 extern "C" {
@@ -2555,12 +2568,12 @@ int mbqc_sv_sample_c64(const mbqc_plan* plan, const float* angles, int64_t angle
   hipStream_t st = static_cast<hipStream_t>(stream);
   hipError_t e = hipSuccess;
   switch (p->ns) {
-    case 1: e = launch_sample<1>(J, tab, o, outcomes, words, seed, st); break;
-    case 2: e = launch_sample<2>(J, tab, o, outcomes, words, seed, st); break;
-    case 3: e = launch_sample<3>(J, tab, o, outcomes, words, seed, st); break;
-    case 4: e = launch_sample<4>(J, tab, o, outcomes, words, seed, st); break;
-    case 5: e = launch_sample<5>(J, tab, o, outcomes, words, seed, st); break;
-    case 6: e = launch_sample<6>(J, tab, o, outcomes, words, seed, st); break;
+    case 1: e = launch_sample<1>(J, tab, o, outcomes, words, seed, p->periodic, st); break;
+    case 2: e = launch_sample<2>(J, tab, o, outcomes, words, seed, p->periodic, st); break;
+    case 3: e = launch_sample<3>(J, tab, o, outcomes, words, seed, p->periodic, st); break;
+    case 4: e = launch_sample<4>(J, tab, o, outcomes, words, seed, p->periodic, st); break;
+    case 5: e = launch_sample<5>(J, tab, o, outcomes, words, seed, p->periodic, st); break;
+    case 6: e = launch_sample<6>(J, tab, o, outcomes, words, seed, p->periodic, st); break;
     default: return fail(MBQC_ERR_NOT_IMPLEMENTED, "sampled outcomes need n_slots <= 6");
   }
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "sv_sample launch");
@@ -2822,6 +2835,7 @@ int mbqc_batch_create(const mbqc_batch_job* jobs, int32_t n_jobs, int32_t sims_p
     return fail(MBQC_ERR_VALUE, "sims_per_item must be a positive multiple of 64");
   auto* bt = new mbqc_batch();
   for (bool& sh : bt->share) sh = true;
+  for (bool& pe : bt->periodic) pe = true;
   struct Key {
     int ns;
     int64_t cost;
@@ -2850,6 +2864,7 @@ int mbqc_batch_create(const mbqc_batch_job* jobs, int32_t n_jobs, int32_t sims_p
     bt->jobs.push_back(J);
     bt->max_ns = std::max(bt->max_ns, p->ns);
     if (p->ns < 2 || J.S % share_group(p->ns) != 0) bt->share[p->ns] = false;
+    if (!p->periodic) bt->periodic[p->ns] = false;
     const int64_t cost = int64_t(p->n_steps) << p->ns;
     for (int64_t f = 0; f < J.total; f += sims_per_item) {
       BatchItem it{};
@@ -3019,15 +3034,16 @@ int mbqc_batch_launch(mbqc_batch* bt, int32_t waves, int32_t wg_waves, uint32_t 
     const int blocks = int(wv / wg_waves);
     const BatchItem* items = bt->d_items + bt->group_begin[ns];
     int* counter = bt->d_counter + ns * kCounterStride;
-const BatchGroupLaunch L{dim3(blocks), dim3(threads), gs, bt->d_jobs, items, n, counter, flags};
+    const BatchGroupLaunch L{dim3(blocks), dim3(threads), gs, bt->d_jobs, items, n, counter, flags};
     const bool sh = bt->share[ns];
+    const bool pe = bt->periodic[ns] && !(flags & MBQC_FLAG_TABLE_STEPS);
     switch (ns) {
-      case 1: launch_batch_group<1>(L, ref, false); break;
-      case 2: launch_batch_group<2>(L, ref, sh); break;
-      case 3: launch_batch_group<3>(L, ref, sh); break;
-      case 4: launch_batch_group<4>(L, ref, sh); break;
-      case 5: launch_batch_group<5>(L, ref, sh); break;
-      default: launch_batch_group<6>(L, ref, sh); break;
+      case 1: launch_batch_group<1>(L, ref, false, pe); break;
+      case 2: launch_batch_group<2>(L, ref, sh, pe); break;
+      case 3: launch_batch_group<3>(L, ref, sh, pe); break;
+      case 4: launch_batch_group<4>(L, ref, sh, pe); break;
+      case 5: launch_batch_group<5>(L, ref, sh, pe); break;
+      default: launch_batch_group<6>(L, ref, sh, pe); break;
     }
     e = hipGetLastError();
     if (e == hipSuccess && concurrent) {

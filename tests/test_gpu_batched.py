@@ -86,3 +86,31 @@ def test_batched_shared_coefficient_groups(S_list, rng):
         assert (single - o).abs().max().item() < 1e-6
         ref = sv_batch(g, th[:3].astype(np.float64), st)
         assert np.abs(o[:3].cpu().numpy() - ref).max() < 2e-5
+
+
+def test_batched_periodic_groups(rng):
+    """Window-size groups of periodic plans (grids) take compile-time pair
+    coefficients; a group holding one non-periodic plan (a 3-row grid on a
+    5-qubit window joins the 4-row grids) reads the tables.  Both are
+    bit-identical to the all-table launch and equal the single launches."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from mentpy_amd.batched import BatchedForward
+    jobs = []
+    for r, c, w in [(2, 6, None), (3, 5, None), (4, 7, None), (5, 4, None), (3, 4, 5), (4, 5, None)]:
+        g = mp.templates.grid_cluster(r, c)
+        sim = mp.PatternSimulator(g, backend="hip-sv", **({"window_size": w} if w else {}))
+        assert sim.simulator.plan.info["periodic"] == (w is None)
+        th = rng.uniform(-7, 7, (40, len(g.trainable_nodes))).astype(np.float32)
+        st = np.stack([haar_state(r, rng) for _ in range(3)])
+        jobs.append((sim, th, st))
+    bf = BatchedForward(jobs, sims_per_item=64)
+    for ref_phase in (False, True):
+        a = [o.clone() for o in bf.launch(waves=256, wg_waves=2, ref_phase=ref_phase)]
+        b = [o.clone() for o in bf.launch(waves=256, wg_waves=2, ref_phase=ref_phase,
+                                          table_steps=True)]
+        torch.cuda.synchronize()
+        for x, y in zip(a, b):
+            assert torch.equal(x, y)
+    for (sim, th, st), o in zip(jobs, a):    # reference phase, as run_batch
+        assert (sim.run_batch(th, st) - o).abs().max().item() < 1e-6

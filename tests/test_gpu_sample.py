@@ -21,18 +21,23 @@ def _torch():
     return torch
 
 
-@pytest.mark.parametrize("rows,cols", [(1, 5), (2, 4), (2, 7), (3, 4), (3, 5), (4, 3)])
-def test_sampled_matches_pennylane_order_oracle(rows, cols, rng):
+@pytest.mark.parametrize("rows,cols,w", [(1, 5, None), (2, 4, None), (2, 7, None), (3, 4, None),
+                                         (3, 5, None), (4, 3, None), (3, 4, 5)])
+def test_sampled_matches_pennylane_order_oracle(rows, cols, w, rng):
+    """Grids in column order run the periodic kernel (compile-time pair
+    coefficients), the wider window the table kernel."""
     torch = _torch()
     import mentpy_amd as mp
     from mentpy_amd.simulators.hip_simulator import (decode_outcomes, sv_forward, sv_sample,
                                                       _as_angles, _as_states)
     from oracle.brute import brute_sv_sampled
     g = mp.templates.linear_cluster(cols) if rows == 1 else mp.templates.grid_cluster(rows, cols)
-    sim = mp.PatternSimulator(g, backend="hip-sv", force0=False, seed=7)
+    sim = mp.PatternSimulator(g, backend="hip-sv", force0=False, seed=7,
+                              **({"window_size": w} if w else {}))
     core = sim.simulator
     plan = core.plan
     assert plan.info["can_sample"] == 1
+    assert plan.info["periodic"] == (w is None)
     B, S = 6, 4
     th = rng.uniform(0, 2 * np.pi, (B, len(g.trainable_nodes)))
     X = np.stack([haar_state(rows, rng) for _ in range(S)])
