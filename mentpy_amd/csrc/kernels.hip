@@ -1063,7 +1063,7 @@ __global__ __launch_bounds__(T * G) void k_dm_forward_lds(Job J, DmNoise nz, uin
 // Forward pass of the adjoint: raw phase, unnormalized output, one checkpoint
 // of the live state at the start of every block of NS steps (after the
 // block's rescale), stored column-fastest for coalesced access.
-template <int NS>
+template <int NS, bool PERIODIC>
 // ckpt is __restrict__ so the checkpoint stores inside the step loop do not
 // count as possible writes to the plan tables (which keeps table reads scalar).
 // A checkpoint is kept every `ck` blocks (1 or 2; 2 halves the checkpoint
@@ -1080,7 +1080,7 @@ __global__ __launch_bounds__(256) void k_sv_adj_forward(Job J, float2* __restric
   sv_load_input<NS>(J, J.psi + si * J.in_stride, t);
   RefPhase ph;
   const int64_t total = J.total;
-  sv_run_steps<NS, false>(t, J, ang, &ph, [&](int blk, const float2 (&tt)[H]) {
+  sv_run_steps<NS, false, PERIODIC>(t, J, ang, &ph, [&](int blk, const float2 (&tt)[H]) {
     if (active && (blk & (ck - 1)) == 0) {
       const int64_t c = blk >> (ck - 1);
 #pragma unroll
@@ -1125,6 +1125,31 @@ __device__ __forceinline__ float sv_adj_step(float2 (&lam)[1 << NS], const float
   return fmaf(b.x, ai, b.y * ar);
 }
 
+// sv_adj_step of a periodic step (alpha = -sign, beta = 2 sign with sign the
+// parity of j & NMASK, as in sv_step_periodic): the same arithmetic.
+template <int NS, int Q, uint32_t NMASK>
+__device__ __forceinline__ float sv_adj_step_periodic(float2 (&lam)[1 << NS],
+                                                      const float2 (&y)[1 << (NS - 1)], float2 b) {
+  constexpr int H = 1 << NS;
+  constexpr int QB = 1 << Q;
+  const float2 bc = cconj(b);
+  float ar = 0.f, ai = 0.f;
+  int i = 0;
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    if (j & QB) continue;
+    const float sg = (__builtin_popcount(uint32_t(j) & NMASK) & 1) ? -1.0f : 1.0f;
+    const float2 l0 = lam[j], l1 = lam[j | QB];
+    const float2 h = make_float2(fmaf(-sg, l1.x, l0.x), fmaf(-sg, l1.y, l0.y));
+    const float2 yy = y[i++];
+    ar = fmaf(yy.x, h.x, fmaf(yy.y, h.y, ar));
+    ai = fmaf(yy.y, h.x, fmaf(-yy.x, h.y, ai));
+    lam[j] = make_float2(fmaf(2.0f * sg, l1.x, h.x), fmaf(2.0f * sg, l1.y, h.y));
+    lam[j | QB] = cmul(bc, h);
+  }
+  return fmaf(b.x, ai, b.y * ar);
+}
+
 template <int NS, int Q>
 __device__ __forceinline__ void take_y(const float2 (&t)[1 << NS], float2 (&y)[1 << (NS - 1)]) {
   constexpr int QB = 1 << Q;
@@ -1145,7 +1170,7 @@ struct AdjCtx {
   int32_t n_angles;
 };
 
-template <int NS, int SUB>
+template <int NS, bool PERIODIC, int SUB>
 __device__ __forceinline__ void adj_recompute(float2 (&t)[1 << NS], float2 (&ys)[NS][1 << (NS - 1)],
                                               float2 (&bs)[NS], const mbqc_step* steps,
                                               const float* coef, int k0, int n,
@@ -1154,7 +1179,12 @@ __device__ __forceinline__ void adj_recompute(float2 (&t)[1 << NS], float2 (&ys)
   if (k < n) {
     bs[SUB] = step_coeff<false>(th[SUB], nullptr);
     take_y<NS, NS - 1 - SUB>(t, ys[SUB]);
-    if (SUB + 1 < NS) sv_step<NS, NS - 1 - SUB>(t, bs[SUB], coef_row<NS>(coef, steps, k));
+    if constexpr (SUB + 1 < NS) {
+      if constexpr (PERIODIC)
+        sv_step_periodic<NS, NS - 1 - SUB, periodic_nmask(NS, SUB)>(t, bs[SUB]);
+      else
+        sv_step<NS, NS - 1 - SUB>(t, bs[SUB], coef_row<NS>(coef, steps, k));
+    }
   }
 }
 
@@ -1187,14 +1217,18 @@ __device__ __forceinline__ float lane_group_sum(float v, int red) {
   return v;
 }
 
-template <int NS, int SUB>
+template <int NS, bool PERIODIC, int SUB>
 __device__ __forceinline__ void adj_back(float2 (&lam)[1 << NS], const float2 (&ys)[NS][1 << (NS - 1)],
                                          const float2 (&bs)[NS], const mbqc_step* steps,
                                          const float* coef, int k0, int n, const AdjCtx& A,
                                          float* prow, bool writer) {
   const int k = k0 + SUB;
   if (k < n) {
-    float d = sv_adj_step<NS, NS - 1 - SUB>(lam, ys[SUB], bs[SUB], coef_row<NS>(coef, steps, k));
+    float d;
+    if constexpr (PERIODIC)
+      d = sv_adj_step_periodic<NS, NS - 1 - SUB, periodic_nmask(NS, SUB)>(lam, ys[SUB], bs[SUB]);
+    else
+      d = sv_adj_step<NS, NS - 1 - SUB>(lam, ys[SUB], bs[SUB], coef_row<NS>(coef, steps, k));
     const int ai = steps[k].angle_index;
     if (ai >= 0) {
       d = lane_group_sum(d, A.red);
@@ -1203,18 +1237,18 @@ __device__ __forceinline__ void adj_back(float2 (&lam)[1 << NS], const float2 (&
   }
 }
 
-template <int NS, int... SUBS>
+template <int NS, bool PERIODIC, int... SUBS>
 __device__ __forceinline__ void adj_block(float2 (&t)[1 << NS], float2 (&lam)[1 << NS],
                                           const mbqc_step* steps, const float* coef, int k0, int n,
                                           const float (&th)[NS], const AdjCtx& A, float* prow,
                                           bool writer, std::integer_sequence<int, SUBS...>) {
   float2 ys[NS][1 << (NS - 1)];
   float2 bs[NS];
-  (adj_recompute<NS, SUBS>(t, ys, bs, steps, coef, k0, n, th), ...);
-  (adj_back<NS, NS - 1 - SUBS>(lam, ys, bs, steps, coef, k0, n, A, prow, writer), ...);
+  (adj_recompute<NS, PERIODIC, SUBS>(t, ys, bs, steps, coef, k0, n, th), ...);
+  (adj_back<NS, PERIODIC, NS - 1 - SUBS>(lam, ys, bs, steps, coef, k0, n, A, prow, writer), ...);
 }
 
-template <int NS>
+template <int NS, bool PERIODIC>
 // The pointers are __restrict__ (and the partial sums come in as their own
 // argument): the per-step partial-sum stores would otherwise count as possible
 // writes to the plan tables, turning every table read into a vector load.
@@ -1266,23 +1300,23 @@ __global__ __launch_bounds__(64) void k_sv_adj_backward(Job J, const float2* __r
       for (int j = 0; j < H; ++j) t[j] = s0[j];
       RefPhase ph;
       NoModify nomod;
-      sv_block<NS, false, true, false, false, false>(t, steps, coef, b0 * NS, n, th0, &ph, nomod,
-                                              std::make_integer_sequence<int, NS>{});
+      sv_block<NS, false, true, false, false, PERIODIC>(t, steps, coef, b0 * NS, n, th0, &ph,
+                                                        nomod, std::make_integer_sequence<int, NS>{});
       if (resc[b0 + 1]) {
 #pragma unroll
         for (int j = 0; j < H; ++j) t[j] = cscale(t[j], 0x1p-32f);
       }
       float th1[NS];
       load_block_angles<NS>(steps, (b0 + 1) * NS, n, ang, J.angle_stride_a, th1);
-      adj_block<NS>(t, lam, steps, coef, (b0 + 1) * NS, n, th1, A, prow, writer,
-                    std::make_integer_sequence<int, NS>{});
+      adj_block<NS, PERIODIC>(t, lam, steps, coef, (b0 + 1) * NS, n, th1, A, prow, writer,
+                              std::make_integer_sequence<int, NS>{});
       if (resc[b0 + 1]) {
 #pragma unroll
         for (int j = 0; j < H; ++j) lam[j] = cscale(lam[j], 0x1p-32f);
       }
     }
-    adj_block<NS>(s0, lam, steps, coef, b0 * NS, n, th0, A, prow, writer,
-                  std::make_integer_sequence<int, NS>{});
+    adj_block<NS, PERIODIC>(s0, lam, steps, coef, b0 * NS, n, th0, A, prow, writer,
+                            std::make_integer_sequence<int, NS>{});
     if (resc[b0]) {
 #pragma unroll
       for (int j = 0; j < H; ++j) lam[j] = cscale(lam[j], 0x1p-32f);
@@ -1337,7 +1371,7 @@ __device__ __forceinline__ float2 cfma_conj(float2 a, float2 b, float2 acc) {
 // inside one segment.  `ckpt` and `ws` are the same workspace: the checkpoint
 // stores go through their own restrict pointer, so the compiler can prove
 // they never touch the plan tables and keeps the table reads on scalar loads.
-template <int NS>
+template <int NS, bool PERIODIC>
 __global__ __launch_bounds__(64) void k_sv_seg_forward(Job J, SegInfo G, float2* __restrict__ ckpt,
                                                        float2* __restrict__ ws) {
   constexpr int H = 1 << NS;
@@ -1364,7 +1398,8 @@ __global__ __launch_bounds__(64) void k_sv_seg_forward(Job J, SegInfo G, float2*
   RefPhase ph;
   float2* ck = ckpt + G.ckpt[p];
   const int b0 = G.blk[p];
-  sv_run_blocks<NS, false, false>(t, J, ang, &ph, b0, G.blk[p + 1], [&](int blk, const float2 (&tt)[H]) {
+  sv_run_blocks<NS, false, false, false, PERIODIC>(t, J, ang, &ph, b0, G.blk[p + 1],
+                                                   [&](int blk, const float2 (&tt)[H]) {
     if (active) {
 #pragma unroll
       for (int j = 0; j < H; ++j) ck[(int64_t(blk - b0) * H + j) * Np + l] = tt[j];
@@ -1524,7 +1559,7 @@ __global__ __launch_bounds__(256) void k_seg_chain_backward(SegInfo G, float2* _
   for (int e = tid; e < C0 * H; e += 256) g0[e] = Lm[(e >> NS) * LD + (e & (H - 1))];
 }
 
-template <int NS>
+template <int NS, bool PERIODIC>
 __global__ __launch_bounds__(64) void k_sv_seg_backward(Job J, SegInfo G, const float2* __restrict__ ws,
                                                         int32_t n_angles, float* __restrict__ grad) {
   constexpr int H = 1 << NS;
@@ -1570,8 +1605,8 @@ __global__ __launch_bounds__(64) void k_sv_seg_backward(Job J, SegInfo G, const 
     for (int j = 0; j < H; ++j) t[j] = cb[int64_t(j) * Np];
     float th[NS];
     load_block_angles<NS>(steps, blk * NS, n, ang, J.angle_stride_a, th);
-    adj_block<NS>(t, lam, steps, coef, blk * NS, n, th, A, prow, writer,
-                  std::make_integer_sequence<int, NS>{});
+    adj_block<NS, PERIODIC>(t, lam, steps, coef, blk * NS, n, th, A, prow, writer,
+                            std::make_integer_sequence<int, NS>{});
     if (resc[blk]) {
 #pragma unroll
       for (int j = 0; j < H; ++j) lam[j] = cscale(lam[j], 0x1p-32f);
@@ -2295,23 +2330,30 @@ hipError_t launch_dm(const Job& J, const DmNoise& nz, uint32_t flags, hipStream_
 inline int adj_ckpt_blocks(int64_t n_columns) { return n_columns >= 65536 ? 2 : 1; }
 
 template <int NS>
-hipError_t launch_adj_fwd(const Job& J, float2* ckpt, int ck, hipStream_t st) {
+hipError_t launch_adj_fwd(const Job& J, float2* ckpt, int ck, bool periodic, hipStream_t st) {
   // one wave per workgroup: at small batches the waves spread over all CUs
   // instead of sharing a few CUs' scalar units and instruction issue
   const int threads = 64;
   const int64_t blocks = (J.total + threads - 1) / threads;
-  hipLaunchKernelGGL((k_sv_adj_forward<NS>), dim3(blocks), dim3(threads),
-                     (threads / 64) * kColumnTile * sizeof(float2), st, J, ckpt, ck);
+  const size_t lds = (threads / 64) * kColumnTile * sizeof(float2);
+  if (periodic)
+    hipLaunchKernelGGL((k_sv_adj_forward<NS, true>), dim3(blocks), dim3(threads), lds, st, J, ckpt, ck);
+  else
+    hipLaunchKernelGGL((k_sv_adj_forward<NS, false>), dim3(blocks), dim3(threads), lds, st, J, ckpt, ck);
   return hipGetLastError();
 }
 
 template <int NS>
 hipError_t launch_adj_bwd(const Job& J, const float2* ckpt, const float2* seed, const AdjCtx& A,
-                          hipStream_t st) {
+                          bool periodic, hipStream_t st) {
   const int threads = 64;
   const int64_t blocks = (J.total + threads - 1) / threads;
-  hipLaunchKernelGGL((k_sv_adj_backward<NS>), dim3(blocks), dim3(threads), 0, st, J, ckpt, seed, A,
-                     A.gpart);
+  if (periodic)
+    hipLaunchKernelGGL((k_sv_adj_backward<NS, true>), dim3(blocks), dim3(threads), 0, st, J, ckpt,
+                       seed, A, A.gpart);
+  else
+    hipLaunchKernelGGL((k_sv_adj_backward<NS, false>), dim3(blocks), dim3(threads), 0, st, J, ckpt,
+                       seed, A, A.gpart);
   return hipGetLastError();
 }
 
@@ -2351,10 +2393,16 @@ size_t seg_layout(const Plan& p, int64_t B, int n, SegInfo* G) {
 }
 
 template <int NS>
-hipError_t launch_seg_forward(const Job& J, const SegInfo& G, float2* ws, float2* out, hipStream_t st) {
-  if (G.wave[G.n] > 0)
-    hipLaunchKernelGGL((k_sv_seg_forward<NS>), dim3(G.wave[G.n]), dim3(64),
-                       kColumnTile * sizeof(float2), st, J, G, ws, ws);
+hipError_t launch_seg_forward(const Job& J, const SegInfo& G, float2* ws, float2* out, bool periodic,
+                              hipStream_t st) {
+  if (G.wave[G.n] > 0) {
+    if (periodic)
+      hipLaunchKernelGGL((k_sv_seg_forward<NS, true>), dim3(G.wave[G.n]), dim3(64),
+                         kColumnTile * sizeof(float2), st, J, G, ws, ws);
+    else
+      hipLaunchKernelGGL((k_sv_seg_forward<NS, false>), dim3(G.wave[G.n]), dim3(64),
+                         kColumnTile * sizeof(float2), st, J, G, ws, ws);
+  }
   hipError_t e = hipGetLastError();
   if (e == hipSuccess && G.B > 0) {
     if constexpr (NS == 4)
@@ -2367,7 +2415,7 @@ hipError_t launch_seg_forward(const Job& J, const SegInfo& G, float2* ws, float2
 
 template <int NS>
 hipError_t launch_seg_backward(const Job& J, const SegInfo& G, float2* ws, const float2* seed,
-                               int32_t n_angles, float* grad, hipStream_t st) {
+                               int32_t n_angles, float* grad, bool periodic, hipStream_t st) {
   if (G.B > 0) {
     if constexpr (NS == 4)
       hipLaunchKernelGGL(k_seg_chain_backward_mfma, dim3(G.B), dim3(64), 0, st, G, ws, seed);
@@ -2375,9 +2423,14 @@ hipError_t launch_seg_backward(const Job& J, const SegInfo& G, float2* ws, const
       hipLaunchKernelGGL((k_seg_chain_backward<NS>), dim3(G.B), dim3(256), 0, st, G, ws, seed);
   }
   hipError_t e = hipGetLastError();
-  if (e == hipSuccess && G.wave[G.n] > 0)
-    hipLaunchKernelGGL((k_sv_seg_backward<NS>), dim3(G.wave[G.n]), dim3(64), 0, st, J, G, ws,
-                       n_angles, grad);
+  if (e == hipSuccess && G.wave[G.n] > 0) {
+    if (periodic)
+      hipLaunchKernelGGL((k_sv_seg_backward<NS, true>), dim3(G.wave[G.n]), dim3(64), 0, st, J, G,
+                         ws, n_angles, grad);
+    else
+      hipLaunchKernelGGL((k_sv_seg_backward<NS, false>), dim3(G.wave[G.n]), dim3(64), 0, st, J, G,
+                         ws, n_angles, grad);
+  }
   return e == hipSuccess ? hipGetLastError() : e;
 }
 
@@ -2717,11 +2770,11 @@ int mbqc_sv_segmented_forward_c64(const mbqc_plan* plan, const float* angles, in
   float2* o = static_cast<float2*>(out);
   hipError_t e = hipSuccess;
   switch (p->ns) {
-    case 1: e = launch_seg_forward<1>(J, G, ws, o, st); break;
-    case 2: e = launch_seg_forward<2>(J, G, ws, o, st); break;
-    case 3: e = launch_seg_forward<3>(J, G, ws, o, st); break;
-    case 4: e = launch_seg_forward<4>(J, G, ws, o, st); break;
-    case 5: e = launch_seg_forward<5>(J, G, ws, o, st); break;
+    case 1: e = launch_seg_forward<1>(J, G, ws, o, p->periodic, st); break;
+    case 2: e = launch_seg_forward<2>(J, G, ws, o, p->periodic, st); break;
+    case 3: e = launch_seg_forward<3>(J, G, ws, o, p->periodic, st); break;
+    case 4: e = launch_seg_forward<4>(J, G, ws, o, p->periodic, st); break;
+    case 5: e = launch_seg_forward<5>(J, G, ws, o, p->periodic, st); break;
   }
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "segmented forward launch");
 }
@@ -2743,11 +2796,11 @@ int mbqc_sv_segmented_backward_c64(const mbqc_plan* plan, const float* angles, i
   const float2* sd = static_cast<const float2*>(seed);
   hipError_t e = hipSuccess;
   switch (p->ns) {
-    case 1: e = launch_seg_backward<1>(J, G, ws, sd, p->n_angles, grad, st); break;
-    case 2: e = launch_seg_backward<2>(J, G, ws, sd, p->n_angles, grad, st); break;
-    case 3: e = launch_seg_backward<3>(J, G, ws, sd, p->n_angles, grad, st); break;
-    case 4: e = launch_seg_backward<4>(J, G, ws, sd, p->n_angles, grad, st); break;
-    case 5: e = launch_seg_backward<5>(J, G, ws, sd, p->n_angles, grad, st); break;
+    case 1: e = launch_seg_backward<1>(J, G, ws, sd, p->n_angles, grad, p->periodic, st); break;
+    case 2: e = launch_seg_backward<2>(J, G, ws, sd, p->n_angles, grad, p->periodic, st); break;
+    case 3: e = launch_seg_backward<3>(J, G, ws, sd, p->n_angles, grad, p->periodic, st); break;
+    case 4: e = launch_seg_backward<4>(J, G, ws, sd, p->n_angles, grad, p->periodic, st); break;
+    case 5: e = launch_seg_backward<5>(J, G, ws, sd, p->n_angles, grad, p->periodic, st); break;
   }
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "segmented backward launch");
 }
@@ -2778,11 +2831,11 @@ int mbqc_sv_adjoint_forward_c64(const mbqc_plan* plan, const float* angles, int6
   float2* ck = static_cast<float2*>(ckpt);
   hipError_t e = hipSuccess;
   switch (p->ns) {
-    case 1: e = launch_adj_fwd<1>(J, ck, adj_ckpt_blocks(B * C), st); break;
-    case 2: e = launch_adj_fwd<2>(J, ck, adj_ckpt_blocks(B * C), st); break;
-    case 3: e = launch_adj_fwd<3>(J, ck, adj_ckpt_blocks(B * C), st); break;
-    case 4: e = launch_adj_fwd<4>(J, ck, adj_ckpt_blocks(B * C), st); break;
-    case 5: e = launch_adj_fwd<5>(J, ck, adj_ckpt_blocks(B * C), st); break;
+    case 1: e = launch_adj_fwd<1>(J, ck, adj_ckpt_blocks(B * C), p->periodic, st); break;
+    case 2: e = launch_adj_fwd<2>(J, ck, adj_ckpt_blocks(B * C), p->periodic, st); break;
+    case 3: e = launch_adj_fwd<3>(J, ck, adj_ckpt_blocks(B * C), p->periodic, st); break;
+    case 4: e = launch_adj_fwd<4>(J, ck, adj_ckpt_blocks(B * C), p->periodic, st); break;
+    case 5: e = launch_adj_fwd<5>(J, ck, adj_ckpt_blocks(B * C), p->periodic, st); break;
   }
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "adjoint forward launch");
 }
@@ -2817,11 +2870,11 @@ int mbqc_sv_adjoint_backward_c64(const mbqc_plan* plan, const float* angles, int
   const float2* sd = static_cast<const float2*>(seed);
   hipError_t e = hipSuccess;
   switch (p->ns) {
-    case 1: e = launch_adj_bwd<1>(J, ck, sd, A, st); break;
-    case 2: e = launch_adj_bwd<2>(J, ck, sd, A, st); break;
-    case 3: e = launch_adj_bwd<3>(J, ck, sd, A, st); break;
-    case 4: e = launch_adj_bwd<4>(J, ck, sd, A, st); break;
-    case 5: e = launch_adj_bwd<5>(J, ck, sd, A, st); break;
+    case 1: e = launch_adj_bwd<1>(J, ck, sd, A, p->periodic, st); break;
+    case 2: e = launch_adj_bwd<2>(J, ck, sd, A, p->periodic, st); break;
+    case 3: e = launch_adj_bwd<3>(J, ck, sd, A, p->periodic, st); break;
+    case 4: e = launch_adj_bwd<4>(J, ck, sd, A, p->periodic, st); break;
+    case 5: e = launch_adj_bwd<5>(J, ck, sd, A, p->periodic, st); break;
   }
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "adjoint backward launch");
 }
