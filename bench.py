@@ -328,7 +328,7 @@ def config_sample(args, dev, stream):
             "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "c64", "data": "synthetic (seeded)",
             "config": {"workload": "grid_cluster(4,20) hip-sv, force0=False, B angle vectors x 1 Haar input",
-                       "batch": B, "kernels": "k_sv_sample<4> (output frame applied in-kernel)"},
+                       "batch": B, "kernels": "k_sv_sample<4,true> (periodic pair coefficients; output frame applied in-kernel)"},
             "max_infidelity_vs_postselected": float((1 - ov).abs().max().item()),
             "outcome_mean": float(bits.mean())}
 
@@ -467,7 +467,8 @@ def main():
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                 "traffic": int(TRAFFIC_BYTES_PER_SIM_4X20 * B),
-                "kernel": "k_sv_forward<4,true,true>", "flops_per_sim": f_sim}
+                "kernel": ("k_sv_forward_periodic<4,true>" if plan.info["periodic"]
+                           else "k_sv_forward<4,true,true>"), "flops_per_sim": f_sim}
 
     line = {
         "metric": "pattern sims/s + adjoint angle-grads/s, 4x20 grid, c64 vs c128 error",
