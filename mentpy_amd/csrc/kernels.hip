@@ -254,28 +254,28 @@ __device__ __forceinline__ void sv_block(float2 (&t)[1 << NS], const mbqc_step* 
    ...);
 }
 
-// Angles of the NS steps of a block (steps past the end get 0).  The index is
-// clamped so the load is always issued from a valid address; fixed-angle
-// steps select their stored angle afterwards.
+// Angles of the NS steps of a block.  Steps past the end (n_steps >= 1) read
+// the last step's angle, so every load is issued from a valid address without
+// a branch (their values are never used); fixed-angle steps select their
+// stored angle afterwards.
 template <int NS>
 __device__ __forceinline__ void fetch_block_angles(const mbqc_step* steps, int k0, int n_steps,
                                                    const float* ang, int64_t sa, float (&th)[NS]) {
 #pragma unroll
   for (int sub = 0; sub < NS; ++sub) {
-    const int k = k0 + sub;
-    float v = 0.0f;
-    if (k < n_steps) {
-      const int ai = steps[k].angle_index;
-      const float a = ang[int64_t(ai >= 0 ? ai : 0) * sa];
-      v = ai >= 0 ? a : steps[k].fixed_angle;
-    }
-    th[sub] = v;
+    const int k = k0 + sub < n_steps ? k0 + sub : n_steps - 1;
+    const int ai = steps[k].angle_index;
+    const float a = ang[int64_t(ai >= 0 ? ai : 0) * sa];
+    th[sub] = ai >= 0 ? a : steps[k].fixed_angle;
   }
 }
 
 // Run blocks [blk0, blk1) of NS steps: full blocks of NS unrolled steps with
 // the next block's angles in flight, then the partial tail block.
 // `on_block(blk, t)` runs at the start of every block (after its rescale).
+// The full blocks run in stretches that end before the next rescaled block,
+// so the inner loop is straight-line code whose loop-carried state needs no
+// register copies (a rescale branch inside it cost 32 moves per block).
 // PERIODIC: the full blocks take the compile-time coefficients of a periodic
 // plan (Plan::periodic), the tail block the table.
 template <int NS, bool REF, bool PER_STEP, bool SH = false, bool PERIODIC = false,
@@ -284,30 +284,43 @@ __device__ __forceinline__ void sv_run_blocks(float2 (&t)[1 << NS], const Job& J
                                               RefPhase* ph, int blk0, int blk1, OnBlock on_block,
                                               Mod&& mod = Mod()) {
   constexpr int H = 1 << NS;
+  if (blk0 >= blk1) return;
   const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
   const int32_t* resc = J.tab + J.off_svres;
   const float* coef = reinterpret_cast<const float*>(J.tab + J.off_coef);
   const int n = J.n_steps;
   const int nfull = n / NS;
+  const int full_end = nfull < blk1 ? nfull : blk1;
   float th_next[NS];
   const int64_t sa = J.angle_stride_a;
   fetch_block_angles<NS>(steps, blk0 * NS, n, ang, sa, th_next);
-  for (int blk = blk0; blk < blk1; ++blk) {
-    float th[NS];
+  int blk = blk0;
+  while (blk < full_end) {
+    int end = blk + 1;
+    while (end < full_end && !resc[end]) ++end;
+    if (resc[blk]) {
 #pragma unroll
-    for (int s2 = 0; s2 < NS; ++s2) th[s2] = th_next[s2];
-    if (blk + 1 < blk1) fetch_block_angles<NS>(steps, (blk + 1) * NS, n, ang, sa, th_next);
+      for (int j = 0; j < H; ++j) t[j] = cscale(t[j], 0x1p-32f);
+    }
+    for (; blk < end; ++blk) {
+      float th[NS];
+#pragma unroll
+      for (int s2 = 0; s2 < NS; ++s2) th[s2] = th_next[s2];
+      fetch_block_angles<NS>(steps, (blk + 1) * NS, n, ang, sa, th_next);
+      on_block(blk, t);
+      sv_block<NS, REF, true, PER_STEP, SH, PERIODIC>(t, steps, coef, blk * NS, n, th, ph, mod,
+                                                      std::make_integer_sequence<int, NS>{});
+    }
+  }
+  if (blk < blk1) {
+    // the partial tail block (blk == nfull)
     if (resc[blk]) {
 #pragma unroll
       for (int j = 0; j < H; ++j) t[j] = cscale(t[j], 0x1p-32f);
     }
     on_block(blk, t);
-    if (blk < nfull)
-      sv_block<NS, REF, true, PER_STEP, SH, PERIODIC>(t, steps, coef, blk * NS, n, th, ph, mod,
-                                                      std::make_integer_sequence<int, NS>{});
-    else
-      sv_block<NS, REF, false, PER_STEP, SH, false>(t, steps, coef, blk * NS, n, th, ph, mod,
-                                                    std::make_integer_sequence<int, NS>{});
+    sv_block<NS, REF, false, PER_STEP, SH, false>(t, steps, coef, blk * NS, n, th_next, ph, mod,
+                                                  std::make_integer_sequence<int, NS>{});
   }
 }
 
