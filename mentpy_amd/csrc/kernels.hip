@@ -39,6 +39,14 @@ __device__ __forceinline__ uint32_t czbit(uint32_t j, uint32_t mask) {
   return uint32_t(__builtin_popcount(j & mask) & 1) << 31;
 }
 
+// Round-to-nearest-even integer of v (|v| < 2^22) by the 1.5 * 2^23 shifter:
+// fl(v + 1.5 * 2^23) has a unit last mantissa bit, so its bits are
+// 0x4B400000 + rint(v) and its lowest bit is the integer's parity.  With the
+// shifter added in an FMA, v = a * c is rounded once (no rounding of a * c
+// first).
+constexpr float kShifter = 12582912.0f;      // 1.5 * 2^23
+constexpr uint32_t kShifterBits = 0x4B400000u;
+
 // sin and cos of x for |x| up to a few thousand: x = k pi + r with a two-term
 // Cody-Waite split of pi (hi part the nearest float), |r| <= pi/2, then
 // polynomials fitted for minimum maximum error on [-pi/2, pi/2] (odd degree 9
@@ -46,7 +54,8 @@ __device__ __forceinline__ uint32_t czbit(uint32_t j, uint32_t mask) {
 // 4,000 Chebyshev points, approximation error below 5e-10, float evaluation
 // error about 1e-7), and the sign (-1)^k on both.
 __device__ __forceinline__ void sincos_poly(float x, float* sn, float* cs) {
-  const float k = rintf(x * 0.318309886183790671538f);
+  const float kb = fmaf(x, 0.318309886183790671538f, kShifter);
+  const float k = kb - kShifter;
   float r = fmaf(-k, 3.14159274101257324219f, x);
   r = fmaf(-k, -8.74227800037248e-8f, r);
   const float r2 = r * r;
@@ -59,7 +68,7 @@ __device__ __forceinline__ void sincos_poly(float x, float* sn, float* cs) {
   pc = fmaf(r2, pc, 4.166670144e-2f);
   pc = fmaf(r2, pc, -0.5f);
   const float cr = fmaf(r2, pc, 1.0f);
-  const uint32_t sgn = uint32_t(int(k) & 1) << 31;
+  const uint32_t sgn = __float_as_uint(kb) << 31;
   *sn = __uint_as_float(__float_as_uint(sr) ^ sgn);
   *cs = __uint_as_float(__float_as_uint(cr) ^ sgn);
 }
@@ -70,13 +79,14 @@ __device__ __forceinline__ void sincos_poly(float x, float* sn, float* cs) {
 // theta_k/2 - pi m_k lies in [-pi/2, pi/2], so sgn(cos(theta_k/2)) = (-1)^m_k
 // (sgn(0) = +1: rint rounds half to even) and the product is
 // exp(i (sum_k theta_k / 2 - pi sum_k m_k)).  The angles are summed in double
-// (full-rate on this GPU) and the integers m_k in float (exact), and the phase
-// is formed once at the end.
+// (full-rate on this GPU) and the integers m_k (from the shifter's bits) in
+// int32, and the phase is formed once at the end.
 struct RefPhase {
   double th_sum = 0.0;
-  float m_sum = 0.0f;
+  int32_t m_sum = 0;
   __device__ __forceinline__ void add(float th) {
-    m_sum += rintf(th * 0.15915494309189533577f);
+    const float mb = fmaf(th, 0.15915494309189533577f, kShifter);
+    m_sum += int32_t(__float_as_uint(mb) - kShifterBits);
     th_sum += double(th);
   }
   __device__ __forceinline__ float2 value() const {
