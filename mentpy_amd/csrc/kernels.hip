@@ -910,16 +910,17 @@ __device__ __forceinline__ void sv_lds_step(float2* st, int H, const mbqc_step& 
   }
 }
 
-// One step on an LDS-resident density matrix (H x H), one quartet of the
-// measured qubit per thread iteration; gb = gamma e^{-i theta}.
+// One step on an LDS-resident density matrix (H x H) measuring slot q, one
+// quartet of the measured qubit per thread iteration; gb = gamma e^{-i theta}.
+// Called with a compile-time q (block sub-steps), the quartet addresses depend
+// on the thread only.
 template <int T>
-__device__ __forceinline__ void dm_lds_step(float2* r, int H, const mbqc_step& step, float2 gb,
-                                            int tid) {
+__device__ __forceinline__ void dm_lds_step(float2* r, int H, int q, uint32_t flags,
+                                            uint32_t nmask, float2 gb, int tid) {
   const int HQ = H / 2;
   const float2 gbc = cconj(gb);
-  const int q = int((step.flags >> 8) & 0xffu);
   const int QB = 1 << q;
-  const bool app = step.flags & 1u, sneg = step.flags & 2u;
+  const bool app = flags & 1u, sneg = flags & 2u;
   for (int i = tid; i < HQ * HQ; i += T) {
     const int j = insert_zero(i / HQ, q), kk = insert_zero(i % HQ, q);
     const float2 B00 = r[j * H + kk], B01 = r[j * H + (kk | QB)];
@@ -928,7 +929,7 @@ __device__ __forceinline__ void dm_lds_step(float2* r, int H, const mbqc_step& s
     const float2 P = cadd(B00, B11);
     const float2 r00 = cadd(P, cadd(U, V));
     if (app) {
-      const uint32_t sj = czbit(j, step.nmask), sk = czbit(kk, step.nmask);
+      const uint32_t sj = czbit(j, nmask), sk = czbit(kk, nmask);
       if (sneg) {
         const float2 M = csub(B00, B11);
         r[j * H + kk] = r00;
@@ -1004,9 +1005,31 @@ __global__ __launch_bounds__(T) void k_sv_forward_lds(Job J, uint32_t flags) {
   if (tid == 0 && (flags & MBQC_FLAG_WRITE_PHASE) && J.phase_out) J.phase_out[g] = ph;
 }
 
+// Sub-step SUB of the block starting at step k0 (measured slot NS - 1 - SUB);
+// c0 = k0's position in the staged coefficient chunk.
+template <int NS, int T, int SUB>
+__device__ __forceinline__ void dm_lds_sub(float2* r, const mbqc_step* steps, const float2* gb_s,
+                                           int k0, int c0, int n, int tid) {
+  const int k = k0 + SUB;
+  if (k < n) {
+    const mbqc_step st = steps[k];
+    const float2 gb = gb_s[c0 + SUB];
+    __syncthreads();
+    dm_lds_step<T>(r, 1 << NS, NS - 1 - SUB, st.flags, st.nmask, gb, tid);
+  }
+}
+
+template <int NS, int T, int... SUBS>
+__device__ __forceinline__ void dm_lds_block(float2* r, const mbqc_step* steps, const float2* gb_s,
+                                             int k0, int c0, int n, int tid,
+                                             std::integer_sequence<int, SUBS...>) {
+  (dm_lds_sub<NS, T, SUBS>(r, steps, gb_s, k0, c0, n, tid), ...);
+}
+
 // Density matrix in LDS: T threads per simulation (one quartet each per step)
 // and G simulations per workgroup, so that small windows (T < 64) still fill
-// whole waves.
+// whole waves.  Steps run in blocks of NS with the measured slot fixed per
+// sub-step; coefficients are staged in chunks of a multiple of NS steps.
 template <int NS, int T, int G>
 __global__ __launch_bounds__(T * G) void k_dm_forward_lds(Job J, DmNoise nz, uint32_t flags) {
   constexpr int H = 1 << NS;
@@ -1034,20 +1057,20 @@ __global__ __launch_bounds__(T * G) void k_dm_forward_lds(Job J, DmNoise nz, uin
     const float2 vk = flip(psi[in_index[k]], sign[k] ? 0x80000000u : 0u);
     r[e] = cmul(vj, cconj(vk));
   }
-  for (int k = 0; k < J.n_steps; ++k) {
-    if (k % NS == 0 && resc[k / NS]) {
+  constexpr int CH = (kAngleChunk / NS) * NS;
+  const int n = J.n_steps;
+  for (int k0 = 0; k0 < n; k0 += NS) {
+    if (resc[k0 / NS]) {
       __syncthreads();
       for (int e = tid; e < H * H; e += T) r[e] = cscale(r[e], 0x1p-32f);
     }
-    if (k % kAngleChunk == 0) {
+    const int c0 = k0 % CH;
+    if (c0 == 0) {
       __syncthreads();
-      stage_coeffs<T>(gb_s, steps, ang, J.angle_stride_a, k, J.n_steps, nz.gamma, tid);
+      stage_coeffs<T>(gb_s, steps, ang, J.angle_stride_a, k0, n, nz.gamma, tid);
       __syncthreads();
     }
-    const mbqc_step step = steps[k];
-    const float2 gb = gb_s[k % kAngleChunk];
-    __syncthreads();
-    dm_lds_step<T>(r, H, step, gb, tid);
+    dm_lds_block<NS, T>(r, steps, gb_s, k0, c0, n, tid, std::make_integer_sequence<int, NS>{});
   }
   // noise of every output qubit
   for (int m = 0; m < NS; ++m) {
@@ -2075,7 +2098,7 @@ __global__ __launch_bounds__(64) void k_dm_window(Job J, const float* __restrict
     const mbqc_step step = steps[k];
     const float2 gb = cscale(step_coeff<false>(ang[k], nullptr), nz.gamma);
     __syncthreads();
-    dm_lds_step<T>(r, H, step, gb, tid);
+    dm_lds_step<T>(r, H, int((step.flags >> 8) & 0xffu), step.flags, step.nmask, gb, tid);
   }
   __syncthreads();
   float tr = 0.0f;
