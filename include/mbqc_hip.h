@@ -34,8 +34,9 @@ extern "C" {
 #define MBQC_FLAG_REF_PHASE 1u    /* reference global phase: prod_k sgn(cos(t_k/2)) e^{i t_k/2} */
 #define MBQC_FLAG_RENORMALIZE 2u  /* divide the output by its norm (SV) / trace (DM) */
 #define MBQC_FLAG_WRITE_PHASE 4u  /* also write the per-sim global phase (SV forward) */
-#define MBQC_FLAG_TABLE_STEPS 8u  /* SV forward: read the pair coefficients from the
-                                     plan's table also for a periodic plan */
+#define MBQC_FLAG_TABLE_STEPS 8u  /* SV / DM forward and batched launch: read the step
+                                     coefficients and CZ masks from the plan's tables
+                                     also for a periodic plan (same results) */
 
 /* Host description of a pattern; all arrays are host memory, copied by
  * mbqc_plan_create.  Node ids are 0..n_nodes-1. */

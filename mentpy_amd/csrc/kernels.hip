@@ -623,21 +623,27 @@ __device__ __forceinline__ void dm_channel(HermReg<1 << NS>& r, const DmNoise& n
   }
 }
 
-template <int NS, bool REF, int SUB>
+// PERIODIC: the step flags of a periodic plan (append, CZ to the measured
+// qubit, neighbour mask periodic_nmask) as compile-time constants, so the CZ
+// signs of the quartets fold into the arithmetic.
+template <int NS, bool PERIODIC, int SUB>
 __device__ __forceinline__ void dm_sub(HermReg<1 << NS>& r, const mbqc_step* steps, int k0,
                                        int n_steps, const float (&th)[NS], float gamma) {
   const int k = k0 + SUB;
   if (k < n_steps) {
     const float2 b = step_coeff<false>(th[SUB], nullptr);
-    dm_step<NS, NS - 1 - SUB>(r, cscale(b, gamma), steps[k].flags, steps[k].nmask);
+    if constexpr (PERIODIC)
+      dm_step<NS, NS - 1 - SUB>(r, cscale(b, gamma), 3u, periodic_nmask(NS, SUB));
+    else
+      dm_step<NS, NS - 1 - SUB>(r, cscale(b, gamma), steps[k].flags, steps[k].nmask);
   }
 }
 
-template <int NS, int... SUBS>
+template <int NS, bool PERIODIC, int... SUBS>
 __device__ __forceinline__ void dm_block(HermReg<1 << NS>& r, const mbqc_step* steps, int k0,
                                          int n_steps, const float (&th)[NS], float gamma,
                                          std::integer_sequence<int, SUBS...>) {
-  (dm_sub<NS, false, SUBS>(r, steps, k0, n_steps, th, gamma), ...);
+  (dm_sub<NS, PERIODIC, SUBS>(r, steps, k0, n_steps, th, gamma), ...);
 }
 
 template <int NS, int... MS>
@@ -646,7 +652,7 @@ __device__ __forceinline__ void dm_output_noise(HermReg<1 << NS>& r, uint32_t li
   ((live & (1u << MS) ? dm_channel<NS, MS>(r, nz) : void()), ...);
 }
 
-template <int NS>
+template <int NS, bool PERIODIC>
 __global__ __launch_bounds__(256) void k_dm_forward(Job J, DmNoise nz, uint32_t flags) {
   constexpr int H = 1 << NS;
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -672,7 +678,8 @@ __global__ __launch_bounds__(256) void k_dm_forward(Job J, DmNoise nz, uint32_t 
     }
     float th[NS];
     load_block_angles<NS>(steps, blk * NS, n, ang, J.angle_stride_a, th);
-    dm_block<NS>(r, steps, blk * NS, n, th, nz.gamma, std::make_integer_sequence<int, NS>{});
+    dm_block<NS, PERIODIC>(r, steps, blk * NS, n, th, nz.gamma,
+                           std::make_integer_sequence<int, NS>{});
   }
   if (nz.out_noise)
     dm_output_noise<NS>(r, uint32_t(J.live_mask), nz, std::make_integer_sequence<int, NS>{});
@@ -1007,30 +1014,35 @@ __global__ __launch_bounds__(T) void k_sv_forward_lds(Job J, uint32_t flags) {
 
 // Sub-step SUB of the block starting at step k0 (measured slot NS - 1 - SUB);
 // c0 = k0's position in the staged coefficient chunk.
-template <int NS, int T, int SUB>
+template <int NS, int T, bool PERIODIC, int SUB>
 __device__ __forceinline__ void dm_lds_sub(float2* r, const mbqc_step* steps, const float2* gb_s,
                                            int k0, int c0, int n, int tid) {
   const int k = k0 + SUB;
   if (k < n) {
-    const mbqc_step st = steps[k];
     const float2 gb = gb_s[c0 + SUB];
-    __syncthreads();
-    dm_lds_step<T>(r, 1 << NS, NS - 1 - SUB, st.flags, st.nmask, gb, tid);
+    if constexpr (PERIODIC) {
+      __syncthreads();
+      dm_lds_step<T>(r, 1 << NS, NS - 1 - SUB, 3u, periodic_nmask(NS, SUB), gb, tid);
+    } else {
+      const mbqc_step st = steps[k];
+      __syncthreads();
+      dm_lds_step<T>(r, 1 << NS, NS - 1 - SUB, st.flags, st.nmask, gb, tid);
+    }
   }
 }
 
-template <int NS, int T, int... SUBS>
+template <int NS, int T, bool PERIODIC, int... SUBS>
 __device__ __forceinline__ void dm_lds_block(float2* r, const mbqc_step* steps, const float2* gb_s,
                                              int k0, int c0, int n, int tid,
                                              std::integer_sequence<int, SUBS...>) {
-  (dm_lds_sub<NS, T, SUBS>(r, steps, gb_s, k0, c0, n, tid), ...);
+  (dm_lds_sub<NS, T, PERIODIC, SUBS>(r, steps, gb_s, k0, c0, n, tid), ...);
 }
 
 // Density matrix in LDS: T threads per simulation (one quartet each per step)
 // and G simulations per workgroup, so that small windows (T < 64) still fill
 // whole waves.  Steps run in blocks of NS with the measured slot fixed per
 // sub-step; coefficients are staged in chunks of a multiple of NS steps.
-template <int NS, int T, int G>
+template <int NS, int T, int G, bool PERIODIC>
 __global__ __launch_bounds__(T * G) void k_dm_forward_lds(Job J, DmNoise nz, uint32_t flags) {
   constexpr int H = 1 << NS;
   constexpr int HQ = H / 2;
@@ -1070,7 +1082,8 @@ __global__ __launch_bounds__(T * G) void k_dm_forward_lds(Job J, DmNoise nz, uin
       stage_coeffs<T>(gb_s, steps, ang, J.angle_stride_a, k0, n, nz.gamma, tid);
       __syncthreads();
     }
-    dm_lds_block<NS, T>(r, steps, gb_s, k0, c0, n, tid, std::make_integer_sequence<int, NS>{});
+    dm_lds_block<NS, T, PERIODIC>(r, steps, gb_s, k0, c0, n, tid,
+                                  std::make_integer_sequence<int, NS>{});
   }
   // noise of every output qubit
   for (int m = 0; m < NS; ++m) {
@@ -2362,10 +2375,14 @@ DmNoise compose_noise(const mbqc_noise& n) {
 }
 
 template <int NS>
-hipError_t launch_dm(const Job& J, const DmNoise& nz, uint32_t flags, hipStream_t st) {
+hipError_t launch_dm(const Job& J, const DmNoise& nz, uint32_t flags, bool periodic,
+                     hipStream_t st) {
   const int threads = 256;
   const int64_t blocks = (J.total + threads - 1) / threads;
-  hipLaunchKernelGGL((k_dm_forward<NS>), dim3(blocks), dim3(threads), 0, st, J, nz, flags);
+  if (periodic)
+    hipLaunchKernelGGL((k_dm_forward<NS, true>), dim3(blocks), dim3(threads), 0, st, J, nz, flags);
+  else
+    hipLaunchKernelGGL((k_dm_forward<NS, false>), dim3(blocks), dim3(threads), 0, st, J, nz, flags);
   return hipGetLastError();
 }
 
@@ -2552,12 +2569,18 @@ hipError_t launch_sv_lds(const Job& J, uint32_t flags, hipStream_t st) {
 }
 
 template <int NS>
-hipError_t launch_dm_lds(const Job& J, const DmNoise& nz, uint32_t flags, hipStream_t st) {
+hipError_t launch_dm_lds(const Job& J, const DmNoise& nz, uint32_t flags, bool periodic,
+                         hipStream_t st) {
   constexpr int Q = 1 << (2 * (NS - 1));          // quartets per step
   constexpr int T = Q < 256 ? Q : 256;            // threads per simulation
   constexpr int G = T < 64 ? 64 / T : 1;          // simulations per workgroup
   const int64_t blocks = (J.total + G - 1) / G;
-  hipLaunchKernelGGL((k_dm_forward_lds<NS, T, G>), dim3(blocks), dim3(T * G), 0, st, J, nz, flags);
+  if (periodic)
+    hipLaunchKernelGGL((k_dm_forward_lds<NS, T, G, true>), dim3(blocks), dim3(T * G), 0, st, J, nz,
+                       flags);
+  else
+    hipLaunchKernelGGL((k_dm_forward_lds<NS, T, G, false>), dim3(blocks), dim3(T * G), 0, st, J, nz,
+                       flags);
   return hipGetLastError();
 }
 
@@ -2765,13 +2788,14 @@ int mbqc_dm_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
   // thread, several small simulations per 64-lane workgroup) spreads the work
   // over the whole chip.
   const bool small = J.total < kDmLaneKernelMinSims;
+  const bool pe = p->periodic && !(flags & MBQC_FLAG_TABLE_STEPS);
   switch (p->ns) {
-    case 1: e = small ? launch_dm_lds<1>(J, nz, flags, st) : launch_dm<1>(J, nz, flags, st); break;
-    case 2: e = small ? launch_dm_lds<2>(J, nz, flags, st) : launch_dm<2>(J, nz, flags, st); break;
-    case 3: e = small ? launch_dm_lds<3>(J, nz, flags, st) : launch_dm<3>(J, nz, flags, st); break;
-    case 4: e = launch_dm_lds<4>(J, nz, flags, st); break;
-    case 5: e = launch_dm_lds<5>(J, nz, flags, st); break;
-    case 6: e = launch_dm_lds<6>(J, nz, flags, st); break;
+    case 1: e = small ? launch_dm_lds<1>(J, nz, flags, pe, st) : launch_dm<1>(J, nz, flags, pe, st); break;
+    case 2: e = small ? launch_dm_lds<2>(J, nz, flags, pe, st) : launch_dm<2>(J, nz, flags, pe, st); break;
+    case 3: e = small ? launch_dm_lds<3>(J, nz, flags, pe, st) : launch_dm<3>(J, nz, flags, pe, st); break;
+    case 4: e = launch_dm_lds<4>(J, nz, flags, pe, st); break;
+    case 5: e = launch_dm_lds<5>(J, nz, flags, pe, st); break;
+    case 6: e = launch_dm_lds<6>(J, nz, flags, pe, st); break;
   }
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "dm_forward launch");
 }

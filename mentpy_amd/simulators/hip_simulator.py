@@ -215,9 +215,12 @@ def decode_outcomes(rows, n_steps: int):
 
 
 def dm_forward(plan: nat.NativePlan, angles, states, *, p_depol: float = 0.0,
-               p_readout: float = 0.0, noise=None, renormalize: bool = True, out=None):
+               p_readout: float = 0.0, noise=None, renormalize: bool = True, out=None,
+               table_steps: bool = False):
     """Batched density-matrix forward -> ``[B, S, 2^|O|, 2^|O|]`` complex64.
-    ``noise`` (a ``NoiseModel``) replaces ``p_depol`` / ``p_readout``."""
+    ``noise`` (a ``NoiseModel``) replaces ``p_depol`` / ``p_readout``;
+    ``table_steps`` makes a periodic plan read its step flags from the tables
+    (same results)."""
     torch = _torch()
     device = states.device
     _require_gpu(device)
@@ -230,7 +233,8 @@ def dm_forward(plan: nat.NativePlan, angles, states, *, p_depol: float = 0.0,
         noise = nat.Noise(p_depol=float(p_depol), p_readout=float(p_readout))
     elif not isinstance(noise, nat.Noise):
         noise = noise.native()
-    flags = nat.FLAG_RENORMALIZE if renormalize else 0
+    flags = ((nat.FLAG_RENORMALIZE if renormalize else 0)
+             | (nat.FLAG_TABLE_STEPS if table_steps else 0))
     sb, sa = _angle_strides(angles)
     with torch.cuda.device(device):
         nat.check(nat.lib().mbqc_dm_forward_c64(

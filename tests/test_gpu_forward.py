@@ -281,3 +281,32 @@ def test_periodic_steps_equal_table_steps(rows, cols, ref_phase, rng):
         assert np.abs(o1.cpu().numpy() - ref).max() < AMP_TOL
     else:
         assert phase_aligned_max_err(o1.cpu().numpy(), ref) < AMP_TOL
+
+
+@pytest.mark.parametrize("rows,cols,B,S", [(2, 6, 37, 3), (3, 5, 301, 2), (3, 4, 12288, 4),
+                                          (2, 5, 16384, 3), (4, 4, 50, 2), (1, 7, 20000, 3)])
+def test_dm_periodic_equals_table_steps(rows, cols, B, S, rng):
+    """Both density-matrix kernels (workgroup kernel below 49,152 simulations,
+    one simulation per lane above) take a periodic plan's step flags as
+    constants; the results are bit-identical to the table path and match the
+    oracle on a sample."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from mentpy_amd.simulators.hip_simulator import dm_forward
+    from oracle.vectorized import dm_batch
+    g = mp.templates.grid_cluster(rows, cols) if rows > 1 else mp.templates.linear_cluster(cols)
+    sim = mp.PatternSimulator(g, backend="hip-dm", p_depol=0.01, p_readout=0.005)
+    plan = sim.simulator.plan
+    assert plan.info["periodic"] == 1
+    th = rng.uniform(-7, 7, (B, len(g.trainable_nodes)))
+    st = np.stack([haar_state(len(g.input_nodes), rng) for _ in range(S)])
+    a = torch.tensor(th, dtype=torch.float32, device="cuda")
+    s = torch.tensor(st, dtype=torch.complex64, device="cuda")
+    nz = sim.simulator.noise
+    o1 = dm_forward(plan, a, s, noise=nz)
+    o2 = dm_forward(plan, a, s, noise=nz, table_steps=True)
+    torch.cuda.synchronize()
+    assert torch.equal(o1, o2)
+    ref = dm_batch(g, th[:4].astype(np.float32).astype(np.float64), st, p_depol=0.01,
+                   p_readout=0.005)
+    assert np.abs(o1[:4].cpu().numpy() - ref).max() < 2e-5
