@@ -289,7 +289,7 @@ def config_dm(args, dev, stream):
             "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": FP32_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
                          "traffic": None,
-                         "kernel": "k_dm_forward<3>" if B >= 49152 else "k_dm_forward_lds<3,16,4>",
+                         "kernel": "k_dm_forward<3,true>" if B >= 32768 else "k_dm_forward_lds<3,16,4,true>",
                          "flops_per_sim": f_sim}}
 
 

@@ -2538,8 +2538,9 @@ void launch_batch_group(const BatchGroupLaunch& L, bool ref, bool share, bool pe
 
 constexpr int kMaxSvSlots = 6;       // register-resident SV kernels
 // below this many simulations the DM path keeps each density matrix in LDS
-// (one quartet per thread, several simulations per workgroup)
-constexpr int64_t kDmLaneKernelMinSims = 49152;
+// (one quartet per thread, several simulations per workgroup); the two
+// kernels take the same time at 32,768 simulations of the 3x12 grid
+constexpr int64_t kDmLaneKernelMinSims = 32768;
 constexpr int kMaxSvLdsSlots = 12;   // LDS-resident SV kernel
 constexpr int kMaxDmSlots = 3;       // register-resident DM kernel
 constexpr int kMaxDmLdsSlots = 6;    // LDS-resident DM kernel

@@ -286,7 +286,7 @@ def test_periodic_steps_equal_table_steps(rows, cols, ref_phase, rng):
 @pytest.mark.parametrize("rows,cols,B,S", [(2, 6, 37, 3), (3, 5, 301, 2), (3, 4, 12288, 4),
                                           (2, 5, 16384, 3), (4, 4, 50, 2), (1, 7, 20000, 3)])
 def test_dm_periodic_equals_table_steps(rows, cols, B, S, rng):
-    """Both density-matrix kernels (workgroup kernel below 49,152 simulations,
+    """Both density-matrix kernels (workgroup kernel below 32,768 simulations,
     one simulation per lane above) take a periodic plan's step flags as
     constants; the results are bit-identical to the table path and match the
     oracle on a sample."""

@@ -80,7 +80,7 @@ def test_dm_composed_channels(rows, cols, rng):
 @pytest.mark.parametrize("rows,cols,composed", [(2, 6, False), (3, 6, False), (3, 5, True),
                                                 (1, 7, True)])
 def test_dm_lane_and_workgroup_kernels_agree(rows, cols, composed, rng):
-    """Batches of >= 49152 simulations run one simulation per lane (Hermitian
+    """Batches of >= 32768 simulations run one simulation per lane (Hermitian
     upper-triangle registers); smaller ones keep the density matrix in LDS.
     Both paths on the same inputs agree, and a sample of the large batch
     matches the oracle."""
