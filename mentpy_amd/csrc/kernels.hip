@@ -1235,7 +1235,7 @@ __device__ __forceinline__ void adj_recompute(float2 (&t)[1 << NS], float2 (&ys)
                                               const float* coef, int k0, int n,
                                               const float (&th)[NS]) {
   const int k = k0 + SUB;
-  if (k < n) {
+  if (PERIODIC || k < n) {   // a periodic plan has only full blocks
     bs[SUB] = step_coeff<false>(th[SUB], nullptr);
     take_y<NS, NS - 1 - SUB>(t, ys[SUB]);
     if constexpr (SUB + 1 < NS) {
@@ -1282,7 +1282,7 @@ __device__ __forceinline__ void adj_back(float2 (&lam)[1 << NS], const float2 (&
                                          const float* coef, int k0, int n, const AdjCtx& A,
                                          float* prow, bool writer) {
   const int k = k0 + SUB;
-  if (k < n) {
+  if (PERIODIC || k < n) {
     float d;
     if constexpr (PERIODIC)
       d = sv_adj_step_periodic<NS, NS - 1 - SUB, periodic_nmask(NS, SUB)>(lam, ys[SUB], bs[SUB]);
