@@ -229,10 +229,12 @@ def test_mbqml_trainer_graph_training_reduces_loss(rng):
     assert (h_eager[-1] < h_eager[0]).mean() > 0.9
 
 
-@pytest.mark.parametrize("rows,cols,S,B", [(1, 5, 4, 3), (2, 4, 9, 40), (3, 5, 40, 17), (4, 6, 300, 33)])
+@pytest.mark.parametrize("rows,cols,S,B", [(1, 5, 4, 3), (2, 4, 9, 40), (3, 5, 40, 17), (4, 6, 300, 33),
+                                          (4, 6, 64, 2100)])
 def test_fused_gram_loss_matches_library_gemm(rows, cols, S, B, rng):
     """The fused Gram loss/seed kernel against the torch GEMM formulation of the
-    same step (K = 4, 16, 64, 256; set counts not multiples of the tile)."""
+    same step (K = 4, 16, 64, 256; set counts not multiples of the tile, up to
+    2,100 sets)."""
     torch = _torch()
     import mentpy_amd as mp
     from mentpy_amd.gradients import MBQMLStep
