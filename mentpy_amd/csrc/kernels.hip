@@ -9,7 +9,9 @@
 // by n_slots.  Per-step control (angle index, append/CZ flags, CZ neighbour
 // mask) is wave-uniform and read with scalar loads, so a wave of 64
 // simulations of one pattern runs with no divergence and no cross-lane
-// traffic.  HBM sees only the angles, the input states and the outputs.
+// traffic; for periodic plans (grids in column order, chains) the CZ signs
+// are compile-time constants instead.  HBM sees only the angles, the input
+// states and the outputs.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
