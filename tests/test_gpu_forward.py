@@ -310,3 +310,39 @@ def test_dm_periodic_equals_table_steps(rows, cols, B, S, rng):
     ref = dm_batch(g, th[:4].astype(np.float32).astype(np.float64), st, p_depol=0.01,
                    p_readout=0.005)
     assert np.abs(o1[:4].cpu().numpy() - ref).max() < 2e-5
+
+
+@pytest.mark.parametrize("rows,cols,w", [(2, 33, 4), (3, 23, 5), (3, 24, 6), (4, 18, 6)])
+def test_partial_tail_block_with_rescale(rows, cols, w, rng):
+    """Patterns whose last, partial block of steps also carries a growth
+    rescale (non-periodic: a window above the minimum): the tail runs outside
+    the full-block stretches and must still rescale.  Forward and adjoint
+    against the oracle."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from oracle.vectorized import sv_batch
+    g = mp.templates.grid_cluster(rows, cols)
+    sim = mp.PatternSimulator(g, backend="hip-sv", window_size=w)
+    plan = sim.simulator.plan
+    ns, n = plan.info["n_slots"], plan.info["n_steps"]
+    assert plan.info["periodic"] == 0 and n % ns != 0 and plan.rescale()[0][-1]
+    th = rng.uniform(0, 2 * np.pi, (23, len(g.trainable_nodes)))
+    st = np.stack([haar_state(rows, rng) for _ in range(2)])
+    out = sim.run_batch(th, st).cpu().numpy()
+    ref = sv_batch(g, th.astype(np.float32).astype(np.float64), st)
+    assert np.abs(out - ref).max() < AMP_TOL
+    if ns <= 5:
+        from mentpy_amd.gradients import mbqml_infidelity
+        from oracle.gradients import exact_shift_gradient
+        from mentpy_amd.operators.gates import haar_unitary
+        U = haar_unitary(2 ** rows, rng)
+        X = np.stack([haar_state(rows, rng) for _ in range(4)])
+        Y = X @ U.T
+        th32 = th[:2].astype(np.float32)
+        loss, grad = mbqml_infidelity(sim, th32, X, Y, mode="transfer")
+
+        def cost(t):
+            o = sv_batch(g, np.asarray(t)[None], X)[0]
+            return 1.0 - np.mean(np.abs(np.sum(np.conj(Y) * o, -1)) ** 2)
+        ref_g = exact_shift_gradient(cost, th32[0].astype(np.float64))
+        assert np.abs(grad[0].cpu().numpy() - ref_g).max() < 2e-4
