@@ -6,7 +6,9 @@
  * `stream` is a hipStream_t passed as void*.  Launches are stream-ordered and
  * asynchronous; a plan is immutable after creation, so concurrent launches of
  * one plan on different streams are safe.  Status codes are negative on error
- * and the message is available from mbqc_last_error() (thread-local).
+ * and the message is available from mbqc_last_error() (thread-local).  A
+ * forward, density-matrix or sampled-outcome launch over B = 0 angle vectors
+ * does nothing and returns MBQC_OK; its data pointers may then be null.
  *
  * The reference functions each entry point replaces are cited below as
  * file:line into mentpy 0.1.0a9.

@@ -2268,10 +2268,12 @@ int device_tables(Plan* p, const int32_t** out) {
 
 int make_job(Plan* p, const float* angles, int64_t angle_stride, int64_t angle_stride_a,
              const void* psi, int64_t in_stride, void* out, int64_t B, int64_t S, Job* J) {
-  if (!angles && p->n_angles > 0) return fail(MBQC_ERR_VALUE, "angles pointer is null");
-  if (!psi) return fail(MBQC_ERR_VALUE, "input pointer is null");
-  if (!out) return fail(MBQC_ERR_VALUE, "output pointer is null");
   if (B < 0 || S <= 0) return fail(MBQC_ERR_VALUE, "batch sizes must be B >= 0, S >= 1");
+  if (B > 0) {   // an empty batch launches nothing and may pass null pointers
+    if (!angles && p->n_angles > 0) return fail(MBQC_ERR_VALUE, "angles pointer is null");
+    if (!psi) return fail(MBQC_ERR_VALUE, "input pointer is null");
+    if (!out) return fail(MBQC_ERR_VALUE, "output pointer is null");
+  }
   if (angle_stride < 0 || angle_stride_a < 0 || in_stride < 0)
     return fail(MBQC_ERR_VALUE, "negative stride");
   if (p->n_angles > 1 && angle_stride_a == 0)
@@ -2652,7 +2654,8 @@ int mbqc_sv_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
   Job J;
   int rc = make_job(p, angles, angle_stride, angle_stride_a, psi_in, in_stride, out, B, S, &J);
   if (rc != MBQC_OK) return rc;
-  if ((flags & MBQC_FLAG_WRITE_PHASE) && !phase_out) return fail(MBQC_ERR_VALUE, "phase_out is null");
+  if ((flags & MBQC_FLAG_WRITE_PHASE) && !phase_out && B > 0)
+    return fail(MBQC_ERR_VALUE, "phase_out is null");
   J.phase_out = static_cast<float2*>(phase_out);
   if (J.total == 0) return MBQC_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -2682,7 +2685,7 @@ int mbqc_sv_sample_c64(const mbqc_plan* plan, const float* angles, int64_t angle
   Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(plan));
   if (!p->can_sample)
     return fail(MBQC_ERR_NOT_IMPLEMENTED, "sampled outcomes unavailable: " + p->sample_reason);
-  if (!outcomes) return fail(MBQC_ERR_VALUE, "outcomes pointer is null");
+  if (!outcomes && B > 0) return fail(MBQC_ERR_VALUE, "outcomes pointer is null");
   Job J;
   int rc = make_job(p, angles, angle_stride, angle_stride_a, psi_in, in_stride, out, B, S, &J);
   if (rc != MBQC_OK) return rc;

@@ -346,3 +346,24 @@ def test_partial_tail_block_with_rescale(rows, cols, w, rng):
             return 1.0 - np.mean(np.abs(np.sum(np.conj(Y) * o, -1)) ** 2)
         ref_g = exact_shift_gradient(cost, th32[0].astype(np.float64))
         assert np.abs(grad[0].cpu().numpy() - ref_g).max() < 2e-4
+
+
+def test_empty_batches_return_empty_outputs():
+    """B = 0 angle vectors: every forward entry point launches nothing and
+    returns empty outputs (torch passes null data pointers for them)."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from mentpy_amd.simulators.hip_simulator import dm_forward, sv_forward, sv_sample
+    g = mp.templates.grid_cluster(2, 4)
+    sv = mp.PatternSimulator(g, backend="hip-sv")
+    dm = mp.PatternSimulator(g, backend="hip-dm", p_depol=0.01)
+    a = torch.empty((0, len(g.trainable_nodes)), dtype=torch.float32, device="cuda")
+    s = torch.ones((3, 4), dtype=torch.complex64, device="cuda") / 2
+    out, ph = sv_forward(sv.simulator.plan, a, s, want_phase=True)
+    assert out.shape == (0, 3, 4) and ph.shape == (0, 3)
+    assert dm_forward(dm.simulator.plan, a, s).shape == (0, 3, 4, 4)
+    smp = mp.PatternSimulator(g, backend="hip-sv", force0=False, seed=1)
+    o, rows = sv_sample(smp.simulator.plan, a, s, seed=3)
+    assert o.shape == (0, 3, 4) and rows.shape[:2] == (0, 3)
+    assert sv.run_batch(np.zeros((0, len(g.trainable_nodes))), s.cpu().numpy()).shape == (0, 3, 4)
+    torch.cuda.synchronize()
