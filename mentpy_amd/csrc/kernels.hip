@@ -273,11 +273,15 @@ __device__ __forceinline__ void sv_block(float2 (&t)[1 << NS], const mbqc_step* 
 template <int NS>
 __device__ __forceinline__ void fetch_block_angles(const mbqc_step* steps, int k0, int n_steps,
                                                    const float* ang, int64_t sa, float (&th)[NS]) {
+  // angles live in device memory: a global-typed pointer keeps the loads
+  // global (a descriptor read from memory otherwise leaves them flat)
+  const __attribute__((address_space(1))) float* gang =
+      (const __attribute__((address_space(1))) float*)ang;
 #pragma unroll
   for (int sub = 0; sub < NS; ++sub) {
     const int k = k0 + sub < n_steps ? k0 + sub : n_steps - 1;
     const int ai = steps[k].angle_index;
-    const float a = ang[int64_t(ai >= 0 ? ai : 0) * sa];
+    const float a = gang[int64_t(ai >= 0 ? ai : 0) * sa];
     th[sub] = ai >= 0 ? a : steps[k].fixed_angle;
   }
 }
