@@ -730,8 +730,12 @@ __device__ __forceinline__ uint32_t draw32(uint64_t seed, uint64_t sim, uint64_t
 // (-1)^x theta, the drawn projection m' (probability 1/2 each) is onto
 // |+/- theta'>, and the physical outcome is m' xor z.  An outcome 1 flips the
 // frames of the flow successor and of its later neighbours (xmask/zmask).
+// Frames are 32-bit: a plan that samples applies every CZ edge within its
+// window of at most 7 positions, so every offset is below 14 (the host
+// checks < 32).
 struct SampleMod {
-  uint64_t fx = 0, fz = 0, seed = 0, sim = 0;
+  uint32_t fx = 0, fz = 0;
+  uint64_t seed = 0, sim = 0;
   const uint64_t* xm;
   const uint64_t* zm;
   int32_t* out_words;   // this simulation's outcome words (null for inactive lanes)
@@ -741,16 +745,16 @@ struct SampleMod {
     const int kb = k & 31;
     if (kb == 0) rbits = draw32(seed, sim, uint64_t(k >> 5));
     const uint32_t mprime = (rbits >> kb) & 1u;
-    const uint32_t xk = uint32_t(fx & 1u), zk = uint32_t(fz & 1u);
+    const uint32_t xk = fx & 1u, zk = fz & 1u;
     const uint32_t mk = mprime ^ zk;
     word |= mk << kb;
     if (kb == 31 || k == n - 1) {
       if (out_words) out_words[k >> 5] = int32_t(word);
       word = 0;
     }
-    const uint64_t sel = uint64_t(0) - uint64_t(mk);
-    fx = (fx ^ (xm[k] & sel)) >> 1;
-    fz = (fz ^ (zm[k] & sel)) >> 1;
+    const uint32_t sel = 0u - mk;
+    fx = (fx ^ (uint32_t(xm[k]) & sel)) >> 1;
+    fz = (fz ^ (uint32_t(zm[k]) & sel)) >> 1;
     float2 bb = xk ? cconj(b) : b;
     return mprime ? make_float2(-bb.x, -bb.y) : bb;
   }
@@ -784,8 +788,8 @@ __global__ __launch_bounds__(256) void k_sv_sample(Job J, const uint64_t* __rest
   uint32_t xo = 0, zo = 0;
   for (int m = 0; m < J.n_out; ++m) {
     const int off = out_off[m];
-    xo |= uint32_t((mod.fx >> off) & 1u) << (J.n_out - 1 - m);
-    zo |= uint32_t((mod.fz >> off) & 1u) << (J.n_out - 1 - m);
+    xo |= ((mod.fx >> off) & 1u) << (J.n_out - 1 - m);
+    zo |= ((mod.fz >> off) & 1u) << (J.n_out - 1 - m);
   }
   // all output positions first: the outcome stores above keep the compiler
   // from moving these table loads across the output stores below

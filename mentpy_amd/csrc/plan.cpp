@@ -268,7 +268,8 @@ int compile_plan(const mbqc_pattern_desc* d, Plan* p) {
   // pennylane_simulator.py:146-153): outcome 1 on schedule[k] flips the X
   // frame of its flow successor and the Z frame of the successor's neighbours
   // measured after schedule[k].  Frames are kept relative to the current
-  // step in 64-bit words, so every offset must be below 64.
+  // step in 32-bit words on the device, so every offset must be below 32
+  // (with no dropped edge and at most 6 live slots they stay below 14).
   p->can_sample = false;
   if (!d->flow) {
     p->sample_reason = "the pattern description has no flow";
@@ -289,12 +290,12 @@ int compile_plan(const mbqc_pattern_desc* d, Plan* p) {
         break;
       }
       const int dx = pos[f] - k;
-      if (dx >= 64) { ok = false; break; }
+      if (dx >= 32) { ok = false; break; }
       p->xmask[k] |= uint64_t(1) << dx;
       for (int nb : nbr[f]) {
         if (nb == v || pos[nb] <= k) continue;
         const int dz = pos[nb] - k;
-        if (dz >= 64) { ok = false; break; }
+        if (dz >= 32) { ok = false; break; }
         p->zmask[k] |= uint64_t(1) << dz;
       }
     }
@@ -302,14 +303,14 @@ int compile_plan(const mbqc_pattern_desc* d, Plan* p) {
       p->out_offset.assign(nout, 0);
       for (int m = 0; m < nout; ++m) {
         const int off = pos[d->output_nodes[m]] - nm;
-        if (off < 0 || off >= 64) { ok = false; break; }
+        if (off < 0 || off >= 32) { ok = false; break; }
         p->out_offset[m] = off;
       }
     }
     if (ok) {
       p->can_sample = true;
     } else {
-      if (p->sample_reason.empty()) p->sample_reason = "flow dependencies span more than 64 positions";
+      if (p->sample_reason.empty()) p->sample_reason = "flow dependencies span more than 32 positions";
       p->xmask.clear();
       p->zmask.clear();
       p->out_offset.clear();
