@@ -28,10 +28,25 @@ sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3     # MI355X published FP32 (vector and matrix) peak
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per simulation of the headline forward kernel, from rocprofv3
-# FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) per dispatch, the scales read off a
-# 1 GiB device copy (profiles/sv_forward_grid4x20.md)
-TRAFFIC_BYTES_PER_SIM_4X20 = 432.1
+# The HBM traffic of the headline kernel is measured with rocprofv3 counters
+# (FETCH_SIZE, WRITE_SIZE per dispatch, scaled by a device copy of known size)
+# and recorded in this profile summary, which names the kernel it measured.
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "sv_forward_grid4x20.md")
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per simulation of ``kernel`` from the profile summary's
+    ``traffic_bytes_per_sim <kernel> <bytes>`` line (None if the summary is
+    absent or was measured on another kernel)."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                parts = line.strip().strip("`").split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_sim" and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
 
 
 def flops_per_sim(n_steps, window):
@@ -52,6 +67,23 @@ def time_launches(fn, steps, warmup, stream):
     end.record(stream)
     torch.cuda.synchronize()
     return start.elapsed_time(end) / 1e3
+
+
+def time_each_launch(fn, steps, warmup, stream):
+    """Mean duration of a single launch: an event pair around each of
+    ``steps`` launches on ``stream`` (the stream the kernel runs on)."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    for a, b in ev:
+        a.record(stream)
+        fn()
+        b.record(stream)
+    torch.cuda.synchronize()
+    return float(np.mean([a.elapsed_time(b) for a, b in ev])) / 1e3
 
 
 def cpu_baseline(circ, psi, seconds=12.0, seed=7):
@@ -92,7 +124,12 @@ def _pool_worker(args):
     return count, time.perf_counter() - t0
 
 
-def cpu_pool_baseline(seconds=8.0, max_workers=16):
+# CPU share of one GPU on the benchmark hosts: worker pools are capped at 16
+# processes whatever the visible core count is
+POOL_CAP = 16
+
+
+def cpu_pool_baseline(seconds=8.0, max_workers=POOL_CAP):
     """The restatement mapped over a process pool of the host's cores, one
     numpy thread each (as mentpy's parallel tutorial maps simulations over a
     pool, intro-to-mbqml-parallel.rst:43-47).  Workers are spawned fresh, so no
@@ -153,8 +190,10 @@ def adjoint_section(circ, sim, dev, steps, stream, sets=(1024, 16384), S=256, se
             secs = time_launches(graph.replay, steps, 2, stream)
             row[mode] = B * steps / secs
             del graph, step
-        secs = time_launches(lambda: fd_loop_infidelity_grad(sim, th, Xd, Yd, h=1e-3), 1, 1, stream)
-        row["fd"] = B / secs
+        for mode in ("direct", "gram"):
+            secs = time_launches(lambda: fd_loop_infidelity_grad(sim, th, Xd, Yd, h=1e-3, mode=mode),
+                                 1, 1, stream)
+            row["fd_" + mode] = B / secs
         res[B] = row
     from oracle.gradients import exact_shift_gradient
     from oracle.vectorized import sv_batch
@@ -170,16 +209,24 @@ def adjoint_section(circ, sim, dev, steps, stream, sets=(1024, 16384), S=256, se
     ref = exact_shift_gradient(cost, t0)
     err = float(np.abs(g[0].double().cpu().numpy() - ref).max())
     big, small = sets[-1], sets[0]
-    out = {"adjoint_grads_per_s": res[big]["gram"], "fd_loop_grads_per_s": res[big]["fd"],
-           "adjoint_vs_fd_loop": res[big]["gram"] / res[big]["fd"],
+    out = {"adjoint_grads_per_s": res[big]["gram"], "fd_loop_grads_per_s": res[big]["fd_direct"],
+           "adjoint_vs_fd_loop": res[big]["gram"] / res[big]["fd_direct"],
            "adjoint_direct_grads_per_s": res[big]["direct"],
-           "adjoint_direct_vs_fd_loop": res[big]["direct"] / res[big]["fd"],
-           "adjoint_grads_per_s_1024": res[small]["gram"], "fd_loop_grads_per_s_1024": res[small]["fd"],
-           "adjoint_vs_fd_loop_1024": res[small]["gram"] / res[small]["fd"],
+           "fd_gram_grads_per_s": res[big]["fd_gram"],
+           "adjoint_gram_vs_fd_gram": res[big]["gram"] / res[big]["fd_gram"],
+           "adjoint_direct_vs_fd_direct": res[big]["direct"] / res[big]["fd_direct"],
+           "adjoint_grads_per_s_1024": res[small]["gram"],
+           "fd_loop_grads_per_s_1024": res[small]["fd_direct"],
+           "adjoint_vs_fd_loop_1024": res[small]["gram"] / res[small]["fd_direct"],
+           "adjoint_gram_vs_fd_gram_1024": res[small]["gram"] / res[small]["fd_gram"],
+           "adjoint_direct_vs_fd_direct_1024": res[small]["direct"] / res[small]["fd_direct"],
            "grad_max_abs_err_vs_c128": err,
            "grad_workload": f"4x20 grid ({n} angles), loss 1-mean_s|<U psi_s|out_s>|^2 over {S} Haar "
                             f"inputs; adjoint = one graph-replayed step (transfer matrix + data Gram "
-                            f"matrix GEMM); {big} parameter sets (and {small}); a grad is the full "
+                            f"matrix GEMM, or the inputs propagated: direct); FD loops = 2 forward "
+                            f"launches + 1 fused loss kernel per angle, propagating the inputs "
+                            f"(fd_loop, the plain loop; direct) or the basis columns with the Gram "
+                            f"loss (fd_gram); {big} parameter sets (and {small}); a grad is the full "
                             f"{n}-angle gradient of one parameter set"}
     return out, (X, Y, t0, cost)
 
@@ -441,6 +488,13 @@ def main():
     # the kernels see it as the [B, n_angles] view angles_t.t()
     angles_t = torch.rand((n_ang, B), generator=torch.Generator().manual_seed(11),
                           dtype=torch.float32).mul_(2 * np.pi).to(dev)
+    # the error sample always holds simulations at float32(pi), where the
+    # reference phase sgn(cos(theta/2)) e^{i theta/2} is -i (float32(pi) > pi):
+    # one with every angle at float32(pi), one with a few such angles
+    idx = np.random.default_rng(5).choice(B, 512, replace=False)
+    pi32 = float(np.float32(np.pi))
+    angles_t[:, int(idx[0])] = pi32
+    angles_t[::7, int(idx[1])] = pi32
     angles = angles_t.t()
     states = torch.as_tensor(psi, dtype=torch.complex64, device=dev).reshape(1, -1)
     out = torch.empty((B, 1, 16), dtype=torch.complex64, device=dev)
@@ -451,10 +505,24 @@ def main():
     secs = time_launches(fwd, args.steps, args.warmup, stream)
     ms = secs / args.steps * 1e3
     sims_per_s = B * args.steps / secs
+    kernel_s = time_each_launch(fwd, args.steps, 1, stream)
+
+    # single-simulation latency: one angle vector per launch (inputs in HBM),
+    # and the host call run(angles) with its copies to and from the device
+    one = angles[:1]
+    out1 = torch.empty((1, 1, 16), dtype=torch.complex64, device=dev)
+    lat_s = time_each_launch(lambda: sv_forward(plan, one, states, out=out1), 50, 5, stream)
+    sim.reset(input_state=psi)
+    th_host = one[0].double().cpu().numpy()
+    sim.run(th_host, output_form="sv")
+    t0 = time.perf_counter()
+    for _ in range(20):
+        sim.reset(input_state=psi)
+        sim.run(th_host, output_form="sv")
+    host_call_s = (time.perf_counter() - t0) / 20
 
     # c64 vs c128 error on a sample
     from oracle.vectorized import sv_batch
-    idx = np.random.default_rng(5).choice(B, 512, replace=False)
     a_s = angles[idx].double().cpu().numpy()
     ref = sv_batch(circ, a_s, psi[None])[:, 0]
     got = out[idx, 0].cpu().numpy()
@@ -463,12 +531,16 @@ def main():
     fid_err = float(np.max(np.abs(1 - fid)))
 
     f_sim = flops_per_sim(plan.info["n_steps"], plan.info["window"])
-    achieved = f_sim * B / (ms / 1e3) / 1e12
+    achieved = f_sim * B / kernel_s / 1e12
+    kernel = ("k_sv_forward_periodic<4,true>" if plan.info["periodic"]
+              else "k_sv_forward<4,true,true>")
+    tps = profiled_traffic(kernel)
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                "traffic": int(TRAFFIC_BYTES_PER_SIM_4X20 * B),
-                "kernel": ("k_sv_forward_periodic<4,true>" if plan.info["periodic"]
-                           else "k_sv_forward<4,true,true>"), "flops_per_sim": f_sim}
+                "traffic": None if tps is None else int(tps * B),
+                "kernel": kernel, "flops_per_sim": f_sim, "kernel_ms": kernel_s * 1e3,
+                "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE per dispatch, "
+                                  "profiles/sv_forward_grid4x20.md"}
 
     line = {
         "metric": "pattern sims/s + adjoint angle-grads/s, 4x20 grid, c64 vs c128 error",
@@ -480,6 +552,10 @@ def main():
                    "angle_layout": "angle-major [n_angles, B]"},
         "max_amp_err_vs_c128": amp_err, "max_infidelity_vs_c128": fid_err,
         "roofline": roofline,
+        "single_sim_latency_us": lat_s * 1e6,
+        "single_sim_host_call_us": host_call_s * 1e6,
+        "latency_note": "one angle vector per launch with the input in HBM; host call = "
+                        "reset + run(angles) with host arrays (copies in and out, sync)",
     }
     X = None
     if not args.no_grad:
@@ -496,8 +572,10 @@ def main():
             prate, workers, pcount = cpu_pool_baseline(args.cpu_pool_seconds)
             line["cpu_baseline_pool"] = {
                 "value": prate, "unit": "sims/s", "cores": workers, "kind": "port",
+                "host_cores_visible": len(os.sched_getaffinity(0)), "pool_cap": POOL_CAP,
                 "sample": f"{pcount} sims of grid_cluster(4,20), {workers} spawned processes x "
-                          f"{args.cpu_pool_seconds:.0f} s, 1 numpy thread each"}
+                          f"{args.cpu_pool_seconds:.0f} s, 1 numpy thread each (pool capped at "
+                          f"{POOL_CAP}, the host's CPU share per GPU)"}
         if X is not None:
             grate, gel = cpu_gradient_baseline(circ, X[:args.cpu_grad_inputs], Y[:args.cpu_grad_inputs], t0)
             grate = grate * args.cpu_grad_inputs / len(X)

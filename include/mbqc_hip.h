@@ -39,6 +39,20 @@ extern "C" {
 #define MBQC_FLAG_TABLE_STEPS 8u  /* SV / DM forward and batched launch: read the step
                                      coefficients and CZ masks from the plan's tables
                                      also for a periodic plan (same results) */
+#define MBQC_FLAG_STEP_PROBS 16u  /* DM forward / window: evaluate every step's outcome
+                                     probability (outcome 1 where outcome 0 has
+                                     probability below 1e-4, division by the outcome
+                                     probability per step), as np_simulator_dm.py:329-346
+                                     does; always on for plans with XZ / YZ / Z nodes */
+
+/* Measurement planes (mbqc_pattern_desc.plane; mentpy/operators/ment.py:218-253).
+ * Planes X and Y are XY at fixed angles 0 and pi/2.  MBQC_PLANE_Z is traced
+ * out without projection: the reference's mode="expectation" branch
+ * (np_simulator_dm.py:318-322); its mode="sample" draw is not provided. */
+#define MBQC_PLANE_XY 0
+#define MBQC_PLANE_XZ 1
+#define MBQC_PLANE_YZ 2
+#define MBQC_PLANE_Z 3
 
 /* Host description of a pattern; all arrays are host memory, copied by
  * mbqc_plan_create.  Node ids are 0..n_nodes-1. */
@@ -58,6 +72,9 @@ typedef struct mbqc_pattern_desc {
   int32_t n_angles;            /* length of one angle vector */
   const int32_t* flow;         /* optional [n_nodes]: causal-flow successor, -1 for outputs;
                                   needed only for sampled outcomes */
+  const int32_t* plane;        /* optional [n_nodes]: MBQC_PLANE_* of each measured node
+                                  (NULL: every node XY); a plan with another plane runs
+                                  only in the density-matrix entry points */
 } mbqc_pattern_desc;
 
 typedef struct mbqc_plan mbqc_plan;
@@ -75,13 +92,18 @@ typedef struct mbqc_plan_info {
                              qubit and the CZ pattern repeats every n_slots steps
                              (grids in column order, chains): the SV forward
                              takes its pair coefficients at compile time */
+  int32_t general_planes; /* 1: some node is measured in XZ / YZ / Z (density-matrix
+                             entry points only) */
+  int32_t shared_angles;  /* 1: some angle-vector entry drives several nodes (the
+                             adjoint entry points refuse the plan) */
 } mbqc_plan_info;
 
 /* Per-step table entry (16 bytes) as stored on the device. */
 typedef struct mbqc_step {
   int32_t angle_index;    /* >= 0: angle vector entry; < 0: use fixed_angle */
   uint32_t flags;         /* bit0 append, bit1 measured qubit is a CZ neighbour of
-                             the appended one, bits 8..15 measured slot, bits
+                             the appended one, bits 2..3 measurement plane
+                             (MBQC_PLANE_*), bits 8..15 measured slot, bits
                              16..31 row of the step's pair coefficients */
   uint32_t nmask;         /* slots that are CZ neighbours of the appended qubit */
   float fixed_angle;
@@ -123,6 +145,13 @@ int mbqc_causal_flow(int32_t n_nodes, int32_t n_edges, const int32_t* edges, int
  * n_measure = n_nodes - n_outputs, a window <= 1 becomes n_inputs + 1 and a
  * NULL flow is the causal flow (a pattern without one is MBQC_ERR_VALUE). */
 int mbqc_plan_create(const mbqc_pattern_desc* desc, mbqc_plan** out);
+/* mbqc_plan_create also copies the plan's tables to the current device when
+ * a GPU is present, so the plan's launches on that device allocate nothing
+ * (and the first one can be captured in a hipGraph).  mbqc_plan_upload does
+ * the same for the current device when it was not done yet (another device);
+ * it allocates device memory and copies synchronously, so call it before any
+ * capture. */
+int mbqc_plan_upload(const mbqc_plan* plan);
 void mbqc_plan_destroy(mbqc_plan* plan);
 int mbqc_plan_get_info(const mbqc_plan* plan, mbqc_plan_info* info);
 /* Copy host tables out (tests): steps[n_steps], in_index[2^n_slots],
@@ -174,14 +203,17 @@ int mbqc_sv_sample_c64(const mbqc_plan* plan, const float* angles, int64_t angle
  * step_angles[b*step_stride + j], so any angle can be given per step; inputs as
  * in the forward calls.  The state is normalised (SV: MBQC_FLAG_REF_PHASE
  * selects the reference's global phase).  out: [B*S][2^W] (SV) or
- * [B*S][2^W][2^W] (DM; noise of the measured qubits as in the full run). */
+ * [B*S][2^W][2^W] (DM; noise of the measured qubits as in the full run).
+ * DM outcomes (optional, may be NULL): [B*S][k] float32, the outcomes of the
+ * k measurements as mbqc_dm_forward_outcomes_c64 reports them; given, or for
+ * a plan with XZ / YZ / Z nodes, every step's probability is evaluated. */
 int32_t mbqc_plan_window_qubits(const mbqc_plan* plan, int32_t k);
 int mbqc_sv_window_c64(const mbqc_plan* plan, const float* step_angles, int64_t step_stride,
                        const void* psi_in, int64_t in_stride, int32_t k, void* out, int64_t B,
                        int64_t S, uint32_t flags, void* stream);
 int mbqc_dm_window_c64(const mbqc_plan* plan, const float* step_angles, int64_t step_stride,
                        const void* psi_in, int64_t in_stride, int32_t k, void* out, int64_t B,
-                       int64_t S, const mbqc_noise* noise, void* stream);
+                       int64_t S, const mbqc_noise* noise, float* outcomes, void* stream);
 
 /* Density-matrix forward, complex64: replaces NumpySimulatorDM.run / measure /
  * measure_ment (np_simulator_dm.py:151-283, 307-346) and
@@ -192,6 +224,15 @@ int mbqc_dm_window_c64(const mbqc_plan* plan, const float* step_angles, int64_t 
 int mbqc_dm_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
                         int64_t angle_stride_a, const void* psi_in, int64_t in_stride, void* out,
                         int64_t B, int64_t S, const mbqc_noise* noise, uint32_t flags, void* stream);
+/* The same with every step's outcome probability evaluated (MBQC_FLAG_STEP_PROBS
+ * implied) and the outcomes written, as NumpySimulatorDM.run records them in
+ * its `outcomes` dict (np_simulator_dm.py:243-250, 318-338): outcomes is
+ * [B*S][n_steps] float32, 0 or 1 for a projected node (1 where outcome 0 had
+ * probability below 1e-4) and the expectation p1 for a Z node. */
+int mbqc_dm_forward_outcomes_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
+                                 int64_t angle_stride_a, const void* psi_in, int64_t in_stride,
+                                 void* out, float* outcomes, int64_t B, int64_t S,
+                                 const mbqc_noise* noise, uint32_t flags, void* stream);
 
 /* Adjoint angle gradient (replaces the 2n cost evaluations of fd_gradient /
  * psr_gradient, mentpy/gradients/_finite_difference.py:9-24,
@@ -199,7 +240,11 @@ int mbqc_dm_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
  * Forward pass: like mbqc_sv_forward_c64 in the raw phase frame, unnormalized,
  * also storing one checkpoint of the live state per block of n_slots steps in
  * `ckpt` (mbqc_sv_adjoint_workspace_bytes bytes).  Columns c = 0..C-1 of
- * parameter set b are simulations b*C + c. */
+ * parameter set b are simulations b*C + c.
+ * All adjoint entry points (these and the segmented ones below) need live
+ * windows of at most 5 qubits and every angle-vector entry used by at most
+ * one measured node (angle_index values unique); other plans are
+ * MBQC_ERR_NOT_IMPLEMENTED. */
 size_t mbqc_sv_adjoint_workspace_bytes(const mbqc_plan* plan, int64_t n_columns);
 int mbqc_sv_adjoint_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
                                 int64_t angle_stride_a, const void* psi_in, int64_t in_stride,
@@ -283,7 +328,18 @@ int mbqc_gram_finish_c64(const void* t, const void* z, int64_t B, int32_t K, int
 int mbqc_gram_loss_seed_c64(const void* t, const void* MT, int64_t B, int32_t K, int32_t S,
                             float* loss, void* seed, float* workspace, void* stream);
 
-/* Device-resident optimizer steps on n float32 parameters (all pointers device
+/* MB-QML infidelity loss from propagated inputs (no reference counterpart as
+ * one kernel: the reference evaluates calculator.fidelity per pair,
+ * mentpy/calculator/borrows.py:4-6, inside the tutorial loss,
+ * docs/tutorials/intro-to-mbqml.rst:35-54): out is [B][S][O] (input s of
+ * parameter set b propagated), targets [S][O], O a power of two;
+ * loss[b] = 1 - (1/S) sum_s |<targets_s|out_bs>|^2 and, if seed is not
+ * NULL, seed[b][s][o] = -2/S <targets_s|out_bs> targets[s][o] (the adjoint
+ * seed dL/d conj(out)). */
+int mbqc_overlap_loss_seed_c64(const void* out, const void* targets, int64_t B, int32_t S,
+                               int32_t O, float* loss, void* seed, void* stream);
+
+/* Device-resident optimizer steps on n float32 parameters/* Device-resident optimizer steps on n float32 parameters (all pointers device
  * memory, updated in place), replacing the host updates of
  * AdamOptimizer.step (mentpy/optimizers/adam.py:54-66) and SGDOptimizer.step
  * (mentpy/optimizers/sgd.py:49-59).  `step` is a device int32 holding the

@@ -14,7 +14,10 @@ from typing import Optional
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "lib" / "libmbqc_hip.so"
+# MENTPY_AMD_LIB selects another build of the same library (the host-sanitized
+# one of ``mentpy_amd/csrc/build.py --sanitize``).
+LIB_PATH = Path(os.environ.get("MENTPY_AMD_LIB")
+                or Path(__file__).resolve().parent / "lib" / "libmbqc_hip.so")
 
 MBQC_OK = 0
 MBQC_ERR_VALUE = -1
@@ -26,12 +29,20 @@ FLAG_REF_PHASE = 1
 FLAG_RENORMALIZE = 2
 FLAG_WRITE_PHASE = 4
 FLAG_TABLE_STEPS = 8
+FLAG_STEP_PROBS = 16
+
+PLANE_XY = 0
+PLANE_XZ = 1
+PLANE_YZ = 2
+PLANE_Z = 3
 
 EXPORTED_SYMBOLS = (
-    "mbqc_causal_flow", "mbqc_plan_create", "mbqc_plan_destroy", "mbqc_plan_get_info", "mbqc_plan_get_tables",
+    "mbqc_causal_flow", "mbqc_plan_create", "mbqc_plan_upload", "mbqc_plan_destroy", "mbqc_plan_get_info", "mbqc_plan_get_tables",
     "mbqc_plan_get_rescale", "mbqc_sv_forward_c64", "mbqc_sv_sample_c64", "mbqc_dm_forward_c64",
+    "mbqc_dm_forward_outcomes_c64",
     "mbqc_plan_window_qubits", "mbqc_sv_window_c64", "mbqc_dm_window_c64",
     "mbqc_gram_workspace_bytes", "mbqc_gram_loss_seed_c64", "mbqc_gram_finish_c64",
+    "mbqc_overlap_loss_seed_c64",
     "mbqc_sv_adjoint_workspace_bytes", "mbqc_sv_adjoint_forward_c64",
     "mbqc_sv_adjoint_partials", "mbqc_sv_adjoint_backward_c64", "mbqc_sv_segments_auto",
     "mbqc_sv_segmented_workspace_bytes", "mbqc_sv_segmented_forward_c64",
@@ -49,13 +60,15 @@ class PatternDesc(C.Structure):
         ("n_outputs", C.c_int32), ("output_nodes", C.POINTER(C.c_int32)),
         ("angle_index", C.POINTER(C.c_int32)), ("fixed_angle", C.POINTER(C.c_float)),
         ("n_angles", C.c_int32), ("flow", C.POINTER(C.c_int32)),
+        ("plane", C.POINTER(C.c_int32)),
     ]
 
 
 class PlanInfo(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "n_slots", "lag", "n_steps", "n_inputs", "n_outputs", "n_angles", "n_nodes", "window",
-        "n_appends", "n_dropped_edges", "min_window", "can_sample", "periodic")]
+        "n_appends", "n_dropped_edges", "min_window", "can_sample", "periodic", "general_planes",
+        "shared_angles")]
 
 
 class Step(C.Structure):
@@ -97,6 +110,8 @@ def lib() -> C.CDLL:
     vp, i32, i64, u32, sz = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_size_t
     L.mbqc_plan_create.argtypes = [C.POINTER(PatternDesc), C.POINTER(vp)]
     L.mbqc_plan_create.restype = C.c_int
+    L.mbqc_plan_upload.argtypes = [vp]
+    L.mbqc_plan_upload.restype = C.c_int
     L.mbqc_plan_destroy.argtypes = [vp]
     L.mbqc_plan_destroy.restype = None
     L.mbqc_plan_get_info.argtypes = [vp, C.POINTER(PlanInfo)]
@@ -115,7 +130,8 @@ def lib() -> C.CDLL:
     L.mbqc_plan_window_qubits.restype = i32
     L.mbqc_sv_window_c64.argtypes = [vp, vp, i64, vp, i64, i32, vp, i64, i64, u32, vp]
     L.mbqc_sv_window_c64.restype = C.c_int
-    L.mbqc_dm_window_c64.argtypes = [vp, vp, i64, vp, i64, i32, vp, i64, i64, C.POINTER(Noise), vp]
+    L.mbqc_dm_window_c64.argtypes = [vp, vp, i64, vp, i64, i32, vp, i64, i64, C.POINTER(Noise), vp,
+                                     vp]
     L.mbqc_dm_window_c64.restype = C.c_int
     L.mbqc_causal_flow.argtypes = [i32, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp]
     L.mbqc_causal_flow.restype = C.c_int
@@ -123,6 +139,9 @@ def lib() -> C.CDLL:
     L.mbqc_sv_sample_c64.restype = C.c_int
     L.mbqc_dm_forward_c64.argtypes = [vp, vp, i64, i64, vp, i64, vp, i64, i64, C.POINTER(Noise), u32, vp]
     L.mbqc_dm_forward_c64.restype = C.c_int
+    L.mbqc_dm_forward_outcomes_c64.argtypes = [vp, vp, i64, i64, vp, i64, vp, vp, i64, i64,
+                                               C.POINTER(Noise), u32, vp]
+    L.mbqc_dm_forward_outcomes_c64.restype = C.c_int
     L.mbqc_sv_adjoint_workspace_bytes.argtypes = [vp, i64]
     L.mbqc_sv_adjoint_workspace_bytes.restype = sz
     L.mbqc_sv_adjoint_forward_c64.argtypes = [vp, vp, i64, i64, vp, i64, vp, i64, i64, vp, sz, vp]
@@ -133,6 +152,8 @@ def lib() -> C.CDLL:
     L.mbqc_sv_adjoint_backward_c64.restype = C.c_int
     L.mbqc_gram_finish_c64.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp]
     L.mbqc_gram_finish_c64.restype = C.c_int
+    L.mbqc_overlap_loss_seed_c64.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp]
+    L.mbqc_overlap_loss_seed_c64.restype = C.c_int
     L.mbqc_sv_segments_auto.argtypes = [vp, i64]
     L.mbqc_sv_segments_auto.restype = i32
     L.mbqc_sv_segmented_workspace_bytes.argtypes = [vp, i64, i32]
@@ -185,7 +206,7 @@ class NativePlan:
 
     def __init__(self, n_nodes: int, edges, schedule, n_measure: int, window: int,
                  input_nodes, output_nodes, angle_index, fixed_angle, n_angles: int,
-                 flow=None):
+                 flow=None, plane=None):
         L = lib()
         self._keep = dict(
             edges=_i32(np.asarray(edges, dtype=np.int32).reshape(-1, 2) if len(edges) else np.zeros((0, 2))),
@@ -194,6 +215,7 @@ class NativePlan:
             angle_index=_i32(angle_index),
             fixed=np.ascontiguousarray(np.asarray(fixed_angle, dtype=np.float32)),
             flow=None if flow is None else _i32(flow),
+            plane=None if plane is None else _i32(plane),
         )
         k = self._keep
         p32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
@@ -205,13 +227,20 @@ class NativePlan:
             n_outputs=len(k["outputs"]), output_nodes=p32(k["outputs"]),
             angle_index=p32(k["angle_index"]),
             fixed_angle=k["fixed"].ctypes.data_as(C.POINTER(C.c_float)), n_angles=n_angles,
-            flow=None if k["flow"] is None else p32(k["flow"]))
+            flow=None if k["flow"] is None else p32(k["flow"]),
+            plane=None if k["plane"] is None else p32(k["plane"]))
         handle = C.c_void_p()
         check(L.mbqc_plan_create(C.byref(desc), C.byref(handle)))
         self.handle = handle
         info = PlanInfo()
         check(L.mbqc_plan_get_info(self.handle, C.byref(info)))
         self.info = {f: getattr(info, f) for f, _ in PlanInfo._fields_}
+
+    def upload(self) -> None:
+        """Copy the plan's tables to the current device (done at creation for
+        the device current then); launches on that device then allocate
+        nothing, so they can be captured in a HIP graph."""
+        check(lib().mbqc_plan_upload(self.handle))
 
     def __del__(self):
         h = getattr(self, "handle", None)

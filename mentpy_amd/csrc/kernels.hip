@@ -42,12 +42,10 @@ __device__ __forceinline__ uint32_t czbit(uint32_t j, uint32_t mask) {
 }
 
 // Round-to-nearest-even integer of v (|v| < 2^22) by the 1.5 * 2^23 shifter:
-// fl(v + 1.5 * 2^23) has a unit last mantissa bit, so its bits are
-// 0x4B400000 + rint(v) and its lowest bit is the integer's parity.  With the
-// shifter added in an FMA, v = a * c is rounded once (no rounding of a * c
-// first).
+// fl(v + 1.5 * 2^23) has a unit last mantissa bit, so its lowest bit is the
+// parity of rint(v).  With the shifter added in an FMA, v = a * c is rounded
+// once (no rounding of a * c first).
 constexpr float kShifter = 12582912.0f;      // 1.5 * 2^23
-constexpr uint32_t kShifterBits = 0x4B400000u;
 
 // sin and cos of x for |x| up to a few thousand: x = k pi + r with a two-term
 // Cody-Waite split of pi (hi part the nearest float), |r| <= pi/2, then
@@ -79,17 +77,27 @@ __device__ __forceinline__ void sincos_poly(float x, float* sn, float* cs) {
 // (the factor its amplitude-sum partial trace introduces,
 // mentpy/calculator/state_ops.py:68-73).  With m_k = rint(theta_k / 2 pi),
 // theta_k/2 - pi m_k lies in [-pi/2, pi/2], so sgn(cos(theta_k/2)) = (-1)^m_k
-// (sgn(0) = +1: rint rounds half to even) and the product is
-// exp(i (sum_k theta_k / 2 - pi sum_k m_k)).  The angles are summed in double
-// (full-rate on this GPU) and the integers m_k (from the shifter's bits) in
-// int32, and the phase is formed once at the end.
+// and the product is exp(i (sum_k theta_k / 2 - pi sum_k m_k)).  The angles
+// are summed in double (full-rate on this GPU) and the integers m_k in int32,
+// and the phase is formed once at the end.
+// m_k must be exact for every float32 angle, also the ones next to an odd
+// multiple of pi where cos(theta/2) changes sign (float32(pi) itself lies
+// 8.7e-8 above pi: m = 1, phase -i).  A float32 product with fl32(1/2pi),
+// 4e-8 too small relative to 1/2pi, rounds those down, so m_k is rounded in
+// double: fma(theta, fl64(1/2pi), 1.5 * 2^52) rounds the exact product to an
+// integer (ties to even) held in the low word of its bits, with a relative
+// error of 1.1e-16 that can only matter within 1e-13 of a half-integer;
+// no float32 angle below 2^12 comes that close (the nearest lies more than
+// 1e-9 away, checked exhaustively on the CPU).
+constexpr double kShifter64 = 6755399441055744.0;   // 1.5 * 2^52
 struct RefPhase {
   double th_sum = 0.0;
   int32_t m_sum = 0;
   __device__ __forceinline__ void add(float th) {
-    const float mb = fmaf(th, 0.15915494309189533577f, kShifter);
-    m_sum += int32_t(__float_as_uint(mb) - kShifterBits);
-    th_sum += double(th);
+    const double d = double(th);
+    const double mb = fma(d, 0.15915494309189533577, kShifter64);
+    m_sum += int32_t(uint32_t(__double_as_longlong(mb)));
+    th_sum += d;
   }
   __device__ __forceinline__ float2 value() const {
     constexpr double kPi = 3.14159265358979323846;
@@ -607,6 +615,7 @@ struct DmNoise {
   float gamma;
   float d00, d01, d10, d11, coh;
   int32_t out_noise;  // 0: the output map is the identity
+  float flip;         // 1 - 2 p_readout
 };
 
 template <int NS, int M>
@@ -1858,6 +1867,47 @@ __global__ __launch_bounds__(256) void k_gram_finish(const float2* __restrict__ 
   if (threadIdx.x == 0) loss[b] = 1.0f - part * inv_S;
 }
 
+// MB-QML loss (and seed) from propagated inputs: for parameter set b (one
+// workgroup), f_bs = sum_o conj(Y[s][o]) out[b][s][o],
+// loss[b] = 1 - (1/S) sum_s |f_bs|^2 and, if seed is given,
+// seed[b][s][o] = -2/S f_bs Y[s][o] (dL/d conj(out)).  G lanes (G = min(O, 64),
+// a power of two) share a row s, so each load instruction of a wave reads
+// contiguous rows; f is summed over the G lanes with shuffles inside the
+// group (all lanes of a group walk the same rows).
+__global__ __launch_bounds__(256) void k_overlap_loss(const float2* __restrict__ out,
+                                                      const float2* __restrict__ Y, int S, int O,
+                                                      int G, float inv_S, float* __restrict__ loss,
+                                                      float2* __restrict__ seed) {
+  __shared__ float red[4];
+  const int64_t b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int o0 = tid & (G - 1), r0 = tid / G, rows = 256 / G;
+  const float sc = -2.0f * inv_S;
+  float acc = 0.0f;
+  for (int s = r0; s < S; s += rows) {
+    const float2* ob = out + (b * S + s) * int64_t(O);
+    const float2* y = Y + int64_t(s) * O;
+    float2 f = make_float2(0.f, 0.f);
+    for (int o = o0; o < O; o += G) {
+      const float2 v = ob[o], w = y[o];
+      f.x = fmaf(w.x, v.x, fmaf(w.y, v.y, f.x));
+      f.y = fmaf(w.x, v.y, fmaf(-w.y, v.x, f.y));
+    }
+    for (int m = G >> 1; m >= 1; m >>= 1) {
+      f.x += __shfl_xor(f.x, m, 64);
+      f.y += __shfl_xor(f.y, m, 64);
+    }
+    if (o0 == 0) acc = fmaf(f.x, f.x, fmaf(f.y, f.y, acc));
+    if (seed) {
+      float2* sb = seed + (b * S + s) * int64_t(O);
+      const float2 fs = cscale(f, sc);
+      for (int o = o0; o < O; o += G) sb[o] = cmul(fs, y[o]);
+    }
+  }
+  acc = block_sum<256>(acc, red);
+  if (tid == 0) loss[b] = 1.0f - acc * inv_S;
+}
+
 __global__ void k_gram_loss(const float* __restrict__ lpart, int64_t B, int nblk, float inv_S,
                             float* __restrict__ loss) {
   const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -2140,6 +2190,221 @@ __global__ __launch_bounds__(64) void k_dm_window(Job J, const float* __restrict
   }
 }
 
+// ------------------------------------------------- measurements in any plane
+// The reference's density-matrix step in every plane of Ment.matrix
+// (mentpy/simulators/np_simulator_dm.py:307-346, mentpy/operators/ment.py:218-253)
+// with the step's outcome probability evaluated on the device: outcome 0
+// unless its probability is below 1e-4 (then 1), the state divided by the
+// outcome probability, and Z-plane nodes traced out without projection (the
+// reference's mode="expectation").  The +1 outcome of a node projects on
+// (I + n.sigma)/2 with the Bloch vector n of its plane at angle theta
+// (XY: (c, s, 0), XZ: (c, 0, s), YZ: (0, c, s)).  With the measured qubit's
+// noise folded in (populations (d00 B00 + d01 B11, d10 B00 + d11 B11),
+// coherences times coh) and the readout flip f = 1 - 2 p_m, outcome o (sign
+// g = +-1) maps the quartet blocks B_ab of the measured slot to
+//   P = W0 B00 + W1 B11,  M = W0 B00 - W1 B11,  V = C B10,  U = conj(C) B01,
+//   W0 = (1 + f g n_z) d00 + (1 - f g n_z) d10,
+//   W1 = (1 + f g n_z) d01 + (1 - f g n_z) d11,  C = f g coh (n_x - i n_y),
+// the same quartet form as the XY kernels (which have W0 = W1 = 1).  The
+// probability of outcome 0 (noise included, readout flip not) is
+//   p0 = (W0' t0 + W1' t1 + 2 coh (n_x x + n_y y)) / (2 (t0 + t1))
+// with W0', W1' at f g = 1, t0 / t1 the traces of B00 / B11 and x + iy that of
+// B10 (zero when the appended qubit's CZ with the measured one comes first);
+// the mixed post-selected probability is p_m + f p_o.  Every step divides
+// by it (and by the trace and the appended |+><+|'s factor 2), so the state
+// keeps unit trace and needs no growth tracking.
+__device__ __forceinline__ void plane_bloch(uint32_t plane, float th, float* nx, float* ny,
+                                            float* nz) {
+  float s, c;
+  sincos_poly(th, &s, &c);
+  *nx = plane == MBQC_PLANE_XY || plane == MBQC_PLANE_XZ ? c : 0.0f;
+  *ny = plane == MBQC_PLANE_XY ? s : (plane == MBQC_PLANE_YZ ? c : 0.0f);
+  *nz = plane == MBQC_PLANE_XZ || plane == MBQC_PLANE_YZ ? s : (plane == MBQC_PLANE_Z ? 1.0f : 0.0f);
+}
+
+// dm_lds_step with the population weights W0, W1 and coherence coefficient C
+// of any plane (see above).
+template <int T>
+__device__ __forceinline__ void dm_lds_step_general(float2* r, int H, int q, uint32_t flags,
+                                                    uint32_t nmask, float2 C, float w0, float w1,
+                                                    int tid) {
+  const int HQ = H / 2;
+  const float2 Cc = cconj(C);
+  const int QB = 1 << q;
+  const bool app = flags & 1u, sneg = flags & 2u;
+  for (int i = tid; i < HQ * HQ; i += T) {
+    const int j = insert_zero(i / HQ, q), kk = insert_zero(i % HQ, q);
+    const float2 B00 = r[j * H + kk], B01 = r[j * H + (kk | QB)];
+    const float2 B10 = r[(j | QB) * H + kk], B11 = r[(j | QB) * H + (kk | QB)];
+    const float2 U = cmul(Cc, B01), V = cmul(C, B10);
+    const float2 P = make_float2(fmaf(w0, B00.x, w1 * B11.x), fmaf(w0, B00.y, w1 * B11.y));
+    const float2 r00 = cadd(P, cadd(U, V));
+    if (app) {
+      const uint32_t sj = czbit(j, nmask), sk = czbit(kk, nmask);
+      if (sneg) {
+        const float2 M = make_float2(fmaf(w0, B00.x, -w1 * B11.x), fmaf(w0, B00.y, -w1 * B11.y));
+        r[j * H + kk] = r00;
+        r[j * H + (kk | QB)] = flip(cadd(M, csub(V, U)), sk);
+        r[(j | QB) * H + kk] = flip(cadd(M, csub(U, V)), sj);
+        r[(j | QB) * H + (kk | QB)] = flip(csub(P, cadd(U, V)), sj ^ sk);
+      } else {
+        r[j * H + kk] = r00;
+        r[j * H + (kk | QB)] = flip(r00, sk);
+        r[(j | QB) * H + kk] = flip(r00, sj);
+        r[(j | QB) * H + (kk | QB)] = flip(r00, sj ^ sk);
+      }
+    } else {
+      r[j * H + kk] = r00;
+      r[j * H + (kk | QB)] = make_float2(0.f, 0.f);
+      r[(j | QB) * H + kk] = make_float2(0.f, 0.f);
+      r[(j | QB) * H + (kk | QB)] = make_float2(0.f, 0.f);
+    }
+  }
+}
+
+// One simulation per 64-thread workgroup, density matrix in dynamic LDS
+// (4^n_slots entries, n_slots <= 6), k_steps steps.  Forward mode
+// (window_mode = 0): angles from the angle table, output noise, output in
+// out_pos order.  Window mode: per-step angles step_angles[b * step_stride + k]
+// and the reference's window after k_steps measurements (as k_dm_window).
+// If `outcomes` is not null, step k's outcome (0 or 1; for a Z node the
+// expectation p1 = 1 - p0) goes to outcomes[g * k_steps + k].
+__global__ __launch_bounds__(64) void k_dm_general(Job J, const float* __restrict__ step_angles,
+                                                   int64_t step_stride, int32_t k_steps,
+                                                   WindowMap map, int32_t window_mode, DmNoise nz,
+                                                   uint32_t flags, float* __restrict__ outcomes) {
+  constexpr int T = 64;
+  extern __shared__ float2 r[];
+  __shared__ float red[4];
+  const int ns = J.ns, H = 1 << ns, HQ = H / 2;
+  const int64_t g = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int64_t bi = g / J.S, si = g - bi * J.S;
+  const float* ang = window_mode ? step_angles + bi * step_stride : J.angles + bi * J.angle_stride;
+  const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
+  const int32_t* in_index = J.tab + J.off_in;
+  const int32_t* sign = J.tab + J.off_sign;
+  const float2* psi = J.psi + si * J.in_stride;
+  for (int e = tid; e < H * H; e += T) {
+    const int j = e / H, k = e % H;
+    const float2 vj = flip(psi[in_index[j]], sign[j] ? 0x80000000u : 0u);
+    const float2 vk = flip(psi[in_index[k]], sign[k] ? 0x80000000u : 0u);
+    r[e] = cmul(vj, cconj(vk));
+  }
+  const float pm = 0.5f * (1.0f - nz.flip);
+  for (int k = 0; k < k_steps; ++k) {
+    const mbqc_step step = steps[k];
+    const int q = int((step.flags >> 8) & 0xffu);
+    const uint32_t plane = (step.flags >> 2) & 3u;
+    const int QB = 1 << q;
+    float th = 0.0f;
+    if (plane != MBQC_PLANE_Z) {
+      if (window_mode) th = ang[k];
+      else th = step.angle_index >= 0 ? ang[int64_t(step.angle_index) * J.angle_stride_a]
+                                      : step.fixed_angle;
+    }
+    __syncthreads();
+    // traces of the measured slot's diagonal blocks
+    float t0 = 0.f, t1 = 0.f, x = 0.f, y = 0.f;
+    for (int a = tid; a < HQ; a += T) {
+      const int j = insert_zero(a, q);
+      t0 += r[j * H + j].x;
+      t1 += r[(j | QB) * H + (j | QB)].x;
+      const float2 b10 = r[(j | QB) * H + j];
+      x += b10.x;
+      y += b10.y;
+    }
+    t0 = block_sum<T>(t0, red);
+    t1 = block_sum<T>(t1, red);
+    x = block_sum<T>(x, red);
+    y = block_sum<T>(y, red);
+    float nx, ny, nzz;
+    plane_bloch(plane, th, &nx, &ny, &nzz);
+    const float tt = t0 + t1;
+    const float w0p = (1.0f + nzz) * nz.d00 + (1.0f - nzz) * nz.d10;
+    const float w1p = (1.0f + nzz) * nz.d01 + (1.0f - nzz) * nz.d11;
+    // a CZ between the appended |+> and the measured qubit before the
+    // measurement (flags bit 1) dephases the measured qubit: its coherences
+    // then have zero trace and do not enter the probability
+    const float coh_term = (step.flags & 2u) ? 0.0f : 2.0f * nz.coh * (nx * x + ny * y);
+    const float p0 = (w0p * t0 + w1p * t1 + coh_term) / (2.0f * tt);
+    float w0, w1, outcome, denom;
+    float2 C;
+    if (plane == MBQC_PLANE_Z) {
+      w0 = w1 = 1.0f;
+      C = make_float2(0.f, 0.f);
+      outcome = 1.0f - p0;
+      denom = tt;
+    } else {
+      const bool one = p0 < 1e-4f;
+      const float gf = one ? -nz.flip : nz.flip;
+      w0 = (1.0f + gf * nzz) * nz.d00 + (1.0f - gf * nzz) * nz.d10;
+      w1 = (1.0f + gf * nzz) * nz.d01 + (1.0f - gf * nzz) * nz.d11;
+      C = make_float2(gf * nz.coh * nx, -gf * nz.coh * ny);
+      outcome = one ? 1.0f : 0.0f;
+      const float po = one ? 1.0f - p0 : p0;
+      denom = 2.0f * (pm + nz.flip * po) * tt;
+    }
+    const float sc = 1.0f / ((step.flags & 1u) ? 2.0f * denom : denom);
+    w0 *= sc;
+    w1 *= sc;
+    C = cscale(C, sc);
+    if (outcomes && tid == 0)
+      outcomes[g * int64_t(window_mode ? k_steps : J.n_steps) + k] = outcome;
+    __syncthreads();
+    dm_lds_step_general<T>(r, H, q, step.flags, step.nmask, C, w0, w1, tid);
+  }
+  __syncthreads();
+  if (window_mode) {
+    float tr = 0.0f;
+    for (int j = tid; j < H; j += T) tr += r[j * H + j].x;
+    tr = block_sum<T>(tr, red);
+    bool appended = false;
+    for (int i = 0; i < map.W; ++i) appended |= map.slot[i] < 0;
+    const float scale = (appended ? 0.5f : 1.0f) / tr;
+    const int D = 1 << map.W;
+    float2* o = J.out + g * int64_t(D) * D;
+    for (int e = tid; e < D * D; e += T) {
+      const int row = e / D, col = e % D;
+      uint32_t sr, sc2;
+      const int jr = window_source(map, row, &sr), jc = window_source(map, col, &sc2);
+      o[e] = flip(cscale(r[jr * H + jc], scale), sr ^ sc2);
+    }
+    return;
+  }
+  // noise of every output qubit
+  for (int m = 0; m < ns; ++m) {
+    if (!(uint32_t(J.live_mask) & (1u << m)) || !nz.out_noise) continue;
+    const int MB = 1 << m;
+    __syncthreads();
+    for (int i = tid; i < HQ * HQ; i += T) {
+      const int j = insert_zero(i / HQ, m), kk = insert_zero(i % HQ, m);
+      const float2 a = r[j * H + kk], d = r[(j | MB) * H + (kk | MB)];
+      r[j * H + kk] = make_float2(fmaf(nz.d00, a.x, nz.d01 * d.x), fmaf(nz.d00, a.y, nz.d01 * d.y));
+      r[(j | MB) * H + (kk | MB)] =
+          make_float2(fmaf(nz.d10, a.x, nz.d11 * d.x), fmaf(nz.d10, a.y, nz.d11 * d.y));
+      r[j * H + (kk | MB)] = cscale(r[j * H + (kk | MB)], nz.coh);
+      r[(j | MB) * H + kk] = cscale(r[(j | MB) * H + kk], nz.coh);
+    }
+  }
+  __syncthreads();
+  const int32_t* out_pos = J.tab + J.off_out;
+  float scale = 1.0f;
+  if (flags & MBQC_FLAG_RENORMALIZE) {
+    float tr = 0.0f;
+    for (int j = tid; j < H; j += T)
+      if (out_pos[j] >= 0) tr += r[j * H + j].x;
+    tr = block_sum<T>(tr, red);
+    scale = 1.0f / tr;
+  }
+  const int64_t d = int64_t(1) << J.n_out;
+  float2* o = J.out + g * d * d;
+  for (int e = tid; e < H * H; e += T) {
+    const int pj = out_pos[e / H], pk = out_pos[e % H];
+    if (pj >= 0 && pk >= 0) o[int64_t(pj) * d + pk] = cscale(r[e], scale);
+  }
+}
+
 // ---------------------------------------------------------------- batched
 struct BatchItem {
   int32_t job;
@@ -2245,7 +2510,18 @@ __global__ void k_reset_counters(int* counter, int n) {
 // ==================================================================== C ABI
 using namespace mbqc;
 
+
 namespace {
+
+constexpr int kMaxSvSlots = 6;       // register-resident SV kernels
+// below this many simulations the DM path keeps each density matrix in LDS
+// (one quartet per thread, several simulations per workgroup); the two
+// kernels take the same time at 32,768 simulations of the 3x12 grid
+constexpr int64_t kDmLaneKernelMinSims = 32768;
+constexpr int kMaxSvLdsSlots = 12;   // LDS-resident SV kernel
+constexpr int kMaxDmSlots = 3;       // register-resident DM kernel
+constexpr int kMaxDmLdsSlots = 6;    // LDS-resident DM kernel
+constexpr int kMaxAdjSlots = 5;
 
 int hip_fail(hipError_t e, const char* what) {
   return fail(MBQC_ERR_RUNTIME, std::string(what) + ": " + hipGetErrorString(e));
@@ -2271,6 +2547,46 @@ int device_tables(Plan* p, const int32_t** out) {
     p->dev_tab[dev] = d;
   }
   *out = static_cast<const int32_t*>(p->dev_tab[dev]);
+  return MBQC_OK;
+}
+
+// Upload at plan creation (current device; nothing to do on a host without
+// a GPU, where launches fail later anyway).
+int upload_at_create(Plan* p) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    (void)hipGetLastError();   // leave no error behind for a later check
+    return MBQC_OK;
+  }
+  const int32_t* tab = nullptr;
+  return device_tables(p, &tab);
+}
+
+struct RegisterUploadHook {
+  RegisterUploadHook() { g_plan_upload_hook = upload_at_create; }
+} register_upload_hook;
+
+// The statevector kernels measure in the XY plane only, as the reference's
+// statevector backend does (np_simulator_sv.py:54-59, 329-332).
+int check_xy_plan(const Plan* p) {
+  if (p->general_planes)
+    return fail(MBQC_ERR_VALUE, "the pattern has XZ, YZ or Z measurements, but only XY plane is "
+                                "supported by the statevector kernels");
+  return MBQC_OK;
+}
+
+// Adjoint sweeps write one gradient entry per step, so an angle shared by two
+// measured nodes would keep only one node's contribution.
+int check_adjoint_plan(const Plan* p) {
+  int rc = check_xy_plan(p);
+  if (rc != MBQC_OK) return rc;
+  if (p->ns > kMaxAdjSlots)
+    return fail(MBQC_ERR_NOT_IMPLEMENTED, "adjoint kernels support live windows of at most " +
+                                              std::to_string(kMaxAdjSlots) + " qubits");
+  if (p->shared_angles)
+    return fail(MBQC_ERR_NOT_IMPLEMENTED,
+                "adjoint gradients need every angle-vector entry to belong to at most one "
+                "measured node");
   return MBQC_OK;
 }
 
@@ -2383,6 +2699,7 @@ DmNoise compose_noise(const mbqc_noise& n) {
   nz.d11 = float((1.0 - ch.c + ch.a) / 2.0);
   nz.coh = float(ch.coh);
   nz.out_noise = (ch.coh != 1.0 || ch.a != 1.0 || ch.c != 0.0) ? 1 : 0;
+  nz.flip = float(1.0 - 2.0 * n.p_readout);
   return nz;
 }
 
@@ -2548,16 +2865,6 @@ void launch_batch_group(const BatchGroupLaunch& L, bool ref, bool share, bool pe
   else launch_batch_group_p<NS, false>(L, ref, share);
 }
 
-constexpr int kMaxSvSlots = 6;       // register-resident SV kernels
-// below this many simulations the DM path keeps each density matrix in LDS
-// (one quartet per thread, several simulations per workgroup); the two
-// kernels take the same time at 32,768 simulations of the 3x12 grid
-constexpr int64_t kDmLaneKernelMinSims = 32768;
-constexpr int kMaxSvLdsSlots = 12;   // LDS-resident SV kernel
-constexpr int kMaxDmSlots = 3;       // register-resident DM kernel
-constexpr int kMaxDmLdsSlots = 6;    // LDS-resident DM kernel
-constexpr int kMaxAdjSlots = 5;
-
 template <int NS>
 hipError_t launch_sample(const Job& J, const int32_t* tab, const TableOffsets& o, int32_t* outcomes,
                          int32_t words, uint64_t seed, bool periodic, hipStream_t st) {
@@ -2602,9 +2909,8 @@ int seg_setup(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
               void* workspace, size_t workspace_bytes, Plan** pp, Job* J, SegInfo* G) {
   if (!plan) return fail(MBQC_ERR_VALUE, "null plan");
   Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(plan));
-  if (p->ns > kMaxAdjSlots)
-    return fail(MBQC_ERR_NOT_IMPLEMENTED, "adjoint kernels support live windows of at most " +
-                                              std::to_string(kMaxAdjSlots) + " qubits");
+  int rc0 = check_adjoint_plan(p);
+  if (rc0 != MBQC_OK) return rc0;
   if (n_segments < 2 || n_segments > std::min(kMaxSegments, p->n_blocks()))
     return fail(MBQC_ERR_VALUE, "n_segments must be in 2.." +
                                     std::to_string(std::min(kMaxSegments, p->n_blocks())));
@@ -2615,6 +2921,93 @@ int seg_setup(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
   if (rc != MBQC_OK) return rc;
   *pp = p;
   return MBQC_OK;
+}
+
+// Validated noise parameters, composed for the kernels.
+int dm_noise(const mbqc_noise* noise, DmNoise* nz) {
+  const mbqc_noise zero{};
+  const mbqc_noise& n = noise ? *noise : zero;
+  const float vals[7] = {n.p_depol, n.p_readout, n.amp_damping, n.phase_damping,
+                         n.phase_flip, n.gad_gamma, n.gad_p};
+  for (float v : vals)
+    if (!(v >= 0.f && v <= 1.f)) return fail(MBQC_ERR_VALUE, "noise parameters must lie in [0, 1]");
+  *nz = compose_noise(n);
+  return MBQC_OK;
+}
+
+int dm_forward(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
+               int64_t angle_stride_a, const void* psi_in, int64_t in_stride, void* out,
+               float* outcomes, int64_t B, int64_t S, const mbqc_noise* noise, uint32_t flags,
+               void* stream) {
+  if (!plan) return fail(MBQC_ERR_VALUE, "null plan");
+  Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(plan));
+  if (p->ns > kMaxDmLdsSlots)
+    return fail(MBQC_ERR_NOT_IMPLEMENTED, "density-matrix kernels support live windows of at most " +
+                                              std::to_string(kMaxDmLdsSlots) + " qubits");
+  DmNoise nz;
+  int rc = dm_noise(noise, &nz);
+  if (rc != MBQC_OK) return rc;
+  Job J;
+  rc = make_job(p, angles, angle_stride, angle_stride_a, psi_in, in_stride, out, B, S, &J);
+  if (rc != MBQC_OK) return rc;
+  if (J.total == 0) return MBQC_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipError_t e = hipSuccess;
+  if (p->general_planes || (flags & MBQC_FLAG_STEP_PROBS)) {
+    if (J.total > 0x7fffffff) return fail(MBQC_ERR_VALUE, "too many simulations for one launch");
+    WindowMap none{};
+    const size_t lds = sizeof(float2) << (2 * p->ns);
+    hipLaunchKernelGGL(k_dm_general, dim3(unsigned(J.total)), dim3(64), lds, st, J,
+                       static_cast<const float*>(nullptr), int64_t(0), int32_t(p->n_steps), none, 0,
+                       nz, flags, outcomes);
+    e = hipGetLastError();
+    return e == hipSuccess ? MBQC_OK : hip_fail(e, "dm_forward launch");
+  }
+  // Small batches of small windows leave the one-simulation-per-lane kernel
+  // with a few waves on a few CUs; there the LDS kernel (one quartet per
+  // thread, several small simulations per 64-lane workgroup) spreads the work
+  // over the whole chip.
+  const bool small = J.total < kDmLaneKernelMinSims;
+  const bool pe = p->periodic && !(flags & MBQC_FLAG_TABLE_STEPS);
+  switch (p->ns) {
+    case 1: e = small ? launch_dm_lds<1>(J, nz, flags, pe, st) : launch_dm<1>(J, nz, flags, pe, st); break;
+    case 2: e = small ? launch_dm_lds<2>(J, nz, flags, pe, st) : launch_dm<2>(J, nz, flags, pe, st); break;
+    case 3: e = small ? launch_dm_lds<3>(J, nz, flags, pe, st) : launch_dm<3>(J, nz, flags, pe, st); break;
+    case 4: e = launch_dm_lds<4>(J, nz, flags, pe, st); break;
+    case 5: e = launch_dm_lds<5>(J, nz, flags, pe, st); break;
+    case 6: e = launch_dm_lds<6>(J, nz, flags, pe, st); break;
+  }
+  return e == hipSuccess ? MBQC_OK : hip_fail(e, "dm_forward launch");
+}
+
+int gram_loss_seed(const void* t, const void* MT, int64_t B, int32_t K, int32_t S, float* loss,
+                   void* seed, float* workspace, void* stream) {
+  if (!t || !MT || !loss || !seed) return fail(MBQC_ERR_VALUE, "null pointer");
+  if (B < 0 || K <= 0 || S <= 0) return fail(MBQC_ERR_VALUE, "sizes must be B >= 0, K >= 1, S >= 1");
+  if (int64_t(kGramSets) * K * int64_t(sizeof(float2)) > 64 * 1024)
+    return fail(MBQC_ERR_NOT_IMPLEMENTED, "Gram loss supports K <= 512");
+  if (B == 0) return MBQC_OK;
+  if ((B + 15) / 16 > 65535) return fail(MBQC_ERR_VALUE, "too many parameter sets");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const float inv_S = 1.0f / float(S);
+  int nblk;
+  if (K % 16 == 0) {
+    nblk = K / 16;
+    hipLaunchKernelGGL(k_gram_seed_mfma, dim3(nblk, unsigned((B + 15) / 16)), dim3(64), 0, st,
+                       static_cast<const float2*>(t), static_cast<const float2*>(MT), B, K, inv_S,
+                       static_cast<float2*>(seed), workspace);
+  } else {
+    nblk = (K + 63) / 64;
+    const size_t lds = size_t(kGramSets) * K * sizeof(float2);
+    hipLaunchKernelGGL(k_gram_seed, dim3(nblk, unsigned((B + kGramSets - 1) / kGramSets)),
+                       dim3(256), lds, st, static_cast<const float2*>(t),
+                       static_cast<const float2*>(MT), B, K, inv_S, static_cast<float2*>(seed),
+                       workspace);
+  }
+  hipLaunchKernelGGL(k_gram_loss, dim3(unsigned((B + 255) / 256)), dim3(256), 0, st, workspace, B,
+                     nblk, inv_S, loss);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? MBQC_OK : hip_fail(e, "gram loss launch");
 }
 
 }  // namespace
@@ -2643,6 +3036,12 @@ struct mbqc_batch {
 
 extern "C" {
 
+int mbqc_plan_upload(const mbqc_plan* plan) {
+  if (!plan) return fail(MBQC_ERR_VALUE, "null plan");
+  const int32_t* tab = nullptr;
+  return device_tables(const_cast<Plan*>(reinterpret_cast<const Plan*>(plan)), &tab);
+}
+
 void mbqc_plan_destroy(mbqc_plan* plan) {
   if (!plan) return;
   Plan* p = reinterpret_cast<Plan*>(plan);
@@ -2659,6 +3058,8 @@ int mbqc_sv_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
   if (p->ns > kMaxSvLdsSlots)
     return fail(MBQC_ERR_NOT_IMPLEMENTED, "statevector kernels support live windows of at most " +
                                               std::to_string(kMaxSvLdsSlots) + " qubits");
+  int rc0 = check_xy_plan(p);
+  if (rc0 != MBQC_OK) return rc0;
   Job J;
   int rc = make_job(p, angles, angle_stride, angle_stride_a, psi_in, in_stride, out, B, S, &J);
   if (rc != MBQC_OK) return rc;
@@ -2691,6 +3092,8 @@ int mbqc_sv_sample_c64(const mbqc_plan* plan, const float* angles, int64_t angle
                        int32_t* outcomes, int64_t B, int64_t S, uint64_t seed, void* stream) {
   if (!plan) return fail(MBQC_ERR_VALUE, "null plan");
   Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(plan));
+  int rc0 = check_xy_plan(p);
+  if (rc0 != MBQC_OK) return rc0;
   if (!p->can_sample)
     return fail(MBQC_ERR_NOT_IMPLEMENTED, "sampled outcomes unavailable: " + p->sample_reason);
   if (!outcomes && B > 0) return fail(MBQC_ERR_VALUE, "outcomes pointer is null");
@@ -2745,6 +3148,8 @@ int mbqc_sv_window_c64(const mbqc_plan* plan, const float* step_angles, int64_t 
   int rc = window_job(plan, step_angles, step_stride, psi_in, in_stride, k, out, B, S,
                       kMaxSvLdsSlots, &J, &map, &p);
   if (rc != MBQC_OK) return rc;
+  rc = check_xy_plan(p);
+  if (rc != MBQC_OK) return rc;
   if (J.total == 0) return MBQC_OK;
   const size_t lds = sizeof(float2) << p->ns;
   hipLaunchKernelGGL(k_sv_window, dim3(J.total), dim3(64), lds, static_cast<hipStream_t>(stream), J,
@@ -2755,19 +3160,26 @@ int mbqc_sv_window_c64(const mbqc_plan* plan, const float* step_angles, int64_t 
 
 int mbqc_dm_window_c64(const mbqc_plan* plan, const float* step_angles, int64_t step_stride,
                        const void* psi_in, int64_t in_stride, int32_t k, void* out, int64_t B,
-                       int64_t S, const mbqc_noise* noise, void* stream) {
+                       int64_t S, const mbqc_noise* noise, float* outcomes, void* stream) {
   Job J;
   WindowMap map;
   Plan* p = nullptr;
   int rc = window_job(plan, step_angles, step_stride, psi_in, in_stride, k, out, B, S,
                       kMaxDmLdsSlots, &J, &map, &p);
   if (rc != MBQC_OK) return rc;
-  const mbqc_noise zero{};
-  const DmNoise nz = compose_noise(noise ? *noise : zero);
+  DmNoise nz;
+  rc = dm_noise(noise, &nz);
+  if (rc != MBQC_OK) return rc;
   if (J.total == 0) return MBQC_OK;
+  if (J.total > 0x7fffffff) return fail(MBQC_ERR_VALUE, "too many simulations for one launch");
   const size_t lds = sizeof(float2) << (2 * p->ns);
-  hipLaunchKernelGGL(k_dm_window, dim3(J.total), dim3(64), lds, static_cast<hipStream_t>(stream), J,
-                     step_angles, step_stride, k, map, nz);
+  if (p->general_planes || outcomes)
+    hipLaunchKernelGGL(k_dm_general, dim3(unsigned(J.total)), dim3(64), lds,
+                       static_cast<hipStream_t>(stream), J, step_angles, step_stride, k, map, 1, nz,
+                       0u, outcomes);
+  else
+    hipLaunchKernelGGL(k_dm_window, dim3(unsigned(J.total)), dim3(64), lds,
+                       static_cast<hipStream_t>(stream), J, step_angles, step_stride, k, map, nz);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "dm_window launch");
 }
@@ -2775,43 +3187,17 @@ int mbqc_dm_window_c64(const mbqc_plan* plan, const float* step_angles, int64_t 
 int mbqc_dm_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
                         int64_t angle_stride_a, const void* psi_in, int64_t in_stride, void* out,
                         int64_t B, int64_t S, const mbqc_noise* noise, uint32_t flags, void* stream) {
-  if (!plan) return fail(MBQC_ERR_VALUE, "null plan");
-  Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(plan));
-  if (p->ns > kMaxDmLdsSlots)
-    return fail(MBQC_ERR_NOT_IMPLEMENTED, "density-matrix kernels support live windows of at most " +
-                                              std::to_string(kMaxDmLdsSlots) + " qubits");
-  DmNoise nz;
-  {
-    const mbqc_noise zero{};
-    const mbqc_noise& n = noise ? *noise : zero;
-    const float vals[7] = {n.p_depol, n.p_readout, n.amp_damping, n.phase_damping,
-                           n.phase_flip, n.gad_gamma, n.gad_p};
-    for (float v : vals)
-      if (!(v >= 0.f && v <= 1.f))
-        return fail(MBQC_ERR_VALUE, "noise parameters must lie in [0, 1]");
-    nz = compose_noise(n);
-  }
-  Job J;
-  int rc = make_job(p, angles, angle_stride, angle_stride_a, psi_in, in_stride, out, B, S, &J);
-  if (rc != MBQC_OK) return rc;
-  if (J.total == 0) return MBQC_OK;
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  hipError_t e = hipSuccess;
-  // Small batches of small windows leave the one-simulation-per-lane kernel
-  // with a few waves on a few CUs; there the LDS kernel (one quartet per
-  // thread, several small simulations per 64-lane workgroup) spreads the work
-  // over the whole chip.
-  const bool small = J.total < kDmLaneKernelMinSims;
-  const bool pe = p->periodic && !(flags & MBQC_FLAG_TABLE_STEPS);
-  switch (p->ns) {
-    case 1: e = small ? launch_dm_lds<1>(J, nz, flags, pe, st) : launch_dm<1>(J, nz, flags, pe, st); break;
-    case 2: e = small ? launch_dm_lds<2>(J, nz, flags, pe, st) : launch_dm<2>(J, nz, flags, pe, st); break;
-    case 3: e = small ? launch_dm_lds<3>(J, nz, flags, pe, st) : launch_dm<3>(J, nz, flags, pe, st); break;
-    case 4: e = launch_dm_lds<4>(J, nz, flags, pe, st); break;
-    case 5: e = launch_dm_lds<5>(J, nz, flags, pe, st); break;
-    case 6: e = launch_dm_lds<6>(J, nz, flags, pe, st); break;
-  }
-  return e == hipSuccess ? MBQC_OK : hip_fail(e, "dm_forward launch");
+  return dm_forward(plan, angles, angle_stride, angle_stride_a, psi_in, in_stride, out, nullptr, B,
+                    S, noise, flags, stream);
+}
+
+int mbqc_dm_forward_outcomes_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
+                                 int64_t angle_stride_a, const void* psi_in, int64_t in_stride,
+                                 void* out, float* outcomes, int64_t B, int64_t S,
+                                 const mbqc_noise* noise, uint32_t flags, void* stream) {
+  if (!outcomes && B > 0) return fail(MBQC_ERR_VALUE, "outcomes pointer is null");
+  return dm_forward(plan, angles, angle_stride, angle_stride_a, psi_in, in_stride, out, outcomes, B,
+                    S, noise, flags | MBQC_FLAG_STEP_PROBS, stream);
 }
 
 size_t mbqc_sv_adjoint_workspace_bytes(const mbqc_plan* plan, int64_t n_columns) {
@@ -2902,9 +3288,8 @@ int mbqc_sv_adjoint_forward_c64(const mbqc_plan* plan, const float* angles, int6
                                 void* stream) {
   if (!plan) return fail(MBQC_ERR_VALUE, "null plan");
   Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(plan));
-  if (p->ns > kMaxAdjSlots)
-    return fail(MBQC_ERR_NOT_IMPLEMENTED, "adjoint kernels support live windows of at most " +
-                                              std::to_string(kMaxAdjSlots) + " qubits");
+  int rc0 = check_adjoint_plan(p);
+  if (rc0 != MBQC_OK) return rc0;
   Job J;
   int rc = make_job(p, angles, angle_stride, angle_stride_a, psi_in, in_stride, out, B, C, &J);
   if (rc != MBQC_OK) return rc;
@@ -2930,9 +3315,8 @@ int mbqc_sv_adjoint_backward_c64(const mbqc_plan* plan, const float* angles, int
                                  void* stream) {
   if (!plan) return fail(MBQC_ERR_VALUE, "null plan");
   Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(plan));
-  if (p->ns > kMaxAdjSlots)
-    return fail(MBQC_ERR_NOT_IMPLEMENTED, "adjoint kernels support live windows of at most " +
-                                              std::to_string(kMaxAdjSlots) + " qubits");
+  int rc0 = check_adjoint_plan(p);
+  if (rc0 != MBQC_OK) return rc0;
   if (!seed || !grad_partial) return fail(MBQC_ERR_VALUE, "null seed or gradient pointer");
   if (!ckpt || ckpt_bytes < mbqc_sv_adjoint_workspace_bytes(plan, B * C))
     return fail(MBQC_ERR_VALUE, "checkpoint workspace too small");
@@ -2990,6 +3374,10 @@ int mbqc_batch_create(const mbqc_batch_job* jobs, int32_t n_jobs, int32_t sims_p
       delete bt;
       return fail(MBQC_ERR_NOT_IMPLEMENTED, "batched kernel supports live windows of at most " +
                                                 std::to_string(kMaxSvSlots) + " qubits");
+    }
+    if (int rc0 = check_xy_plan(p); rc0 != MBQC_OK) {
+      delete bt;
+      return rc0;
     }
     Job J;
     int rc = make_job(p, bj.angles, bj.angle_stride, bj.angle_stride_a, bj.psi_in, bj.in_stride,
@@ -3082,6 +3470,21 @@ int mbqc_gram_finish_c64(const void* t, const void* z, int64_t B, int32_t K, int
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "gram finish launch");
 }
 
+int mbqc_overlap_loss_seed_c64(const void* out, const void* targets, int64_t B, int32_t S,
+                               int32_t O, float* loss, void* seed, void* stream) {
+  if (B < 0 || S <= 0 || O <= 0 || (O & (O - 1)) != 0)
+    return fail(MBQC_ERR_VALUE, "sizes must be B >= 0, S >= 1 and O a power of two");
+  if (B == 0) return MBQC_OK;
+  if (!out || !targets || !loss) return fail(MBQC_ERR_VALUE, "null pointer");
+  if (B > 0x7fffffff) return fail(MBQC_ERR_VALUE, "too many parameter sets");
+  const int G = O < 64 ? O : 64;
+  hipLaunchKernelGGL(k_overlap_loss, dim3(unsigned(B)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     static_cast<const float2*>(out), static_cast<const float2*>(targets), S, O, G,
+                     1.0f / float(S), loss, static_cast<float2*>(seed));
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? MBQC_OK : hip_fail(e, "overlap loss launch");
+}
+
 size_t mbqc_gram_workspace_bytes(int64_t B, int32_t K) {
   if (B < 0 || K <= 0) return 0;
   return size_t(B) * size_t((K + 15) / 16) * sizeof(float);
@@ -3089,32 +3492,8 @@ size_t mbqc_gram_workspace_bytes(int64_t B, int32_t K) {
 
 int mbqc_gram_loss_seed_c64(const void* t, const void* MT, int64_t B, int32_t K, int32_t S,
                             float* loss, void* seed, float* workspace, void* stream) {
-  if (!t || !MT || !loss || !seed || !workspace) return fail(MBQC_ERR_VALUE, "null pointer");
-  if (B < 0 || K <= 0 || S <= 0) return fail(MBQC_ERR_VALUE, "sizes must be B >= 0, K >= 1, S >= 1");
-  if (int64_t(kGramSets) * K * int64_t(sizeof(float2)) > 64 * 1024)
-    return fail(MBQC_ERR_NOT_IMPLEMENTED, "Gram loss supports K <= 512");
-  if (B == 0) return MBQC_OK;
-  if ((B + 15) / 16 > 65535) return fail(MBQC_ERR_VALUE, "too many parameter sets");
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  const float inv_S = 1.0f / float(S);
-  int nblk;
-  if (K % 16 == 0) {
-    nblk = K / 16;
-    hipLaunchKernelGGL(k_gram_seed_mfma, dim3(nblk, unsigned((B + 15) / 16)), dim3(64), 0, st,
-                       static_cast<const float2*>(t), static_cast<const float2*>(MT), B, K, inv_S,
-                       static_cast<float2*>(seed), workspace);
-  } else {
-    nblk = (K + 63) / 64;
-    const size_t lds = size_t(kGramSets) * K * sizeof(float2);
-    hipLaunchKernelGGL(k_gram_seed, dim3(nblk, unsigned((B + kGramSets - 1) / kGramSets)),
-                       dim3(256), lds, st, static_cast<const float2*>(t),
-                       static_cast<const float2*>(MT), B, K, inv_S, static_cast<float2*>(seed),
-                       workspace);
-  }
-  hipLaunchKernelGGL(k_gram_loss, dim3(unsigned((B + 255) / 256)), dim3(256), 0, st, workspace, B,
-                     nblk, inv_S, loss);
-  hipError_t e = hipGetLastError();
-  return e == hipSuccess ? MBQC_OK : hip_fail(e, "gram loss launch");
+  if (!workspace) return fail(MBQC_ERR_VALUE, "null pointer");
+  return gram_loss_seed(t, MT, B, K, S, loss, seed, workspace, stream);
 }
 
 int mbqc_adam_step_f32(float* x, const float* grad, float* m, float* v, int64_t n, float lr, float b1,

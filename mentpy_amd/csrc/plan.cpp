@@ -29,6 +29,7 @@
 namespace mbqc {
 
 thread_local std::string g_last_error;
+int (*g_plan_upload_hook)(Plan*) = nullptr;
 
 int fail(int code, const std::string& msg) {
   g_last_error = msg;
@@ -164,15 +165,24 @@ int compile_plan(const mbqc_pattern_desc* d, Plan* p) {
   int applied = 0;
   for (int a = 0; a < ns; ++a)
     for (int b = a + 1; b < ns; ++b) applied += has_edge(d->schedule[a], d->schedule[b]);
+  std::vector<char> angle_used(d->n_angles > 0 ? d->n_angles : 0, 0);
   for (int k = 0; k < nm; ++k) {
     mbqc_step st{};
     const int v = d->schedule[k];
     const int q = slot_of_pos(k);
     int ai = d->angle_index[v];
     if (ai >= d->n_angles) return fail(MBQC_ERR_VALUE, "angle index out of range");
+    if (ai >= 0) {
+      if (angle_used[ai]) p->shared_angles = true;
+      angle_used[ai] = 1;
+    }
     st.angle_index = ai < 0 ? -1 : ai;
     st.fixed_angle = ai < 0 ? d->fixed_angle[v] : 0.0f;
-    uint32_t flags = uint32_t(q) << 8;
+    const int plane = d->plane ? d->plane[v] : MBQC_PLANE_XY;
+    if (plane < MBQC_PLANE_XY || plane > MBQC_PLANE_Z)
+      return fail(MBQC_ERR_VALUE, "measurement plane code out of range");
+    if (plane != MBQC_PLANE_XY) p->general_planes = true;
+    uint32_t flags = (uint32_t(q) << 8) | (uint32_t(plane) << 2);
     uint32_t nmask = 0;
     const int jn = k + ns;
     if (jn <= N - 1) {
@@ -423,6 +433,7 @@ int mbqc_plan_create(const mbqc_pattern_desc* desc, mbqc_plan** out) {
   }
   auto* p = new Plan();
   int rc = compile_plan(&d, p);
+  if (rc == MBQC_OK && g_plan_upload_hook) rc = g_plan_upload_hook(p);
   if (rc != MBQC_OK) {
     delete p;
     return rc;
@@ -454,6 +465,8 @@ int mbqc_plan_get_info(const mbqc_plan* plan, mbqc_plan_info* info) {
   info->min_window = p->min_window;
   info->can_sample = p->can_sample ? 1 : 0;
   info->periodic = p->periodic ? 1 : 0;
+  info->general_planes = p->general_planes ? 1 : 0;
+  info->shared_angles = p->shared_angles ? 1 : 0;
   return MBQC_OK;
 }
 
@@ -483,6 +496,6 @@ int mbqc_plan_get_rescale(const mbqc_plan* plan, int32_t* sv_flags, int32_t* dm_
 
 const char* mbqc_last_error(void) { return g_last_error.c_str(); }
 
-int32_t mbqc_abi_version(void) { return 1; }
+int32_t mbqc_abi_version(void) { return 2; }
 
 }  // extern "C"

@@ -45,6 +45,14 @@ struct Plan {
   // neighbours periodic_nmask(ns, k % ns), and n_steps % ns == 0: the forward
   // kernel can take its coefficients at compile time
   bool periodic = false;
+  // some angle-vector entry is used by more than one measured node: the
+  // forward kernels allow it, the adjoint entry points reject it (each writes
+  // one gradient entry per step)
+  bool shared_angles = false;
+  // some measured node is not in the XY plane (MBQC_PLANE_XZ / YZ / Z): only
+  // the density-matrix entry points run the plan, with per-step outcome
+  // probabilities
+  bool general_planes = false;
   // Pauli-frame tables of the sampled mode (empty when not available): per step
   // the 64-bit masks, relative to the step's schedule position, of the nodes
   // whose X (flow successor) / Z (its later neighbours) frame bit an outcome 1
@@ -59,6 +67,11 @@ struct Plan {
   void* dev_tab[kMaxDevices] = {};
   int n_blocks() const { return (n_steps + ns - 1) / ns; }
 };
+
+// Set by the device code (when linked in): uploads a new plan's tables to the
+// current device at creation, so that its first launch allocates nothing.
+// A build without device code leaves it null.
+extern int (*g_plan_upload_hook)(Plan*);
 
 // Device table layout (int32 words), built by pack_device_tables():
 //   [steps: 4*n_steps][in_index: H][init_sign: H][out_pos: H]

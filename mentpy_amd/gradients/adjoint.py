@@ -38,58 +38,67 @@ def _torch():
     return torch
 
 
+class AdjointState:
+    """What one forward pass leaves for its backward pass: the angles it used,
+    the batch shape and its own checkpoint buffer (so several forward passes
+    can be pending before their backward passes)."""
+
+    __slots__ = ("angles", "B", "C", "ckpt", "need")
+
+    def __init__(self, angles, B, C_, ckpt, need):
+        self.angles, self.B, self.C, self.ckpt, self.need = angles, B, C_, ckpt, need
+
+
 class AdjointEngine:
-    """Forward-with-checkpoints and backward sweep for one plan on one device."""
+    """Forward-with-checkpoints and backward sweep for one plan on one device.
+    The engine holds no per-call state: ``forward`` returns an
+    ``AdjointState`` that ``backward`` takes back."""
 
     def __init__(self, plan: nat.NativePlan, device="cuda"):
         torch = _torch()
         self.plan = plan
         self.device = torch.device(device)
-        self._ws = None
-        self._ws_cols = -1
 
-    def _workspace(self, n_cols: int):
-        torch = _torch()
-        need = int(nat.lib().mbqc_sv_adjoint_workspace_bytes(self.plan.handle, n_cols))
-        if self._ws is None or self._ws.numel() < need:
-            self._ws = torch.empty(max(need, 8), dtype=torch.uint8, device=self.device)
-        return self._ws, need
+    def workspace_bytes(self, n_cols: int) -> int:
+        return int(nat.lib().mbqc_sv_adjoint_workspace_bytes(self.plan.handle, n_cols))
 
     def forward(self, angles, cols):
         """``angles [B, n]`` (float32) and input columns ``cols [C, 2^|I|]``
-        shared by every parameter set -> raw outputs ``[B, C, 2^|O|]``
-        (simulation b*C + c uses angle vector b and column c)."""
+        shared by every parameter set -> ``(out, state)`` with raw outputs
+        ``out [B, C, 2^|O|]`` (simulation b*C + c uses angle vector b and
+        column c)."""
         torch = _torch()
         info = self.plan.info
         B, C_ = angles.shape[0], cols.shape[0]
         out = torch.empty((B, C_, 1 << info["n_outputs"]), dtype=torch.complex64, device=self.device)
-        ws, need = self._workspace(B * C_)
+        need = self.workspace_bytes(B * C_)
+        ckpt = torch.empty(max(need, 8), dtype=torch.uint8, device=self.device)
         stream = C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
         with torch.cuda.device(self.device):
             sb, sa = _angle_strides(angles)
             nat.check(nat.lib().mbqc_sv_adjoint_forward_c64(
                 self.plan.handle, C.c_void_p(angles.data_ptr()), sb, sa,
                 C.c_void_p(cols.data_ptr()), cols.shape[1], C.c_void_p(out.data_ptr()), B, C_,
-                C.c_void_p(ws.data_ptr()), need, stream))
-        self._B, self._C = B, C_
-        self._angles = angles
-        return out
+                C.c_void_p(ckpt.data_ptr()), need, stream))
+        return out, AdjointState(angles, B, C_, ckpt, need)
 
-    def backward(self, seed):
-        """``seed [B, C, 2^|O|]`` = dL/d(conj out) -> ``grad [B, n]`` (float32)."""
+    def backward(self, state: AdjointState, seed):
+        """``seed [B, C, 2^|O|]`` = dL/d(conj out) of the forward pass that
+        returned ``state`` -> ``grad [B, n]`` (float32)."""
         torch = _torch()
         info = self.plan.info
-        B, C_ = self._B, self._C
+        B, C_ = state.B, state.C
         seed = seed.to(torch.complex64).contiguous()
+        if tuple(seed.shape) != (B, C_, 1 << info["n_outputs"]):
+            raise ValueError(f"seed must have shape {(B, C_, 1 << info['n_outputs'])}")
         P = int(nat.lib().mbqc_sv_adjoint_partials(C_))
         part = torch.zeros((B, P, info["n_angles"]), dtype=torch.float32, device=self.device)
-        ws, need = self._workspace(B * C_)
         stream = C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
         with torch.cuda.device(self.device):
-            sb, sa = _angle_strides(self._angles)
+            sb, sa = _angle_strides(state.angles)
             nat.check(nat.lib().mbqc_sv_adjoint_backward_c64(
-                self.plan.handle, C.c_void_p(self._angles.data_ptr()), sb, sa,
-                C.c_void_p(seed.data_ptr()), B, C_, C.c_void_p(ws.data_ptr()), need,
+                self.plan.handle, C.c_void_p(state.angles.data_ptr()), sb, sa,
+                C.c_void_p(seed.data_ptr()), B, C_, C.c_void_p(state.ckpt.data_ptr()), state.need,
                 C.c_void_p(part.data_ptr()), stream))
         return part.sum(1) if P > 1 else part[:, 0]
 
@@ -105,7 +114,8 @@ def _engine_for(sim) -> AdjointEngine:
 
 class PatternSVFunction:
     """``torch.autograd.Function`` over the hip-sv adjoint (built lazily so the
-    module imports without torch)."""
+    module imports without torch).  Each application keeps its own angles and
+    checkpoints on its autograd context."""
 
     _cls = None
 
@@ -117,14 +127,18 @@ class PatternSVFunction:
             class _Fn(torch.autograd.Function):
                 @staticmethod
                 def forward(ctx, angles, cols, engine):
-                    out = engine.forward(angles.detach().to(torch.float32).contiguous(),
-                                         cols.detach().to(torch.complex64).contiguous())
+                    # a private copy: a later in-place change of the caller's
+                    # angles must not reach this call's backward pass
+                    a = angles.detach().to(torch.float32).contiguous().clone()
+                    out, state = engine.forward(a, cols.detach().to(torch.complex64).contiguous())
                     ctx.engine = engine
+                    ctx.state = state
                     return out
 
                 @staticmethod
                 def backward(ctx, grad_out):
-                    g = ctx.engine.backward(grad_out)
+                    g = ctx.engine.backward(ctx.state, grad_out)
+                    ctx.state = None
                     return g, None, None
 
             cls._cls = _Fn
@@ -178,10 +192,14 @@ class MBQMLStep:
         eng = self.engine
         dev = eng.device
         info = eng.plan.info
-        self.X = torch.as_tensor(inputs, device=dev).to(torch.complex64).contiguous()
-        self.Y = torch.as_tensor(targets, device=dev).to(torch.complex64).contiguous()
+        self.X = torch.as_tensor(inputs, device=dev).to(torch.complex64).contiguous().clone()
+        self.Y = torch.as_tensor(targets, device=dev).to(torch.complex64).contiguous().clone()
         S = self.X.shape[0]
         din, dout = 1 << info["n_inputs"], 1 << info["n_outputs"]
+        if self.X.dim() != 2 or self.X.shape[1] != din:
+            raise ValueError(f"inputs must have shape [S, {din}], got {tuple(self.X.shape)}")
+        if self.Y.dim() != 2 or tuple(self.Y.shape) != (S, dout):
+            raise ValueError(f"targets must have shape [{S}, {dout}], got {tuple(self.Y.shape)}")
         K = din * dout
         if mode == "auto":
             if S <= din:
@@ -199,9 +217,8 @@ class MBQMLStep:
         self.part = torch.zeros((self.B, self.P, info["n_angles"]), dtype=torch.float32, device=dev)
         self.loss = torch.empty(self.B, dtype=torch.float32, device=dev)
         if mode == "gram":
-            D = (self.X.unsqueeze(2) * self.Y.conj().unsqueeze(1)).reshape(S, K)
-            M = torch.matmul(D.conj().transpose(0, 1).contiguous(), D)
-            self.MT = M.transpose(0, 1).contiguous()               # Z = t @ M^T = (M t)^T
+            self.MT = torch.empty((K, K), dtype=torch.complex64, device=dev)
+            self._gram_matrix()
             self.Z = torch.empty((self.B, K), dtype=torch.complex64, device=dev)
             if gram_kernel == "auto":
                 gram_kernel = "fused" if (K <= 512 and self.B <= _FUSED_GRAM_MAX_SETS) else "blas"
@@ -224,7 +241,30 @@ class MBQMLStep:
                 raise ValueError(f"segments={self.segments} is out of range for this pattern")
             self._ws = torch.empty(self._need, dtype=torch.uint8, device=dev)
         else:
-            self._ws, self._need = eng._workspace(self.B * C_)
+            self._need = eng.workspace_bytes(self.B * C_)
+            self._ws = torch.empty(max(self._need, 8), dtype=torch.uint8, device=dev)
+
+    def _gram_matrix(self):
+        """M = D^H D of the current data, stored transposed (Z = t M^T = (M t)^T)."""
+        torch = _torch()
+        D = (self.X.unsqueeze(2) * self.Y.conj().unsqueeze(1)).reshape(self.S, self.K)
+        M = torch.matmul(D.conj().transpose(0, 1).contiguous(), D)
+        self.MT.copy_(M.transpose(0, 1))
+
+    def set_data(self, inputs, targets):
+        """Replace the data set (same shapes) in place, recomputing the Gram
+        matrix; a captured graph of this step then uses the new data."""
+        torch = _torch()
+        dev = self.engine.device
+        X = torch.as_tensor(inputs, device=dev).to(torch.complex64)
+        Y = torch.as_tensor(targets, device=dev).to(torch.complex64)
+        if X.shape != self.X.shape or Y.shape != self.Y.shape:
+            raise ValueError(f"data shapes {tuple(X.shape)}, {tuple(Y.shape)} differ from the "
+                             f"step's {tuple(self.X.shape)}, {tuple(self.Y.shape)}")
+        self.X.copy_(X)
+        self.Y.copy_(Y)
+        if self.mode == "gram":
+            self._gram_matrix()
 
     def _launch_forward(self):
         torch = _torch()
@@ -267,39 +307,45 @@ class MBQMLStep:
                               .reshape(self.angles.shape))
         with torch.cuda.device(self.engine.device):
             self._launch_forward()
-            S = self.S
-            if self.mode == "gram" and self.gram_kernel == "fused":
-                stream = C.c_void_p(torch.cuda.current_stream(self.engine.device).cuda_stream)
-                nat.check(nat.lib().mbqc_gram_loss_seed_c64(
-                    C.c_void_p(self.out.data_ptr()), C.c_void_p(self.MT.data_ptr()), self.B,
-                    self.K, S, C.c_void_p(self.loss.data_ptr()), C.c_void_p(self.seed.data_ptr()),
-                    C.c_void_p(self._gws.data_ptr()), stream))
-            elif self.mode == "gram":
-                t = self.out.reshape(self.B, self.K)
-                torch.matmul(t, self.MT, out=self.Z)
-                stream = C.c_void_p(torch.cuda.current_stream(self.engine.device).cuda_stream)
-                nat.check(nat.lib().mbqc_gram_finish_c64(
-                    C.c_void_p(t.data_ptr()), C.c_void_p(self.Z.data_ptr()), self.B, self.K, S,
-                    C.c_void_p(self.loss.data_ptr()), C.c_void_p(self.seed.data_ptr()), stream))
-            elif self.mode == "transfer":
-                Tm = self.out.transpose(1, 2)                                # [B, dout, din]
-                outs = torch.matmul(Tm, self.X.transpose(0, 1))             # [B, dout, S]
-                Phi = self.Y.transpose(0, 1)
-                f = (Phi.conj().unsqueeze(0) * outs).sum(1)                 # [B, S]
-                self.loss.copy_(1.0 - (f.abs() ** 2).mean(1))
-                if grad:
-                    G = torch.matmul(Phi.unsqueeze(0) * f.unsqueeze(1), self.X.conj())
-                    self.seed.copy_(G.transpose(1, 2) * (-2.0 / S))
-            else:
-                f = (self.Y.conj().unsqueeze(0) * self.out).sum(2)          # [B, S]
-                self.loss.copy_(1.0 - (f.abs() ** 2).mean(1))
-                if grad:
-                    torch.mul(f.unsqueeze(2) * self.Y.unsqueeze(0), -2.0 / S, out=self.seed)
+            self._loss_seed(grad)
             if not grad:
                 return self.loss, None
             self._launch_backward()
         g = self.part.sum(1) if self.P > 1 else self.part[:, 0]
         return self.loss, g
+
+    def _loss_seed(self, grad: bool = True):
+        """Loss (and, with ``grad``, the backward seed) from ``self.out``."""
+        torch = _torch()
+        S = self.S
+        if self.mode == "gram" and self.gram_kernel == "fused":
+            stream = C.c_void_p(torch.cuda.current_stream(self.engine.device).cuda_stream)
+            nat.check(nat.lib().mbqc_gram_loss_seed_c64(
+                C.c_void_p(self.out.data_ptr()), C.c_void_p(self.MT.data_ptr()), self.B,
+                self.K, S, C.c_void_p(self.loss.data_ptr()), C.c_void_p(self.seed.data_ptr()),
+                C.c_void_p(self._gws.data_ptr()), stream))
+        elif self.mode == "gram":
+            t = self.out.reshape(self.B, self.K)
+            torch.matmul(t, self.MT, out=self.Z)
+            stream = C.c_void_p(torch.cuda.current_stream(self.engine.device).cuda_stream)
+            nat.check(nat.lib().mbqc_gram_finish_c64(
+                C.c_void_p(t.data_ptr()), C.c_void_p(self.Z.data_ptr()), self.B, self.K, S,
+                C.c_void_p(self.loss.data_ptr()), C.c_void_p(self.seed.data_ptr()), stream))
+        elif self.mode == "transfer":
+            Tm = self.out.transpose(1, 2)                                # [B, dout, din]
+            outs = torch.matmul(Tm, self.X.transpose(0, 1))             # [B, dout, S]
+            Phi = self.Y.transpose(0, 1)
+            f = (Phi.conj().unsqueeze(0) * outs).sum(1)                 # [B, S]
+            self.loss.copy_(1.0 - (f.abs() ** 2).mean(1))
+            if grad:
+                G = torch.matmul(Phi.unsqueeze(0) * f.unsqueeze(1), self.X.conj())
+                self.seed.copy_(G.transpose(1, 2) * (-2.0 / S))
+        else:
+            stream = C.c_void_p(torch.cuda.current_stream(self.engine.device).cuda_stream)
+            nat.check(nat.lib().mbqc_overlap_loss_seed_c64(
+                C.c_void_p(self.out.data_ptr()), C.c_void_p(self.Y.data_ptr()), self.B, S,
+                self.Y.shape[1], C.c_void_p(self.loss.data_ptr()),
+                C.c_void_p(self.seed.data_ptr() if grad else 0), stream))
 
     def capture(self):
         """Capture one step (both kernels and the loss/seed GEMM) in a HIP graph."""
@@ -313,6 +359,24 @@ class MBQMLStep:
         return g
 
 
+def _data_token(x):
+    """What identifies a data argument between calls: a torch tensor by
+    object identity and version counter (in-place changes bump it), anything
+    else by a host copy of its values."""
+    torch = _torch()
+    if isinstance(x, torch.Tensor):
+        return ("tensor", x, x._version)
+    return ("array", np.array(x, copy=True))
+
+
+def _same_data(a, b) -> bool:
+    if a[0] != b[0]:
+        return False
+    if a[0] == "tensor":
+        return a[1] is b[1] and a[2] == b[2]
+    return a[1].shape == b[1].shape and a[1].dtype == b[1].dtype and np.array_equal(a[1], b[1])
+
+
 def mbqml_infidelity(sim, angles, inputs, targets, mode: str = "auto", grad: bool = True):
     """Loss ``L_b = 1 - mean_s |<target_s | out_{b,s}>|^2`` and ``dL_b/dtheta_b``.
 
@@ -320,33 +384,46 @@ def mbqml_infidelity(sim, angles, inputs, targets, mode: str = "auto", grad: boo
     ``(loss [B], grad [B, n])`` float32 device tensors (``grad`` None when
     ``grad=False``).  ``mode`` as in ``MBQMLStep`` (``"auto"`` picks
     ``"direct"`` when S <= 2^|I|, else the cheaper of ``"gram"`` and
-    ``"transfer"``).  Steps (with their Gram matrix) are cached per shape and
-    data set on the simulator.
+    ``"transfer"``).  Steps (buffers and Gram matrix) are cached per shape on
+    the simulator; a call whose data differ from the cached step's (another
+    tensor, an in-place change, other array values) loads them into the step
+    first.
     """
     torch = _torch()
     eng = _engine_for(sim)
     a = torch.as_tensor(angles, dtype=torch.float32, device=eng.device)
     if a.dim() == 1:
         a = a.unsqueeze(0)
+    tx, ty = _data_token(inputs), _data_token(targets)
     X = torch.as_tensor(inputs, device=eng.device)
     Y = torch.as_tensor(targets, device=eng.device)
     core = getattr(sim, "simulator", sim)
     cache = core.__dict__.setdefault("_mbqml_steps", {})
-    key = (a.shape[0], mode, X.data_ptr(), tuple(X.shape), Y.data_ptr(), tuple(Y.shape))
-    step = cache.get(key)
-    if step is None:
+    key = (a.shape[0], mode, tuple(X.shape), tuple(Y.shape))
+    entry = cache.get(key)
+    if entry is None:
         if len(cache) > 8:
             cache.clear()
-        step = MBQMLStep(sim, X, Y, a.shape[0], mode)
-        cache[key] = step
-    loss, g = step.run(a, grad=grad)
+        entry = [MBQMLStep(sim, X, Y, a.shape[0], mode), tx, ty]
+        cache[key] = entry
+    elif not (_same_data(entry[1], tx) and _same_data(entry[2], ty)):
+        entry[0].set_data(X, Y)
+        entry[1], entry[2] = tx, ty
+    loss, g = entry[0].run(a, grad=grad)
     return loss.clone(), (None if g is None else g.clone())
 
 
-def fd_loop_infidelity_grad(sim, angles, inputs, targets, h: float = 1e-3):
+def fd_loop_infidelity_grad(sim, angles, inputs, targets, h: float = 1e-3, mode: str = "direct"):
     """The plain GPU finite-difference loop: for every angle, two batched
-    hip-sv forward launches over all parameter sets and inputs (central
-    differences, as mentpy's ``fd_gradient`` with the cost on the GPU)."""
+    hip-sv forward launches over all parameter sets (central differences, as
+    mentpy's ``fd_gradient`` with the cost on the GPU), each followed by one
+    fused loss kernel.
+
+    ``mode="direct"`` propagates the S inputs of every parameter set (the
+    plain loop); ``mode="gram"`` propagates the 2^|I| basis columns and takes
+    the loss from the transfer matrix and the data Gram matrix, the same
+    reduction the adjoint's ``"gram"`` mode uses (a like-for-like baseline for
+    it)."""
     torch = _torch()
     from ..simulators.hip_simulator import sv_forward
     core = getattr(sim, "simulator", sim)
@@ -357,20 +434,35 @@ def fd_loop_infidelity_grad(sim, angles, inputs, targets, h: float = 1e-3):
     X = torch.as_tensor(inputs, device=dev).to(torch.complex64).contiguous()
     Y = torch.as_tensor(targets, device=dev).to(torch.complex64).contiguous()
     B, n = a.shape
-    out = torch.empty((B, X.shape[0], Y.shape[1]), dtype=torch.complex64, device=dev)
     grad = torch.empty((B, n), dtype=torch.float32, device=dev)
     work = a.clone()
+    if mode == "gram":
+        step = MBQMLStep(sim, X, Y, n_sets=B, mode="gram", segments=1)
+        cols = step.cols
 
-    def cost():
-        sv_forward(core.plan, work, X, ref_phase=False, renormalize=True, out=out)
-        f = (Y.conj().unsqueeze(0) * out).sum(2)
-        return 1.0 - (f.abs() ** 2).mean(1)
+        def cost():
+            sv_forward(core.plan, work, cols, ref_phase=False, renormalize=False, out=step.out)
+            step._loss_seed(grad=False)
+            return step.loss.clone()
+    elif mode == "direct":
+        out = torch.empty((B, X.shape[0], Y.shape[1]), dtype=torch.complex64, device=dev)
+        loss = torch.empty(B, dtype=torch.float32, device=dev)
 
-    for i in range(n):
-        work[:, i] = a[:, i] + h
-        cp = cost()
-        work[:, i] = a[:, i] - h
-        cm = cost()
-        work[:, i] = a[:, i]
-        grad[:, i] = (cp - cm) / (2 * h)
+        def cost():
+            sv_forward(core.plan, work, X, ref_phase=False, renormalize=True, out=out)
+            stream = C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+            nat.check(nat.lib().mbqc_overlap_loss_seed_c64(
+                C.c_void_p(out.data_ptr()), C.c_void_p(Y.data_ptr()), B, X.shape[0], Y.shape[1],
+                C.c_void_p(loss.data_ptr()), None, stream))
+            return loss.clone()
+    else:
+        raise ValueError("mode must be 'direct' or 'gram'")
+    with torch.cuda.device(dev):
+        for i in range(n):
+            work[:, i] = a[:, i] + h
+            cp = cost()
+            work[:, i] = a[:, i] - h
+            cm = cost()
+            work[:, i] = a[:, i]
+            grad[:, i] = (cp - cm) / (2 * h)
     return grad

@@ -35,13 +35,21 @@ def _torch():
 
 
 # ----------------------------------------------------------------- compile
+_PLANE_CODES = {"XY": nat.PLANE_XY, "X": nat.PLANE_XY, "Y": nat.PLANE_XY, "XZ": nat.PLANE_XZ,
+                "YZ": nat.PLANE_YZ, "Z": nat.PLANE_Z}
+
+
 def compile_pattern(mbqcircuit, schedule=None, window_size: int = 1, exclude: str = "output",
-                    xy_only_error=ValueError):
+                    xy_only_error=ValueError, all_planes: bool = False):
     """Flatten an ``MBQCircuit`` into the arrays of the C plan and compile it.
 
     ``exclude`` selects the unmeasured nodes: ``"output"`` (statevector
     backend, `np_simulator_sv.py:62-75`) or ``"quantum_output"`` (density
-    matrix backend, `np_simulator_dm.py:48-61`).  Returns
+    matrix backend, `np_simulator_dm.py:48-61`).  With ``all_planes`` the
+    XZ, YZ and Z planes are compiled too (density-matrix backend, which has
+    no plane check, `np_simulator_dm.py:33-115`); otherwise they raise
+    ``xy_only_error`` as the statevector backend does
+    (`np_simulator_sv.py:54-59`).  Returns
     ``(NativePlan, schedule, schedule_measure, window)``.
     """
     graph = mbqcircuit.graph
@@ -69,17 +77,26 @@ def compile_pattern(mbqcircuit, schedule=None, window_size: int = 1, exclude: st
     tpos = {v: k for k, v in enumerate(trainable)}
     angle_index = np.full(len(nodes), -1, np.int32)
     fixed = np.zeros(len(nodes), np.float32)
+    plane = np.zeros(len(nodes), np.int32)
     for v in sched_meas:
         m = mbqcircuit[v]
         if not isinstance(m, Ment):
             raise ValueError(f"Ment has no matrix representation at qubit {v}")
         if m.plane not in ("X", "Y", "XY"):
-            raise xy_only_error(
-                f"Node {v} has plane {m.plane}, but only XY plane is supported.")
+            if not all_planes:
+                raise xy_only_error(
+                    f"Node {v} has plane {m.plane}, but only XY plane is supported.")
+            if m.plane == "XYZ":
+                raise NotImplementedError(
+                    f"Node {v} is measured in the XYZ plane, which takes a pair of angles per node; "
+                    "the angle vectors of the hip backends carry one angle per node")
+        plane[idx[v]] = _PLANE_CODES[m.plane]
         if v in tpos:
             angle_index[idx[v]] = tpos[v]
-        else:
+        elif m.plane in ("X", "Y", "XY"):
             fixed[idx[v]] = m.xy_angle()
+        elif m.plane != "Z":
+            fixed[idx[v]] = float(m.angle)
     edges = np.array([(idx[a], idx[b]) for a, b in graph.edges()], np.int32).reshape(-1, 2)
     outputs = [v for v in schedule[n_meas:]]
     target_out = list(mbqcircuit.output_nodes if exclude == "output"
@@ -101,7 +118,7 @@ def compile_pattern(mbqcircuit, schedule=None, window_size: int = 1, exclude: st
         n_nodes=len(nodes), edges=edges, schedule=[idx[v] for v in schedule], n_measure=n_meas,
         window=int(window_size), input_nodes=[idx[v] for v in mbqcircuit.input_nodes],
         output_nodes=[idx[v] for v in target_out], angle_index=angle_index, fixed_angle=fixed,
-        n_angles=len(trainable), flow=flow)
+        n_angles=len(trainable), flow=flow, plane=plane if plane.any() else None)
     return plan, schedule, sched_meas, int(window_size)
 
 
@@ -216,11 +233,13 @@ def decode_outcomes(rows, n_steps: int):
 
 def dm_forward(plan: nat.NativePlan, angles, states, *, p_depol: float = 0.0,
                p_readout: float = 0.0, noise=None, renormalize: bool = True, out=None,
-               table_steps: bool = False):
+               table_steps: bool = False, step_probs: bool = False, outcomes=None):
     """Batched density-matrix forward -> ``[B, S, 2^|O|, 2^|O|]`` complex64.
     ``noise`` (a ``NoiseModel``) replaces ``p_depol`` / ``p_readout``;
     ``table_steps`` makes a periodic plan read its step flags from the tables
-    (same results)."""
+    (same results).  ``step_probs`` evaluates every step's outcome
+    probability (always done for plans with XZ / YZ / Z nodes); ``outcomes``
+    (a float32 ``[B, S, n_steps]`` device tensor) also receives the outcomes."""
     torch = _torch()
     device = states.device
     _require_gpu(device)
@@ -234,13 +253,22 @@ def dm_forward(plan: nat.NativePlan, angles, states, *, p_depol: float = 0.0,
     elif not isinstance(noise, nat.Noise):
         noise = noise.native()
     flags = ((nat.FLAG_RENORMALIZE if renormalize else 0)
-             | (nat.FLAG_TABLE_STEPS if table_steps else 0))
+             | (nat.FLAG_TABLE_STEPS if table_steps else 0)
+             | (nat.FLAG_STEP_PROBS if step_probs else 0))
     sb, sa = _angle_strides(angles)
     with torch.cuda.device(device):
-        nat.check(nat.lib().mbqc_dm_forward_c64(
-            plan.handle, C.c_void_p(angles.data_ptr()), sb, sa,
-            C.c_void_p(states.data_ptr()), states.shape[1], C.c_void_p(out.data_ptr()), B, S,
-            C.byref(noise), flags, _stream_ptr(device)))
+        if outcomes is not None:
+            if tuple(outcomes.shape) != (B, S, info["n_steps"]) or outcomes.dtype != torch.float32:
+                raise ValueError(f"outcomes must be a float32 tensor of shape {(B, S, info['n_steps'])}")
+            nat.check(nat.lib().mbqc_dm_forward_outcomes_c64(
+                plan.handle, C.c_void_p(angles.data_ptr()), sb, sa,
+                C.c_void_p(states.data_ptr()), states.shape[1], C.c_void_p(out.data_ptr()),
+                C.c_void_p(outcomes.data_ptr()), B, S, C.byref(noise), flags, _stream_ptr(device)))
+        else:
+            nat.check(nat.lib().mbqc_dm_forward_c64(
+                plan.handle, C.c_void_p(angles.data_ptr()), sb, sa,
+                C.c_void_p(states.data_ptr()), states.shape[1], C.c_void_p(out.data_ptr()), B, S,
+                C.byref(noise), flags, _stream_ptr(device)))
     return out
 
 
@@ -266,8 +294,15 @@ class _HipBackend(BaseSimulator):
             raise NotImplementedError("dev_mode scheduling is not supported by the hip backends.")
         self._check_planes(mbqcircuit)
         plan, schedule, sched_meas, window = compile_pattern(
-            mbqcircuit, self.schedule, self.window_size, self._exclude, self._plane_error)
+            mbqcircuit, self.schedule, self.window_size, self._exclude, self._plane_error,
+            all_planes=self._all_planes)
         self.plan = plan
+        torch = _torch()
+        dev = self.device
+        if dev.type == "cuda" and torch.cuda.is_available():
+            # tables on the backend's device now, so no launch allocates
+            with torch.cuda.device(dev):
+                plan.upload()
         self.schedule = schedule
         self.schedule_measure = sched_meas
         self.window_size = window
@@ -289,6 +324,7 @@ class _HipBackend(BaseSimulator):
 
     # hooks
     _plane_error = ValueError
+    _all_planes = False
     _can_sample = False
 
     def _backend_kwargs(self, kwargs):
@@ -334,17 +370,23 @@ class _HipBackend(BaseSimulator):
         node = self.schedule_measure[self.current_measurement]
         ment = self.mbqcircuit[node]
         ment.matrix(angle)                 # the reference's angle checks (ment.py:218-229)
+        self._check_step_mode(ment, kwargs.get("mode", "sample"))
         if ment.plane == "X":
             theta = 0.0
         elif ment.plane == "Y":
             theta = np.pi / 2
+        elif ment.plane == "Z":
+            theta = 0.0
         else:
             theta = float(angle if angle is not None else ment.angle)
         self._step_angles.append(theta)
         self.current_measurement += 1
-        self.qstate = self._window(len(self._step_angles))
-        self.outcomes[node] = 0
-        return self.qstate, 0
+        self.qstate, outcome = self._window(len(self._step_angles))
+        self.outcomes[node] = outcome
+        return self.qstate, outcome
+
+    def _check_step_mode(self, ment, mode):
+        pass
 
     def _window_args(self, k):
         torch = _torch()
@@ -448,7 +490,7 @@ class HipSimulatorSV(_HipBackend):
             nat.check(nat.lib().mbqc_sv_window_c64(
                 self.plan.handle, C.c_void_p(a.data_ptr()), 0, C.c_void_p(s.data_ptr()), 0, k,
                 C.c_void_p(out.data_ptr()), 1, 1, flags, _stream_ptr(s.device)))
-        return out[0, 0].cpu().numpy().astype(np.complex128)
+        return out[0, 0].cpu().numpy().astype(np.complex128), 0
 
     def run_batch(self, angles, input_states=None, *, phase: Optional[str] = None,
                   renormalize: Optional[bool] = None, want_phase: bool = False):
@@ -485,6 +527,7 @@ class HipSimulatorDM(_HipBackend):
 
     _exclude = "quantum_output"
     _plane_error = NotImplementedError
+    _all_planes = True
 
     def _backend_kwargs(self, kwargs):
         from ..noise import NoiseModel
@@ -508,12 +551,45 @@ class HipSimulatorDM(_HipBackend):
         self.noise = noise
         self.renormalize = kwargs.pop("renormalize", True)
 
+    @property
+    def _has_z(self):
+        return any(self.mbqcircuit[v].plane == "Z" for v in self.schedule_measure)
+
+    @property
+    def _exact_probs(self):
+        # Every outcome probability is 1/2 on a pattern with a causal flow and
+        # XY measurements; otherwise the reference's per-step probabilities
+        # (and its outcome-1 rule) are evaluated on the device.
+        return bool(self.plan.info["general_planes"]) or self.mbqcircuit.flow is None
+
+    def _check_mode(self, mode):
+        if self._has_z and mode not in ("expectation", "exp"):
+            raise NotImplementedError(
+                "Z-plane nodes: in mode 'sample' the reference draws their outcome with an unseeded "
+                "np.random.choice (np_simulator_dm.py:323-327); run them with mode='expectation', "
+                "which traces them out unprojected")
+
+    def _check_step_mode(self, ment, mode):
+        if ment.plane == "Z" and mode not in ("expectation", "exp"):
+            self._check_mode(mode)
+
     def run(self, angles, mode: str = "sample", input_state=None):
         self._check_run(angles, input_state)
+        self._check_mode(mode)
         a, s = self._device_inputs(angles)
-        rho = dm_forward(self.plan, a, s, noise=self.noise, renormalize=self.renormalize)
+        if self._exact_probs:
+            torch = _torch()
+            oc = torch.empty((1, 1, self.plan.info["n_steps"]), dtype=torch.float32, device=s.device)
+            rho = dm_forward(self.plan, a, s, noise=self.noise, renormalize=self.renormalize,
+                             outcomes=oc)
+            vals = oc[0, 0].cpu().numpy()
+            self._finish_run()
+            self.outcomes = {v: (float(o) if self.mbqcircuit[v].plane == "Z" else int(o))
+                             for v, o in zip(self.schedule_measure, vals)}
+        else:
+            rho = dm_forward(self.plan, a, s, noise=self.noise, renormalize=self.renormalize)
+            self._finish_run()
         out = rho[0, 0].cpu().numpy().astype(np.complex128)
-        self._finish_run()
         self.qstate = out
         return out
 
@@ -522,18 +598,32 @@ class HipSimulatorDM(_HipBackend):
         a, s, W = self._window_args(k)
         out = torch.empty((1, 1, 1 << W, 1 << W), dtype=torch.complex64, device=s.device)
         noise = self.noise.native()
+        oc = None
+        if self._exact_probs and k > 0:
+            oc = torch.empty((1, 1, k), dtype=torch.float32, device=s.device)
         with torch.cuda.device(s.device):
             nat.check(nat.lib().mbqc_dm_window_c64(
                 self.plan.handle, C.c_void_p(a.data_ptr()), 0, C.c_void_p(s.data_ptr()), 0, k,
-                C.c_void_p(out.data_ptr()), 1, 1, C.byref(noise), _stream_ptr(s.device)))
-        return out[0, 0].cpu().numpy().astype(np.complex128)
+                C.c_void_p(out.data_ptr()), 1, 1, C.byref(noise),
+                C.c_void_p(oc.data_ptr() if oc is not None else 0), _stream_ptr(s.device)))
+        outcome = 0
+        if oc is not None:
+            last = float(oc[0, 0, k - 1].item())
+            node = self.schedule_measure[k - 1]
+            outcome = last if self.mbqcircuit[node].plane == "Z" else int(last)
+        return out[0, 0].cpu().numpy().astype(np.complex128), outcome
 
-    def run_batch(self, angles, input_states=None, renormalize: Optional[bool] = None):
-        """``angles [B, n]`` x pure ``input_states [S, 2^|I|]`` -> ``[B, S, d, d]``."""
+    def run_batch(self, angles, input_states=None, renormalize: Optional[bool] = None,
+                  mode: str = "sample", outcomes=None):
+        """``angles [B, n]`` x pure ``input_states [S, 2^|I|]`` -> ``[B, S, d, d]``.
+        ``outcomes``: optional float32 ``[B, S, n_steps]`` device tensor that
+        receives every simulation's outcomes."""
+        self._check_mode(mode)
         dev = self.device
         _require_gpu(dev)
         a = _as_angles(angles, self.plan.info["n_angles"], dev)
         st = self._input_np if input_states is None else input_states
         s = _as_states(st, self.plan.info["n_inputs"], dev)
         rn = self.renormalize if renormalize is None else renormalize
-        return dm_forward(self.plan, a, s, noise=self.noise, renormalize=rn)
+        return dm_forward(self.plan, a, s, noise=self.noise, renormalize=rn,
+                          step_probs=self._exact_probs, outcomes=outcomes)

@@ -10,7 +10,12 @@ convention): the reference's convention multiplies it by
 ``prod_k sgn(cos(theta_k/2)) exp(i theta_k/2)`` (see ``closed_forms``).
 
 ``brute_dm`` does the same on density matrices with depolarizing noise on
-every qubit after all CZs and readout flips at the measurements.
+every qubit after all CZs and readout flips at the measurements.  It projects
+each node onto the +1 eigenvector of its ``Ment.matrix`` (any plane but XYZ,
+`mentpy/operators/ment.py:218-253`, from a numerical eigendecomposition, so it
+shares no closed form with the window oracles), takes outcome 1 when outcome
+0 has probability below 1e-4 and, with ``mode="expectation"``, traces Z-plane
+nodes out unprojected, as `mentpy/simulators/np_simulator_dm.py:307-346` does.
 Register size is 2^N (4^N for DM), so only patterns of up to ~14 (7) nodes.
 """
 from __future__ import annotations
@@ -99,7 +104,26 @@ def brute_sv_sampled(circ, angles, input_state, outcomes):
     return np.transpose(full, out_perm).reshape(-1), prob, fx, fz
 
 
-def brute_dm(circ, angles, input_state, p_depol=0.0, p_readout=0.0, kraus=None):
+def _ment_eigvecs(circ, node, angles):
+    """(+1, -1) eigenvectors of the node's measurement observable."""
+    m = circ[node]
+    trainable = list(circ.trainable_nodes)
+    th = float(angles[trainable.index(node)]) if node in trainable else m.angle
+    if m.plane == "XY":
+        M = np.cos(th) * X + np.sin(th) * Y
+    elif m.plane in ("X", "Y", "Z"):
+        M = {"X": X, "Y": Y, "Z": Z}[m.plane]
+    elif m.plane == "XZ":
+        M = np.cos(th) * X + np.sin(th) * Z
+    elif m.plane == "YZ":
+        M = np.cos(th) * Y + np.sin(th) * Z
+    else:
+        raise NotImplementedError(f"plane {m.plane}")
+    w, V = np.linalg.eigh(M)
+    return V[:, int(np.argmax(w))], V[:, int(np.argmin(w))]
+
+
+def brute_dm(circ, angles, input_state, p_depol=0.0, p_readout=0.0, kraus=None, mode="sample"):
     """Output density matrix (quantum_output_nodes order) with noise: the
     depolarizing channel ``p_depol`` and then the Kraus channel ``kraus`` (a list of
     2x2 operators) on every qubit after all CZs, readout flips at the measurements."""
@@ -114,15 +138,23 @@ def brute_dm(circ, angles, input_state, p_depol=0.0, p_readout=0.0, kraus=None):
     measured = [v for v in circ.measurement_order if v not in circ.quantum_output_nodes]
     live = list(nodes)
     for v in measured:
-        th = _node_angle(circ, v, angles)
         ax = live.index(v)
         n = len(live)
-        plus = np.array([1.0, np.exp(1j * th)]) / np.sqrt(2)
-        minus = np.array([1.0, -np.exp(1j * th)]) / np.sqrt(2)
+        tr_all = np.trace(rho.reshape(2 ** n, 2 ** n)).real
+        if circ[v].plane == "Z" and mode in ("expectation", "exp"):
+            rho = np.trace(rho, axis1=ax, axis2=n + ax) / tr_all
+            live.pop(ax)
+            continue
+        if circ[v].plane == "Z":
+            raise NotImplementedError("the reference draws Z-plane outcomes unseeded")
+        plus, minus = _ment_eigvecs(circ, v, angles)
+
         def proj(vec):
             r = np.tensordot(vec.conj(), rho, axes=([0], [ax]))          # row
             return np.tensordot(r, vec, axes=([n - 1 + ax], [0]))        # col (shifted)
         a, b = proj(plus), proj(minus)
+        if np.trace(a.reshape(2 ** (n - 1), 2 ** (n - 1))).real / tr_all < 1e-4:
+            a, b = b, a                                                  # outcome 1
         mix = (1 - p_readout) * a + p_readout * b
         tr = np.trace(mix.reshape(2 ** (n - 1), 2 ** (n - 1)))
         rho = mix / tr

@@ -8,7 +8,12 @@ Structure follows `mentpy/simulators/np_simulator_dm.py`:
   ``partial_trace_mixed_state`` (`state_ops.py:77-119`), append |+><+|,
   dense CZ conjugation with window neighbours.
 * ``measure_ment`` (:307-346): ``prob = Re tr(rho P_ext)``, outcome 0 unless
-  ``prob0 < 1e-4`` (then 1), ``rho <- P rho P^dag / prob``.
+  ``prob0 < 1e-4`` (then 1), ``rho <- P rho P^dag / prob``; any plane of
+  ``Ment.matrix`` but XYZ (`mentpy/operators/ment.py:218-253`).  A Z-plane
+  node in ``mode="expectation"`` is not projected: its outcome is
+  ``prob1 / (prob0 + prob1)`` and the qubit is traced out (:318-322, 339-340).
+  In ``mode="sample"`` the reference draws a Z outcome with an unseeded
+  ``np.random.choice`` (:323-327); here a draw needs an explicit ``rng``.
 * ``run`` (:218-283): schedule without ``quantum_output_nodes``, output
   reordered to ``quantum_output_nodes``.
 
@@ -116,7 +121,7 @@ class ReferenceDM:
         self.qstate = self.initial_czs @ rho @ self.initial_czs.conj().T
         self.outcomes = {}
 
-    def measure_ment(self, plane, stored, angle, i):
+    def measure_ment(self, plane, stored, angle, i, mode="sample", rng=None):
         n = self.current_number_simulated_nodes()
         rho = depolarize(self.qstate, i, n, self.p_depol)
         rho = apply_channel(rho, i, n, self.kraus)
@@ -124,18 +129,25 @@ class ReferenceDM:
         p0e, p1e = embed(p0, i, n), embed(p1, i, n)
         prob0 = np.real(np.trace(rho @ p0e))
         prob1 = np.real(np.trace(rho @ p1e))
-        outcome = 1 if prob0 < 1e-4 else 0
+        if plane == "Z":
+            if mode in ("expectation", "exp"):
+                return rho, prob1 / (prob0 + prob1)
+            if rng is None:
+                raise NotImplementedError("the reference draws Z-plane outcomes unseeded; pass rng")
+            outcome = int(rng.choice([0, 1], p=np.array([prob0, prob1]) / (prob0 + prob1)))
+        else:
+            outcome = 1 if prob0 < 1e-4 else 0
         pa, pb = (p0e, p1e) if outcome == 0 else (p1e, p0e)
         qa, qb = (prob0, prob1) if outcome == 0 else (prob1, prob0)
         pm = self.p_readout
         num = (1 - pm) * (pa @ rho @ pa.conj().T) + pm * (pb @ rho @ pb.conj().T)
         return num / ((1 - pm) * qa + pm * qb), outcome
 
-    def measure(self, angle):
+    def measure(self, angle, mode="sample", rng=None):
         if self.current_measurement >= len(self.schedule_measure):
             raise ValueError("No more measurements to be done.")
         m = self.circ[self.schedule_measure[self.current_measurement]]
-        self.qstate, outcome = self.measure_ment(m.plane, m.angle, angle, 0)
+        self.qstate, outcome = self.measure_ment(m.plane, m.angle, angle, 0, mode, rng)
         if np.isnan(self.qstate).any():
             raise ValueError("qstate has nan, you might want to increase the window size")
         self.current_measurement += 1
@@ -151,13 +163,13 @@ class ReferenceDM:
                     self.qstate = cz @ self.qstate @ cz.conj().T
         return self.qstate, outcome
 
-    def run(self, angles, mode="sample", input_state=None):
+    def run(self, angles, mode="sample", input_state=None, rng=None):
         if input_state is not None:
             self.reset(input_state)
         if len(angles) != len(self.circ.trainable_nodes):
             raise ValueError("Number of angles does not match number of trainable nodes.")
         for node in self.schedule_measure:
-            self.qstate, out = self.measure(_node_angle(self.circ, node, angles))
+            self.qstate, out = self.measure(_node_angle(self.circ, node, angles), mode, rng)
             self.outcomes[node] = out
         current = [i for i in self.schedule if i not in self.schedule_measure]
         n = len(current)

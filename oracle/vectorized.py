@@ -26,11 +26,13 @@ class _Setup:
         self.circ = circ
         trainable = list(circ.trainable_nodes)
         self.angle_src = []
+        self.planes = []        # "XY" (also X, Y), "XZ", "YZ" or "Z"
         for v in self.sched_meas:
+            m = circ[v]
+            self.planes.append("XY" if m.plane in ("X", "Y", "XY") else m.plane)
             if v in trainable:
                 self.angle_src.append(("t", trainable.index(v)))
             else:
-                m = circ[v]
                 a = 0.0 if m.plane == "X" else (np.pi / 2 if m.plane == "Y" else float(m.angle))
                 self.angle_src.append(("f", a))
 
@@ -122,11 +124,31 @@ def _kraus_axis(x, kraus, a, b):
     return out
 
 
+def _bloch(plane, th):
+    """Bloch vector (n_x, n_y, n_z) of the +1 outcome of ``Ment.matrix``
+    (`mentpy/operators/ment.py:218-253`) in a continuous plane; Z is (0, 0, 1)."""
+    c, s = np.cos(th), np.sin(th)
+    z = np.zeros_like(c)
+    if plane == "XY":
+        return c, s, z
+    if plane == "XZ":
+        return c, z, s
+    if plane == "YZ":
+        return z, c, s
+    if plane == "Z":
+        return z, z, np.ones_like(c)
+    raise NotImplementedError(f"plane {plane}")
+
+
 def dm_batch(circ, angles, states, window_size=1, schedule=None, p_depol=0.0, p_readout=0.0,
-             kraus=None):
+             kraus=None, mode="sample"):
     """``angles [B, n]``, pure ``states [S, 2^|I|]`` -> ``[B, S, d, d]`` with noise
     (depolarizing ``p_depol``, then the Kraus channel ``kraus`` if given, on
-    every qubit after its CZs; readout flip ``p_readout``)."""
+    every qubit after its CZs; readout flip ``p_readout``).  Every plane but
+    XYZ: the projector ``(I + n.sigma)/2`` of the node's Bloch vector, outcome
+    1 where outcome 0 has probability below 1e-4 (per simulation), Z-plane
+    nodes traced out unprojected with ``mode="expectation"``
+    (`mentpy/simulators/np_simulator_dm.py:307-346`)."""
     su = _Setup(circ, window_size, schedule, "quantum_output")
     angles = np.atleast_2d(np.asarray(angles, dtype=np.float64))
     states = np.atleast_2d(np.asarray(states, dtype=complex))
@@ -154,10 +176,23 @@ def dm_batch(circ, angles, states, window_size=1, schedule=None, p_depol=0.0, p_
             blk = np.stack([np.stack([B00, B01], 1), np.stack([B10, B11], 1)], 1)  # [N,2,2,h,h]
             blk = _kraus_axis(blk, kraus, 1, 2)
             B00, B01, B10, B11 = blk[:, 0, 0], blk[:, 0, 1], blk[:, 1, 0], blk[:, 1, 1]
-        e = np.exp(1j * th)[:, None, None]
-        plus_part = 0.5 * (B00 + B11 + e * B01 + np.conj(e) * B10)
-        minus_part = 0.5 * (B00 + B11 - e * B01 - np.conj(e) * B10)
-        num = (1 - pm) * plus_part + pm * minus_part
+        plane = su.planes[k]
+        if plane == "Z" and mode not in ("expectation", "exp"):
+            raise NotImplementedError("the reference draws Z-plane outcomes unseeded")
+        if plane == "Z":
+            num = B00 + B11
+        else:
+            nx, ny, nz = (a[:, None, None] for a in _bloch(plane, th))
+            e = nx + 1j * ny
+            # sum_ab P0[b, a] B_ab with P0 = (I + n.sigma)/2
+            plus_part = 0.5 * ((1 + nz) * B00 + (1 - nz) * B11 + e * B01 + np.conj(e) * B10)
+            minus_part = 0.5 * ((1 - nz) * B00 + (1 + nz) * B11 - e * B01 - np.conj(e) * B10)
+            tr_in = np.real(np.trace(B00 + B11, axis1=1, axis2=2))[:, None, None]
+            p0 = np.real(np.trace(plus_part, axis1=1, axis2=2))[:, None, None] / tr_in
+            one = p0 < 1e-4
+            plus_part, minus_part = (np.where(one, minus_part, plus_part),
+                                     np.where(one, plus_part, minus_part))
+            num = (1 - pm) * plus_part + pm * minus_part
         tr = np.real(np.trace(num, axis1=1, axis2=2))[:, None, None]
         rho = num / tr
         cm = k + 1

@@ -67,7 +67,10 @@ def test_adjoint_matches_exact_shift(rows, cols, S, mode, rng):
         assert err < 2e-4
 
 
-def test_adjoint_matches_gpu_fd_loop(rng):
+@pytest.mark.parametrize("fd_mode", ["direct", "gram"])
+def test_adjoint_matches_gpu_fd_loop(fd_mode, rng):
+    """Both GPU finite-difference loops (inputs propagated with the fused
+    overlap loss; basis columns with the Gram loss) agree with the adjoint."""
     torch = _torch()
     import mentpy_amd as mp
     from mentpy_amd.gradients import fd_loop_infidelity_grad, mbqml_infidelity
@@ -76,10 +79,38 @@ def test_adjoint_matches_gpu_fd_loop(rng):
     X, Y = _dataset(3, 24, rng)
     th = rng.uniform(0, 2 * np.pi, (8, len(g.trainable_nodes))).astype(np.float32)
     _, ga = mbqml_infidelity(sim, th, X, Y)
-    gf = fd_loop_infidelity_grad(sim, th, X, Y, h=1e-3)
+    gf = fd_loop_infidelity_grad(sim, th, X, Y, h=1e-3, mode=fd_mode)
     err = (ga - gf).abs().max().item()
-    print(f"adjoint vs GPU FD (h=1e-3): {err:.2e}", flush=True)
+    print(f"adjoint vs GPU FD ({fd_mode}, h=1e-3): {err:.2e}", flush=True)
     assert err < 5e-3
+
+
+def test_overlap_loss_kernel_matches_oracle(rng):
+    """The fused loss/seed kernel of the direct mode, for output sizes below,
+    at and above one wave per row, against a complex128 evaluation."""
+    torch = _torch()
+    import ctypes as C
+    from mentpy_amd import _native as nat
+    for B, S, O in ((3, 5, 1), (4, 37, 16), (2, 300, 64), (2, 9, 128)):
+        out = rng.normal(size=(B, S, O)) + 1j * rng.normal(size=(B, S, O))
+        Y = rng.normal(size=(S, O)) + 1j * rng.normal(size=(S, O))
+        od = torch.as_tensor(out, dtype=torch.complex64, device="cuda")
+        yd = torch.as_tensor(Y, dtype=torch.complex64, device="cuda")
+        loss = torch.empty(B, dtype=torch.float32, device="cuda")
+        seed = torch.empty_like(od)
+        nat.check(nat.lib().mbqc_overlap_loss_seed_c64(
+            C.c_void_p(od.data_ptr()), C.c_void_p(yd.data_ptr()), B, S, O,
+            C.c_void_p(loss.data_ptr()), C.c_void_p(seed.data_ptr()),
+            C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        torch.cuda.synchronize()
+        out32 = out.astype(np.complex64).astype(np.complex128)
+        Y32 = Y.astype(np.complex64).astype(np.complex128)
+        f = np.einsum("so,bso->bs", Y32.conj(), out32)
+        want_loss = 1 - (np.abs(f) ** 2).mean(1)
+        want_seed = -2.0 / S * f[:, :, None] * Y32[None]
+        scale = max(1.0, np.abs(want_loss).max())
+        assert np.abs(loss.cpu().numpy() - want_loss).max() < 1e-5 * scale * O, (B, S, O)
+        assert np.abs(seed.cpu().numpy() - want_seed).max() < 1e-5 * np.abs(want_seed).max(), (B, S, O)
 
 
 def test_adjoint_4x16_mbqml_vs_oracle_fd(rng):
@@ -158,6 +189,76 @@ def test_autograd_function_custom_loss(rng):
     err = np.abs(th.grad.cpu().numpy()[0] - ref).max()
     print(f"autograd custom loss grad err {err:.2e}", flush=True)
     assert err < 2e-4
+
+
+def test_two_pattern_sv_calls_before_one_backward(rng):
+    """Two applications summed into one loss before backward(): each keeps
+    its own angles and checkpoints, so both terms differentiate correctly."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from mentpy_amd.gradients import pattern_sv
+    from oracle.gradients import exact_shift_gradient
+    from oracle.vectorized import sv_batch
+    g = mp.templates.grid_cluster(3, 5)
+    sim = mp.PatternSimulator(g, backend="hip-sv")
+    n = len(g.trainable_nodes)
+    c1 = np.stack([haar_state(3, rng) for _ in range(2)])
+    c2 = np.stack([haar_state(3, rng) for _ in range(5)])
+    w1, w2 = rng.uniform(0.5, 1.5, 8), rng.uniform(0.5, 1.5, 8)
+    a1 = torch.tensor(rng.uniform(0, 2 * np.pi, (1, n)), dtype=torch.float32, device="cuda",
+                      requires_grad=True)
+    a2 = torch.tensor(rng.uniform(0, 2 * np.pi, (1, n)), dtype=torch.float32, device="cuda",
+                      requires_grad=True)
+    dev = lambda x: torch.as_tensor(x, dtype=torch.complex64, device="cuda")
+    o1 = pattern_sv(sim, a1, dev(c1))
+    o2 = pattern_sv(sim, a2, dev(c2))
+    wt1 = torch.as_tensor(w1, dtype=torch.float32, device="cuda")
+    wt2 = torch.as_tensor(w2, dtype=torch.float32, device="cuda")
+    loss = ((o1.abs() ** 2) * wt1).sum() + ((o2.abs() ** 2) * wt2).sum()
+    loss.backward()
+
+    def cost(x, cols, w):
+        o = sv_batch(g, x[None], cols)[0]
+        return float(((np.abs(o) ** 2) * w).sum())
+    for a, cols, w in ((a1, c1, w1), (a2, c2, w2)):
+        x0 = a.detach().cpu().numpy()[0].astype(np.float64)
+        ref = exact_shift_gradient(lambda x: cost(x, cols, w), x0)
+        err = np.abs(a.grad.cpu().numpy()[0] - ref).max()
+        print(f"two pending forwards: grad err {err:.2e}", flush=True)
+        assert err < 2e-4
+
+
+@pytest.mark.parametrize("mode", ["gram", "transfer", "direct"])
+def test_mbqml_infidelity_new_data_same_shape(mode, rng):
+    """Calls with different data of the same shape (fresh numpy arrays, and a
+    device tensor changed in place) use the new data, not a cached step's."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from mentpy_amd.gradients import mbqml_infidelity
+    from oracle.gradients import exact_shift_gradient
+    g = mp.templates.grid_cluster(2, 5)
+    sim = mp.PatternSimulator(g, backend="hip-sv")
+    th = rng.uniform(0, 2 * np.pi, (2, len(g.trainable_nodes))).astype(np.float32)
+    S = 3 if mode == "direct" else 12
+    sets = [_dataset(2, S, rng) for _ in range(3)]
+
+    def check(X, Y, loss, grad):
+        cost = _oracle_cost(g, X, Y)
+        t0 = th[1].astype(np.float64)
+        assert abs(loss[1].item() - cost(t0)) < 2e-5
+        err = np.abs(grad[1].cpu().numpy() - exact_shift_gradient(cost, t0)).max()
+        assert err < 2e-4, err
+    for X, Y in sets[:2]:                      # numpy arrays, new values each call
+        loss, grad = mbqml_infidelity(sim, th, X, Y, mode=mode)
+        check(X, Y, loss, grad)
+    Xd = torch.as_tensor(sets[0][0], dtype=torch.complex64, device="cuda")
+    Yd = torch.as_tensor(sets[0][1], dtype=torch.complex64, device="cuda")
+    loss, grad = mbqml_infidelity(sim, th, Xd, Yd, mode=mode)
+    check(*sets[0], loss, grad)
+    Xd.copy_(torch.as_tensor(sets[2][0], dtype=torch.complex64))   # in place: same tensor
+    Yd.copy_(torch.as_tensor(sets[2][1], dtype=torch.complex64))
+    loss, grad = mbqml_infidelity(sim, th, Xd, Yd, mode=mode)
+    check(*sets[2], loss, grad)
 
 
 def test_adam_with_adjoint_cost_decreases_loss(rng):

@@ -367,3 +367,33 @@ def test_empty_batches_return_empty_outputs():
     assert o.shape == (0, 3, 4) and rows.shape[:2] == (0, 3)
     assert sv.run_batch(np.zeros((0, len(g.trainable_nodes))), s.cpu().numpy()).shape == (0, 3, 4)
     torch.cuda.synchronize()
+
+
+def test_fresh_plan_first_launch_inside_graph_capture(rng):
+    """A plan's tables are on the device from its creation, so its very first
+    launch can be captured in a HIP graph (replayed several times, with eager
+    work in between, against the eager result)."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from mentpy_amd.simulators.hip_simulator import _as_angles, _as_states, sv_forward
+    g = mp.templates.grid_cluster(3, 9)
+    sim = mp.PatternSimulator(g, backend="hip-sv")
+    core = sim.simulator
+    th = _as_angles(rng.uniform(0, 2 * np.pi, (70, len(g.trainable_nodes))), core.plan.info["n_angles"],
+                    core.device)
+    st = _as_states(np.stack([haar_state(3, rng) for _ in range(2)]), 3, core.device)
+    out = torch.empty((70, 2, 8), dtype=torch.complex64, device="cuda")
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(graph):
+            sv_forward(core.plan, th, st, out=out)
+    torch.cuda.synchronize()
+    want = sv_forward(core.plan, th, st).clone()
+    for rep in range(3):
+        out.zero_()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, want), f"replay {rep} differs from the eager launch"
+        _ = (want * 2).sum().item()          # eager work between replays

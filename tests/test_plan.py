@@ -129,7 +129,7 @@ def test_library_exports_every_header_symbol(native):
     for nm in sorted(names):
         assert hasattr(L, nm), f"{nm} not exported"
     assert set(native.EXPORTED_SYMBOLS) == names
-    assert L.mbqc_abi_version() == 1
+    assert L.mbqc_abi_version() == 2
 
 
 def test_validation_errors_map_to_reference_exceptions(native):
@@ -149,8 +149,21 @@ def test_validation_errors_map_to_reference_exceptions(native):
     g2[1] = mp.Ment("XZ")
     with pytest.raises(ValueError, match="only XY plane"):
         mp.PatternSimulator(g2, backend="hip-sv")
-    with pytest.raises(NotImplementedError):
-        mp.PatternSimulator(g2, backend="hip-dm")
+    sim = mp.PatternSimulator(g2, backend="hip-dm")       # the DM backend has no plane check
+    assert sim.simulator.plan.info["general_planes"] == 1
+    L = native.lib()
+    rc = L.mbqc_sv_forward_c64(sim.simulator.plan.handle, None, 0, 1, None, 0, None, None, 1, 1, 0,
+                               None)
+    assert rc == native.MBQC_ERR_VALUE and "only XY plane" in L.mbqc_last_error().decode()
+    g3 = mp.templates.grid_cluster(2, 3)
+    with pytest.warns(UserWarning):
+        g3[1] = mp.Ment((0.1, 0.2), "XYZ")
+    with pytest.raises(NotImplementedError, match="pair of angles"):
+        mp.PatternSimulator(g3, backend="hip-dm")
+    g4 = mp.templates.grid_cluster(2, 3)
+    g4[1] = mp.Ment("Z")
+    with pytest.raises(NotImplementedError, match="expectation"):
+        mp.PatternSimulator(g4, backend="hip-dm").run([0.1] * len(g4.trainable_nodes))
     with pytest.raises(ValueError, match="Number of angles"):
         mp.PatternSimulator(g, backend="hip-sv").run([0.1])
 
@@ -168,6 +181,36 @@ def test_kernel_limits_are_reported(native):
     assert rc == native.MBQC_ERR_NOT_IMPLEMENTED
     with pytest.raises(NotImplementedError):
         compile_pattern(mp.templates.grid_cluster(13, 14))
+
+
+def test_adjoint_rejects_shared_angle_entries(native):
+    """An angle-vector entry used by two measured nodes runs in the forward
+    kernels, but the adjoint entry points (which write one gradient entry per
+    step) reject it before any device work."""
+    import numpy as np
+    g = mp.templates.grid_cluster(2, 4)
+    from mentpy_amd.simulators import compile_pattern
+    plan, sched, sched_meas, w = compile_pattern(g)
+    nodes = list(g.graph.nodes())
+    tr = list(g.trainable_nodes)
+    ai = np.full(len(nodes), -1, np.int32)
+    for v in sched_meas:
+        ai[nodes.index(v)] = tr.index(v) % 3          # three angles, shared
+    shared = native.NativePlan(
+        n_nodes=len(nodes), edges=[(nodes.index(a), nodes.index(b)) for a, b in g.graph.edges()],
+        schedule=[nodes.index(v) for v in sched], n_measure=len(sched_meas), window=w,
+        input_nodes=[nodes.index(v) for v in g.input_nodes],
+        output_nodes=[nodes.index(v) for v in g.output_nodes], angle_index=ai,
+        fixed_angle=np.zeros(len(nodes), np.float32), n_angles=3)
+    L = native.lib()
+    for p, want in ((shared, native.MBQC_ERR_NOT_IMPLEMENTED),):
+        rc = L.mbqc_sv_adjoint_forward_c64(p.handle, None, 0, 1, None, 0, None, 1, 1, None, 0, None)
+        assert rc == want
+        assert "at most one measured node" in L.mbqc_last_error().decode()
+        rc = L.mbqc_sv_adjoint_backward_c64(p.handle, None, 0, 1, None, 1, 1, None, 0, None, None)
+        assert rc == want
+        rc = L.mbqc_sv_segmented_forward_c64(p.handle, None, 0, 1, None, 1, 2, None, 0, None)
+        assert rc == want
 
 
 def test_circuit_noise_names_map_to_noise_model():
