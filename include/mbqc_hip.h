@@ -339,7 +339,7 @@ int mbqc_gram_loss_seed_c64(const void* t, const void* MT, int64_t B, int32_t K,
 int mbqc_overlap_loss_seed_c64(const void* out, const void* targets, int64_t B, int32_t S,
                                int32_t O, float* loss, void* seed, void* stream);
 
-/* Device-resident optimizer steps on n float32 parameters/* Device-resident optimizer steps on n float32 parameters (all pointers device
+/* Device-resident optimizer steps on n float32 parameters (all pointers device
  * memory, updated in place), replacing the host updates of
  * AdamOptimizer.step (mentpy/optimizers/adam.py:54-66) and SGDOptimizer.step
  * (mentpy/optimizers/sgd.py:49-59).  `step` is a device int32 holding the
