@@ -416,3 +416,36 @@ def test_haar_sampler_matches_scipy_unitary_group():
     g = np.random.default_rng(7)
     cols = np.stack([random_su(3, g)[:, 0] for _ in range(5)])
     assert np.array_equal(st, cols)
+
+
+def test_dense_gate_builders_follow_the_reference_conventions(rng):
+    """mentpy's dense builders (gates.py:62-143) on the MSB-first kron order:
+    the single-qubit embedding acts on the chosen tensor axis, swap_ij
+    exchanges two axes of a random state, controlled_z negates the amplitudes
+    with both bits set, and cnot_ij (1-based qubits) flips the target where
+    the control is set; each checked on random states by moving tensor axes."""
+    from mentpy_amd.operators import gates
+    n = 4
+    psi = rng.normal(size=2 ** n) + 1j * rng.normal(size=2 ** n)
+    T = psi.reshape([2] * n)
+    u = rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2))
+    for i in range(n):
+        want = np.moveaxis(np.tensordot(u, T, axes=([1], [i])), 0, i).reshape(-1)
+        assert np.allclose(gates.arbitrary_qubit_gate(u, i, n) @ psi, want)
+    for i in range(n):
+        for j in range(n):
+            if i == j:
+                continue
+            assert np.allclose(gates.swap_ij(i, j, n) @ psi, np.swapaxes(T, i, j).reshape(-1))
+            Z = T.copy()
+            sl = [slice(None)] * n
+            sl[i], sl[j] = 1, 1
+            Z[tuple(sl)] *= -1
+            assert np.allclose(gates.controlled_z(i, j, n) @ psi, Z.reshape(-1))
+            C_ = T.copy()
+            sl = [slice(None)] * n
+            sl[i] = 1
+            C_[tuple(sl)] = np.flip(T[tuple(sl)], axis=j - 1 if j > i else j)
+            assert np.allclose(gates.cnot_ij(i + 1, j + 1, n) @ psi, C_.reshape(-1))
+    assert np.array_equal(gates.cnot_ij(1, 2, 2), gates.CNOT)
+    assert np.array_equal(gates.swap_ij(0, 1, 2), gates.SWAP)
