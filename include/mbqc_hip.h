@@ -327,6 +327,18 @@ int mbqc_gram_finish_c64(const void* t, const void* z, int64_t B, int32_t K, int
                          void* seed, void* stream);
 int mbqc_gram_loss_seed_c64(const void* t, const void* MT, int64_t B, int32_t K, int32_t S,
                             float* loss, void* seed, float* workspace, void* stream);
+/* mbqc_gram_loss_seed_c64 without a workspace argument and without any
+ * allocation (no counterpart in the reference): for callers that cannot
+ * allocate device memory, such as code under HIP graph capture or a serving
+ * loop.  The loss partials (B x K/16 floats, B x ceil(K/64) when K is not a
+ * multiple of 16) go to a static 1 MiB device array (262,144 floats) of the
+ * library on each device, held until the process ends; larger B is
+ * MBQC_ERR_VALUE.  The first call on a device looks the array up (no
+ * allocation).  Every partial is written before it is read, so nothing is
+ * cleared between calls; calls that overlap on one device share the array and
+ * race. */
+int mbqc_gram_loss_seed_c64_noalloc(const void* t, const void* MT, int64_t B, int32_t K,
+                                    int32_t S, float* loss, void* seed, void* stream);
 
 /* MB-QML infidelity loss from propagated inputs (no reference counterpart as
  * one kernel: the reference evaluates calculator.fidelity per pair,

@@ -41,7 +41,8 @@ EXPORTED_SYMBOLS = (
     "mbqc_plan_get_rescale", "mbqc_sv_forward_c64", "mbqc_sv_sample_c64", "mbqc_dm_forward_c64",
     "mbqc_dm_forward_outcomes_c64",
     "mbqc_plan_window_qubits", "mbqc_sv_window_c64", "mbqc_dm_window_c64",
-    "mbqc_gram_workspace_bytes", "mbqc_gram_loss_seed_c64", "mbqc_gram_finish_c64",
+    "mbqc_gram_workspace_bytes", "mbqc_gram_loss_seed_c64", "mbqc_gram_loss_seed_c64_noalloc",
+    "mbqc_gram_finish_c64",
     "mbqc_overlap_loss_seed_c64",
     "mbqc_sv_adjoint_workspace_bytes", "mbqc_sv_adjoint_forward_c64",
     "mbqc_sv_adjoint_partials", "mbqc_sv_adjoint_backward_c64", "mbqc_sv_segments_auto",
@@ -126,6 +127,8 @@ def lib() -> C.CDLL:
     L.mbqc_gram_workspace_bytes.restype = sz
     L.mbqc_gram_loss_seed_c64.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp, vp]
     L.mbqc_gram_loss_seed_c64.restype = C.c_int
+    L.mbqc_gram_loss_seed_c64_noalloc.argtypes = [vp, vp, i64, i32, i32, vp, vp, vp]
+    L.mbqc_gram_loss_seed_c64_noalloc.restype = C.c_int
     L.mbqc_plan_window_qubits.argtypes = [vp, i32]
     L.mbqc_plan_window_qubits.restype = i32
     L.mbqc_sv_window_c64.argtypes = [vp, vp, i64, vp, i64, i32, vp, i64, i64, u32, vp]

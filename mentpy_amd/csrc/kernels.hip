@@ -2509,6 +2509,11 @@ __global__ void k_reset_counters(int* counter, int n) {
 
 // ==================================================================== C ABI
 using namespace mbqc;
+// Static device scratch of mbqc_gram_loss_seed_c64_noalloc (loss partials):
+// 1 MiB on every device the library runs on, held until the process ends.
+// Calls that overlap on one device share it and race.
+constexpr int64_t kScratchFloats = int64_t(1) << 18;
+static __device__ __align__(256) float scratch[kScratchFloats];
 
 
 namespace {
@@ -2980,6 +2985,11 @@ int dm_forward(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "dm_forward launch");
 }
 
+// Loss partials of the Gram loss: one per (parameter set, tile of indices n).
+int64_t gram_partials(int64_t B, int32_t K) {
+  return B * int64_t(K % 16 == 0 ? K / 16 : (K + 63) / 64);
+}
+
 int gram_loss_seed(const void* t, const void* MT, int64_t B, int32_t K, int32_t S, float* loss,
                    void* seed, float* workspace, void* stream) {
   if (!t || !MT || !loss || !seed) return fail(MBQC_ERR_VALUE, "null pointer");
@@ -3008,6 +3018,26 @@ int gram_loss_seed(const void* t, const void* MT, int64_t B, int32_t K, int32_t 
                      nblk, inv_S, loss);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "gram loss launch");
+}
+
+// Address of this device's copy of `scratch`, looked up once per device with
+// that device current (the array's name is only a host-side stand-in).
+int scratch_ptr(float** out) {
+  static std::mutex mu;
+  static float* ptr[kMaxDevices] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return fail(MBQC_ERR_RUNTIME, "device id out of range");
+  std::lock_guard<std::mutex> lk(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    ptr[dev] = static_cast<float*>(p);
+  }
+  *out = ptr[dev];
+  return MBQC_OK;
 }
 
 }  // namespace
@@ -3494,6 +3524,20 @@ int mbqc_gram_loss_seed_c64(const void* t, const void* MT, int64_t B, int32_t K,
                             float* loss, void* seed, float* workspace, void* stream) {
   if (!workspace) return fail(MBQC_ERR_VALUE, "null pointer");
   return gram_loss_seed(t, MT, B, K, S, loss, seed, workspace, stream);
+}
+
+int mbqc_gram_loss_seed_c64_noalloc(const void* t, const void* MT, int64_t B, int32_t K,
+                                    int32_t S, float* loss, void* seed, void* stream) {
+  if (B < 0 || K <= 0) return fail(MBQC_ERR_VALUE, "sizes must be B >= 0, K >= 1, S >= 1");
+  if (gram_partials(B, K) > kScratchFloats)
+    return fail(MBQC_ERR_VALUE, "B x " + std::to_string(gram_partials(1, K)) + " = " +
+                                    std::to_string(gram_partials(B, K)) +
+                                    " loss partials exceed the static scratch of " +
+                                    std::to_string(kScratchFloats) + " floats");
+  float* ws = nullptr;
+  int rc = scratch_ptr(&ws);
+  if (rc != MBQC_OK) return rc;
+  return gram_loss_seed(t, MT, B, K, S, loss, seed, ws, stream);
 }
 
 int mbqc_adam_step_f32(float* x, const float* grad, float* m, float* v, int64_t n, float lr, float b1,
