@@ -319,7 +319,19 @@ def config_dm(args, dev, stream):
 
     def run():
         dm_forward(plan, th.t(), st, p_depol=0.01, p_readout=0.005, out=out)
-    secs = time_launches(run, args.steps, args.warmup, stream)
+    # a 4096-simulation launch is shorter than one Python call of the wrapper:
+    # the K launches are captured in one HIP graph and replayed (the eager
+    # call loop is reported beside it)
+    eager_ms = time_launches(run, args.steps, args.warmup, stream) / args.steps * 1e3
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(graph):
+            for _ in range(args.steps):
+                run()
+    torch.cuda.synchronize()
+    secs = time_launches(graph.replay, 1, 2, stream)
     ms = secs / args.steps * 1e3
     k = min(B, 4096)
     ref = dm_batch(g, th.t()[:k].double().cpu().numpy(), psi[None], p_depol=0.01, p_readout=0.005)[:, 0]
@@ -333,6 +345,8 @@ def config_dm(args, dev, stream):
             "config": {"workload": "grid_cluster(3,12) hip-dm, depolarizing 0.01, readout flip 0.005",
                        "batch": B},
             "max_trace_distance_vs_c128": float(td.max()),
+            "timing": f"{args.steps} launches captured in one HIP graph, replayed",
+            "eager_ms_per_launch": eager_ms,
             "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": FP32_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
                          "traffic": None,
@@ -456,6 +470,11 @@ def main():
     ap.add_argument("--sample-batch", type=int, default=1 << 20)
     ap.add_argument("--no-grad", action="store_true", help="skip the adjoint section (profiling runs)")
     ap.add_argument("--dm-batch", type=int, default=4096)
+    ap.add_argument("--angles", default="per-sim", choices=["per-sim", "shared"],
+                    help="headline workload variant for attribution profiles: one angle vector "
+                         "per simulation (the benchmark) or one vector shared by all")
+    ap.add_argument("--phase", default="reference", choices=["reference", "raw"],
+                    help="output phase convention of the headline forward (reference: mentpy's)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 (simulations "
@@ -496,11 +515,14 @@ def main():
     angles_t[:, int(idx[0])] = pi32
     angles_t[::7, int(idx[1])] = pi32
     angles = angles_t.t()
+    if args.angles == "shared":
+        angles = angles_t[:, int(idx[2]):int(idx[2]) + 1].t().expand(B, n_ang)   # angle_stride 0
+    ref_phase = args.phase == "reference"
     states = torch.as_tensor(psi, dtype=torch.complex64, device=dev).reshape(1, -1)
     out = torch.empty((B, 1, 16), dtype=torch.complex64, device=dev)
 
     def fwd():
-        sv_forward(plan, angles, states, ref_phase=True, renormalize=True, out=out)
+        sv_forward(plan, angles, states, ref_phase=ref_phase, renormalize=True, out=out)
 
     secs = time_launches(fwd, args.steps, args.warmup, stream)
     ms = secs / args.steps * 1e3
@@ -524,16 +546,21 @@ def main():
     # c64 vs c128 error on a sample
     from oracle.vectorized import sv_batch
     a_s = angles[idx].double().cpu().numpy()
-    ref = sv_batch(circ, a_s, psi[None])[:, 0]
+    ref = sv_batch(circ, a_s, psi[None], phase=args.phase)[:, 0]
     got = out[idx, 0].cpu().numpy()
-    amp_err = float(np.abs(got - ref).max())
+    if ref_phase:
+        amp_err = float(np.abs(got - ref).max())
+    else:   # raw frame: the global phase is not fixed; compare after aligning it
+        ov = np.sum(np.conj(ref) * got, -1)
+        amp_err = float(np.abs(got - (ov / np.abs(ov))[:, None] * ref).max())
     fid = np.abs(np.sum(np.conj(ref) * got, -1)) ** 2
     fid_err = float(np.max(np.abs(1 - fid)))
 
     f_sim = flops_per_sim(plan.info["n_steps"], plan.info["window"])
     achieved = f_sim * B / kernel_s / 1e12
-    kernel = ("k_sv_forward_periodic<4,true>" if plan.info["periodic"]
-              else "k_sv_forward<4,true,true>")
+    r = "true" if ref_phase else "false"
+    kernel = (f"k_sv_forward_periodic<4,{r}>" if plan.info["periodic"]
+              else f"k_sv_forward<4,{r},true>")
     tps = profiled_traffic(kernel)
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
@@ -549,7 +576,10 @@ def main():
         "vs_baseline": None, "dtype": "c64", "data": "synthetic (seeded uniform angles, Haar input)",
         "config": {"workload": "grid_cluster(4,20) hip-sv forward, B angle vectors x 1 Haar input",
                    "batch": B, "n_angles": n_ang, "window": plan.info["window"],
-                   "angle_layout": "angle-major [n_angles, B]"},
+                   "angle_layout": "angle-major [n_angles, B]"
+                                   + ("" if args.angles == "per-sim" else
+                                      " (attribution variant: one vector shared by all)"),
+                   "phase": args.phase},
         "max_amp_err_vs_c128": amp_err, "max_infidelity_vs_c128": fid_err,
         "roofline": roofline,
         "single_sim_latency_us": lat_s * 1e6,

@@ -75,6 +75,16 @@ typedef struct mbqc_pattern_desc {
   const int32_t* plane;        /* optional [n_nodes]: MBQC_PLANE_* of each measured node
                                   (NULL: every node XY); a plan with another plane runs
                                   only in the density-matrix entry points */
+  const int32_t* measure_order;/* optional [n_measure]: the node measured at step k (NULL:
+                                  schedule[k]).  A permutation of schedule[0..n_measure)
+                                  in which step k's node has entered the window
+                                  (position <= k + window - 1): the order mentpy's
+                                  dev_mode scheduler picks (np_simulator_sv.py:176-197,
+                                  254-276).  Nodes still enter the window in schedule
+                                  order.  A plan whose measured slots leave the fixed
+                                  rotation (plan_info.rotating = 0) runs on the LDS
+                                  kernels; its adjoint, sampled and batched launches
+                                  are not available. */
 } mbqc_pattern_desc;
 
 typedef struct mbqc_plan mbqc_plan;
@@ -96,6 +106,9 @@ typedef struct mbqc_plan_info {
                              entry points only) */
   int32_t shared_angles;  /* 1: some angle-vector entry drives several nodes (the
                              adjoint entry points refuse the plan) */
+  int32_t rotating;       /* 1: step k measures slot (n_slots-1-k) mod n_slots (always
+                             so in schedule order); 0: a measure_order leaves that
+                             rotation */
 } mbqc_plan_info;
 
 /* Per-step table entry (16 bytes) as stored on the device. */
@@ -310,6 +323,17 @@ int64_t mbqc_batch_items(const mbqc_batch* batch);
  * in a hipGraph), `wg_waves` wavefronts per workgroup. */
 int mbqc_batch_launch(mbqc_batch* batch, int32_t waves, int32_t wg_waves, uint32_t flags,
                       void* stream);
+/* Work items of the group of live-window size n_slots (0..6; -1 for a null
+ * batch or n_slots out of range).  Each non-empty group runs its own
+ * persistent kernel, concurrently with the others. */
+int64_t mbqc_batch_group_items(const mbqc_batch* batch, int32_t n_slots);
+/* mbqc_batch_launch with the resident wavefront count chosen per group:
+ * group_waves[ns] (n_entries <= 7 entries, indexed by live-window size) for
+ * the group of size ns; 0 skips the group (its outputs are not written), a
+ * positive count must be at least wg_waves.  Sizing the groups by their
+ * work makes the concurrent kernels finish together. */
+int mbqc_batch_launch_groups(mbqc_batch* batch, const int32_t* group_waves, int32_t n_entries,
+                             int32_t wg_waves, uint32_t flags, void* stream);
 
 /* MB-QML infidelity loss and adjoint seed from the transfer matrices, fused
  * (the Gram form of 1 - mean_s |<phi_s|T psi_s>|^2; the reference evaluates

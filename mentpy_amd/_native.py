@@ -48,7 +48,8 @@ EXPORTED_SYMBOLS = (
     "mbqc_sv_adjoint_partials", "mbqc_sv_adjoint_backward_c64", "mbqc_sv_segments_auto",
     "mbqc_sv_segmented_workspace_bytes", "mbqc_sv_segmented_forward_c64",
     "mbqc_sv_segmented_backward_c64", "mbqc_batch_create",
-    "mbqc_batch_destroy", "mbqc_batch_items", "mbqc_batch_launch", "mbqc_adam_step_f32",
+    "mbqc_batch_destroy", "mbqc_batch_items", "mbqc_batch_launch", "mbqc_batch_group_items",
+    "mbqc_batch_launch_groups", "mbqc_adam_step_f32",
     "mbqc_sgd_step_f32", "mbqc_last_error", "mbqc_abi_version",
 )
 
@@ -61,7 +62,7 @@ class PatternDesc(C.Structure):
         ("n_outputs", C.c_int32), ("output_nodes", C.POINTER(C.c_int32)),
         ("angle_index", C.POINTER(C.c_int32)), ("fixed_angle", C.POINTER(C.c_float)),
         ("n_angles", C.c_int32), ("flow", C.POINTER(C.c_int32)),
-        ("plane", C.POINTER(C.c_int32)),
+        ("plane", C.POINTER(C.c_int32)), ("measure_order", C.POINTER(C.c_int32)),
     ]
 
 
@@ -69,7 +70,7 @@ class PlanInfo(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "n_slots", "lag", "n_steps", "n_inputs", "n_outputs", "n_angles", "n_nodes", "window",
         "n_appends", "n_dropped_edges", "min_window", "can_sample", "periodic", "general_planes",
-        "shared_angles")]
+        "shared_angles", "rotating")]
 
 
 class Step(C.Structure):
@@ -173,6 +174,10 @@ def lib() -> C.CDLL:
     L.mbqc_batch_items.restype = i64
     L.mbqc_batch_launch.argtypes = [vp, i32, i32, u32, vp]
     L.mbqc_batch_launch.restype = C.c_int
+    L.mbqc_batch_group_items.argtypes = [vp, i32]
+    L.mbqc_batch_group_items.restype = i64
+    L.mbqc_batch_launch_groups.argtypes = [vp, C.POINTER(i32), i32, i32, u32, vp]
+    L.mbqc_batch_launch_groups.restype = C.c_int
     f32 = C.c_float
     L.mbqc_adam_step_f32.argtypes = [vp, vp, vp, vp, i64, f32, f32, f32, f32, vp, vp]
     L.mbqc_adam_step_f32.restype = C.c_int
@@ -209,7 +214,7 @@ class NativePlan:
 
     def __init__(self, n_nodes: int, edges, schedule, n_measure: int, window: int,
                  input_nodes, output_nodes, angle_index, fixed_angle, n_angles: int,
-                 flow=None, plane=None):
+                 flow=None, plane=None, measure_order=None):
         L = lib()
         self._keep = dict(
             edges=_i32(np.asarray(edges, dtype=np.int32).reshape(-1, 2) if len(edges) else np.zeros((0, 2))),
@@ -219,6 +224,7 @@ class NativePlan:
             fixed=np.ascontiguousarray(np.asarray(fixed_angle, dtype=np.float32)),
             flow=None if flow is None else _i32(flow),
             plane=None if plane is None else _i32(plane),
+            order=None if measure_order is None else _i32(measure_order),
         )
         k = self._keep
         p32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
@@ -231,7 +237,8 @@ class NativePlan:
             angle_index=p32(k["angle_index"]),
             fixed_angle=k["fixed"].ctypes.data_as(C.POINTER(C.c_float)), n_angles=n_angles,
             flow=None if k["flow"] is None else p32(k["flow"]),
-            plane=None if k["plane"] is None else p32(k["plane"]))
+            plane=None if k["plane"] is None else p32(k["plane"]),
+            measure_order=None if k["order"] is None else p32(k["order"]))
         handle = C.c_void_p()
         check(L.mbqc_plan_create(C.byref(desc), C.byref(handle)))
         self.handle = handle

@@ -9,8 +9,10 @@ whose wavefronts pull work items from a device counter.  The launch (counter
 reset kernel + persistent kernel) can be captured in a HIP graph and replayed.
 
 ``autotune`` picks the resident wavefront count, wavefronts per workgroup and
-simulations per work item by timing candidate launches with HIP events, and
-caches the choice per (device, job signature) in a JSON file.
+simulations per work item by timing candidate launches with HIP events, then
+(with several live-window groups) wavefront counts per group in proportion to
+each group's solo time, and caches the choice per (device, job signature) in a
+JSON file.
 """
 from __future__ import annotations
 
@@ -25,6 +27,9 @@ from . import _native as nat
 from .simulators.hip_simulator import _angle_strides, _as_angles, _as_states, _require_gpu
 
 __all__ = ["BatchedForward", "autotune", "default_cache_path"]
+
+# live-window sizes 0..6 of the batched statevector kernel
+MAX_GROUPS = 7
 
 
 def _torch():
@@ -102,25 +107,49 @@ class BatchedForward:
                        for p, a, s, _ in self._jobs)
         return hashlib.sha1(repr(parts).encode()).hexdigest()[:16]
 
+    def groups(self) -> dict:
+        """Work items per live-window size (the groups that run as separate,
+        concurrent persistent kernels)."""
+        L = nat.lib()
+        out = {}
+        for ns in range(MAX_GROUPS):
+            n = int(L.mbqc_batch_group_items(self._handle, ns))
+            if n > 0:
+                out[ns] = n
+        return out
+
     def default_config(self):
         torch = _torch()
         cus = torch.cuda.get_device_properties(self.device).multi_processor_count
         return {"waves": cus * 4 * 4, "wg_waves": 4, "sims_per_item": self.sims_per_item}
 
     def launch(self, waves: Optional[int] = None, wg_waves: Optional[int] = None,
-               ref_phase: bool = True, renormalize: bool = True, table_steps: bool = False):
-        """One persistent launch over every job.  ``table_steps`` makes groups
-        of periodic plans read the pair coefficients from the tables (same
-        results)."""
+               group_waves: Optional[Sequence[int]] = None, ref_phase: bool = True,
+               renormalize: bool = True, table_steps: bool = False):
+        """One persistent launch over every job.  ``waves`` resident
+        wavefronts per live-window group, or ``group_waves[ns]`` for the group
+        of size ns (0 skips that group); without either, the tuned (or
+        default) configuration.  ``table_steps`` makes groups of periodic plans
+        read the pair coefficients from the tables (same results)."""
         torch = _torch()
         cfg = self.config or self.default_config()
-        waves = int(waves or cfg["waves"])
         wg = int(wg_waves or cfg["wg_waves"])
+        if group_waves is None and waves is None:
+            group_waves = cfg.get("group_waves")
         flags = ((nat.FLAG_REF_PHASE if ref_phase else 0) | (nat.FLAG_RENORMALIZE if renormalize else 0)
                  | (nat.FLAG_TABLE_STEPS if table_steps else 0))
         stream = C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
         with torch.cuda.device(self.device):
-            nat.check(nat.lib().mbqc_batch_launch(self._handle, waves, wg, flags, stream))
+            if group_waves is not None:
+                gw = [int(x) for x in group_waves]
+                if len(gw) > MAX_GROUPS:
+                    raise ValueError(f"group_waves has at most {MAX_GROUPS} entries")
+                arr = (C.c_int32 * len(gw))(*gw)
+                nat.check(nat.lib().mbqc_batch_launch_groups(self._handle, arr, len(gw), wg, flags,
+                                                             stream))
+            else:
+                waves = int(waves or cfg["waves"])
+                nat.check(nat.lib().mbqc_batch_launch(self._handle, waves, wg, flags, stream))
         return self.outputs
 
     def capture_graph(self, **launch_kwargs):
@@ -181,9 +210,32 @@ def autotune(batch: BatchedForward, cache: Optional[Path] = None, reps: int = 3,
             print(f"autotune {cfg}: {t:.3f} ms", flush=True)
         if t < best_t:
             best, best_t = dict(cfg), t
-    best["ms"] = best_t
     if best["sims_per_item"] != batch.sims_per_item:
         batch._build(best["sims_per_item"])
+    # second pass: with several window-size groups, size each group's
+    # resident waves by its solo time, so that the concurrent kernels finish
+    # together (equal counts leave the cheapest-per-wave group running alone)
+    groups = batch.groups()
+    if len(groups) > 1:
+        wg, W = best["wg_waves"], best["waves"]
+        total = W * len(groups)
+        solo = {}
+        for ns in groups:
+            gw = [0] * MAX_GROUPS
+            gw[ns] = total
+            solo[ns] = _time(lambda: batch.launch(wg_waves=wg, group_waves=gw), reps)
+        norm = sum(solo.values())
+        for f in (0.5, 0.75, 1.0, 1.5):
+            gw = [0] * MAX_GROUPS
+            for ns, t in solo.items():
+                gw[ns] = max(wg, int(round(f * total * t / norm / wg)) * wg)
+            t = _time(lambda: batch.launch(wg_waves=wg, group_waves=gw), reps)
+            if verbose:
+                print(f"autotune group_waves {gw}: {t:.3f} ms", flush=True)
+            if t < best_t:
+                best = {**best, "group_waves": gw}
+                best_t = t
+    best["ms"] = best_t
     batch.config = best
     table[key] = best
     try:

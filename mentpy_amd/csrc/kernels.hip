@@ -283,14 +283,21 @@ __device__ __forceinline__ void fetch_block_angles(const mbqc_step* steps, int k
                                                    const float* ang, int64_t sa, float (&th)[NS]) {
   // angles live in device memory: a global-typed pointer keeps the loads
   // global (a descriptor read from memory otherwise leaves them flat)
-  const __attribute__((address_space(1))) float* gang =
-      (const __attribute__((address_space(1))) float*)ang;
+  // Every step issues exactly one load, from its angle-vector entry or from
+  // its stored fixed angle (the address is selected, not the value): with a
+  // fixed number of loads in flight the wait before a block's math covers
+  // only that block's angles, not the prefetch issued for the next block.
+  using gfloat = const __attribute__((address_space(1))) float;
+  gfloat* gang = (gfloat*)ang;
+  int ai[NS];
+#pragma unroll
+  for (int sub = 0; sub < NS; ++sub)   // the step records first: one scalar wait
+    ai[sub] = steps[k0 + sub < n_steps ? k0 + sub : n_steps - 1].angle_index;
 #pragma unroll
   for (int sub = 0; sub < NS; ++sub) {
     const int k = k0 + sub < n_steps ? k0 + sub : n_steps - 1;
-    const int ai = steps[k].angle_index;
-    const float a = gang[int64_t(ai >= 0 ? ai : 0) * sa];
-    th[sub] = ai >= 0 ? a : steps[k].fixed_angle;
+    gfloat* src = ai[sub] >= 0 ? gang + int64_t(ai[sub]) * sa : (gfloat*)&steps[k].fixed_angle;
+    th[sub] = *src;
   }
 }
 
@@ -2592,6 +2599,9 @@ int check_adjoint_plan(const Plan* p) {
     return fail(MBQC_ERR_NOT_IMPLEMENTED,
                 "adjoint gradients need every angle-vector entry to belong to at most one "
                 "measured node");
+  if (!p->rotating)
+    return fail(MBQC_ERR_NOT_IMPLEMENTED,
+                "adjoint gradients need the nodes measured in schedule order");
   return MBQC_OK;
 }
 
@@ -2958,7 +2968,9 @@ int dm_forward(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
   if (J.total == 0) return MBQC_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
   hipError_t e = hipSuccess;
-  if (p->general_planes || (flags & MBQC_FLAG_STEP_PROBS)) {
+  // other planes, per-step probabilities or a measurement order off the
+  // slot rotation: the kernel that reads each step's slot and plane
+  if (p->general_planes || !p->rotating || (flags & MBQC_FLAG_STEP_PROBS)) {
     if (J.total > 0x7fffffff) return fail(MBQC_ERR_VALUE, "too many simulations for one launch");
     WindowMap none{};
     const size_t lds = sizeof(float2) << (2 * p->ns);
@@ -3064,6 +3076,64 @@ struct mbqc_batch {
   bool periodic[kMaxSvSlots + 1] = {};
 };
 
+// One launch of a batch: the counter reset, then one persistent kernel per
+// live-window size ns with group_waves[ns] > 0 resident wavefronts (capped at
+// the group's item count); with several groups each runs on its own stream,
+// forked from and joined into `stream`.  Groups given no waves are skipped
+// (their outputs are not written).
+static int batch_launch(mbqc_batch* bt, const int32_t* group_waves, int32_t wg_waves,
+                        uint32_t flags, void* stream) {
+  if (wg_waves < 1 || wg_waves > 4) return fail(MBQC_ERR_VALUE, "wg_waves must be 1..4");
+  int launched = 0;
+  for (int ns = 1; ns <= kMaxSvSlots; ++ns) {
+    if (group_waves[ns] == 0) continue;
+    if (group_waves[ns] < wg_waves) return fail(MBQC_ERR_VALUE, "waves must be at least wg_waves");
+    ++launched;
+  }
+  if (bt->items.empty() || launched == 0) return MBQC_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, bt->d_counter, kMaxSvSlots + 1);
+  const int threads = 64 * wg_waves;
+  const bool ref = flags & MBQC_FLAG_REF_PHASE;
+  const bool concurrent = bt->fork != nullptr && launched > 1;
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && concurrent) e = hipEventRecord(bt->fork, st);
+  // largest windows first
+  for (int ns = kMaxSvSlots; ns >= 1 && e == hipSuccess; --ns) {
+    const int64_t n = bt->group_count[ns];
+    if (n == 0 || group_waves[ns] == 0) continue;
+    hipStream_t gs = st;
+    if (concurrent) {
+      gs = bt->side[ns];
+      e = hipStreamWaitEvent(gs, bt->fork, 0);
+      if (e != hipSuccess) break;
+    }
+    // no more resident waves than the group has items
+    const int64_t wv =
+        std::min<int64_t>(group_waves[ns], ((n + wg_waves - 1) / wg_waves) * wg_waves);
+    const int blocks = int(wv / wg_waves);
+    const BatchItem* items = bt->d_items + bt->group_begin[ns];
+    int* counter = bt->d_counter + ns * kCounterStride;
+    const BatchGroupLaunch L{dim3(blocks), dim3(threads), gs, bt->d_jobs, items, n, counter, flags};
+    const bool sh = bt->share[ns];
+    const bool pe = bt->periodic[ns] && !(flags & MBQC_FLAG_TABLE_STEPS);
+    switch (ns) {
+      case 1: launch_batch_group<1>(L, ref, false, pe); break;
+      case 2: launch_batch_group<2>(L, ref, sh, pe); break;
+      case 3: launch_batch_group<3>(L, ref, sh, pe); break;
+      case 4: launch_batch_group<4>(L, ref, sh, pe); break;
+      case 5: launch_batch_group<5>(L, ref, sh, pe); break;
+      default: launch_batch_group<6>(L, ref, sh, pe); break;
+    }
+    e = hipGetLastError();
+    if (e == hipSuccess && concurrent) {
+      e = hipEventRecord(bt->join[ns], gs);
+      if (e == hipSuccess) e = hipStreamWaitEvent(st, bt->join[ns], 0);
+    }
+  }
+  return e == hipSuccess ? MBQC_OK : hip_fail(e, "batch launch");
+}
+
 extern "C" {
 
 int mbqc_plan_upload(const mbqc_plan* plan) {
@@ -3100,6 +3170,24 @@ int mbqc_sv_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
   hipStream_t st = static_cast<hipStream_t>(stream);
   hipError_t e = hipSuccess;
   const bool periodic = p->periodic && !(flags & MBQC_FLAG_TABLE_STEPS);
+  if (!p->rotating) {
+    // measured slots off the rotation: the LDS kernel reads them per step
+    switch (p->ns) {
+      case 1: e = launch_sv_lds<1>(J, flags, st); break;
+      case 2: e = launch_sv_lds<2>(J, flags, st); break;
+      case 3: e = launch_sv_lds<3>(J, flags, st); break;
+      case 4: e = launch_sv_lds<4>(J, flags, st); break;
+      case 5: e = launch_sv_lds<5>(J, flags, st); break;
+      case 6: e = launch_sv_lds<6>(J, flags, st); break;
+      case 7: e = launch_sv_lds<7>(J, flags, st); break;
+      case 8: e = launch_sv_lds<8>(J, flags, st); break;
+      case 9: e = launch_sv_lds<9>(J, flags, st); break;
+      case 10: e = launch_sv_lds<10>(J, flags, st); break;
+      case 11: e = launch_sv_lds<11>(J, flags, st); break;
+      case 12: e = launch_sv_lds<12>(J, flags, st); break;
+    }
+    return e == hipSuccess ? MBQC_OK : hip_fail(e, "sv_forward launch");
+  }
   switch (p->ns) {
     case 1: e = launch_sv<1>(J, p->coef_per_step, periodic, flags, st); break;
     case 2: e = launch_sv<2>(J, p->coef_per_step, periodic, flags, st); break;
@@ -3409,6 +3497,11 @@ int mbqc_batch_create(const mbqc_batch_job* jobs, int32_t n_jobs, int32_t sims_p
       delete bt;
       return rc0;
     }
+    if (!p->rotating) {
+      delete bt;
+      return fail(MBQC_ERR_NOT_IMPLEMENTED,
+                  "batched launches need the nodes measured in schedule order");
+    }
     Job J;
     int rc = make_job(p, bj.angles, bj.angle_stride, bj.angle_stride_a, bj.psi_in, bj.in_stride,
                       bj.out, bj.B, bj.S, &J);
@@ -3568,50 +3661,29 @@ int mbqc_sgd_step_f32(float* x, const float* grad, float* v, int64_t n, float lr
 
 int mbqc_batch_launch(mbqc_batch* bt, int32_t waves, int32_t wg_waves, uint32_t flags, void* stream) {
   if (!bt) return fail(MBQC_ERR_VALUE, "null batch");
-  if (wg_waves < 1 || wg_waves > 4) return fail(MBQC_ERR_VALUE, "wg_waves must be 1..4");
-  if (waves < wg_waves) return fail(MBQC_ERR_VALUE, "waves must be at least wg_waves");
-  if (bt->items.empty()) return MBQC_OK;
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, bt->d_counter, kMaxSvSlots + 1);
-  const int threads = 64 * wg_waves;
-  const bool ref = flags & MBQC_FLAG_REF_PHASE;
-  const bool concurrent = bt->fork != nullptr;
-  hipError_t e = hipSuccess;
-  if (concurrent) e = hipEventRecord(bt->fork, st);
-  // one persistent launch per live-window size, largest windows first; with
-  // several sizes each runs on its own stream, forked from and joined into st
-  for (int ns = kMaxSvSlots; ns >= 1 && e == hipSuccess; --ns) {
-    const int64_t n = bt->group_count[ns];
-    if (n == 0) continue;
-    hipStream_t gs = st;
-    if (concurrent) {
-      gs = bt->side[ns];
-      e = hipStreamWaitEvent(gs, bt->fork, 0);
-      if (e != hipSuccess) break;
-    }
-    // no more resident waves than the group has items
-    const int64_t wv = std::min<int64_t>(waves, ((n + wg_waves - 1) / wg_waves) * wg_waves);
-    const int blocks = int(wv / wg_waves);
-    const BatchItem* items = bt->d_items + bt->group_begin[ns];
-    int* counter = bt->d_counter + ns * kCounterStride;
-    const BatchGroupLaunch L{dim3(blocks), dim3(threads), gs, bt->d_jobs, items, n, counter, flags};
-    const bool sh = bt->share[ns];
-    const bool pe = bt->periodic[ns] && !(flags & MBQC_FLAG_TABLE_STEPS);
-    switch (ns) {
-      case 1: launch_batch_group<1>(L, ref, false, pe); break;
-      case 2: launch_batch_group<2>(L, ref, sh, pe); break;
-      case 3: launch_batch_group<3>(L, ref, sh, pe); break;
-      case 4: launch_batch_group<4>(L, ref, sh, pe); break;
-      case 5: launch_batch_group<5>(L, ref, sh, pe); break;
-      default: launch_batch_group<6>(L, ref, sh, pe); break;
-    }
-    e = hipGetLastError();
-    if (e == hipSuccess && concurrent) {
-      e = hipEventRecord(bt->join[ns], gs);
-      if (e == hipSuccess) e = hipStreamWaitEvent(st, bt->join[ns], 0);
-    }
+  int32_t gw[kMaxSvSlots + 1];
+  for (int ns = 0; ns <= kMaxSvSlots; ++ns) gw[ns] = bt->group_count[ns] > 0 ? waves : 0;
+  return batch_launch(bt, gw, wg_waves, flags, stream);
+}
+
+int mbqc_batch_launch_groups(mbqc_batch* bt, const int32_t* group_waves, int32_t n_entries,
+                             int32_t wg_waves, uint32_t flags, void* stream) {
+  if (!bt) return fail(MBQC_ERR_VALUE, "null batch");
+  if (!group_waves) return fail(MBQC_ERR_VALUE, "group_waves is null");
+  if (n_entries < 1 || n_entries > kMaxSvSlots + 1)
+    return fail(MBQC_ERR_VALUE, "group_waves must have 1.." + std::to_string(kMaxSvSlots + 1) +
+                                    " entries");
+  int32_t gw[kMaxSvSlots + 1] = {};
+  for (int ns = 0; ns < n_entries; ++ns) {
+    if (group_waves[ns] < 0) return fail(MBQC_ERR_VALUE, "negative wave count");
+    gw[ns] = bt->group_count[ns] > 0 ? group_waves[ns] : 0;
   }
-  return e == hipSuccess ? MBQC_OK : hip_fail(e, "batch launch");
+  return batch_launch(bt, gw, wg_waves, flags, stream);
+}
+
+int64_t mbqc_batch_group_items(const mbqc_batch* bt, int32_t n_slots) {
+  if (!bt || n_slots < 0 || n_slots > kMaxSvSlots) return -1;
+  return bt->group_count[n_slots];
 }
 
 }  // extern "C"

@@ -124,7 +124,29 @@ int compile_plan(const mbqc_pattern_desc* d, Plan* p) {
 
   p->schedule.assign(d->schedule, d->schedule + N);
   p->nbr = nbr;
-  const int ns = (w >= 2 && nin <= w - 1 && nout <= w - 1) ? w - 1 : w;
+  // measurement order: schedule order, or the description's, which must
+  // measure each of schedule[0..nm) once, after it has entered the window
+  std::vector<int> order(d->schedule, d->schedule + nm);
+  if (d->measure_order) {
+    std::vector<char> seen(N, 0);
+    for (int k = 0; k < nm; ++k) {
+      const int v = d->measure_order[k];
+      if (v < 0 || v >= N || pos[v] >= nm)
+        return fail(MBQC_ERR_VALUE, "measure_order names a node that is not measured");
+      if (seen[v]) return fail(MBQC_ERR_VALUE, "measure_order repeats a node");
+      seen[v] = 1;
+      if (pos[v] > k + w - 1)
+        return fail(MBQC_ERR_VALUE, "measure_order[" + std::to_string(k) + "] = " +
+                                        std::to_string(v) + " has not entered the window yet");
+      order[k] = v;
+    }
+  }
+  p->order = order;
+  // With lag 1 the node that enters the window with step k (schedule[k+w-1])
+  // is appended by step k itself, so no step may measure it right away.
+  bool lag_ok = true;
+  for (int k = 0; k < nm; ++k) lag_ok = lag_ok && pos[order[k]] != k + w - 1;
+  const int ns = (w >= 2 && nin <= w - 1 && nout <= w - 1 && lag_ok) ? w - 1 : w;
   if (ns > kMaxSlots)
     return fail(MBQC_ERR_NOT_IMPLEMENTED, "live window of " + std::to_string(ns) +
                                               " qubits exceeds the supported maximum of " +
@@ -166,10 +188,18 @@ int compile_plan(const mbqc_pattern_desc* d, Plan* p) {
   for (int a = 0; a < ns; ++a)
     for (int b = a + 1; b < ns; ++b) applied += has_edge(d->schedule[a], d->schedule[b]);
   std::vector<char> angle_used(d->n_angles > 0 ? d->n_angles : 0, 0);
+  // register slot of each live node: the initial ones by position, then each
+  // appended node takes the slot its step's measured node frees
+  std::vector<int> slot(N, -1);
+  for (int j = 0; j < ns && j < N; ++j) slot[d->schedule[j]] = slot_of_pos(j);
+  p->rotating = true;
   for (int k = 0; k < nm; ++k) {
     mbqc_step st{};
-    const int v = d->schedule[k];
-    const int q = slot_of_pos(k);
+    const int v = order[k];
+    const int q = slot[v];
+    if (q < 0) return fail(MBQC_ERR_VALUE, "measured node is not live");
+    slot[v] = -1;
+    p->rotating = p->rotating && q == slot_of_pos(k);
     int ai = d->angle_index[v];
     if (ai >= d->n_angles) return fail(MBQC_ERR_VALUE, "angle index out of range");
     if (ai >= 0) {
@@ -193,11 +223,12 @@ int compile_plan(const mbqc_pattern_desc* d, Plan* p) {
         flags |= 2u;
         ++applied;
       }
-      for (int j = k + 1; j <= k + ns - 1; ++j)
-        if (has_edge(nv, d->schedule[j])) {
-          nmask |= 1u << slot_of_pos(j);
+      for (int u : nbr[nv])
+        if (slot[u] >= 0) {   // live, the measured node excluded
+          nmask |= 1u << slot[u];
           ++applied;
         }
+      slot[nv] = q;
     }
     st.flags = flags;
     st.nmask = nmask;
@@ -221,7 +252,8 @@ int compile_plan(const mbqc_pattern_desc* d, Plan* p) {
   std::vector<int> live_slot(nout);
   uint32_t live_mask = 0;
   for (int m = 0; m < nout; ++m) {
-    live_slot[m] = slot_of_pos(pos[d->output_nodes[m]]);
+    live_slot[m] = slot[d->output_nodes[m]];
+    if (live_slot[m] < 0) return fail(MBQC_ERR_VALUE, "output node is not live at the end");
     live_mask |= 1u << live_slot[m];
   }
   for (int j = 0; j < H; ++j) {
@@ -269,7 +301,7 @@ int compile_plan(const mbqc_pattern_desc* d, Plan* p) {
       st.flags = (st.flags & 0xffffu) | (r << 16);
     }
   }
-  p->periodic = ns <= kMaxCoefSlots && nm > 0 && nm % ns == 0;
+  p->periodic = p->rotating && ns <= kMaxCoefSlots && nm > 0 && nm % ns == 0;
   for (int k = 0; k < nm && p->periodic; ++k) {
     const mbqc_step& st = p->steps[k];
     p->periodic = (st.flags & 3u) == 3u && st.nmask == periodic_nmask(ns, k % ns);
@@ -283,6 +315,8 @@ int compile_plan(const mbqc_pattern_desc* d, Plan* p) {
   p->can_sample = false;
   if (!d->flow) {
     p->sample_reason = "the pattern description has no flow";
+  } else if (order != std::vector<int>(d->schedule, d->schedule + nm)) {
+    p->sample_reason = "nodes are measured out of schedule order";
   } else if (p->n_dropped > 0) {
     p->sample_reason = "the window drops CZ edges";
   } else if (ns > kMaxCoefSlots) {
@@ -379,19 +413,29 @@ int window_map(const Plan& p, int k, WindowMap* m) {
   if (W > kMaxSlots + 1) return fail(MBQC_ERR_NOT_IMPLEMENTED, "window too large");
   m->W = W;
   m->nb = 0;
-  // live nodes are schedule[k .. k+ns-1] (fewer at the end of the schedule);
-  // with lag 1 the reference window holds one more node, already appended
-  const int n_live = std::min(ns, N - k);
-  for (int i = 0; i < W; ++i) {
-    if (i < n_live) {
-      m->slot[i] = ((ns - 1 - (k + i)) % ns + ns) % ns;
-    } else {
-      m->slot[i] = -1;
-      const int v = p.schedule[k + i];
-      for (int x : p.nbr[v])
-        for (int t = 0; t < n_live; ++t)
-          if (p.schedule[k + t] == x) m->nb |= 1u << (W - 1 - t);
-    }
+  // replay the slot assignment of the first k steps
+  std::vector<int> slot(N, -1);
+  for (int j = 0; j < ns && j < N; ++j) slot[p.schedule[j]] = ((ns - 1 - j) % ns + ns) % ns;
+  for (int t = 0; t < k; ++t) {
+    const int q = slot[p.order[t]];
+    slot[p.order[t]] = -1;
+    if (t + ns <= N - 1) slot[p.schedule[t + ns]] = q;
+  }
+  // the reference window lists its nodes in the order they entered: the live
+  // ones in schedule order, then (lag 1) the node already appended as |+>
+  std::vector<int> live;
+  for (int j = 0; j < N; ++j)
+    if (slot[p.schedule[j]] >= 0) live.push_back(p.schedule[j]);
+  const int n_live = int(live.size());
+  const bool pending = p.lag == 1 && k + ns <= N - 1;
+  if (n_live + (pending ? 1 : 0) != W) return fail(MBQC_ERR_RUNTIME, "window bookkeeping mismatch");
+  for (int i = 0; i < n_live; ++i) m->slot[i] = slot[live[i]];
+  if (pending) {
+    m->slot[n_live] = -1;
+    const int v = p.schedule[k + ns];
+    for (int x : p.nbr[v])
+      for (int t = 0; t < n_live; ++t)
+        if (live[t] == x) m->nb |= 1u << (W - 1 - t);
   }
   for (int i = W; i < kMaxSlots + 1; ++i) m->slot[i] = -1;
   return MBQC_OK;
@@ -467,6 +511,7 @@ int mbqc_plan_get_info(const mbqc_plan* plan, mbqc_plan_info* info) {
   info->periodic = p->periodic ? 1 : 0;
   info->general_planes = p->general_planes ? 1 : 0;
   info->shared_angles = p->shared_angles ? 1 : 0;
+  info->rotating = p->rotating ? 1 : 0;
   return MBQC_OK;
 }
 
@@ -496,6 +541,6 @@ int mbqc_plan_get_rescale(const mbqc_plan* plan, int32_t* sv_flags, int32_t* dm_
 
 const char* mbqc_last_error(void) { return g_last_error.c_str(); }
 
-int32_t mbqc_abi_version(void) { return 2; }
+int32_t mbqc_abi_version(void) { return 3; }
 
 }  // extern "C"

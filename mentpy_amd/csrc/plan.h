@@ -34,6 +34,13 @@ struct Plan {
   std::vector<mbqc_step> steps;
   std::vector<int32_t> in_index, init_sign, out_pos, out_slot;
   std::vector<int32_t> schedule;          // node order, measured nodes first
+  // node measured at each step: schedule[k] unless the description gave a
+  // measurement order (the reference's dev_mode picks nodes out of order)
+  std::vector<int32_t> order;
+  // step k measures slot (n_slots - 1 - k) mod n_slots (always the case in
+  // schedule order): the register kernels take the measured slot at compile
+  // time; other plans run on the kernels that read it from the step flags
+  bool rotating = true;
   std::vector<std::vector<int>> nbr;      // adjacency (applied edges and dropped ones)
   // Pair coefficients (statevector kernels, n_slots <= kMaxCoefSlots): for
   // step k and pair i (the i-th slot index with the measured bit clear),
@@ -86,10 +93,11 @@ TableOffsets table_offsets(const Plan& p);
 std::vector<int32_t> pack_device_tables(const Plan& p);
 
 // Reference window after k measurements, in terms of the live slots: window
-// qubit i (0 = oldest, the most significant bit) is node schedule[k + i]; it
-// lives in register slot slot[i], or is the node the reference has already
-// appended as |+> (slot -1, the last qubit) whose CZ neighbours among the
-// window are the window-index bits nb.
+// qubit i (0 = oldest, the most significant bit) is the i-th unmeasured node
+// of the window in schedule order (schedule[k + i] when nodes are measured in
+// schedule order); it lives in register slot slot[i], or is the node the
+// reference has already appended as |+> (slot -1, the last qubit) whose CZ
+// neighbours among the window are the window-index bits nb.
 struct WindowMap {
   int32_t W;
   int32_t slot[kMaxSlots + 1];

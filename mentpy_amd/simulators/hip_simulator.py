@@ -39,8 +39,53 @@ _PLANE_CODES = {"XY": nat.PLANE_XY, "X": nat.PLANE_XY, "Y": nat.PLANE_XY, "XZ": 
                 "YZ": nat.PLANE_YZ, "Z": nat.PLANE_Z}
 
 
+def dev_mode_order(mbqcircuit, schedule, schedule_measure, window: int, wires):
+    """Node measured at each step by mentpy's ``dev_mode`` scheduler
+    (`np_simulator_sv.py:130-162, 176-197, 254-276`; the same in
+    `np_simulator_dm.py`): the window is a list of nodes, initially
+    ``schedule[:window]``; each step measures the first listed node whose
+    first later neighbour on its wire is listed too (or that has none), drops
+    it and appends the next scheduled node.  The choice depends on the graph
+    and wires only, so it is made once, on the host."""
+    if wires is None:
+        raise TypeError("dev_mode needs wires: a list of node lists, one per wire")
+    wires = [list(w) for w in wires]
+    graph = mbqcircuit.graph
+
+    def wire_of(node):
+        for wire in wires:
+            if node in wire:
+                return wire
+        raise TypeError(f"node {node} is on none of the wires")
+
+    measured = set(schedule_measure)
+    n_nodes = len(schedule)
+    window_nodes = list(schedule[:window])
+    order = []
+    for cm in range(len(schedule_measure)):
+        picked = None
+        for node in window_nodes:
+            later = [n for n in wire_of(node) if graph.has_edge(node, n) and n > node]
+            if not later or later[0] in window_nodes:
+                picked = node
+                break
+        if picked is None:
+            raise ValueError("dev_mode: no node of the window can be measured "
+                             f"(window {window_nodes})")
+        if picked not in measured:
+            # the reference would measure an output node here (and fail on its
+            # missing measurement)
+            raise ValueError(f"dev_mode picks unmeasured node {picked}")
+        order.append(picked)
+        window_nodes = [v for v in window_nodes if v != picked]
+        if cm + 1 + window <= n_nodes:
+            window_nodes.append(schedule[cm + window])
+    return order
+
+
 def compile_pattern(mbqcircuit, schedule=None, window_size: int = 1, exclude: str = "output",
-                    xy_only_error=ValueError, all_planes: bool = False):
+                    xy_only_error=ValueError, all_planes: bool = False, dev_mode: bool = False,
+                    wires=None):
     """Flatten an ``MBQCircuit`` into the arrays of the C plan and compile it.
 
     ``exclude`` selects the unmeasured nodes: ``"output"`` (statevector
@@ -49,8 +94,9 @@ def compile_pattern(mbqcircuit, schedule=None, window_size: int = 1, exclude: st
     XZ, YZ and Z planes are compiled too (density-matrix backend, which has
     no plane check, `np_simulator_dm.py:33-115`); otherwise they raise
     ``xy_only_error`` as the statevector backend does
-    (`np_simulator_sv.py:54-59`).  Returns
-    ``(NativePlan, schedule, schedule_measure, window)``.
+    (`np_simulator_sv.py:54-59`).  With ``dev_mode`` the nodes are measured
+    in ``dev_mode_order``.  Returns ``(NativePlan, schedule, schedule_measure,
+    window)``; ``NativePlan.measure_order`` lists the node of every step.
     """
     graph = mbqcircuit.graph
     nodes = list(graph.nodes())
@@ -114,11 +160,16 @@ def compile_pattern(mbqcircuit, schedule=None, window_size: int = 1, exclude: st
                 flow[idx[v]] = idx[f] if f in idx else -1
         except (KeyError, IndexError, TypeError, ValueError):
             flow = None
+    order = list(sched_meas)
+    if dev_mode:
+        order = dev_mode_order(mbqcircuit, schedule, sched_meas, int(window_size), wires)
     plan = nat.NativePlan(
         n_nodes=len(nodes), edges=edges, schedule=[idx[v] for v in schedule], n_measure=n_meas,
         window=int(window_size), input_nodes=[idx[v] for v in mbqcircuit.input_nodes],
         output_nodes=[idx[v] for v in target_out], angle_index=angle_index, fixed_angle=fixed,
-        n_angles=len(trainable), flow=flow, plane=plane if plane.any() else None)
+        n_angles=len(trainable), flow=flow, plane=plane if plane.any() else None,
+        measure_order=None if order == sched_meas else [idx[v] for v in order])
+    plan.measure_order = order
     return plan, schedule, sched_meas, int(window_size)
 
 
@@ -290,13 +341,13 @@ class _HipBackend(BaseSimulator):
             raise TypeError(f"unexpected keyword arguments {sorted(kwargs)}")
         if not self.force0 and not self._can_sample:
             raise NotImplementedError("Numpy simulator does not support force0=False.")
-        if self.dev_mode:
-            raise NotImplementedError("dev_mode scheduling is not supported by the hip backends.")
         self._check_planes(mbqcircuit)
         plan, schedule, sched_meas, window = compile_pattern(
             mbqcircuit, self.schedule, self.window_size, self._exclude, self._plane_error,
-            all_planes=self._all_planes)
+            all_planes=self._all_planes, dev_mode=self.dev_mode, wires=self.wires)
         self.plan = plan
+        # node measured at each step (schedule_measure unless dev_mode reorders)
+        self.measure_order = plan.measure_order
         torch = _torch()
         dev = self.device
         if dev.type == "cuda" and torch.cuda.is_available():
@@ -367,7 +418,7 @@ class _HipBackend(BaseSimulator):
             raise ValueError("No more measurements to be done.")
         if not self.force0:
             raise NotImplementedError("step-by-step measure() draws no outcomes; use run()")
-        node = self.schedule_measure[self.current_measurement]
+        node = self.measure_order[self.current_measurement]
         ment = self.mbqcircuit[node]
         ment.matrix(angle)                 # the reference's angle checks (ment.py:218-229)
         self._check_step_mode(ment, kwargs.get("mode", "sample"))
@@ -414,9 +465,22 @@ class _HipBackend(BaseSimulator):
     def _finish_run(self, bits=None):
         self.current_measurement = len(self.schedule_measure)
         if bits is None:
-            self.outcomes = {v: 0 for v in self.schedule_measure}
+            self.outcomes = {v: 0 for v in self.measure_order}
         else:
-            self.outcomes = {v: int(b) for v, b in zip(self.schedule_measure, bits)}
+            self.outcomes = {v: int(b) for v, b in zip(self.measure_order, bits)}
+
+    def current_simulated_nodes(self):
+        """Nodes of the reference's window, oldest first
+        (`np_simulator_sv.py:130-136`): ``schedule[k:k+w]`` after k steps, or in
+        dev_mode the entered nodes not yet measured."""
+        k, w = self.current_measurement, self.window_size
+        if not self.dev_mode:
+            return self.schedule[k:k + w]
+        done = set(self.measure_order[:k])
+        return [v for v in self.schedule[:k + w] if v not in done]
+
+    def current_number_simulated_nodes(self) -> int:
+        return min(self.window_size, len(self.mbqcircuit) - self.current_measurement)
 
     def _next_seed(self) -> int:
         return int(self._rng.integers(0, 2 ** 63, dtype=np.int64))
@@ -560,7 +624,8 @@ class HipSimulatorDM(_HipBackend):
         # Every outcome probability is 1/2 on a pattern with a causal flow and
         # XY measurements; otherwise the reference's per-step probabilities
         # (and its outcome-1 rule) are evaluated on the device.
-        return bool(self.plan.info["general_planes"]) or self.mbqcircuit.flow is None
+        return (bool(self.plan.info["general_planes"]) or self.mbqcircuit.flow is None
+                or not self.plan.info["rotating"])
 
     def _check_mode(self, mode):
         if self._has_z and mode not in ("expectation", "exp"):
@@ -585,7 +650,7 @@ class HipSimulatorDM(_HipBackend):
             vals = oc[0, 0].cpu().numpy()
             self._finish_run()
             self.outcomes = {v: (float(o) if self.mbqcircuit[v].plane == "Z" else int(o))
-                             for v, o in zip(self.schedule_measure, vals)}
+                             for v, o in zip(self.measure_order, vals)}
         else:
             rho = dm_forward(self.plan, a, s, noise=self.noise, renormalize=self.renormalize)
             self._finish_run()
@@ -609,7 +674,7 @@ class HipSimulatorDM(_HipBackend):
         outcome = 0
         if oc is not None:
             last = float(oc[0, 0, k - 1].item())
-            node = self.schedule_measure[k - 1]
+            node = self.measure_order[k - 1]
             outcome = last if self.mbqcircuit[node].plane == "Z" else int(last)
         return out[0, 0].cpu().numpy().astype(np.complex128), outcome
 

@@ -16,6 +16,9 @@ Structure follows `mentpy/simulators/np_simulator_dm.py`:
   ``np.random.choice`` (:323-327); here a draw needs an explicit ``rng``.
 * ``run`` (:218-283): schedule without ``quantum_output_nodes``, output
   reordered to ``quantum_output_nodes``.
+* ``dev_mode`` (:151-216, 240-262): the window is a node list; each step
+  measures the first listed node whose first later wire neighbour is listed
+  (or that has none), at its list index (as in ``reference_sv``).
 
 Noise (absent from mentpy's numpy backends; placement of the PennyLane backend,
 `pennylane_simulator.py:118-136`): every qubit is depolarized with ``p`` after
@@ -67,8 +70,10 @@ class ReferenceDM:
     with optional depolarizing / readout-flip noise."""
 
     def __init__(self, circ, input_state=None, window_size=1, schedule=None,
-                 p_depol=0.0, p_readout=0.0, kraus=None):
+                 p_depol=0.0, p_readout=0.0, kraus=None, dev_mode=False, wires=None):
         self.circ = circ
+        self.dev_mode = dev_mode
+        self.wires = wires
         self.kraus = kraus
         self.window_size = window_size
         self.schedule = schedule
@@ -104,13 +109,31 @@ class ReferenceDM:
         self.reset(input_state)
 
     def current_simulated_nodes(self):
+        if self.dev_mode:
+            return self._current_simulated_nodes
         return self.schedule[self.current_measurement:self.current_measurement + self.window_size]
+
+    def _dev_pick(self):
+        """dev_mode (np_simulator_dm.py:151-175, 240-262): the first window
+        node whose first later wire neighbour is in the window (or that has
+        none), as (node, index); (None, None) if there is none."""
+        window = self.current_simulated_nodes()
+        for node in window:
+            wire = next((w for w in self.wires if node in w), None)
+            if wire is None:
+                raise TypeError(f"node {node} is on none of the wires")
+            later = [n for n in wire if self.circ.graph.has_edge(node, n) and n > node]
+            if not later or later[0] in window:
+                return node, window.index(node)
+        return None, None
 
     def current_number_simulated_nodes(self):
         return min(self.window_size, self.n_nodes - self.current_measurement)
 
     def reset(self, input_state=None):
         self.current_measurement = 0
+        if self.dev_mode:
+            self._current_simulated_nodes = list(self.schedule[:self.window_size])
         if input_state is not None:
             st = reorder_qubits(np.asarray(input_state, dtype=complex), self.circ.input_nodes,
                                 self.schedule[:len(self.circ.input_nodes)])
@@ -146,12 +169,23 @@ class ReferenceDM:
     def measure(self, angle, mode="sample", rng=None):
         if self.current_measurement >= len(self.schedule_measure):
             raise ValueError("No more measurements to be done.")
-        m = self.circ[self.schedule_measure[self.current_measurement]]
-        self.qstate, outcome = self.measure_ment(m.plane, m.angle, angle, 0, mode, rng)
+        if self.dev_mode:
+            node, indx = self._dev_pick()
+            if node is None:
+                raise ValueError("no window node can be measured")
+        else:
+            node, indx = self.schedule_measure[self.current_measurement], 0
+        m = self.circ[node]
+        self.qstate, outcome = self.measure_ment(m.plane, m.angle, angle, indx, mode, rng)
         if np.isnan(self.qstate).any():
             raise ValueError("qstate has nan, you might want to increase the window size")
         self.current_measurement += 1
-        self.qstate = partial_trace_mixed_state(self.qstate, [0])
+        self.qstate = partial_trace_mixed_state(self.qstate, [indx])
+        if self.dev_mode:
+            self._current_simulated_nodes = [v for v in self._current_simulated_nodes if v != node]
+            if self.current_measurement + self.window_size <= self.n_nodes:
+                self._current_simulated_nodes.append(
+                    self.schedule[self.current_measurement + self.window_size - 1])
         if self.current_measurement + self.window_size <= self.n_nodes:
             self.qstate = np.kron(self.qstate, pure2density(PLUS))
             window = self.current_simulated_nodes()
@@ -168,9 +202,17 @@ class ReferenceDM:
             self.reset(input_state)
         if len(angles) != len(self.circ.trainable_nodes):
             raise ValueError("Number of angles does not match number of trainable nodes.")
-        for node in self.schedule_measure:
-            self.qstate, out = self.measure(_node_angle(self.circ, node, angles), mode, rng)
-            self.outcomes[node] = out
+        if self.dev_mode:
+            while self.current_measurement < len(self.schedule_measure):
+                node, _ = self._dev_pick()
+                if node is None:
+                    raise ValueError("no window node can be measured")
+                self.qstate, out = self.measure(_node_angle(self.circ, node, angles), mode, rng)
+                self.outcomes[node] = out
+        else:
+            for node in self.schedule_measure:
+                self.qstate, out = self.measure(_node_angle(self.circ, node, angles), mode, rng)
+                self.outcomes[node] = out
         current = [i for i in self.schedule if i not in self.schedule_measure]
         n = len(current)
         for i in range(n):

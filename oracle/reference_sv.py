@@ -14,6 +14,11 @@ Follows `mentpy/simulators/np_simulator_sv.py` line by line in structure:
 * ``run`` (:227-297): angle of node i is ``angles[trainable_nodes.index(i)]``,
   fixed-angle nodes use their own angle; output reordered to ``output_nodes``;
   returns ``outer(psi, psi*)`` or ``psi``.
+* ``dev_mode`` (:130-162, 176-197, 254-276): the window is a list of nodes,
+  initially ``schedule[:w]``; each step measures the first listed node whose
+  first later neighbour on its wire (``wires``) is listed too (or that has
+  none), at its list index, drops it from the list and appends the next
+  scheduled node as before.
 
 Dense operators are built exactly as the reference builds them (kron chains of
 2x2 factors), so the cost profile of this class is the reference's; it is the
@@ -157,10 +162,13 @@ def _node_angle(circ, node, angles):
 class ReferenceSV:
     """Faithful complex128 restatement of ``NumpySimulatorSV`` (force0 only)."""
 
-    def __init__(self, circ, input_state=None, window_size=1, schedule=None):
+    def __init__(self, circ, input_state=None, window_size=1, schedule=None, dev_mode=False,
+                 wires=None):
         self.circ = circ
         self.window_size = window_size
         self.schedule = schedule
+        self.dev_mode = dev_mode
+        self.wires = wires
         for node in circ.graph.nodes:
             m = circ[node]
             if m is not None and m.plane not in ("X", "Y", "XY"):
@@ -195,13 +203,34 @@ class ReferenceSV:
         self.reset(input_state)
 
     def current_simulated_nodes(self) -> List:
+        if self.dev_mode:
+            return self._current_simulated_nodes
         return self.schedule[self.current_measurement:self.current_measurement + self.window_size]
+
+    def _wire_of(self, node):
+        for wire in self.wires:
+            if node in wire:
+                return wire
+        raise TypeError(f"node {node} is on none of the wires")
+
+    def _dev_pick(self):
+        """First window node whose first later wire neighbour is in the window
+        (or that has none): (node, index), or (None, None)."""
+        window = self.current_simulated_nodes()
+        for node in window:
+            wire = self._wire_of(node)
+            later = [n for n in wire if self.circ.graph.has_edge(node, n) and n > node]
+            if not later or later[0] in window:
+                return node, window.index(node)
+        return None, None
 
     def current_number_simulated_nodes(self) -> int:
         return min(self.window_size, self.n_nodes - self.current_measurement)
 
     def reset(self, input_state=None):
         self.current_measurement = 0
+        if self.dev_mode:
+            self._current_simulated_nodes = list(self.schedule[:self.window_size])
         if input_state is not None:
             st = reorder_qubits(np.asarray(input_state, dtype=complex), self.circ.input_nodes,
                                 self.schedule[:len(self.circ.input_nodes)])
@@ -223,10 +252,21 @@ class ReferenceSV:
     def measure(self, angle):
         if self.current_measurement >= len(self.schedule_measure):
             raise ValueError("No more measurements to be done.")
-        m = self.circ[self.schedule_measure[self.current_measurement]]
-        self.qstate = self.measure_ment(m.plane, m.angle, angle, 0)
+        if self.dev_mode:
+            node, indx = self._dev_pick()
+            if node is None:
+                raise ValueError("no window node can be measured")
+        else:
+            node, indx = self.schedule_measure[self.current_measurement], 0
+        m = self.circ[node]
+        self.qstate = self.measure_ment(m.plane, m.angle, angle, indx)
         self.current_measurement += 1
-        self.qstate = partial_trace(self.qstate, [0])
+        self.qstate = partial_trace(self.qstate, [indx])
+        if self.dev_mode:
+            self._current_simulated_nodes = [v for v in self._current_simulated_nodes if v != node]
+            if self.current_measurement + self.window_size <= self.n_nodes:
+                self._current_simulated_nodes.append(
+                    self.schedule[self.current_measurement + self.window_size - 1])
         if self.current_measurement + self.window_size <= self.n_nodes:
             self.qstate = np.kron(self.qstate, PLUS)
             window = self.current_simulated_nodes()
@@ -242,9 +282,17 @@ class ReferenceSV:
             self.reset(input_state)
         if len(angles) != len(self.circ.trainable_nodes):
             raise ValueError("Number of angles does not match number of trainable nodes.")
-        for node in self.schedule_measure:
-            self.qstate, out = self.measure(_node_angle(self.circ, node, angles))
-            self.outcomes[node] = out
+        if self.dev_mode:
+            while self.current_measurement < len(self.schedule_measure):
+                node, _ = self._dev_pick()
+                if node is None:
+                    raise ValueError("no window node can be measured")
+                self.qstate, out = self.measure(_node_angle(self.circ, node, angles))
+                self.outcomes[node] = out
+        else:
+            for node in self.schedule_measure:
+                self.qstate, out = self.measure(_node_angle(self.circ, node, angles))
+                self.outcomes[node] = out
         current = self.current_simulated_nodes()
         if list(self.circ.quantum_output_nodes) != current:
             self.qstate = reorder_qubits(self.qstate, current, self.circ.output_nodes)

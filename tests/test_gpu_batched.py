@@ -114,3 +114,39 @@ def test_batched_periodic_groups(rng):
             assert torch.equal(x, y)
     for (sim, th, st), o in zip(jobs, a):    # reference phase, as run_batch
         assert (sim.run_batch(th, st) - o).abs().max().item() < 1e-6
+
+
+def test_batched_group_waves(rng):
+    """Resident waves chosen per live-window group: every sizing gives the
+    uniform launch's outputs bit for bit; a group given 0 waves is skipped
+    (its outputs keep what they held) while the others are written; counts
+    below the workgroup size and too many entries are rejected."""
+    torch = _torch()
+    from mentpy_amd.batched import MAX_GROUPS, BatchedForward
+    jobs = _jobs(rng, n_pat=8)
+    bf = BatchedForward([(s, th, st) for s, th, st, _ in jobs], sims_per_item=64)
+    groups = bf.groups()
+    assert sum(groups.values()) == bf.n_items and len(groups) > 1
+    want = [o.clone() for o in bf.launch(waves=512, wg_waves=2)]
+    torch.cuda.synchronize()
+    for sizes in ([4, 8, 12, 16, 20, 24, 28], [64] * MAX_GROUPS, [2] * MAX_GROUPS):
+        for o in bf.outputs:
+            o.fill_(complex(float("nan"), 0.0))
+        got = bf.launch(wg_waves=2, group_waves=sizes)
+        torch.cuda.synchronize()
+        for x, y in zip(got, want):
+            assert torch.equal(x, y), f"group_waves {sizes} differs from the uniform launch"
+    skip = min(groups)
+    for o in bf.outputs:
+        o.zero_()
+    bf.launch(wg_waves=2, group_waves=[0 if ns == skip else 8 for ns in range(MAX_GROUPS)])
+    torch.cuda.synchronize()
+    for (sim, _, _, _), o, w in zip(jobs, bf.outputs, want):
+        if sim.simulator.plan.info["n_slots"] == skip:
+            assert o.abs().max().item() == 0.0
+        else:
+            assert torch.equal(o, w)
+    with pytest.raises(ValueError):
+        bf.launch(wg_waves=2, group_waves=[1] * MAX_GROUPS)
+    with pytest.raises(ValueError):
+        bf.launch(wg_waves=2, group_waves=[8] * (MAX_GROUPS + 1))

@@ -129,7 +129,7 @@ def test_library_exports_every_header_symbol(native):
     for nm in sorted(names):
         assert hasattr(L, nm), f"{nm} not exported"
     assert set(native.EXPORTED_SYMBOLS) == names
-    assert L.mbqc_abi_version() == 2
+    assert L.mbqc_abi_version() == 3
 
 
 def test_validation_errors_map_to_reference_exceptions(native):
