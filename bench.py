@@ -275,7 +275,7 @@ def config_batched(args, dev, stream):
         pats.append((g, th, st))
         flops += 1024 * 16 * flops_per_sim(sim.simulator.plan.info["n_steps"], r + 1)
     bf = BatchedForward(jobs)
-    cfg = autotune(bf, reps=2)
+    cfg = autotune(bf)
     graph = bf.capture_graph()
     secs = time_launches(graph.replay, args.steps, args.warmup, stream)
     ms = secs / args.steps * 1e3

@@ -9,7 +9,8 @@ whose wavefronts pull work items from a device counter.  The launch (counter
 reset kernel + persistent kernel) can be captured in a HIP graph and replayed.
 
 ``autotune`` picks the resident wavefront count, wavefronts per workgroup and
-simulations per work item by timing candidate launches with HIP events, then
+simulations per work item by timing graph replays of candidate launches with
+HIP events, then
 (with several live-window groups) wavefront counts per group in proportion to
 each group's solo time, and caches the choice per (device, job signature) in a
 JSON file.
@@ -30,6 +31,8 @@ __all__ = ["BatchedForward", "autotune", "default_cache_path"]
 
 # live-window sizes 0..6 of the batched statevector kernel
 MAX_GROUPS = 7
+# tag of the cached tuning results (entries timed another way are not reused)
+_TUNE_VERSION = "graph-replay"
 
 
 def _torch():
@@ -165,26 +168,39 @@ class BatchedForward:
 
 
 def _time(fn, reps=5):
+    """Mean ms of ``fn``'s launch captured in a HIP graph and replayed
+    ``reps`` times (how the batch runs in production: the replays carry no
+    host-side launch cost, which would otherwise blur candidates apart)."""
     torch = _torch()
     fn()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(g):
+            fn()
+    torch.cuda.synchronize()
+    g.replay()
     torch.cuda.synchronize()
     s = torch.cuda.Event(enable_timing=True)
     e = torch.cuda.Event(enable_timing=True)
     s.record()
     for _ in range(reps):
-        fn()
+        g.replay()
     e.record()
     torch.cuda.synchronize()
+    del g
     return s.elapsed_time(e) / reps
 
 
-def autotune(batch: BatchedForward, cache: Optional[Path] = None, reps: int = 3,
+def autotune(batch: BatchedForward, cache: Optional[Path] = None, reps: int = 10,
              candidates: Optional[List[dict]] = None, verbose: bool = False) -> dict:
     """Time candidate launch shapes, keep the fastest, cache it per signature."""
     torch = _torch()
     cache = Path(cache) if cache is not None else default_cache_path()
     name = torch.cuda.get_device_properties(batch.device).name
-    key = f"{name}|{batch.signature()}"
+    key = f"{name}|{batch.signature()}|{_TUNE_VERSION}"
     table = {}
     if cache.exists():
         try:

@@ -176,6 +176,7 @@ struct Job {
   const int32_t* tab;       // packed device tables of the plan
   int32_t off_in, off_sign, off_out, off_svres, off_dmres, e_sv, e_dm;
   int32_t ns, n_steps, n_out, live_mask, off_coef;
+  int32_t off_aidx, off_run;  // contiguous angle indices, stretch ends
   const float* angles;
   int64_t angle_stride;     // elements between angle vectors (0: shared)
   int64_t angle_stride_a;   // elements between the angles of one vector
@@ -274,73 +275,111 @@ __device__ __forceinline__ void sv_block(float2 (&t)[1 << NS], const mbqc_step* 
    ...);
 }
 
-// Angles of the NS steps of a block.  Steps past the end (n_steps >= 1) read
-// the last step's angle, so every load is issued from a valid address without
-// a branch (their values are never used); fixed-angle steps select their
-// stored angle afterwards.
-template <int NS>
-__device__ __forceinline__ void fetch_block_angles(const mbqc_step* steps, int k0, int n_steps,
-                                                   const float* ang, int64_t sa, float (&th)[NS]) {
+// Angles of steps [k0, k0 + M).  Steps past the end (n_steps >= 1) read the
+// last step's angle, so every load is issued from a valid address (their
+// values are never used).  Every step issues exactly one load, from its
+// angle-vector entry or from its stored fixed angle (the address is selected,
+// not the value): with a fixed number of loads in flight, the wait before a
+// block's math covers only that block's angles, not loads issued for later
+// blocks.
+// The angle indices come from the plan's contiguous, padded index table
+// (aidx, M <= kAngleIdxPad entries past the last step at most), so they are
+// adjacent scalar loads.
+template <int M>
+__device__ __forceinline__ void fetch_step_angles(const mbqc_step* steps, const int32_t* aidx,
+                                                  int k0, int n_steps, const float* ang,
+                                                  int64_t sa, float (&th)[M]) {
+  static_assert(M <= kAngleIdxPad, "index table padding");
   // angles live in device memory: a global-typed pointer keeps the loads
   // global (a descriptor read from memory otherwise leaves them flat)
-  // Every step issues exactly one load, from its angle-vector entry or from
-  // its stored fixed angle (the address is selected, not the value): with a
-  // fixed number of loads in flight the wait before a block's math covers
-  // only that block's angles, not the prefetch issued for the next block.
   using gfloat = const __attribute__((address_space(1))) float;
   gfloat* gang = (gfloat*)ang;
-  int ai[NS];
+  int ai[M];
 #pragma unroll
-  for (int sub = 0; sub < NS; ++sub)   // the step records first: one scalar wait
-    ai[sub] = steps[k0 + sub < n_steps ? k0 + sub : n_steps - 1].angle_index;
+  for (int m = 0; m < M; ++m) ai[m] = aidx[k0 + m];
 #pragma unroll
-  for (int sub = 0; sub < NS; ++sub) {
-    const int k = k0 + sub < n_steps ? k0 + sub : n_steps - 1;
-    gfloat* src = ai[sub] >= 0 ? gang + int64_t(ai[sub]) * sa : (gfloat*)&steps[k].fixed_angle;
-    th[sub] = *src;
+  for (int m = 0; m < M; ++m) {
+    const int k = k0 + m < n_steps ? k0 + m : n_steps - 1;
+    gfloat* src = ai[m] >= 0 ? gang + int64_t(ai[m]) * sa : (gfloat*)&steps[k].fixed_angle;
+    th[m] = *src;
   }
 }
 
-// Run blocks [blk0, blk1) of NS steps: full blocks of NS unrolled steps with
-// the next block's angles in flight, then the partial tail block.
-// `on_block(blk, t)` runs at the start of every block (after its rescale).
-// The full blocks run in stretches that end before the next rescaled block,
-// so the inner loop is straight-line code whose loop-carried state needs no
-// register copies (a rescale branch inside it cost 32 moves per block).
-// PERIODIC: the full blocks take the compile-time coefficients of a periodic
-// plan (Plan::periodic), the tail block the table.
+// Blocks whose angles one fetch brings in: small windows do little work per
+// block, so their loads are issued several blocks ahead of their use.
+template <int NS>
+constexpr int kFetchBlocks = NS <= 2 ? 4 : (NS == 3 ? 2 : 1);
+
+// Run blocks [blk0, blk1) of NS steps: full blocks of NS unrolled steps, with
+// the angles of the next kFetchBlocks blocks in flight, then the partial tail
+// block.  `on_block(blk, t)` runs at the start of every block (after its
+// rescale).  The full blocks run in stretches that end before the next
+// rescaled block, so the inner loop is straight-line code whose loop-carried
+// state needs no register copies (a rescale branch inside it cost 32 moves
+// per block).  PERIODIC: the full blocks take the compile-time coefficients
+// of a periodic plan (Plan::periodic), the tail block the table.
 template <int NS, bool REF, bool PER_STEP, bool SH = false, bool PERIODIC = false,
           typename OnBlock, typename Mod = NoModify>
 __device__ __forceinline__ void sv_run_blocks(float2 (&t)[1 << NS], const Job& J, const float* ang,
                                               RefPhase* ph, int blk0, int blk1, OnBlock on_block,
                                               Mod&& mod = Mod()) {
   constexpr int H = 1 << NS;
+  constexpr int K = kFetchBlocks<NS>;
+  using Seq = std::make_integer_sequence<int, NS>;
   if (blk0 >= blk1) return;
   const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
   const int32_t* resc = J.tab + J.off_svres;
+  const int32_t* run_end = J.tab + J.off_run;
+  const int32_t* aidx = J.tab + J.off_aidx;
   const float* coef = reinterpret_cast<const float*>(J.tab + J.off_coef);
   const int n = J.n_steps;
   const int nfull = n / NS;
   const int full_end = nfull < blk1 ? nfull : blk1;
-  float th_next[NS];
+  // th_next holds the angles of blocks blk .. blk + K - 1
+  float th_next[K * NS];
   const int64_t sa = J.angle_stride_a;
-  fetch_block_angles<NS>(steps, blk0 * NS, n, ang, sa, th_next);
+  fetch_step_angles<K * NS>(steps, aidx, blk0 * NS, n, ang, sa, th_next);
   int blk = blk0;
   while (blk < full_end) {
-    int end = blk + 1;
-    while (end < full_end && !resc[end]) ++end;
+    const int re = run_end[blk];
+    const int end = re < full_end ? re : full_end;
     if (resc[blk]) {
 #pragma unroll
       for (int j = 0; j < H; ++j) t[j] = cscale(t[j], 0x1p-32f);
     }
-    for (; blk < end; ++blk) {
-      float th[NS];
+    for (; blk + K <= end; blk += K) {
+      float th[K * NS];
 #pragma unroll
-      for (int s2 = 0; s2 < NS; ++s2) th[s2] = th_next[s2];
-      fetch_block_angles<NS>(steps, (blk + 1) * NS, n, ang, sa, th_next);
-      on_block(blk, t);
-      sv_block<NS, REF, true, PER_STEP, SH, PERIODIC>(t, steps, coef, blk * NS, n, th, ph, mod,
-                                                      std::make_integer_sequence<int, NS>{});
+      for (int m = 0; m < K * NS; ++m) th[m] = th_next[m];
+      fetch_step_angles<K * NS>(steps, aidx, (blk + K) * NS, n, ang, sa, th_next);
+#pragma unroll
+      for (int b = 0; b < K; ++b) {
+        float tb[NS];
+#pragma unroll
+        for (int s2 = 0; s2 < NS; ++s2) tb[s2] = th[b * NS + s2];
+        on_block(blk + b, t);
+        sv_block<NS, REF, true, PER_STEP, SH, PERIODIC>(t, steps, coef, (blk + b) * NS, n, tb, ph,
+                                                        mod, Seq{});
+      }
+    }
+    if constexpr (K > 1) {
+      if (blk < end) {
+        // fewer than K blocks left in the stretch: run them from th_next,
+        // then fetch from the stretch's end
+#pragma unroll
+        for (int b = 0; b < K - 1; ++b) {
+          if (blk + b < end) {
+            float tb[NS];
+#pragma unroll
+            for (int s2 = 0; s2 < NS; ++s2) tb[s2] = th_next[b * NS + s2];
+            on_block(blk + b, t);
+            sv_block<NS, REF, true, PER_STEP, SH, PERIODIC>(t, steps, coef, (blk + b) * NS, n, tb,
+                                                            ph, mod, Seq{});
+          }
+        }
+        blk = end;
+        fetch_step_angles<K * NS>(steps, aidx, blk * NS, n, ang, sa, th_next);
+      }
     }
   }
   if (blk < blk1) {
@@ -349,9 +388,11 @@ __device__ __forceinline__ void sv_run_blocks(float2 (&t)[1 << NS], const Job& J
 #pragma unroll
       for (int j = 0; j < H; ++j) t[j] = cscale(t[j], 0x1p-32f);
     }
+    float tb[NS];
+#pragma unroll
+    for (int s2 = 0; s2 < NS; ++s2) tb[s2] = th_next[s2];
     on_block(blk, t);
-    sv_block<NS, REF, false, PER_STEP, SH, false>(t, steps, coef, blk * NS, n, th_next, ph, mod,
-                                                  std::make_integer_sequence<int, NS>{});
+    sv_block<NS, REF, false, PER_STEP, SH, false>(t, steps, coef, blk * NS, n, tb, ph, mod, Seq{});
   }
 }
 
@@ -2435,6 +2476,7 @@ struct GlobalJob {
   const __attribute__((address_space(4))) int32_t* tab;
   int32_t off_in, off_sign, off_out, off_svres, off_dmres, e_sv, e_dm;
   int32_t ns, n_steps, n_out, live_mask, off_coef;
+  int32_t off_aidx, off_run;
   const __attribute__((address_space(1))) float* angles;
   int64_t angle_stride;
   int64_t angle_stride_a;
@@ -2462,6 +2504,8 @@ __device__ __forceinline__ Job load_job(const Job* jobs, int i) {
   j.n_out = g.n_out;
   j.live_mask = g.live_mask;
   j.off_coef = g.off_coef;
+  j.off_aidx = g.off_aidx;
+  j.off_run = g.off_run;
   j.angles = (const float*)g.angles;
   j.angle_stride = g.angle_stride;
   j.angle_stride_a = g.angle_stride_a;
@@ -2629,6 +2673,8 @@ int make_job(Plan* p, const float* angles, int64_t angle_stride, int64_t angle_s
   j.off_sign = o.init_sign;
   j.off_out = o.out_pos;
   j.off_svres = o.sv_rescale;
+  j.off_aidx = o.angle_idx;
+  j.off_run = o.sv_run_end;
   j.off_dmres = o.dm_rescale;
   j.off_coef = o.coef;
   j.e_sv = p->e_sv;

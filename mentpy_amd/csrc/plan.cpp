@@ -381,7 +381,9 @@ TableOffsets table_offsets(const Plan& p) {
   o.xmask += o.xmask & 1;  // 8-byte alignment of the 64-bit masks
   o.zmask = o.xmask + 2 * int(p.xmask.size());
   o.out_offset = o.zmask + 2 * int(p.zmask.size());
-  o.total = o.out_offset + int(p.out_offset.size());
+  o.angle_idx = o.out_offset + int(p.out_offset.size());
+  o.sv_run_end = o.angle_idx + p.n_steps + kAngleIdxPad;
+  o.total = o.sv_run_end + p.n_blocks();
   return o;
 }
 
@@ -402,6 +404,16 @@ std::vector<int32_t> pack_device_tables(const Plan& p) {
     std::memcpy(t.data() + o.xmask, p.xmask.data(), sizeof(uint64_t) * p.xmask.size());
     std::memcpy(t.data() + o.zmask, p.zmask.data(), sizeof(uint64_t) * p.zmask.size());
     std::memcpy(t.data() + o.out_offset, p.out_offset.data(), sizeof(int32_t) * p.out_offset.size());
+  }
+  for (int k = 0; k < p.n_steps + kAngleIdxPad; ++k) {
+    const int src = std::min(k, p.n_steps - 1);
+    t[o.angle_idx + k] = src >= 0 ? p.steps[src].angle_index : 0;
+  }
+  const int nb = p.n_blocks();
+  int next = nb;
+  for (int b = nb - 1; b >= 0; --b) {
+    t[o.sv_run_end + b] = next;
+    if (t[o.sv_rescale + b]) next = b;
   }
   return t;
 }

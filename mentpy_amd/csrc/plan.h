@@ -85,9 +85,16 @@ extern int (*g_plan_upload_hook)(Plan*);
 //   [sv_rescale: n_blocks][dm_rescale: n_blocks][sv_e_final, dm_e_final]
 //   [coef: n_rows * 2^ns floats (distinct rows), only when ns <= kMaxCoefSlots]
 //   [xmask, zmask: n_steps uint64 each, out_offset: n_out, only when sampling is possible]
+//   [angle_idx: n_steps + kAngleIdxPad][sv_run_end: n_blocks]
+// angle_idx[k] is steps[k].angle_index, contiguous (the statevector kernels
+// read a run of them at once), padded with the last step's entry so a read of
+// up to kAngleIdxPad entries past the last step stays in the table.
+// sv_run_end[b] is the first block after b whose statevector rescale flag is
+// set (n_blocks if none): the end of the stretch of blocks starting at b.
+constexpr int kAngleIdxPad = 16;
 struct TableOffsets {
   int steps, in_index, init_sign, out_pos, sv_rescale, dm_rescale, e_final, coef, xmask, zmask,
-      out_offset, total;
+      out_offset, angle_idx, sv_run_end, total;
 };
 TableOffsets table_offsets(const Plan& p);
 std::vector<int32_t> pack_device_tables(const Plan& p);
