@@ -28,21 +28,29 @@ sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3     # MI355X published FP32 (vector and matrix) peak
 HBM_PEAK_GBS = 8000.0
-# The HBM traffic of the headline kernel is measured with rocprofv3 counters
-# (FETCH_SIZE, WRITE_SIZE per dispatch, scaled by a device copy of known size)
-# and recorded in this profile summary, which names the kernel it measured.
+# HBM traffic is measured with rocprofv3 counters (FETCH_SIZE, WRITE_SIZE per
+# dispatch, scaled by a device copy of known size) and recorded in the profile
+# summaries, each naming the kernel and workload it measured.
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "sv_forward_grid4x20.md")
+BATCHED_PROFILE = os.path.join(ROOT, "profiles", "batched_64grids.md")
+DM_PROFILE = os.path.join(ROOT, "profiles", "dm_forward_grid3x12.md")
+MBQML_PROFILE = os.path.join(ROOT, "profiles", "mbqml_step_grid4x16.md")
 
 
-def profiled_traffic(kernel):
-    """HBM bytes per simulation of ``kernel`` from the profile summary's
-    ``traffic_bytes_per_sim <kernel> <bytes>`` line (None if the summary is
-    absent or was measured on another kernel)."""
+def _rel(path):
+    return os.path.relpath(path, ROOT)
+
+
+def profiled_traffic(kernel, profile=TRAFFIC_PROFILE, per="sim"):
+    """HBM bytes per simulation (``per="sim"``) or per launch (``"launch"``)
+    of ``kernel`` from a profile summary's ``traffic_bytes_per_<per> <kernel>
+    <bytes>`` line (None if the summary is absent or measured another kernel)."""
+    tag = f"traffic_bytes_per_{per}"
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(profile) as f:
             for line in f:
                 parts = line.strip().strip("`").split()
-                if len(parts) == 3 and parts[0] == "traffic_bytes_per_sim" and parts[1] == kernel:
+                if len(parts) == 3 and parts[0] == tag and parts[1] == kernel:
                     return float(parts[2])
     except OSError:
         pass
@@ -296,7 +304,10 @@ def config_batched(args, dev, stream):
             "max_amp_err_vs_c128": max(errs),
             "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                         "traffic": None, "kernel": "k_batch"}}
+                         "traffic": profiled_traffic("k_batch", BATCHED_PROFILE, per="launch"),
+                         "traffic_note": "HBM bytes per graph replay (the four window-size "
+                                         "kernels), rocprof counters in " + _rel(BATCHED_PROFILE),
+                         "kernel": "k_batch"}}
 
 
 def config_dm(args, dev, stream):
@@ -338,6 +349,9 @@ def config_dm(args, dev, stream):
     td = 0.5 * np.abs(np.linalg.eigvalsh(out[:k, 0].cpu().numpy() - ref)).sum(-1)
     f_sim = plan.info["n_steps"] * 20 * 4 ** (plan.info["window"] - 1)
     achieved = f_sim * B / (ms / 1e3) / 1e12
+    dm_kernel = "k_dm_forward<3,true>" if B >= 32768 else "k_dm_forward_lds<3,16,4,true>"
+    bps = profiled_traffic(dm_kernel, DM_PROFILE)
+    dm_traffic = None if bps is None else round(bps * B)
     return {"metric": "density-matrix pattern sims/s, 3x12 grid, noisy, trace distance vs c128",
             "value": B / (ms / 1e3), "unit": "sims/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
@@ -349,9 +363,8 @@ def config_dm(args, dev, stream):
             "eager_ms_per_launch": eager_ms,
             "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": FP32_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
-                         "traffic": None,
-                         "kernel": "k_dm_forward<3,true>" if B >= 32768 else "k_dm_forward_lds<3,16,4,true>",
-                         "flops_per_sim": f_sim}}
+                         "traffic": dm_traffic,
+                         "kernel": dm_kernel, "flops_per_sim": f_sim}}
 
 
 def config_sample(args, dev, stream):
@@ -415,6 +428,16 @@ def config_mbqml(args, dev, stream):
     graph = step.capture()
     secs = time_launches(graph.replay, args.steps, args.warmup, stream)
     ms = secs / args.steps * 1e3
+    # roofline of the largest part of the step, the segmented forward (the
+    # segment sweeps and the chain products), timed alone with HIP events
+    fwd_s = time_each_launch(step._launch_forward, args.steps, args.warmup, stream)
+    info = sim.simulator.plan.info
+    n_cols = step.cols.shape[0]
+    f_sweep = B * n_cols * flops_per_sim(info["n_steps"], info["window"])
+    f_chain = B * (step.segments - 1) * n_cols ** 3 * 8
+    fwd_flops = f_sweep + f_chain
+    fwd_kernel = "k_sv_seg_forward<4,true>+k_seg_chain_forward_mfma"
+    fwd_tflops = fwd_flops / fwd_s / 1e12
     fd_secs = time_launches(lambda: fd_loop_infidelity_grad(sim, th, step.X, step.Y, h=1e-3), 1, 1, stream)
     _, gr = step.run(th)
     # a full training iteration (adjoint step + device Adam) as one graph replay
@@ -437,7 +460,12 @@ def config_mbqml(args, dev, stream):
             "fd_loop_grads_per_s": B / fd_secs, "adjoint_vs_fd_loop": (B / (ms / 1e3)) / (B / fd_secs),
             "train_iterations_per_s": args.steps / it_secs,
             "train_note": "one graph replay = loss+gradient of all 1024 parameter sets + device Adam update",
-            "grad_max_abs_err_vs_c128": err}
+            "grad_max_abs_err_vs_c128": err,
+            "roofline": {"bound": "valu", "achieved": round(fwd_tflops, 3), "peak": FP32_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(fwd_tflops / FP32_PEAK_TFLOPS, 4),
+                         "traffic": profiled_traffic(fwd_kernel, MBQML_PROFILE, per="launch"),
+                         "kernel": fwd_kernel, "kernel_ms": fwd_s * 1e3,
+                         "flops": fwd_flops, "segments": step.segments}}
 
 
 def config_copy(args, dev, stream):

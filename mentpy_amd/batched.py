@@ -199,8 +199,9 @@ def autotune(batch: BatchedForward, cache: Optional[Path] = None, reps: int = 10
     """Time candidate launch shapes, keep the fastest, cache it per signature."""
     torch = _torch()
     cache = Path(cache) if cache is not None else default_cache_path()
-    name = torch.cuda.get_device_properties(batch.device).name
-    key = f"{name}|{batch.signature()}|{_TUNE_VERSION}"
+    props = torch.cuda.get_device_properties(batch.device)
+    # architecture and CU count (the marketing name can be empty, e.g. under a profiler)
+    key = f"{props.gcnArchName}|{props.multi_processor_count}|{batch.signature()}|{_TUNE_VERSION}"
     table = {}
     if cache.exists():
         try:

@@ -906,6 +906,18 @@ __global__ __launch_bounds__(256) void k_sv_sample(Job J, const uint64_t* __rest
 // workgroup per simulation keeps the state in LDS; the measured slot is a
 // runtime value, each thread updates a strided subset of the amplitude pairs
 // (DM: quartets) and the workgroup synchronises between steps.
+// Workgroups are dealt to the chip's kXcds XCDs round-robin (workgroup b runs
+// on XCD b % kXcds), each XCD with its own L2.  xcd_block maps workgroup b of
+// n to a logical block so that every XCD takes a contiguous run of logical
+// blocks (a bijection of [0, n)): neighbouring simulations, which share cache
+// lines of the angle table, then meet in one L2 instead of each XCD fetching
+// the line.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned n) {
+  const unsigned q = n / kXcds, r = n % kXcds, x = b % kXcds;
+  return x * q + (x < r ? x : r) + b / kXcds;
+}
+
 __device__ __forceinline__ int insert_zero(int v, int q) {
   return ((v >> q) << (q + 1)) | (v & ((1 << q) - 1));
 }
@@ -1030,7 +1042,7 @@ __global__ __launch_bounds__(T) void k_sv_forward_lds(Job J, uint32_t flags) {
   constexpr int H = 1 << NS;
   __shared__ float2 st[H];
   __shared__ float red[4];
-  const int64_t g = blockIdx.x;
+  const int64_t g = xcd_block(blockIdx.x, gridDim.x);   // XCD-aware order
   const int tid = threadIdx.x;
   const int64_t bi = g / J.S, si = g - bi * J.S;
   const float* ang = J.angles + bi * J.angle_stride;
@@ -1117,7 +1129,7 @@ __global__ __launch_bounds__(T * G) void k_dm_forward_lds(Job J, DmNoise nz, uin
   __shared__ float2 gb_all[G][kAngleChunk];
   __shared__ float red[4];
   const int gs = threadIdx.x / T, tid = threadIdx.x % T;
-  const int64_t g = int64_t(blockIdx.x) * G + gs;
+  const int64_t g = int64_t(xcd_block(blockIdx.x, gridDim.x)) * G + gs;
   const bool active = g < J.total;
   const int64_t gg = active ? g : J.total - 1;
   float2* r = rs[gs];
@@ -2325,7 +2337,7 @@ __global__ __launch_bounds__(64) void k_dm_general(Job J, const float* __restric
   extern __shared__ float2 r[];
   __shared__ float red[4];
   const int ns = J.ns, H = 1 << ns, HQ = H / 2;
-  const int64_t g = blockIdx.x;
+  const int64_t g = xcd_block(blockIdx.x, gridDim.x);   // XCD-aware order
   const int tid = threadIdx.x;
   const int64_t bi = g / J.S, si = g - bi * J.S;
   const float* ang = window_mode ? step_angles + bi * step_stride : J.angles + bi * J.angle_stride;
