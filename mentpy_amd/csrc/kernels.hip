@@ -1197,6 +1197,195 @@ __global__ __launch_bounds__(T * G) void k_dm_forward_lds(Job J, DmNoise nz, uin
   }
 }
 
+// ------------------------------------------- DM, one entry per lane
+// Value of lane (lane ^ MASK) in the same wave: DPP quad permutations for
+// MASK 1 and 2, ds_swizzle (no LDS allocation, no barrier) within 32-lane
+// halves, a bpermute shuffle across them.
+template <int MASK>
+__device__ __forceinline__ float lane_xor(float v) {
+  if constexpr (MASK == 1) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xf, 0xf, false));
+  } else if constexpr (MASK == 2) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xf, 0xf, false));
+  } else if constexpr (MASK < 32) {
+    // bit mode: lane' = ((lane & 0x1f) ^ MASK) within each 32-lane half
+    return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), (MASK << 10) | 0x1f));
+  } else {
+    return __shfl_xor(v, MASK, 64);
+  }
+}
+template <int MASK>
+__device__ __forceinline__ float2 lane_xor2(float2 v) {
+  return make_float2(lane_xor<MASK>(v.x), lane_xor<MASK>(v.y));
+}
+
+// One step of the density matrix held one entry per lane: lane entry
+// (a, b) of an H x H matrix, H = 2^NS, lanes numbered a * H + b within the
+// simulation's group of H^2 lanes.  The quartet of measured slot Q that holds
+// the entry is {(a,b), (a,b^Q), (a^Q,b), (a^Q,b^Q)}: column, row and diagonal
+// partners are lanes xor Q, xor Q<<NS and both.  Same update as dm_lds_step
+// (quartet form of Tr_Q[(D^dag P) CZ (rho (x) |+><+|) CZ], gb = gamma e^{-i theta}).
+template <int NS, int Q>
+__device__ __forceinline__ float2 dm_wave_step(float2 x, int a, int b, float2 gb, uint32_t flags,
+                                               uint32_t nmask) {
+  constexpr int QB = 1 << Q;
+  const float2 c = lane_xor2<QB>(x);           // (a, b^Q)
+  const float2 r = lane_xor2<(QB << NS)>(x);   // (a^Q, b)
+  const float2 d = lane_xor2<QB>(r);           // (a^Q, b^Q)
+  const bool ra = (a >> Q) & 1, rb = (b >> Q) & 1;
+  // the quartet's entries by position
+  const float2 b00 = ra ? (rb ? d : r) : (rb ? c : x);
+  const float2 b01 = ra ? (rb ? r : d) : (rb ? x : c);
+  const float2 b10 = ra ? (rb ? c : x) : (rb ? d : r);
+  const float2 b11 = ra ? (rb ? x : c) : (rb ? r : d);
+  const float2 U = cmul(cconj(gb), b01), V = cmul(gb, b10);
+  const float2 P = cadd(b00, b11);
+  const float2 r00 = cadd(P, cadd(U, V));
+  if (!(flags & 1u)) return (ra | rb) ? make_float2(0.f, 0.f) : r00;
+  // nmask holds no bit of slot Q, so the row / column CZ signs are those of a / b
+  const uint32_t sa = czbit(uint32_t(a), nmask), sb = czbit(uint32_t(b), nmask);
+  if (flags & 2u) {
+    const float2 M = csub(b00, b11);
+    if (!ra && !rb) return r00;
+    if (!ra) return flip(cadd(M, csub(V, U)), sb);
+    if (!rb) return flip(cadd(M, csub(U, V)), sa);
+    return flip(csub(P, cadd(U, V)), sa ^ sb);
+  }
+  return flip(r00, (ra ? sa : 0u) ^ (rb ? sb : 0u));
+}
+
+// Coefficient of chunk entry i of this simulation's group (lanes gbase .. gbase+E-1).
+template <int E>
+__device__ __forceinline__ float2 group_coeff(float2 cf, int gbase, int i) {
+  if constexpr (E == 64) {
+    return make_float2(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(cf.x), i)),
+                       __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cf.y), i)));
+  } else {
+    return make_float2(__shfl(cf.x, gbase + i, 64), __shfl(cf.y, gbase + i, 64));
+  }
+}
+
+template <int NS, bool PERIODIC, int SUB>
+__device__ __forceinline__ void dm_wave_sub(float2& x, int a, int b, const mbqc_step* steps,
+                                            float2 cf, int gbase, int k0, int c0, int n) {
+  const int k = k0 + SUB;
+  if (k < n) {
+    const float2 gb = group_coeff<(1 << (2 * NS))>(cf, gbase, c0 + SUB);
+    if constexpr (PERIODIC) {
+      x = dm_wave_step<NS, NS - 1 - SUB>(x, a, b, gb, 3u, periodic_nmask(NS, SUB));
+    } else {
+      const mbqc_step st = steps[k];
+      x = dm_wave_step<NS, NS - 1 - SUB>(x, a, b, gb, st.flags, st.nmask);
+    }
+  }
+}
+
+template <int NS, bool PERIODIC, int... SUBS>
+__device__ __forceinline__ void dm_wave_block(float2& x, int a, int b, const mbqc_step* steps,
+                                              float2 cf, int gbase, int k0, int c0, int n,
+                                              std::integer_sequence<int, SUBS...>) {
+  (dm_wave_sub<NS, PERIODIC, SUBS>(x, a, b, steps, cf, gbase, k0, c0, n), ...);
+}
+
+// Output channel of slot M on the entry held by this lane (as dm_channel).
+template <int NS, int M>
+__device__ __forceinline__ void dm_wave_channel(float2& x, int a, int b, const DmNoise& nz,
+                                                uint32_t live) {
+  constexpr int MB = 1 << M;
+  if (!(live & MB)) return;
+  const float2 p = lane_xor2<(MB | (MB << NS))>(x);   // (a^M, b^M)
+  const bool am = (a >> M) & 1, bm = (b >> M) & 1;
+  if (am != bm) {
+    x = cscale(x, nz.coh);
+  } else {
+    const float2 A = am ? p : x, D = am ? x : p;
+    const float w0 = am ? nz.d10 : nz.d00, w1 = am ? nz.d11 : nz.d01;
+    x = make_float2(fmaf(w0, A.x, w1 * D.x), fmaf(w0, A.y, w1 * D.y));
+  }
+}
+
+template <int NS, int... MS>
+__device__ __forceinline__ void dm_wave_channels(float2& x, int a, int b, const DmNoise& nz,
+                                                 uint32_t live, std::integer_sequence<int, MS...>) {
+  (dm_wave_channel<NS, MS>(x, a, b, nz, live), ...);
+}
+
+// Density matrix with one entry per lane (windows of at most 3 slots: 64
+// entries, so one simulation per wave at 3 slots, 4 at 2 slots, 16 at 1).
+// The step's quartet partners are cross-lane moves at compile-time distances
+// (the measured slot is fixed per sub-step of a block), so a step needs no LDS
+// round trip and no barrier; a 4096-simulation batch of 3-slot windows runs
+// 4096 waves, four per SIMD.  Each lane of a simulation's group computes one
+// step's coefficient gamma e^{-i theta} per chunk of steps, broadcast to the
+// group step by step.
+template <int NS, bool PERIODIC>
+__global__ __launch_bounds__(256) void k_dm_forward_wave(Job J, DmNoise nz, uint32_t flags) {
+  constexpr int H = 1 << NS, E = H * H;
+  static_assert(E <= 64, "one entry per lane");
+  constexpr int SPW = 64 / E;                 // simulations per wave
+  constexpr int CH = (E / NS) * NS;           // steps per coefficient chunk
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = int64_t(xcd_block(blockIdx.x, gridDim.x)) * (blockDim.x >> 6) +
+                       (threadIdx.x >> 6);
+  const int e = lane & (E - 1), gbase = lane & ~(E - 1);
+  const int a = e >> NS, b = e & (H - 1);
+  const int64_t g = wave * SPW + lane / E;
+  const bool active = g < J.total;
+  const int64_t gg = active ? g : J.total - 1;
+  const int64_t bi = gg / J.S, si = gg - bi * J.S;
+  const float* ang = J.angles + bi * J.angle_stride;
+  const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
+  const int32_t* in_index = J.tab + J.off_in;
+  const int32_t* sign = J.tab + J.off_sign;
+  const int32_t* resc = J.tab + J.off_dmres;
+  const int32_t* out_pos = J.tab + J.off_out;
+  const float2* psi = J.psi + si * J.in_stride;
+  float2 x;
+  {
+    const float2 va = flip(psi[in_index[a]], sign[a] ? 0x80000000u : 0u);
+    const float2 vb = flip(psi[in_index[b]], sign[b] ? 0x80000000u : 0u);
+    x = cmul(va, cconj(vb));
+  }
+  const int n = J.n_steps;
+  float2 cf = make_float2(0.f, 0.f);
+  for (int k0 = 0; k0 < n; k0 += NS) {
+    if (resc[k0 / NS]) x = cscale(x, 0x1p-32f);
+    const int c0 = k0 % CH;
+    if (c0 == 0) {
+      // this lane's share of the chunk: the coefficient of step k0 + e
+      const int k = k0 + e;
+      float th = 0.f;
+      if (e < CH && k < n) {
+        const mbqc_step st = steps[k];
+        th = st.angle_index >= 0 ? ang[int64_t(st.angle_index) * J.angle_stride_a] : st.fixed_angle;
+      }
+      cf = cscale(step_coeff<false>(th, nullptr), nz.gamma);
+    }
+    dm_wave_block<NS, PERIODIC>(x, a, b, steps, cf, gbase, k0, c0, n,
+                                std::make_integer_sequence<int, NS>{});
+  }
+  if (nz.out_noise)
+    dm_wave_channels<NS>(x, a, b, nz, uint32_t(J.live_mask), std::make_integer_sequence<int, NS>{});
+  float scale = ldexpf(1.0f, -J.e_dm);
+  if (flags & MBQC_FLAG_RENORMALIZE) {
+    float tr = (a == b && out_pos[a] >= 0) ? x.x : 0.0f;
+    // sum over the E lanes of the group (xor distances below E stay inside it)
+    if constexpr (E > 1) tr += lane_xor<1>(tr);
+    if constexpr (E > 2) tr += lane_xor<2>(tr);
+    if constexpr (E > 4) tr += lane_xor<4>(tr);
+    if constexpr (E > 8) tr += lane_xor<8>(tr);
+    if constexpr (E > 16) tr += lane_xor<16>(tr);
+    if constexpr (E > 32) tr += lane_xor<32>(tr);
+    scale = 1.0f / tr;
+  }
+  if (!active) return;
+  const int pj = out_pos[a], pk = out_pos[b];
+  if (pj >= 0 && pk >= 0) {
+    const int64_t d = int64_t(1) << J.n_out;
+    J.out[g * d * d + int64_t(pj) * d + pk] = cscale(x, scale);
+  }
+}
+
 // ---------------------------------------------------------------- adjoint
 // Forward pass of the adjoint: raw phase, unnormalized output, one checkpoint
 // of the live state at the start of every block of NS steps (after the
@@ -2586,6 +2775,13 @@ constexpr int kMaxSvSlots = 6;       // register-resident SV kernels
 // (one quartet per thread, several simulations per workgroup); the two
 // kernels take the same time at 32,768 simulations of the 3x12 grid
 constexpr int64_t kDmLaneKernelMinSims = 32768;
+// up to this many simulations (by live-window size 0..3) the DM path holds
+// one density-matrix entry per lane (k_dm_forward_wave).  Measured per launch
+// (wave / LDS / one-simulation-per-lane kernels, µs): 3 slots at 1,024
+// simulations 9.8 / 10.4 / 38.6 and at 4,096 16.1 / 12.4 / 39.1; 2 slots at
+// 16,384 10.2 / 10.5 / 12.4 and at 32,768 15.9 / 13.1 / 13.1; 1 slot at 32,768
+// 4.8 / - / 5.1.
+constexpr int64_t kDmWaveKernelMaxSims[4] = {0, 32768, 16384, 1024};
 constexpr int kMaxSvLdsSlots = 12;   // LDS-resident SV kernel
 constexpr int kMaxDmSlots = 3;       // register-resident DM kernel
 constexpr int kMaxDmLdsSlots = 6;    // LDS-resident DM kernel
@@ -2962,6 +3158,21 @@ hipError_t launch_sv_lds(const Job& J, uint32_t flags, hipStream_t st) {
 }
 
 template <int NS>
+hipError_t launch_dm_wave(const Job& J, const DmNoise& nz, uint32_t flags, bool periodic,
+                          hipStream_t st) {
+  constexpr int SPW = 64 >> (2 * NS);               // simulations per wave
+  const int64_t waves = (J.total + SPW - 1) / SPW;
+  const int64_t blocks = (waves + 3) / 4;           // 4 waves per workgroup
+  if (periodic)
+    hipLaunchKernelGGL((k_dm_forward_wave<NS, true>), dim3(unsigned(blocks)), dim3(256), 0, st, J,
+                       nz, flags);
+  else
+    hipLaunchKernelGGL((k_dm_forward_wave<NS, false>), dim3(unsigned(blocks)), dim3(256), 0, st, J,
+                       nz, flags);
+  return hipGetLastError();
+}
+
+template <int NS>
 hipError_t launch_dm_lds(const Job& J, const DmNoise& nz, uint32_t flags, bool periodic,
                          hipStream_t st) {
   constexpr int Q = 1 << (2 * (NS - 1));          // quartets per step
@@ -3039,15 +3250,19 @@ int dm_forward(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
     return e == hipSuccess ? MBQC_OK : hip_fail(e, "dm_forward launch");
   }
   // Small batches of small windows leave the one-simulation-per-lane kernel
-  // with a few waves on a few CUs; there the LDS kernel (one quartet per
-  // thread, several small simulations per 64-lane workgroup) spreads the work
+  // with a few waves on a few CUs; there the one-entry-per-lane kernel (up to
+  // kDmWaveKernelMaxSims[ns] simulations) or the LDS kernel spread the work
   // over the whole chip.
   const bool small = J.total < kDmLaneKernelMinSims;
+  const bool wave = p->ns <= 3 && J.total <= kDmWaveKernelMaxSims[p->ns];
   const bool pe = p->periodic && !(flags & MBQC_FLAG_TABLE_STEPS);
   switch (p->ns) {
-    case 1: e = small ? launch_dm_lds<1>(J, nz, flags, pe, st) : launch_dm<1>(J, nz, flags, pe, st); break;
-    case 2: e = small ? launch_dm_lds<2>(J, nz, flags, pe, st) : launch_dm<2>(J, nz, flags, pe, st); break;
-    case 3: e = small ? launch_dm_lds<3>(J, nz, flags, pe, st) : launch_dm<3>(J, nz, flags, pe, st); break;
+    case 1: e = wave ? launch_dm_wave<1>(J, nz, flags, pe, st) : launch_dm<1>(J, nz, flags, pe, st); break;
+    case 2: e = wave ? launch_dm_wave<2>(J, nz, flags, pe, st) : launch_dm<2>(J, nz, flags, pe, st); break;
+    case 3:
+      e = wave ? launch_dm_wave<3>(J, nz, flags, pe, st)
+               : (small ? launch_dm_lds<3>(J, nz, flags, pe, st) : launch_dm<3>(J, nz, flags, pe, st));
+      break;
     case 4: e = launch_dm_lds<4>(J, nz, flags, pe, st); break;
     case 5: e = launch_dm_lds<5>(J, nz, flags, pe, st); break;
     case 6: e = launch_dm_lds<6>(J, nz, flags, pe, st); break;
