@@ -2782,13 +2782,21 @@ constexpr int64_t kDmLaneKernelMinSims = 32768;
 // 16,384 10.2 / 10.5 / 12.4 and at 32,768 15.9 / 13.1 / 13.1; 1 slot at 32,768
 // 4.8 / - / 5.1.
 constexpr int64_t kDmWaveKernelMaxSims[4] = {0, 32768, 16384, 1024};
-constexpr int kMaxSvLdsSlots = 12;   // LDS-resident SV kernel
+constexpr int kMaxSvLdsSlots = 14;   // LDS-resident SV kernel (2^14 amplitudes: 128 KB)
 constexpr int kMaxDmSlots = 3;       // register-resident DM kernel
-constexpr int kMaxDmLdsSlots = 6;    // LDS-resident DM kernel
+constexpr int kMaxDmLdsSlots = 7;    // LDS-resident DM kernel (4^7 entries: 128 KB)
 constexpr int kMaxAdjSlots = 5;
 
 int hip_fail(hipError_t e, const char* what) {
   return fail(MBQC_ERR_RUNTIME, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Dynamic LDS above 64 KB (windows of 14 SV / 7 DM slots take 128 KB of the
+// 160 KB per workgroup) is requested from the runtime for the kernel first.
+int allow_lds(const void* kernel, size_t bytes) {
+  if (bytes <= 65536) return MBQC_OK;
+  hipError_t e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
+  return e == hipSuccess ? MBQC_OK : hip_fail(e, "hipFuncSetAttribute(dynamic LDS)");
 }
 
 // Upload (once per device) and return the plan's packed tables.
@@ -3243,6 +3251,8 @@ int dm_forward(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
     if (J.total > 0x7fffffff) return fail(MBQC_ERR_VALUE, "too many simulations for one launch");
     WindowMap none{};
     const size_t lds = sizeof(float2) << (2 * p->ns);
+    if (int rl = allow_lds(reinterpret_cast<const void*>(&k_dm_general), lds); rl != MBQC_OK)
+      return rl;
     hipLaunchKernelGGL(k_dm_general, dim3(unsigned(J.total)), dim3(64), lds, st, J,
                        static_cast<const float*>(nullptr), int64_t(0), int32_t(p->n_steps), none, 0,
                        nz, flags, outcomes);
@@ -3266,6 +3276,7 @@ int dm_forward(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
     case 4: e = launch_dm_lds<4>(J, nz, flags, pe, st); break;
     case 5: e = launch_dm_lds<5>(J, nz, flags, pe, st); break;
     case 6: e = launch_dm_lds<6>(J, nz, flags, pe, st); break;
+    case 7: e = launch_dm_lds<7>(J, nz, flags, pe, st); break;
   }
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "dm_forward launch");
 }
@@ -3458,6 +3469,8 @@ int mbqc_sv_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
       case 10: e = launch_sv_lds<10>(J, flags, st); break;
       case 11: e = launch_sv_lds<11>(J, flags, st); break;
       case 12: e = launch_sv_lds<12>(J, flags, st); break;
+      case 13: e = launch_sv_lds<13>(J, flags, st); break;
+      case 14: e = launch_sv_lds<14>(J, flags, st); break;
     }
     return e == hipSuccess ? MBQC_OK : hip_fail(e, "sv_forward launch");
   }
@@ -3474,6 +3487,8 @@ int mbqc_sv_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
     case 10: e = launch_sv_lds<10>(J, flags, st); break;
     case 11: e = launch_sv_lds<11>(J, flags, st); break;
     case 12: e = launch_sv_lds<12>(J, flags, st); break;
+    case 13: e = launch_sv_lds<13>(J, flags, st); break;
+    case 14: e = launch_sv_lds<14>(J, flags, st); break;
   }
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "sv_forward launch");
 }
@@ -3543,6 +3558,7 @@ int mbqc_sv_window_c64(const mbqc_plan* plan, const float* step_angles, int64_t 
   if (rc != MBQC_OK) return rc;
   if (J.total == 0) return MBQC_OK;
   const size_t lds = sizeof(float2) << p->ns;
+  if (int rl = allow_lds(reinterpret_cast<const void*>(&k_sv_window), lds); rl != MBQC_OK) return rl;
   hipLaunchKernelGGL(k_sv_window, dim3(J.total), dim3(64), lds, static_cast<hipStream_t>(stream), J,
                      step_angles, step_stride, k, map, flags);
   hipError_t e = hipGetLastError();
@@ -3564,6 +3580,10 @@ int mbqc_dm_window_c64(const mbqc_plan* plan, const float* step_angles, int64_t 
   if (J.total == 0) return MBQC_OK;
   if (J.total > 0x7fffffff) return fail(MBQC_ERR_VALUE, "too many simulations for one launch");
   const size_t lds = sizeof(float2) << (2 * p->ns);
+  rc = allow_lds(p->general_planes || outcomes ? reinterpret_cast<const void*>(&k_dm_general)
+                                                : reinterpret_cast<const void*>(&k_dm_window),
+                 lds);
+  if (rc != MBQC_OK) return rc;
   if (p->general_planes || outcomes)
     hipLaunchKernelGGL(k_dm_general, dim3(unsigned(J.total)), dim3(64), lds,
                        static_cast<hipStream_t>(stream), J, step_angles, step_stride, k, map, 1, nz,

@@ -8,7 +8,7 @@
 
 namespace mbqc {
 
-constexpr int kMaxSlots = 12;   // host tables; kernels support fewer (see launchers)
+constexpr int kMaxSlots = 14;   // host tables; kernels support fewer (see launchers)
 constexpr int kMaxDevices = 16;
 constexpr int kMaxCoefSlots = 6;
 // largest pair-coefficient table kept with one row per step

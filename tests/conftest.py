@@ -27,6 +27,12 @@ def rng():
 
 
 def haar_state(n_qubits, rng):
+    """Haar-random pure state: the first column of a Haar unitary, or (above
+    10 qubits, where the unitary's QR would dominate a test) a normalised
+    complex Gaussian vector, which has the same distribution."""
+    if n_qubits > 10:
+        v = rng.normal(size=2 ** n_qubits) + 1j * rng.normal(size=2 ** n_qubits)
+        return v / np.linalg.norm(v)
     from mentpy_amd.operators.gates import haar_unitary
     return haar_unitary(2 ** n_qubits, rng)[:, 0]
 

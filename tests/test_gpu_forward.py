@@ -43,9 +43,10 @@ def test_sv_grid_matches_oracle(rows, cols, rng):
     assert infidelity(out, ref).max() < FID_TOL
 
 
-@pytest.mark.parametrize("rows,cols", [(7, 3), (8, 3), (9, 3), (10, 3), (12, 3)])
+@pytest.mark.parametrize("rows,cols", [(7, 3), (8, 3), (9, 3), (10, 3), (12, 3), (13, 3), (14, 3)])
 def test_sv_large_windows_lds_kernel(rows, cols, rng):
-    """Live windows of 7..12 qubits run in the LDS-resident kernel."""
+    """Live windows of 7..14 qubits run in the LDS-resident kernel (14 slots:
+    2^14 amplitudes, 128 KB of LDS)."""
     torch = _torch()
     import mentpy_amd as mp
     from oracle.vectorized import sv_batch
@@ -61,8 +62,9 @@ def test_sv_large_windows_lds_kernel(rows, cols, rng):
     assert err < AMP_TOL
 
 
-@pytest.mark.parametrize("rows,cols", [(4, 4), (5, 3), (6, 3)])
+@pytest.mark.parametrize("rows,cols", [(4, 4), (5, 3), (6, 3), (7, 3)])
 def test_dm_large_windows_lds_kernel(rows, cols, rng):
+    """Density matrices of 4..7 live qubits in LDS (7 slots: 4^7 entries, 128 KB)."""
     torch = _torch()
     import mentpy_amd as mp
     from oracle.vectorized import dm_batch
@@ -397,3 +399,47 @@ def test_fresh_plan_first_launch_inside_graph_capture(rng):
         torch.cuda.synchronize()
         assert torch.equal(out, want), f"replay {rep} differs from the eager launch"
         _ = (want * 2).sum().item()          # eager work between replays
+
+
+def test_sv_14_slot_measure_windows_end_at_run_output(rng):
+    """Step-by-step measure() on a 14-slot window (the window kernel's state
+    takes 128 KB of dynamic LDS): after the last measurement the window is the
+    output state, equal to run(output_form="sv") with the same angles."""
+    _torch()
+    import mentpy_amd as mp
+    g = mp.templates.grid_cluster(14, 3)
+    psi = haar_state(14, rng)
+    th = rng.uniform(0, 2 * np.pi, len(g.trainable_nodes)).astype(np.float32)
+    sim = mp.PatternSimulator(g, input_state=psi, backend="hip-sv")
+    assert sim.simulator.plan.info["n_slots"] == 14
+    want = sim.run(th, output_form="sv")
+    sim.reset(input_state=psi)
+    tpos = {v: i for i, v in enumerate(g.trainable_nodes)}
+    got = None
+    for node in sim.simulator.schedule_measure:
+        got, out = sim.measure(float(th[tpos[node]]))
+        assert out == 0
+    err = float(np.abs(got - want).max())
+    print(f"14-slot windows: final window vs run() {err:.2e}", flush=True)
+    assert got.shape == want.shape and err < 1e-5
+
+
+def test_dm_7_slot_measure_windows_end_at_run_output(rng):
+    """Step-by-step measure() on a 7-slot density matrix (window kernel: 128 KB
+    of dynamic LDS): without noise the last window is run()'s output."""
+    _torch()
+    import mentpy_amd as mp
+    g = mp.templates.grid_cluster(7, 3)
+    psi = haar_state(7, rng)
+    th = rng.uniform(0, 2 * np.pi, len(g.trainable_nodes)).astype(np.float32)
+    sim = mp.PatternSimulator(g, input_state=psi, backend="hip-dm")
+    assert sim.simulator.plan.info["n_slots"] == 7
+    want = sim.run(th)
+    sim.reset(input_state=psi)
+    tpos = {v: i for i, v in enumerate(g.trainable_nodes)}
+    got = None
+    for node in sim.simulator.schedule_measure:
+        got, _ = sim.measure(float(th[tpos[node]]))
+    err = float(np.abs(got - want).max())
+    print(f"7-slot DM windows: final window vs run() {err:.2e}", flush=True)
+    assert got.shape == want.shape and err < 1e-5

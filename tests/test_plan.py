@@ -172,15 +172,17 @@ def test_kernel_limits_are_reported(native):
     """Windows beyond the kernels' reach fail with NotImplementedError before
     any device work (the reference has no limit, but no configuration needs more)."""
     from mentpy_amd.simulators import compile_pattern
-    plan, *_ = compile_pattern(mp.templates.grid_cluster(7, 8), exclude="quantum_output")
-    assert plan.info["n_slots"] == 7
+    plan, *_ = compile_pattern(mp.templates.grid_cluster(8, 9), exclude="quantum_output")
+    assert plan.info["n_slots"] == 8
     L = native.lib()
     rc = L.mbqc_dm_forward_c64(plan.handle, None, 0, 1, None, 0, None, 1, 1, None, 0, None)
-    assert rc == native.MBQC_ERR_NOT_IMPLEMENTED
+    assert rc == native.MBQC_ERR_NOT_IMPLEMENTED          # density matrices: at most 7 slots
     rc = L.mbqc_sv_adjoint_forward_c64(plan.handle, None, 0, 1, None, 0, None, 1, 1, None, 0, None)
-    assert rc == native.MBQC_ERR_NOT_IMPLEMENTED
+    assert rc == native.MBQC_ERR_NOT_IMPLEMENTED          # adjoint: at most 5 slots
+    plan14, *_ = compile_pattern(mp.templates.grid_cluster(14, 15))
+    assert plan14.info["n_slots"] == 14                     # statevectors: up to 14 slots
     with pytest.raises(NotImplementedError):
-        compile_pattern(mp.templates.grid_cluster(13, 14))
+        compile_pattern(mp.templates.grid_cluster(15, 16))
 
 
 def test_adjoint_rejects_shared_angle_entries(native):
