@@ -1651,6 +1651,193 @@ __global__ __launch_bounds__(64) void k_sv_adj_backward(Job J, const float2* __r
   }
 }
 
+// ---------------------------------------------------- adjoint, LDS-resident
+// Windows of 6..kMaxAdjLdsSlots live slots: one workgroup per simulation in
+// the forward, the state in LDS (as k_sv_forward_lds, raw phase), a
+// checkpoint of the state at the start of every ck-th block (after the
+// block's rescale) stored contiguously per simulation:
+// ckpt[(c * total + g) * 2^NS + j].
+constexpr int kMaxAdjLdsSlots = 10;
+
+template <int NS, int T>
+__global__ __launch_bounds__(T) void k_sv_adj_forward_lds(Job J, float2* __restrict__ ckpt, int ck) {
+  constexpr int H = 1 << NS;
+  __shared__ float2 st[H];
+  __shared__ float th_s[kAngleChunk];
+  const int64_t g = xcd_block(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+  const int64_t bi = g / J.S, si = g - bi * J.S;
+  const float* ang = J.angles + bi * J.angle_stride;
+  const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
+  const int32_t* in_index = J.tab + J.off_in;
+  const int32_t* sign = J.tab + J.off_sign;
+  const int32_t* resc = J.tab + J.off_svres;
+  const int32_t* out_pos = J.tab + J.off_out;
+  const float2* psi = J.psi + si * J.in_stride;
+  for (int j = tid; j < H; j += T) st[j] = flip(psi[in_index[j]], sign[j] ? 0x80000000u : 0u);
+  for (int k = 0; k < J.n_steps; ++k) {
+    if (k % NS == 0) {
+      const int blk = k / NS;
+      if (resc[blk]) {
+        __syncthreads();
+        for (int j = tid; j < H; j += T) st[j] = cscale(st[j], 0x1p-32f);
+      }
+      if ((blk & (ck - 1)) == 0) {
+        __syncthreads();
+        float2* cp = ckpt + (int64_t(blk >> (ck - 1)) * J.total + g) * H;
+        for (int j = tid; j < H; j += T) cp[j] = st[j];
+      }
+    }
+    if (k % kAngleChunk == 0) {
+      __syncthreads();
+      stage_angles<T>(th_s, steps, ang, J.angle_stride_a, k, J.n_steps, tid);
+      __syncthreads();
+    }
+    const mbqc_step step = steps[k];
+    const float2 b = step_coeff<false>(th_s[k % kAngleChunk], nullptr);
+    __syncthreads();
+    sv_lds_step<T>(st, H, step, b, tid);
+  }
+  __syncthreads();
+  const float scale = sv_final_scale(J.e_sv);
+  float2* o = J.out + g * (int64_t(1) << J.n_out);
+  for (int j = tid; j < H; j += T) {
+    const int p = out_pos[j];
+    if (p >= 0) o[p] = cscale(st[j], scale);
+  }
+}
+
+// The angle of step k of this simulation (its angle-vector entry or its fixed angle).
+__device__ __forceinline__ float step_angle(const mbqc_step& st, const float* ang, int64_t sa) {
+  return st.angle_index >= 0 ? ang[int64_t(st.angle_index) * sa] : st.fixed_angle;
+}
+
+// Block [k0, k0 + NS) of the LDS adjoint: recompute the block forward from
+// `cur` (its state at the block start), keeping every step's measured half
+// (the pre-step amplitudes with slot Q = 1) in ys, then run the adjoint steps
+// in reverse on lam, adding each angle's contribution to acc (one writer).
+// Adjoint of new0 = u0 = x + b y, new1 = al u0 + be x (al, be from the step
+// flags as in sv_lds_step): h = lam0 + al lam1, lam_x = h + be lam1,
+// lam_y = conj(b) h, dL/dtheta += Im(b sum y conj(h)).
+template <int NS, int T>
+__device__ void adj_block_lds(float2* cur, float2* lam, float2* ys, float* red, float* acc,
+                              const mbqc_step* steps, const float* ang, int64_t sa, int k0, int n,
+                              int tid) {
+  constexpr int H = 1 << NS, HP = H / 2;
+  const int kend = k0 + NS < n ? k0 + NS : n;
+  for (int k = k0; k < kend; ++k) {
+    const mbqc_step st = steps[k];
+    const int q = int((st.flags >> 8) & 0xffu), QB = 1 << q;
+    __syncthreads();
+    for (int i = tid; i < HP; i += T) ys[(k - k0) * HP + i] = cur[insert_zero(i, q) | QB];
+    if (k + 1 < kend) {
+      const float2 b = step_coeff<false>(step_angle(st, ang, sa), nullptr);
+      __syncthreads();
+      sv_lds_step<T>(cur, H, st, b, tid);
+    }
+  }
+  for (int k = kend - 1; k >= k0; --k) {
+    const mbqc_step st = steps[k];
+    const int q = int((st.flags >> 8) & 0xffu), QB = 1 << q;
+    const float s = (st.flags & 2u) ? -1.0f : 1.0f;
+    const float app = (st.flags & 1u) ? 1.0f : 0.0f;
+    const float2 b = step_coeff<false>(step_angle(st, ang, sa), nullptr);
+    const float2 bc = cconj(b);
+    float ar = 0.f, ai = 0.f;
+    __syncthreads();
+    for (int i = tid; i < HP; i += T) {
+      const int j = insert_zero(i, q);
+      const float c = (__builtin_popcount(unsigned(j) & st.nmask) & 1) ? -app : app;
+      const float al = c * s, be = c * (1.0f - s);
+      const float2 l0 = lam[j], l1 = lam[j | QB];
+      const float2 h = make_float2(fmaf(al, l1.x, l0.x), fmaf(al, l1.y, l0.y));
+      const float2 yy = ys[(k - k0) * HP + i];
+      ar = fmaf(yy.x, h.x, fmaf(yy.y, h.y, ar));
+      ai = fmaf(yy.y, h.x, fmaf(-yy.x, h.y, ai));
+      lam[j] = make_float2(fmaf(be, l1.x, h.x), fmaf(be, l1.y, h.y));
+      lam[j | QB] = cmul(bc, h);
+    }
+    float d = fmaf(b.x, ai, b.y * ar);
+    d = block_sum<T>(d, red);
+    if (tid == 0 && st.angle_index >= 0) acc[st.angle_index] += d;
+  }
+}
+
+// Backward sweep of the LDS adjoint: one workgroup per (parameter set,
+// partial-sum row), walking its A.red columns (all of them when A.red == C)
+// one after the other; the per-angle sums stay in LDS and the row is stored
+// once.  Dynamic LDS: lam, cur (2^NS each), ys (NS x 2^(NS-1)), acc (n_angles).
+template <int NS, int T>
+__global__ __launch_bounds__(T) void k_sv_adj_backward_lds(Job J, const float2* __restrict__ ckpt,
+                                                           const float2* __restrict__ seed, AdjCtx A,
+                                                           float* __restrict__ gpart) {
+  constexpr int H = 1 << NS, HP = H / 2;
+  extern __shared__ float2 dyn[];
+  float2* lam = dyn;
+  float2* cur = lam + H;
+  float2* ys = cur + H;
+  float* acc = reinterpret_cast<float*>(ys + NS * HP);
+  __shared__ float red[4];
+  const int tid = threadIdx.x;
+  const int64_t row = xcd_block(blockIdx.x, gridDim.x);   // bi * P + r
+  const int64_t bi = row / A.P, r = row - bi * A.P;
+  const int ncol = A.red;
+  const float* ang = J.angles + bi * J.angle_stride;
+  const int64_t sa = J.angle_stride_a;
+  const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
+  const int32_t* resc = J.tab + J.off_svres;
+  const int32_t* out_pos = J.tab + J.off_out;
+  const float scale = sv_final_scale(J.e_sv);
+  const int n = J.n_steps;
+  const int nblk = (n + NS - 1) / NS;
+  const int ck = A.ck;
+  const int ngrp = (nblk + ck - 1) / ck;
+  for (int a = tid; a < A.n_angles; a += T) acc[a] = 0.0f;
+  for (int cc = 0; cc < ncol; ++cc) {
+    const int64_t g = bi * A.C + r * ncol + cc;   // simulation (set bi, column)
+    const float2* sd = seed + g * (int64_t(1) << J.n_out);
+    __syncthreads();
+    for (int j = tid; j < H; j += T) {
+      const int p = out_pos[j];
+      lam[j] = p >= 0 ? cscale(sd[p], scale) : make_float2(0.f, 0.f);
+    }
+    for (int grp = ngrp - 1; grp >= 0; --grp) {
+      const float2* cp = ckpt + (int64_t(grp) * J.total + g) * H;
+      const int b0 = grp * ck;
+      if (ck == 2 && b0 + 1 < nblk) {
+        // the state at block b0 + 1: block b0 from the checkpoint, then b0 + 1's rescale
+        __syncthreads();
+        for (int j = tid; j < H; j += T) cur[j] = cp[j];
+        for (int k = b0 * NS; k < (b0 + 1) * NS; ++k) {
+          const mbqc_step st = steps[k];
+          const float2 b = step_coeff<false>(step_angle(st, ang, sa), nullptr);
+          __syncthreads();
+          sv_lds_step<T>(cur, H, st, b, tid);
+        }
+        if (resc[b0 + 1]) {
+          __syncthreads();
+          for (int j = tid; j < H; j += T) cur[j] = cscale(cur[j], 0x1p-32f);
+        }
+        adj_block_lds<NS, T>(cur, lam, ys, red, acc, steps, ang, sa, (b0 + 1) * NS, n, tid);
+        if (resc[b0 + 1]) {
+          __syncthreads();
+          for (int j = tid; j < H; j += T) lam[j] = cscale(lam[j], 0x1p-32f);
+        }
+      }
+      __syncthreads();
+      for (int j = tid; j < H; j += T) cur[j] = cp[j];
+      adj_block_lds<NS, T>(cur, lam, ys, red, acc, steps, ang, sa, b0 * NS, n, tid);
+      if (resc[b0]) {
+        __syncthreads();
+        for (int j = tid; j < H; j += T) lam[j] = cscale(lam[j], 0x1p-32f);
+      }
+    }
+  }
+  __syncthreads();
+  float* prow = gpart + row * A.n_angles;
+  for (int a = tid; a < A.n_angles; a += T) prow[a] = acc[a];
+}
+
 // ---------------------------------------------------------------- segmented adjoint
 // Transfer-matrix adjoint cut in time.  The blocks of the pattern are split
 // into segments p = 0..n-1 at block boundaries; segment p's transfer matrix
@@ -2852,9 +3039,9 @@ int check_xy_plan(const Plan* p) {
 int check_adjoint_plan(const Plan* p) {
   int rc = check_xy_plan(p);
   if (rc != MBQC_OK) return rc;
-  if (p->ns > kMaxAdjSlots)
+  if (p->ns > kMaxAdjLdsSlots)
     return fail(MBQC_ERR_NOT_IMPLEMENTED, "adjoint kernels support live windows of at most " +
-                                              std::to_string(kMaxAdjSlots) + " qubits");
+                                              std::to_string(kMaxAdjLdsSlots) + " qubits");
   if (p->shared_angles)
     return fail(MBQC_ERR_NOT_IMPLEMENTED,
                 "adjoint gradients need every angle-vector entry to belong to at most one "
@@ -3023,6 +3210,38 @@ hipError_t launch_adj_bwd(const Job& J, const float2* ckpt, const float2* seed, 
   else
     hipLaunchKernelGGL((k_sv_adj_backward<NS, false>), dim3(blocks), dim3(threads), 0, st, J, ckpt,
                        seed, A, A.gpart);
+  return hipGetLastError();
+}
+
+template <int NS>
+hipError_t launch_adj_fwd_lds(const Job& J, float2* ckpt, int ck, hipStream_t st) {
+  constexpr int T = (1 << (NS - 1)) < 256 ? (1 << (NS - 1)) : 256;
+  hipLaunchKernelGGL((k_sv_adj_forward_lds<NS, T>), dim3(unsigned(J.total)), dim3(T), 0, st, J, ckpt,
+                     ck);
+  return hipGetLastError();
+}
+
+// Dynamic LDS of k_sv_adj_backward_lds: lam and cur (2^ns each), a block's
+// measured halves (ns x 2^(ns-1)) and the per-angle sums.
+inline size_t adj_lds_bytes(int ns, int n_angles) {
+  const size_t H = size_t(1) << ns;
+  return (2 * H + size_t(ns) * (H / 2)) * sizeof(float2) + size_t(n_angles) * sizeof(float);
+}
+constexpr size_t kMaxDynLds = 160 * 1024 - 256;   // gfx950: 160 KiB per workgroup, less static LDS
+
+template <int NS>
+hipError_t launch_adj_bwd_lds(const Job& J, const float2* ckpt, const float2* seed, const AdjCtx& A,
+                              hipStream_t st) {
+  constexpr int T = (1 << (NS - 1)) < 256 ? (1 << (NS - 1)) : 256;
+  const size_t lds = adj_lds_bytes(NS, A.n_angles);
+  const void* kern = reinterpret_cast<const void*>(&k_sv_adj_backward_lds<NS, T>);
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (e != hipSuccess) return e;
+  }
+  const int64_t rows = (J.total / A.C) * A.P;
+  hipLaunchKernelGGL((k_sv_adj_backward_lds<NS, T>), dim3(unsigned(rows)), dim3(T), lds, st, J, ckpt,
+                     seed, A, A.gpart);
   return hipGetLastError();
 }
 
@@ -3203,6 +3422,9 @@ int seg_setup(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
   Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(plan));
   int rc0 = check_adjoint_plan(p);
   if (rc0 != MBQC_OK) return rc0;
+  if (p->ns > kMaxAdjSlots)
+    return fail(MBQC_ERR_NOT_IMPLEMENTED, "the segmented adjoint supports live windows of at most " +
+                                              std::to_string(kMaxAdjSlots) + " qubits");
   if (n_segments < 2 || n_segments > std::min(kMaxSegments, p->n_blocks()))
     return fail(MBQC_ERR_VALUE, "n_segments must be in 2.." +
                                     std::to_string(std::min(kMaxSegments, p->n_blocks())));
@@ -3621,6 +3843,7 @@ size_t mbqc_sv_adjoint_workspace_bytes(const mbqc_plan* plan, int64_t n_columns)
 int32_t mbqc_sv_segments_auto(const mbqc_plan* plan, int64_t B) {
   if (!plan || B <= 0) return 1;
   const Plan* p = reinterpret_cast<const Plan*>(plan);
+  if (p->ns > kMaxAdjSlots) return 1;   // larger windows: the unsegmented LDS sweeps
   // Measured on the 4x16 and 4x20 grids (16 lanes per set): the best count
   // falls from 4 segments at 16,384 lanes to 1 at 131,072 and more, about as
   // 4 sqrt(16384 / lanes); more segments add chain work per set.
@@ -3716,6 +3939,11 @@ int mbqc_sv_adjoint_forward_c64(const mbqc_plan* plan, const float* angles, int6
     case 3: e = launch_adj_fwd<3>(J, ck, adj_ckpt_blocks(B * C), p->periodic, st); break;
     case 4: e = launch_adj_fwd<4>(J, ck, adj_ckpt_blocks(B * C), p->periodic, st); break;
     case 5: e = launch_adj_fwd<5>(J, ck, adj_ckpt_blocks(B * C), p->periodic, st); break;
+    case 6: e = launch_adj_fwd_lds<6>(J, ck, adj_ckpt_blocks(B * C), st); break;
+    case 7: e = launch_adj_fwd_lds<7>(J, ck, adj_ckpt_blocks(B * C), st); break;
+    case 8: e = launch_adj_fwd_lds<8>(J, ck, adj_ckpt_blocks(B * C), st); break;
+    case 9: e = launch_adj_fwd_lds<9>(J, ck, adj_ckpt_blocks(B * C), st); break;
+    case 10: e = launch_adj_fwd_lds<10>(J, ck, adj_ckpt_blocks(B * C), st); break;
   }
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "adjoint forward launch");
 }
@@ -3744,6 +3972,10 @@ int mbqc_sv_adjoint_backward_c64(const mbqc_plan* plan, const float* angles, int
   A.red = (C <= 64 && (C & (C - 1)) == 0) ? int32_t(C) : (C % 64 == 0 ? 64 : 1);
   A.n_angles = p->n_angles;
   A.ck = adj_ckpt_blocks(B * C);
+  if (p->ns > kMaxAdjSlots && adj_lds_bytes(p->ns, p->n_angles) > kMaxDynLds)
+    return fail(MBQC_ERR_NOT_IMPLEMENTED,
+                "adjoint of a " + std::to_string(p->ns) + "-slot window with " +
+                    std::to_string(p->n_angles) + " angles exceeds the workgroup's LDS");
   hipStream_t st = static_cast<hipStream_t>(stream);
   const float2* ck = static_cast<const float2*>(ckpt);
   const float2* sd = static_cast<const float2*>(seed);
@@ -3754,6 +3986,11 @@ int mbqc_sv_adjoint_backward_c64(const mbqc_plan* plan, const float* angles, int
     case 3: e = launch_adj_bwd<3>(J, ck, sd, A, p->periodic, st); break;
     case 4: e = launch_adj_bwd<4>(J, ck, sd, A, p->periodic, st); break;
     case 5: e = launch_adj_bwd<5>(J, ck, sd, A, p->periodic, st); break;
+    case 6: e = launch_adj_bwd_lds<6>(J, ck, sd, A, st); break;
+    case 7: e = launch_adj_bwd_lds<7>(J, ck, sd, A, st); break;
+    case 8: e = launch_adj_bwd_lds<8>(J, ck, sd, A, st); break;
+    case 9: e = launch_adj_bwd_lds<9>(J, ck, sd, A, st); break;
+    case 10: e = launch_adj_bwd_lds<10>(J, ck, sd, A, st); break;
   }
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "adjoint backward launch");
 }
