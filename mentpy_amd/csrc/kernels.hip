@@ -1091,6 +1091,81 @@ __global__ __launch_bounds__(T) void k_sv_forward_lds(Job J, uint32_t flags) {
   if (tid == 0 && (flags & MBQC_FLAG_WRITE_PHASE) && J.phase_out) J.phase_out[g] = ph;
 }
 
+// Sampled outcomes for windows of 7..14 slots: k_sv_sample's frame and draw
+// arithmetic (SampleMod, evaluated identically by every thread of the
+// workgroup, so the frame is uniform) on the LDS-resident state of
+// k_sv_forward_lds.  Raw phase; the output is renormalised and leaves with the
+// output frame applied, as in k_sv_sample.
+template <int NS, int T>
+__global__ __launch_bounds__(T) void k_sv_sample_lds(Job J, const uint64_t* __restrict__ xm,
+                                                     const uint64_t* __restrict__ zm,
+                                                     const int32_t* __restrict__ out_off,
+                                                     int32_t* __restrict__ outcomes, int32_t words,
+                                                     uint64_t seed) {
+  constexpr int H = 1 << NS;
+  __shared__ float2 st[H];
+  __shared__ float red[4];
+  __shared__ float th_s[kAngleChunk];
+  const int64_t g = xcd_block(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+  const int64_t bi = g / J.S, si = g - bi * J.S;
+  const float* ang = J.angles + bi * J.angle_stride;
+  const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
+  const int32_t* in_index = J.tab + J.off_in;
+  const int32_t* sign = J.tab + J.off_sign;
+  const int32_t* resc = J.tab + J.off_svres;
+  const int32_t* out_pos = J.tab + J.off_out;
+  const float2* psi = J.psi + si * J.in_stride;
+  for (int j = tid; j < H; j += T) st[j] = flip(psi[in_index[j]], sign[j] ? 0x80000000u : 0u);
+  int32_t* row = outcomes + g * (words + 2);
+  SampleMod mod;
+  mod.seed = seed;
+  mod.sim = uint64_t(g);
+  mod.xm = xm;
+  mod.zm = zm;
+  mod.n = J.n_steps;
+  mod.out_words = tid == 0 ? row : nullptr;
+  for (int k = 0; k < J.n_steps; ++k) {
+    if (k % NS == 0 && resc[k / NS]) {
+      __syncthreads();
+      for (int j = tid; j < H; j += T) st[j] = cscale(st[j], 0x1p-32f);
+    }
+    if (k % kAngleChunk == 0) {
+      __syncthreads();
+      stage_angles<T>(th_s, steps, ang, J.angle_stride_a, k, J.n_steps, tid);
+      __syncthreads();
+    }
+    const mbqc_step step = steps[k];
+    const float2 b = mod(k, step_coeff<false>(th_s[k % kAngleChunk], nullptr));
+    __syncthreads();
+    sv_lds_step<T>(st, H, step, b, tid);
+  }
+  __syncthreads();
+  uint32_t xo = 0, zo = 0;
+  for (int m = 0; m < J.n_out; ++m) {
+    const int off = out_off[m];
+    xo |= ((mod.fx >> off) & 1u) << (J.n_out - 1 - m);
+    zo |= ((mod.fz >> off) & 1u) << (J.n_out - 1 - m);
+  }
+  float n2 = 0.0f;
+  for (int j = tid; j < H; j += T)
+    if (out_pos[j] >= 0) n2 = fmaf(st[j].x, st[j].x, fmaf(st[j].y, st[j].y, n2));
+  n2 = block_sum<T>(n2, red);
+  const float scale = 1.0f / sqrtf(n2);
+  float2* o = J.out + g * (int64_t(1) << J.n_out);
+  for (int j = tid; j < H; j += T) {
+    const int p = out_pos[j];
+    if (p >= 0) {
+      const uint32_t q = uint32_t(p) ^ xo;
+      o[q] = flip(cscale(st[j], scale), czbit(q, zo));
+    }
+  }
+  if (tid == 0) {
+    row[words] = int32_t(xo);
+    row[words + 1] = int32_t(zo);
+  }
+}
+
 // Sub-step SUB of the block starting at step k0 (measured slot NS - 1 - SUB);
 // c0 = k0's position in the staged coefficient chunk.
 template <int NS, int T, bool PERIODIC, int SUB>
@@ -3385,6 +3460,17 @@ hipError_t launch_sv_lds(const Job& J, uint32_t flags, hipStream_t st) {
 }
 
 template <int NS>
+hipError_t launch_sample_lds(const Job& J, const int32_t* tab, const TableOffsets& o,
+                             int32_t* outcomes, int32_t words, uint64_t seed, hipStream_t st) {
+  constexpr int T = (1 << (NS - 1)) < 256 ? (1 << (NS - 1)) : 256;
+  const uint64_t* xm = reinterpret_cast<const uint64_t*>(tab + o.xmask);
+  const uint64_t* zm = reinterpret_cast<const uint64_t*>(tab + o.zmask);
+  hipLaunchKernelGGL((k_sv_sample_lds<NS, T>), dim3(unsigned(J.total)), dim3(T), 0, st, J, xm, zm,
+                     tab + o.out_offset, outcomes, words, seed);
+  return hipGetLastError();
+}
+
+template <int NS>
 hipError_t launch_dm_wave(const Job& J, const DmNoise& nz, uint32_t flags, bool periodic,
                           hipStream_t st) {
   constexpr int SPW = 64 >> (2 * NS);               // simulations per wave
@@ -3741,7 +3827,15 @@ int mbqc_sv_sample_c64(const mbqc_plan* plan, const float* angles, int64_t angle
     case 4: e = launch_sample<4>(J, tab, o, outcomes, words, seed, p->periodic, st); break;
     case 5: e = launch_sample<5>(J, tab, o, outcomes, words, seed, p->periodic, st); break;
     case 6: e = launch_sample<6>(J, tab, o, outcomes, words, seed, p->periodic, st); break;
-    default: return fail(MBQC_ERR_NOT_IMPLEMENTED, "sampled outcomes need n_slots <= 6");
+    case 7: e = launch_sample_lds<7>(J, tab, o, outcomes, words, seed, st); break;
+    case 8: e = launch_sample_lds<8>(J, tab, o, outcomes, words, seed, st); break;
+    case 9: e = launch_sample_lds<9>(J, tab, o, outcomes, words, seed, st); break;
+    case 10: e = launch_sample_lds<10>(J, tab, o, outcomes, words, seed, st); break;
+    case 11: e = launch_sample_lds<11>(J, tab, o, outcomes, words, seed, st); break;
+    case 12: e = launch_sample_lds<12>(J, tab, o, outcomes, words, seed, st); break;
+    case 13: e = launch_sample_lds<13>(J, tab, o, outcomes, words, seed, st); break;
+    case 14: e = launch_sample_lds<14>(J, tab, o, outcomes, words, seed, st); break;
+    default: return fail(MBQC_ERR_NOT_IMPLEMENTED, "sampled outcomes need n_slots <= 14");
   }
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "sv_sample launch");
 }

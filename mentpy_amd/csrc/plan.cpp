@@ -311,7 +311,7 @@ int compile_plan(const mbqc_pattern_desc* d, Plan* p) {
   // frame of its flow successor and the Z frame of the successor's neighbours
   // measured after schedule[k].  Frames are kept relative to the current
   // step in 32-bit words on the device, so every offset must be below 32
-  // (with no dropped edge and at most 6 live slots they stay below 14).
+  // (with no dropped edge and n live slots they stay below 2n + 2).
   p->can_sample = false;
   if (!d->flow) {
     p->sample_reason = "the pattern description has no flow";
@@ -319,8 +319,8 @@ int compile_plan(const mbqc_pattern_desc* d, Plan* p) {
     p->sample_reason = "nodes are measured out of schedule order";
   } else if (p->n_dropped > 0) {
     p->sample_reason = "the window drops CZ edges";
-  } else if (ns > kMaxCoefSlots) {
-    p->sample_reason = "sampled outcomes support live windows of at most 6 qubits";
+  } else if (ns > kMaxSampleSlots) {
+    p->sample_reason = "sampled outcomes support live windows of at most 14 qubits";
   } else {
     bool ok = true;
     p->xmask.assign(nm, 0);

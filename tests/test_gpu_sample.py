@@ -121,3 +121,62 @@ def test_force0_false_api(rng):
     with pytest.raises(NotImplementedError):
         mp.PatternSimulator(g3, backend="hip-sv", force0=False, window_size=3,
                             schedule=list(g3.measurement_order))
+
+
+@pytest.mark.parametrize("rows,cols,w", [(3, 5, 8), (4, 6, 9), (2, 12, 12), (5, 6, 15)])
+def test_sampled_lds_windows_match_register_kernel(rows, cols, w, rng):
+    """Windows of 7..14 slots run the LDS kernel: with the same seed the
+    outcome rows (bits and output frames) equal the register kernel's on the
+    minimal window bit for bit, and the outputs agree (the draws depend on
+    the seed, the simulation and the step only)."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from conftest import phase_aligned_max_err
+    from mentpy_amd.simulators.hip_simulator import sv_sample, _as_angles, _as_states
+    g = mp.templates.grid_cluster(rows, cols)
+    wide = mp.PatternSimulator(g, backend="hip-sv", force0=False, window_size=w).simulator.plan
+    base = mp.PatternSimulator(g, backend="hip-sv", force0=False).simulator.plan
+    assert wide.info["n_slots"] == w - 1 and wide.info["can_sample"] == 1
+    B, S = 24, 3
+    dev = torch.device("cuda")
+    th = _as_angles(rng.uniform(0, 2 * np.pi, (B, len(g.trainable_nodes))), wide.info["n_angles"], dev)
+    st = _as_states(np.stack([haar_state(rows, rng) for _ in range(S)]), rows, dev)
+    o1, r1 = sv_sample(wide, th, st, seed=2024)
+    o2, r2 = sv_sample(base, th, st, seed=2024)
+    torch.cuda.synchronize()
+    assert torch.equal(r1, r2)
+    err = phase_aligned_max_err(o1.cpu().numpy(), o2.cpu().numpy())
+    print(f"{rows}x{cols} w={w}: max output diff (phase aligned) {err:.2e}", flush=True)
+    assert err < 2e-5
+
+
+def test_sampled_seven_slot_grid_matches_oracle(rng):
+    """7x3 grid (7 live slots, LDS kernel) against the oracle's PennyLane-order
+    circuit with the kernel's outcome strings (21 qubits, brute force)."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from mentpy_amd.simulators.hip_simulator import decode_outcomes, sv_sample, _as_angles, _as_states
+    from oracle.brute import brute_sv_sampled
+    g = mp.templates.grid_cluster(7, 3)
+    sim = mp.PatternSimulator(g, backend="hip-sv", force0=False)
+    plan = sim.simulator.plan
+    assert plan.info["n_slots"] == 7 and plan.info["can_sample"] == 1
+    B = 2
+    th = rng.uniform(0, 2 * np.pi, (B, len(g.trainable_nodes)))
+    X = haar_state(7, rng)[None]
+    dev = torch.device("cuda")
+    out, rows_t = sv_sample(plan, _as_angles(th, plan.info["n_angles"], dev),
+                            _as_states(X, 7, dev), seed=77)
+    torch.cuda.synchronize()
+    n = plan.info["n_steps"]
+    W = (n + 31) // 32
+    bits = decode_outcomes(rows_t, n)
+    frames = rows_t[..., W:].cpu().numpy()
+    meas = sim.simulator.schedule_measure
+    for b in range(B):
+        want, prob, fx, fz = brute_sv_sampled(g, th[b], X[0],
+                                              {v: int(bits[b, 0, k]) for k, v in enumerate(meas)})
+        inf = float(infidelity(out[b, 0].cpu().numpy(), want))
+        print(f"7x3 sampled b={b}: 1-F {inf:.2e}", flush=True)
+        assert inf < 1e-5
+        assert (int(frames[b, 0, 0]), int(frames[b, 0, 1])) == (fx, fz)
