@@ -1037,13 +1037,12 @@ __device__ __forceinline__ void dm_lds_step(float2* r, int H, int q, uint32_t fl
   }
 }
 
+// Simulation g of job J on the workgroup's LDS state st (2^NS amplitudes),
+// with red (4 floats) and th_s (kAngleChunk floats) as LDS scratch.
 template <int NS, int T>
-__global__ __launch_bounds__(T) void k_sv_forward_lds(Job J, uint32_t flags) {
+__device__ __forceinline__ void sv_lds_run(const Job& J, int64_t g, uint32_t flags, float2* st,
+                                           float* red, float* th_s, int tid) {
   constexpr int H = 1 << NS;
-  __shared__ float2 st[H];
-  __shared__ float red[4];
-  const int64_t g = xcd_block(blockIdx.x, gridDim.x);   // XCD-aware order
-  const int tid = threadIdx.x;
   const int64_t bi = g / J.S, si = g - bi * J.S;
   const float* ang = J.angles + bi * J.angle_stride;
   const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
@@ -1055,7 +1054,6 @@ __global__ __launch_bounds__(T) void k_sv_forward_lds(Job J, uint32_t flags) {
   for (int j = tid; j < H; j += T) st[j] = flip(psi[in_index[j]], sign[j] ? 0x80000000u : 0u);
   RefPhase rp;
   const bool ref = flags & MBQC_FLAG_REF_PHASE;
-  __shared__ float th_s[kAngleChunk];
   for (int k = 0; k < J.n_steps; ++k) {
     if (k % NS == 0 && resc[k / NS]) {
       __syncthreads();
@@ -1089,6 +1087,14 @@ __global__ __launch_bounds__(T) void k_sv_forward_lds(Job J, uint32_t flags) {
     if (p >= 0) o[p] = cmul(st[j], f);
   }
   if (tid == 0 && (flags & MBQC_FLAG_WRITE_PHASE) && J.phase_out) J.phase_out[g] = ph;
+}
+
+template <int NS, int T>
+__global__ __launch_bounds__(T) void k_sv_forward_lds(Job J, uint32_t flags) {
+  __shared__ float2 st[1 << NS];
+  __shared__ float red[4];
+  __shared__ float th_s[kAngleChunk];
+  sv_lds_run<NS, T>(J, xcd_block(blockIdx.x, gridDim.x), flags, st, red, th_s, threadIdx.x);
 }
 
 // Sampled outcomes for windows of 7..14 slots: k_sv_sample's frame and draw
@@ -3011,6 +3017,36 @@ __global__ __launch_bounds__(256) void k_batch(const Job* __restrict__ jobs,
   }
 }
 
+// Batched groups of 7..14 live slots: a persistent workgroup per resident
+// slot pulls items from the group's counter and runs their simulations one
+// after the other with the state in LDS (sv_lds_run, as k_sv_forward_lds).
+// Every thread reads the item index from LDS, so all leave the loop together
+// once the counter passes n_items.
+template <int NS, int T>
+__global__ __launch_bounds__(T) void k_batch_lds(const Job* __restrict__ jobs,
+                                                 const BatchItem* __restrict__ items,
+                                                 int64_t n_items, int* counter, uint32_t flags) {
+  __shared__ float2 st[1 << NS];
+  __shared__ float red[4];
+  __shared__ float th_s[kAngleChunk];
+  __shared__ int item_s;
+  const int tid = threadIdx.x;
+  for (;;) {
+    __syncthreads();   // the previous item's last reads of item_s and st are done
+    if (tid == 0) item_s = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = item_s;
+    if (it >= n_items) break;
+    const BatchItem I = items[it];
+    const Job J = load_job(jobs, I.job);
+    const int64_t end = std::min<int64_t>(I.first + I.count, J.total);
+    for (int64_t g = I.first; g < end; ++g) {
+      __syncthreads();
+      sv_lds_run<NS, T>(J, g, flags, st, red, th_s, tid);
+    }
+  }
+}
+
 // group counters live kCounterStride ints apart (separate cache lines)
 constexpr int kCounterStride = 64;
 
@@ -3033,6 +3069,7 @@ static __device__ __align__(256) float scratch[kScratchFloats];
 namespace {
 
 constexpr int kMaxSvSlots = 6;       // register-resident SV kernels
+constexpr int kMaxBatchSlots = 14;   // batched groups (LDS-resident above kMaxSvSlots)
 // below this many simulations the DM path keeps each density matrix in LDS
 // (one quartet per thread, several simulations per workgroup); the two
 // kernels take the same time at 32,768 simulations of the 3x12 grid
@@ -3644,12 +3681,23 @@ int scratch_ptr(float** out) {
   return MBQC_OK;
 }
 
+// Groups above kMaxSvSlots: one persistent workgroup of T threads (T / 64
+// waves) per resident slot, `waves` / (T / 64) of them, at least one and at
+// most one per item.
+template <int NS>
+void launch_batch_lds_group(int64_t waves, const BatchGroupLaunch& L) {
+  constexpr int T = (1 << (NS - 1)) < 256 ? (1 << (NS - 1)) : 256;
+  const int64_t wgs = std::max<int64_t>(1, std::min<int64_t>(waves / (T / 64), L.n_items));
+  hipLaunchKernelGGL((k_batch_lds<NS, T>), dim3(unsigned(wgs)), dim3(T), 0, L.stream, L.jobs,
+                     L.items, L.n_items, L.counter, L.flags);
+}
+
 }  // namespace
 
 struct mbqc_batch {
   int max_ns = 1;
-  int64_t group_begin[kMaxSvSlots + 1] = {};  // items of live-window size ns
-  int64_t group_count[kMaxSvSlots + 1] = {};
+  int64_t group_begin[kMaxBatchSlots + 1] = {};  // items of live-window size ns
+  int64_t group_count[kMaxBatchSlots + 1] = {};
   std::vector<Job> jobs;
   std::vector<BatchItem> items;
   Job* d_jobs = nullptr;
@@ -3659,13 +3707,13 @@ struct mbqc_batch {
   // side streams and events that run the window-size groups concurrently
   // (fork from and join back into the caller's stream, so the pattern is
   // also valid under graph capture)
-  hipStream_t side[kMaxSvSlots + 1] = {};
+  hipStream_t side[kMaxBatchSlots + 1] = {};
   hipEvent_t fork = nullptr;
-  hipEvent_t join[kMaxSvSlots + 1] = {};
+  hipEvent_t join[kMaxBatchSlots + 1] = {};
   // every job of the group has S a multiple of share_group(ns)
-  bool share[kMaxSvSlots + 1] = {};
+  bool share[kMaxBatchSlots + 1] = {};
   // every plan of the group is periodic (Plan::periodic)
-  bool periodic[kMaxSvSlots + 1] = {};
+  bool periodic[kMaxBatchSlots + 1] = {};
 };
 
 // One launch of a batch: the counter reset, then one persistent kernel per
@@ -3677,21 +3725,21 @@ static int batch_launch(mbqc_batch* bt, const int32_t* group_waves, int32_t wg_w
                         uint32_t flags, void* stream) {
   if (wg_waves < 1 || wg_waves > 4) return fail(MBQC_ERR_VALUE, "wg_waves must be 1..4");
   int launched = 0;
-  for (int ns = 1; ns <= kMaxSvSlots; ++ns) {
+  for (int ns = 1; ns <= kMaxBatchSlots; ++ns) {
     if (group_waves[ns] == 0) continue;
     if (group_waves[ns] < wg_waves) return fail(MBQC_ERR_VALUE, "waves must be at least wg_waves");
     ++launched;
   }
   if (bt->items.empty() || launched == 0) return MBQC_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, bt->d_counter, kMaxSvSlots + 1);
+  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, bt->d_counter, kMaxBatchSlots + 1);
   const int threads = 64 * wg_waves;
   const bool ref = flags & MBQC_FLAG_REF_PHASE;
   const bool concurrent = bt->fork != nullptr && launched > 1;
   hipError_t e = hipGetLastError();
   if (e == hipSuccess && concurrent) e = hipEventRecord(bt->fork, st);
   // largest windows first
-  for (int ns = kMaxSvSlots; ns >= 1 && e == hipSuccess; --ns) {
+  for (int ns = kMaxBatchSlots; ns >= 1 && e == hipSuccess; --ns) {
     const int64_t n = bt->group_count[ns];
     if (n == 0 || group_waves[ns] == 0) continue;
     hipStream_t gs = st;
@@ -3715,7 +3763,15 @@ static int batch_launch(mbqc_batch* bt, const int32_t* group_waves, int32_t wg_w
       case 3: launch_batch_group<3>(L, ref, sh, pe); break;
       case 4: launch_batch_group<4>(L, ref, sh, pe); break;
       case 5: launch_batch_group<5>(L, ref, sh, pe); break;
-      default: launch_batch_group<6>(L, ref, sh, pe); break;
+      case 6: launch_batch_group<6>(L, ref, sh, pe); break;
+      case 7: launch_batch_lds_group<7>(group_waves[ns], L); break;
+      case 8: launch_batch_lds_group<8>(group_waves[ns], L); break;
+      case 9: launch_batch_lds_group<9>(group_waves[ns], L); break;
+      case 10: launch_batch_lds_group<10>(group_waves[ns], L); break;
+      case 11: launch_batch_lds_group<11>(group_waves[ns], L); break;
+      case 12: launch_batch_lds_group<12>(group_waves[ns], L); break;
+      case 13: launch_batch_lds_group<13>(group_waves[ns], L); break;
+      default: launch_batch_lds_group<14>(group_waves[ns], L); break;
     }
     e = hipGetLastError();
     if (e == hipSuccess && concurrent) {
@@ -4112,10 +4168,10 @@ int mbqc_batch_create(const mbqc_batch_job* jobs, int32_t n_jobs, int32_t sims_p
       return fail(MBQC_ERR_VALUE, "job has a null plan");
     }
     Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(bj.plan));
-    if (p->ns > kMaxSvSlots) {
+    if (p->ns > kMaxBatchSlots) {
       delete bt;
       return fail(MBQC_ERR_NOT_IMPLEMENTED, "batched kernel supports live windows of at most " +
-                                                std::to_string(kMaxSvSlots) + " qubits");
+                                                std::to_string(kMaxBatchSlots) + " qubits");
     }
     if (int rc0 = check_xy_plan(p); rc0 != MBQC_OK) {
       delete bt;
@@ -4165,12 +4221,12 @@ int mbqc_batch_create(const mbqc_batch_job* jobs, int32_t n_jobs, int32_t sims_p
   if (e == hipSuccess) e = hipMalloc(&bt->d_jobs, sizeof(Job) * bt->jobs.size());
   if (e == hipSuccess && !bt->items.empty())
     e = hipMalloc(&bt->d_items, sizeof(BatchItem) * bt->items.size());
-  if (e == hipSuccess) e = hipMalloc(&bt->d_counter, sizeof(int) * kCounterStride * (kMaxSvSlots + 1));
+  if (e == hipSuccess) e = hipMalloc(&bt->d_counter, sizeof(int) * kCounterStride * (kMaxBatchSlots + 1));
   int n_groups = 0;
-  for (int ns = 1; ns <= kMaxSvSlots; ++ns) n_groups += bt->group_count[ns] > 0;
+  for (int ns = 1; ns <= kMaxBatchSlots; ++ns) n_groups += bt->group_count[ns] > 0;
   if (e == hipSuccess && n_groups > 1) {
     e = hipEventCreateWithFlags(&bt->fork, hipEventDisableTiming);
-    for (int ns = 1; ns <= kMaxSvSlots && e == hipSuccess; ++ns) {
+    for (int ns = 1; ns <= kMaxBatchSlots && e == hipSuccess; ++ns) {
       if (bt->group_count[ns] == 0) continue;
       e = hipStreamCreateWithFlags(&bt->side[ns], hipStreamNonBlocking);
       if (e == hipSuccess) e = hipEventCreateWithFlags(&bt->join[ns], hipEventDisableTiming);
@@ -4194,7 +4250,7 @@ void mbqc_batch_destroy(mbqc_batch* bt) {
   if (bt->d_jobs) (void)hipFree(bt->d_jobs);
   if (bt->d_items) (void)hipFree(bt->d_items);
   if (bt->d_counter) (void)hipFree(bt->d_counter);
-  for (int ns = 0; ns <= kMaxSvSlots; ++ns) {
+  for (int ns = 0; ns <= kMaxBatchSlots; ++ns) {
     if (bt->side[ns]) (void)hipStreamDestroy(bt->side[ns]);
     if (bt->join[ns]) (void)hipEventDestroy(bt->join[ns]);
   }
@@ -4285,8 +4341,8 @@ int mbqc_sgd_step_f32(float* x, const float* grad, float* v, int64_t n, float lr
 
 int mbqc_batch_launch(mbqc_batch* bt, int32_t waves, int32_t wg_waves, uint32_t flags, void* stream) {
   if (!bt) return fail(MBQC_ERR_VALUE, "null batch");
-  int32_t gw[kMaxSvSlots + 1];
-  for (int ns = 0; ns <= kMaxSvSlots; ++ns) gw[ns] = bt->group_count[ns] > 0 ? waves : 0;
+  int32_t gw[kMaxBatchSlots + 1];
+  for (int ns = 0; ns <= kMaxBatchSlots; ++ns) gw[ns] = bt->group_count[ns] > 0 ? waves : 0;
   return batch_launch(bt, gw, wg_waves, flags, stream);
 }
 
@@ -4294,10 +4350,10 @@ int mbqc_batch_launch_groups(mbqc_batch* bt, const int32_t* group_waves, int32_t
                              int32_t wg_waves, uint32_t flags, void* stream) {
   if (!bt) return fail(MBQC_ERR_VALUE, "null batch");
   if (!group_waves) return fail(MBQC_ERR_VALUE, "group_waves is null");
-  if (n_entries < 1 || n_entries > kMaxSvSlots + 1)
-    return fail(MBQC_ERR_VALUE, "group_waves must have 1.." + std::to_string(kMaxSvSlots + 1) +
+  if (n_entries < 1 || n_entries > kMaxBatchSlots + 1)
+    return fail(MBQC_ERR_VALUE, "group_waves must have 1.." + std::to_string(kMaxBatchSlots + 1) +
                                     " entries");
-  int32_t gw[kMaxSvSlots + 1] = {};
+  int32_t gw[kMaxBatchSlots + 1] = {};
   for (int ns = 0; ns < n_entries; ++ns) {
     if (group_waves[ns] < 0) return fail(MBQC_ERR_VALUE, "negative wave count");
     gw[ns] = bt->group_count[ns] > 0 ? group_waves[ns] : 0;
@@ -4306,7 +4362,7 @@ int mbqc_batch_launch_groups(mbqc_batch* bt, const int32_t* group_waves, int32_t
 }
 
 int64_t mbqc_batch_group_items(const mbqc_batch* bt, int32_t n_slots) {
-  if (!bt || n_slots < 0 || n_slots > kMaxSvSlots) return -1;
+  if (!bt || n_slots < 0 || n_slots > kMaxBatchSlots) return -1;
   return bt->group_count[n_slots];
 }
 

@@ -150,3 +150,54 @@ def test_batched_group_waves(rng):
         bf.launch(wg_waves=2, group_waves=[1] * MAX_GROUPS)
     with pytest.raises(ValueError):
         bf.launch(wg_waves=2, group_waves=[8] * (MAX_GROUPS + 1))
+
+
+def test_batched_lds_groups(rng):
+    """Groups of 7..14 live slots run persistent workgroups with the state in
+    LDS, concurrently with the register-kernel groups: every job equals its
+    single launch bit for bit (the same LDS simulation code), a sample matches
+    the oracle, every group sizing gives the same outputs, and a graph replay
+    reproduces the launch."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from mentpy_amd.batched import MAX_GROUPS, BatchedForward
+    from oracle.vectorized import sv_batch
+    jobs = []
+    for r, c, w, B, S in [(3, 5, None, 40, 3), (7, 3, None, 9, 2), (3, 5, 9, 30, 4), (8, 3, None, 5, 3),
+                          (4, 6, None, 64, 2), (10, 3, None, 3, 2), (5, 6, 15, 4, 1)]:
+        g = mp.templates.grid_cluster(r, c)
+        sim = mp.PatternSimulator(g, backend="hip-sv", **({"window_size": w} if w else {}))
+        th = rng.uniform(-7, 7, (B, len(g.trainable_nodes))).astype(np.float32)
+        st = np.stack([haar_state(r, rng) for _ in range(S)])
+        jobs.append((sim, th, st, g))
+    bf = BatchedForward([(s, th, st) for s, th, st, _ in jobs], sims_per_item=64)
+    groups = bf.groups()
+    assert {7, 8, 10, 14} <= set(groups) and {3, 4} <= set(groups)
+    want = [o.clone() for o in bf.launch(waves=512, wg_waves=1)]
+    torch.cuda.synchronize()
+    for (sim, th, st, g), o in zip(jobs, want):
+        single = sim.run_batch(th, st)
+        ns = sim.simulator.plan.info["n_slots"]
+        if ns > 6:
+            assert torch.equal(single, o), f"{ns}-slot job differs from its single launch"
+        else:
+            assert (single - o).abs().max().item() < 1e-6
+        ref = sv_batch(g, th[:2].astype(np.float64), st)
+        err = np.abs(o[:2].cpu().numpy() - ref).max()
+        print(f"{ns} slots: max amp err vs oracle {err:.2e}", flush=True)
+        assert err < 2e-5
+    for sizes in ([1] * MAX_GROUPS, [4, 8, 12, 16, 20, 24, 28, 3, 5, 1, 64, 2, 2, 9, 1000]):
+        for o in bf.outputs:
+            o.fill_(complex(float("nan"), 0.0))
+        got = bf.launch(wg_waves=1, group_waves=sizes)
+        torch.cuda.synchronize()
+        for x, y in zip(got, want):
+            assert torch.equal(x, y), f"group_waves {sizes} differs"
+    graph = bf.capture_graph(waves=256, wg_waves=1)
+    for rep in range(3):
+        for o in bf.outputs:
+            o.zero_()
+        graph.replay()
+        torch.cuda.synchronize()
+        for o, w in zip(bf.outputs, want):
+            assert torch.equal(o, w), f"graph replay {rep} differs"
