@@ -184,7 +184,11 @@ int mbqc_plan_get_rescale(const mbqc_plan* plan, int32_t* sv_flags, int32_t* dm_
  * (angle_stride 0 = one shared vector; angle-major storage, angle_stride 1 and
  * angle_stride_a = B, gives coalesced loads) and input s at
  * psi_in + s*in_stride amplitudes (in_stride 0 = shared).  out is
- * [B*S][2^n_outputs]; phase_out (optional, MBQC_FLAG_WRITE_PHASE) is [B*S]. */
+ * [B*S][2^n_outputs]; phase_out (optional, MBQC_FLAG_WRITE_PHASE) is [B*S].
+ * Live windows up to 20 slots: up to 6 with the state in registers, 7..14 in
+ * LDS, 15..20 in a workspace of 2^n_slots complex64 per simulation that the
+ * call allocates and frees in stream order (hipMallocAsync / hipFreeAsync on
+ * `stream`, at most 4 GiB per launch, larger batches in chunks). */
 int mbqc_sv_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
                         int64_t angle_stride_a, const void* psi_in, int64_t in_stride, void* out,
                         void* phase_out, int64_t B, int64_t S, uint32_t flags, void* stream);
@@ -234,7 +238,11 @@ int mbqc_dm_window_c64(const mbqc_plan* plan, const float* step_angles, int64_t 
  * partial_trace_mixed_state (state_ops.py:77-119); the noise channels follow
  * the placement of the PennyLane backend (pennylane_simulator.py:118-136).
  * Inputs are state vectors (the reference purifies them, np_simulator_dm.py:89-92).
- * out is [B*S][2^n_outputs][2^n_outputs]. */
+ * out is [B*S][2^n_outputs][2^n_outputs].  Live windows up to 10 slots: up to
+ * 7 with the density matrix in registers or LDS, 8..10 in a workspace of
+ * 4^n_slots complex64 per simulation allocated and freed in stream order
+ * (as mbqc_sv_forward_c64).  The window calls below take up to 14 (SV) and
+ * 7 (DM) slots. */
 int mbqc_dm_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
                         int64_t angle_stride_a, const void* psi_in, int64_t in_stride, void* out,
                         int64_t B, int64_t S, const mbqc_noise* noise, uint32_t flags, void* stream);

@@ -1097,6 +1097,18 @@ __global__ __launch_bounds__(T) void k_sv_forward_lds(Job J, uint32_t flags) {
   sv_lds_run<NS, T>(J, xcd_block(blockIdx.x, gridDim.x), flags, st, red, th_s, threadIdx.x);
 }
 
+// Windows above the LDS: the same simulation on statevectors in a global
+// workspace, ws[(g - g0) * 2^NS + j] for simulations g = g0 + blockIdx (each
+// workgroup its own slice, synchronised by the barrier as in LDS).
+template <int NS>
+__global__ __launch_bounds__(256) void k_sv_forward_global(Job J, uint32_t flags,
+                                                           float2* __restrict__ ws, int64_t g0) {
+  __shared__ float red[4];
+  __shared__ float th_s[kAngleChunk];
+  const int64_t l = xcd_block(blockIdx.x, gridDim.x);
+  sv_lds_run<NS, 256>(J, g0 + l, flags, ws + (l << NS), red, th_s, threadIdx.x);
+}
+
 // Sampled outcomes for windows of 7..14 slots: k_sv_sample's frame and draw
 // arithmetic (SampleMod, evaluated identically by every thread of the
 // workgroup, so the frame is uniform) on the LDS-resident state of
@@ -2786,16 +2798,16 @@ __device__ __forceinline__ void dm_lds_step_general(float2* r, int H, int q, uin
 // and the reference's window after k_steps measurements (as k_dm_window).
 // If `outcomes` is not null, step k's outcome (0 or 1; for a Z node the
 // expectation p1 = 1 - p0) goes to outcomes[g * k_steps + k].
-__global__ __launch_bounds__(64) void k_dm_general(Job J, const float* __restrict__ step_angles,
-                                                   int64_t step_stride, int32_t k_steps,
-                                                   WindowMap map, int32_t window_mode, DmNoise nz,
-                                                   uint32_t flags, float* __restrict__ outcomes) {
-  constexpr int T = 64;
-  extern __shared__ float2 r[];
-  __shared__ float red[4];
+// The body, on a density matrix r (LDS or a global-memory slice) of
+// simulation g, with T threads and red (4 floats of LDS) as scratch.
+template <int T>
+__device__ __forceinline__ void dm_general_run(float2* r, float* red, const Job& J,
+                                               const float* __restrict__ step_angles,
+                                               int64_t step_stride, int32_t k_steps,
+                                               const WindowMap& map, int32_t window_mode,
+                                               const DmNoise& nz, uint32_t flags,
+                                               float* __restrict__ outcomes, int64_t g, int tid) {
   const int ns = J.ns, H = 1 << ns, HQ = H / 2;
-  const int64_t g = xcd_block(blockIdx.x, gridDim.x);   // XCD-aware order
-  const int tid = threadIdx.x;
   const int64_t bi = g / J.S, si = g - bi * J.S;
   const float* ang = window_mode ? step_angles + bi * step_stride : J.angles + bi * J.angle_stride;
   const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
@@ -2921,6 +2933,31 @@ __global__ __launch_bounds__(64) void k_dm_general(Job J, const float* __restric
     if (pj >= 0 && pk >= 0) o[int64_t(pj) * d + pk] = cscale(r[e], scale);
   }
 }
+
+__global__ __launch_bounds__(64) void k_dm_general(Job J, const float* __restrict__ step_angles,
+                                                   int64_t step_stride, int32_t k_steps,
+                                                   WindowMap map, int32_t window_mode, DmNoise nz,
+                                                   uint32_t flags, float* __restrict__ outcomes) {
+  extern __shared__ float2 r[];
+  __shared__ float red[4];
+  dm_general_run<64>(r, red, J, step_angles, step_stride, k_steps, map, window_mode, nz, flags,
+                     outcomes, xcd_block(blockIdx.x, gridDim.x), threadIdx.x);
+}
+
+// Windows above the LDS: the same body on density matrices in a global
+// workspace, ws[(g - g0) * 4^n_slots + e] for simulations g0 + blockIdx (the
+// workgroup's own slice; its threads synchronise through the barrier as in
+// LDS).
+__global__ __launch_bounds__(256) void k_dm_general_global(Job J, DmNoise nz, uint32_t flags,
+                                                           float* __restrict__ outcomes,
+                                                           float2* __restrict__ ws, int64_t g0) {
+  __shared__ float red[4];
+  const int64_t l = xcd_block(blockIdx.x, gridDim.x);
+  WindowMap none{};
+  dm_general_run<256>(ws + l * (int64_t(1) << (2 * J.ns)), red, J, nullptr, 0, J.n_steps, none, 0,
+                      nz, flags, outcomes, g0 + l, threadIdx.x);
+}
+
 
 // ---------------------------------------------------------------- batched
 struct BatchItem {
@@ -3084,6 +3121,47 @@ constexpr int64_t kDmWaveKernelMaxSims[4] = {0, 32768, 16384, 1024};
 constexpr int kMaxSvLdsSlots = 14;   // LDS-resident SV kernel (2^14 amplitudes: 128 KB)
 constexpr int kMaxDmSlots = 3;       // register-resident DM kernel
 constexpr int kMaxDmLdsSlots = 7;    // LDS-resident DM kernel (4^7 entries: 128 KB)
+// Above the LDS the state lives in a stream-ordered workspace (hipMallocAsync
+// on the caller's stream, freed on it after the launch), at most
+// kGlobalWorkspaceBytes per launch (larger batches run in chunks).
+constexpr int kMaxSvGlobalSlots = 20;   // 2^20 amplitudes: 8 MiB per simulation
+constexpr int kMaxDmGlobalSlots = 10;   // 4^10 entries: 8 MiB per simulation
+constexpr size_t kGlobalWorkspaceBytes = size_t(4) << 30;
+
+// Launch `kernel_chunk(ws, g0, n)` over the job's simulations in chunks whose
+// states (state_bytes each) fit the workspace.
+template <typename F>
+hipError_t run_global_chunks(const Job& J, size_t state_bytes, hipStream_t st, F&& kernel_chunk) {
+  const int64_t per = std::max<int64_t>(
+      1, std::min<int64_t>(int64_t(kGlobalWorkspaceBytes / state_bytes), 0x7fffffff));
+  const int64_t n0 = std::min<int64_t>(per, J.total);
+  void* ws = nullptr;
+  hipError_t e = hipMallocAsync(&ws, size_t(n0) * state_bytes, st);
+  if (e != hipSuccess) return e;
+  for (int64_t g0 = 0; g0 < J.total && e == hipSuccess; g0 += n0) {
+    kernel_chunk(static_cast<float2*>(ws), g0, std::min<int64_t>(n0, J.total - g0));
+    e = hipGetLastError();
+  }
+  const hipError_t ef = hipFreeAsync(ws, st);
+  return e != hipSuccess ? e : ef;
+}
+
+template <int NS>
+hipError_t launch_sv_global(const Job& J, uint32_t flags, hipStream_t st) {
+  return run_global_chunks(J, sizeof(float2) << NS, st, [&](float2* ws, int64_t g0, int64_t n) {
+    hipLaunchKernelGGL((k_sv_forward_global<NS>), dim3(unsigned(n)), dim3(256), 0, st, J, flags, ws,
+                       g0);
+  });
+}
+
+hipError_t launch_dm_global(const Job& J, const DmNoise& nz, uint32_t flags, float* outcomes,
+                            hipStream_t st) {
+  return run_global_chunks(J, sizeof(float2) << (2 * J.ns), st,
+                           [&](float2* ws, int64_t g0, int64_t n) {
+    hipLaunchKernelGGL(k_dm_general_global, dim3(unsigned(n)), dim3(256), 0, st, J, nz, flags,
+                       outcomes, ws, g0);
+  });
+}
 constexpr int kMaxAdjSlots = 5;
 
 int hip_fail(hipError_t e, const char* what) {
@@ -3578,9 +3656,9 @@ int dm_forward(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
                void* stream) {
   if (!plan) return fail(MBQC_ERR_VALUE, "null plan");
   Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(plan));
-  if (p->ns > kMaxDmLdsSlots)
+  if (p->ns > kMaxDmGlobalSlots)
     return fail(MBQC_ERR_NOT_IMPLEMENTED, "density-matrix kernels support live windows of at most " +
-                                              std::to_string(kMaxDmLdsSlots) + " qubits");
+                                              std::to_string(kMaxDmGlobalSlots) + " qubits");
   DmNoise nz;
   int rc = dm_noise(noise, &nz);
   if (rc != MBQC_OK) return rc;
@@ -3590,6 +3668,11 @@ int dm_forward(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
   if (J.total == 0) return MBQC_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
   hipError_t e = hipSuccess;
+  if (p->ns > kMaxDmLdsSlots) {
+    // density matrices beyond the LDS: the general kernel on a global workspace
+    e = launch_dm_global(J, nz, flags, outcomes, st);
+    return e == hipSuccess ? MBQC_OK : hip_fail(e, "dm_forward launch (global workspace)");
+  }
   // other planes, per-step probabilities or a measurement order off the
   // slot rotation: the kernel that reads each step's slot and plane
   if (p->general_planes || !p->rotating || (flags & MBQC_FLAG_STEP_PROBS)) {
@@ -3803,9 +3886,9 @@ int mbqc_sv_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
                         void* phase_out, int64_t B, int64_t S, uint32_t flags, void* stream) {
   if (!plan) return fail(MBQC_ERR_VALUE, "null plan");
   Plan* p = const_cast<Plan*>(reinterpret_cast<const Plan*>(plan));
-  if (p->ns > kMaxSvLdsSlots)
+  if (p->ns > kMaxSvGlobalSlots)
     return fail(MBQC_ERR_NOT_IMPLEMENTED, "statevector kernels support live windows of at most " +
-                                              std::to_string(kMaxSvLdsSlots) + " qubits");
+                                              std::to_string(kMaxSvGlobalSlots) + " qubits");
   int rc0 = check_xy_plan(p);
   if (rc0 != MBQC_OK) return rc0;
   Job J;
@@ -3818,6 +3901,18 @@ int mbqc_sv_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
   hipStream_t st = static_cast<hipStream_t>(stream);
   hipError_t e = hipSuccess;
   const bool periodic = p->periodic && !(flags & MBQC_FLAG_TABLE_STEPS);
+  if (p->ns > kMaxSvLdsSlots) {
+    // statevectors beyond the LDS: the LDS kernel's simulation on a global workspace
+    switch (p->ns) {
+      case 15: e = launch_sv_global<15>(J, flags, st); break;
+      case 16: e = launch_sv_global<16>(J, flags, st); break;
+      case 17: e = launch_sv_global<17>(J, flags, st); break;
+      case 18: e = launch_sv_global<18>(J, flags, st); break;
+      case 19: e = launch_sv_global<19>(J, flags, st); break;
+      case 20: e = launch_sv_global<20>(J, flags, st); break;
+    }
+    return e == hipSuccess ? MBQC_OK : hip_fail(e, "sv_forward launch (global workspace)");
+  }
   if (!p->rotating) {
     // measured slots off the rotation: the LDS kernel reads them per step
     switch (p->ns) {

@@ -8,7 +8,7 @@
 
 namespace mbqc {
 
-constexpr int kMaxSlots = 14;   // host tables; kernels support fewer (see launchers)
+constexpr int kMaxSlots = 20;   // host tables; kernels support fewer (see launchers)
 constexpr int kMaxDevices = 16;
 constexpr int kMaxCoefSlots = 6;
 constexpr int kMaxSampleSlots = 14;   // sampled outcomes (LDS kernel above 6 slots)

@@ -80,6 +80,51 @@ def test_dm_large_windows_lds_kernel(rows, cols, rng):
     assert err < AMP_TOL
 
 
+@pytest.mark.parametrize("rows,cols,B", [(15, 3, 3), (16, 3, 2), (20, 3, 1)])
+def test_sv_windows_beyond_lds(rows, cols, B, rng):
+    """Live windows of 15..20 qubits: the LDS kernel's simulation on a
+    stream-ordered global workspace (20 slots: 2^20 amplitudes, 8 MiB per
+    simulation)."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from conftest import infidelity
+    from oracle.vectorized import sv_batch
+    g = mp.templates.grid_cluster(rows, cols)
+    sim = mp.PatternSimulator(g, backend="hip-sv")
+    assert sim.simulator.plan.info["n_slots"] == rows
+    th = rng.uniform(0, 2 * np.pi, (B, len(g.trainable_nodes)))
+    st = haar_state(rows, rng)[None]
+    out = sim.run_batch(th, st).cpu().numpy()
+    ref = sv_batch(g, th.astype(np.float32).astype(np.float64), st)
+    err = np.abs(out - ref).max()
+    inf = float(np.max(infidelity(out, ref)))
+    print(f"grid {rows}x{cols} (global workspace): max amp err {err:.2e}, 1-F {inf:.2e}", flush=True)
+    assert err < AMP_TOL and inf < 1e-5
+
+
+@pytest.mark.parametrize("rows,cols,B,xz", [(8, 3, 3, False), (8, 3, 2, True), (9, 3, 2, False),
+                                            (10, 3, 1, False)])
+def test_dm_windows_beyond_lds(rows, cols, B, xz, rng):
+    """Density matrices of 8..10 live qubits: the general kernel on a
+    stream-ordered global workspace (10 slots: 4^10 entries, 8 MiB per
+    simulation), XY and XZ planes, with noise."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from oracle.vectorized import dm_batch
+    g = mp.templates.grid_cluster(rows, cols)
+    if xz:
+        g[1] = mp.Ment("XZ")
+    sim = mp.PatternSimulator(g, backend="hip-dm", p_depol=0.02, p_readout=0.01)
+    assert sim.simulator.plan.info["n_slots"] == rows
+    th = rng.uniform(0, 2 * np.pi, (B, len(g.trainable_nodes)))
+    st = haar_state(rows, rng)[None]
+    out = sim.run_batch(th, st).cpu().numpy()
+    ref = dm_batch(g, th.astype(np.float32).astype(np.float64), st, p_depol=0.02, p_readout=0.01)
+    err = np.abs(out - ref).max()
+    print(f"dm grid {rows}x{cols} xz={xz} (global workspace): max err {err:.2e}", flush=True)
+    assert err < AMP_TOL
+
+
 @pytest.mark.parametrize("template", ["grid-2-4", "grid-2-5", "linear-4", "linear-5", "muta-2-1"])
 def test_random_measurements_equal_across_backends(template, rng):
     """mentpy tests/test_simulators.py:33-64 (grid_cluster(2, 3+i),

@@ -175,19 +175,19 @@ def test_kernel_limits_are_reported(native):
     plan, *_ = compile_pattern(mp.templates.grid_cluster(8, 9), exclude="quantum_output")
     assert plan.info["n_slots"] == 8
     L = native.lib()
-    rc = L.mbqc_dm_forward_c64(plan.handle, None, 0, 1, None, 0, None, 1, 1, None, 0, None)
-    assert rc == native.MBQC_ERR_NOT_IMPLEMENTED          # density matrices: at most 7 slots
+    plan11, *_ = compile_pattern(mp.templates.grid_cluster(11, 3))
+    assert plan11.info["n_slots"] == 11
+    rc = L.mbqc_dm_forward_c64(plan11.handle, None, 0, 1, None, 0, None, 1, 1, None, 0, None)
+    assert rc == native.MBQC_ERR_NOT_IMPLEMENTED          # density matrices: at most 10 slots
     rc = L.mbqc_sv_segmented_forward_c64(plan.handle, None, 0, 1, None, 1, 2, None, 0, None)
     assert rc == native.MBQC_ERR_NOT_IMPLEMENTED          # time segments: at most 5 slots
     assert L.mbqc_sv_segments_auto(plan.handle, 1) == 1
-    plan11, *_ = compile_pattern(mp.templates.grid_cluster(11, 3))
-    assert plan11.info["n_slots"] == 11
     rc = L.mbqc_sv_adjoint_forward_c64(plan11.handle, None, 0, 1, None, 0, None, 1, 1, None, 0, None)
     assert rc == native.MBQC_ERR_NOT_IMPLEMENTED          # adjoint: at most 10 slots
-    plan14, *_ = compile_pattern(mp.templates.grid_cluster(14, 15))
-    assert plan14.info["n_slots"] == 14                     # statevectors: up to 14 slots
+    plan20, *_ = compile_pattern(mp.templates.grid_cluster(20, 3))
+    assert plan20.info["n_slots"] == 20                     # statevectors: up to 20 slots
     with pytest.raises(NotImplementedError):
-        compile_pattern(mp.templates.grid_cluster(15, 16))
+        compile_pattern(mp.templates.grid_cluster(21, 3))
 
 
 def test_adjoint_rejects_shared_angle_entries(native):
