@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import threading
 from pathlib import Path
 from typing import Optional
 
@@ -212,6 +213,35 @@ def _i32(a) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
 
 
+# Native objects whose Python owners were garbage-collected.  Finalizers only
+# queue them: the cyclic garbage collector can run a finalizer at any
+# allocation, including while a stream is being captured into a HIP graph,
+# where freeing device memory or destroying streams is not allowed (it breaks
+# the capture).  The queue is emptied by release_pending() at points that
+# allocate device memory themselves (plan and batch construction), which are
+# never inside a capture.
+_pending_lock = threading.Lock()
+_pending: list = []
+
+
+def defer_destroy(fn_name: str, handle_value: int) -> None:
+    with _pending_lock:
+        _pending.append((fn_name, handle_value))
+
+
+def release_pending() -> int:
+    """Destroy the queued native objects; returns how many were destroyed."""
+    with _pending_lock:
+        items = list(_pending)
+        _pending.clear()
+    L = _lib
+    if L is None:
+        return 0
+    for fn_name, h in items:
+        getattr(L, fn_name)(C.c_void_p(h))
+    return len(items)
+
+
 class NativePlan:
     """Owner of an ``mbqc_plan*`` compiled from flat pattern arrays."""
 
@@ -219,6 +249,7 @@ class NativePlan:
                  input_nodes, output_nodes, angle_index, fixed_angle, n_angles: int,
                  flow=None, plane=None, measure_order=None):
         L = lib()
+        release_pending()
         self._keep = dict(
             edges=_i32(np.asarray(edges, dtype=np.int32).reshape(-1, 2) if len(edges) else np.zeros((0, 2))),
             schedule=None if schedule is None else _i32(schedule), inputs=_i32(input_nodes),
@@ -257,9 +288,8 @@ class NativePlan:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        lib_ = globals().get("_lib")
-        if h is not None and h.value and lib_ is not None:
-            lib_.mbqc_plan_destroy(h)
+        if h is not None and h.value and defer_destroy is not None:
+            defer_destroy("mbqc_plan_destroy", h.value)
             self.handle = None
 
     def tables(self):

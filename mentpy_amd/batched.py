@@ -88,6 +88,7 @@ class BatchedForward:
         if (spi == [self.sims_per_item] if len(spi) == 1 else spi == self.group_items) \
                 and self._handle is not None:
             return
+        nat.release_pending()
         arr = (nat.BatchJob * len(self._jobs))()
         for k, (plan, a, s, out) in enumerate(self._jobs):
             sb, sa = _angle_strides(a)
@@ -107,10 +108,10 @@ class BatchedForward:
         self.group_items = None if len(spi) == 1 else spi
 
     def __del__(self):
+        # queued, not destroyed here: a finalizer may run inside a graph capture
         h = getattr(self, "_handle", None)
-        lib_ = getattr(nat, "_lib", None) if nat is not None else None
-        if h is not None and lib_ is not None:
-            lib_.mbqc_batch_destroy(h)
+        if h is not None and h.value and nat is not None:
+            nat.defer_destroy("mbqc_batch_destroy", h.value)
             self._handle = None
 
     @property

@@ -338,3 +338,25 @@ def test_periodic_plans(native):
             assert np.array_equal(nmask, np.tile(nmask[:ns], n // ns))
             # the appended qubit's earlier neighbour sits in the slot appended one step before
             assert nmask[0] == 0 and all(nmask[s] == 1 << (ns - s) for s in range(1, ns))
+
+
+def test_finalizers_defer_native_destruction(native):
+    """A plan held in a reference cycle (simulators and their backends form
+    one) dies in the cyclic garbage collector, which can run at any allocation,
+    including inside a HIP graph capture, where freeing device memory breaks
+    the capture: its finalizer only queues the native handle, and the next plan
+    construction (never inside a capture) destroys it."""
+    import gc
+    from mentpy_amd.simulators import compile_pattern
+    native.release_pending()
+    g = mp.templates.grid_cluster(2, 3)
+    plan, *_ = compile_pattern(g)
+    h = plan.handle.value
+    holder = {"plan": plan}
+    holder["self"] = holder
+    del plan, holder
+    gc.collect()
+    assert ("mbqc_plan_destroy", h) in native._pending
+    plan2, *_ = compile_pattern(g)
+    assert ("mbqc_plan_destroy", h) not in native._pending
+    assert plan2.info["n_slots"] == 2
