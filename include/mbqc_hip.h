@@ -329,13 +329,6 @@ typedef struct mbqc_batch_job {
 } mbqc_batch_job;
 int mbqc_batch_create(const mbqc_batch_job* jobs, int32_t n_jobs, int32_t sims_per_item,
                       mbqc_batch** out);
-/* mbqc_batch_create with the work-item size chosen per live-window group:
- * the group of size ns takes sims_per_item[ns] (n_entries <= 15 entries;
- * groups of size >= n_entries take the last entry), each a positive multiple
- * of 64.  Short patterns amortise the per-item fetch over larger items, long
- * ones balance better over smaller items. */
-int mbqc_batch_create_groups(const mbqc_batch_job* jobs, int32_t n_jobs,
-                             const int32_t* sims_per_item, int32_t n_entries, mbqc_batch** out);
 void mbqc_batch_destroy(mbqc_batch* batch);
 int64_t mbqc_batch_items(const mbqc_batch* batch);
 /* Launch the persistent kernel: `waves` resident wavefronts pull work items

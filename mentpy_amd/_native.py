@@ -48,7 +48,7 @@ EXPORTED_SYMBOLS = (
     "mbqc_sv_adjoint_workspace_bytes", "mbqc_sv_adjoint_forward_c64",
     "mbqc_sv_adjoint_partials", "mbqc_sv_adjoint_backward_c64", "mbqc_sv_segments_auto",
     "mbqc_sv_segmented_workspace_bytes", "mbqc_sv_segmented_forward_c64",
-    "mbqc_sv_segmented_backward_c64", "mbqc_batch_create", "mbqc_batch_create_groups",
+    "mbqc_sv_segmented_backward_c64", "mbqc_batch_create",
     "mbqc_batch_destroy", "mbqc_batch_items", "mbqc_batch_launch", "mbqc_batch_group_items",
     "mbqc_batch_launch_groups", "mbqc_adam_step_f32",
     "mbqc_sgd_step_f32", "mbqc_last_error", "mbqc_abi_version",
@@ -169,9 +169,6 @@ def lib() -> C.CDLL:
     L.mbqc_sv_segmented_backward_c64.restype = C.c_int
     L.mbqc_batch_create.argtypes = [C.POINTER(BatchJob), i32, i32, C.POINTER(vp)]
     L.mbqc_batch_create.restype = C.c_int
-    L.mbqc_batch_create_groups.argtypes = [C.POINTER(BatchJob), i32, C.POINTER(i32), i32,
-                                           C.POINTER(vp)]
-    L.mbqc_batch_create_groups.restype = C.c_int
     L.mbqc_batch_destroy.argtypes = [vp]
     L.mbqc_batch_destroy.restype = None
     L.mbqc_batch_items.argtypes = [vp]

@@ -33,7 +33,7 @@ __all__ = ["BatchedForward", "autotune", "default_cache_path"]
 # lane up to 6 slots, one per workgroup with the state in LDS above)
 MAX_GROUPS = 15
 # tag of the cached tuning results (entries timed another way are not reused)
-_TUNE_VERSION = "graph-replay-group-items"
+_TUNE_VERSION = "graph-replay"
 
 
 def _torch():
@@ -55,10 +55,11 @@ class BatchedForward:
     ``self.outputs`` (same order as the jobs).
     """
 
-    def __init__(self, jobs: Sequence[Tuple], sims_per_item=64, device="cuda"):
+    def __init__(self, jobs: Sequence[Tuple], sims_per_item: int = 64, device="cuda"):
         torch = _torch()
         self.device = torch.device(device)
         _require_gpu(self.device)
+        self.sims_per_item = int(sims_per_item)
         self._jobs = []
         self.outputs = []
         for sim, angles, inputs in jobs:
@@ -71,24 +72,14 @@ class BatchedForward:
             self._jobs.append((plan, a, s, out))
             self.outputs.append(out)
         self._handle = None
-        self.sims_per_item = self.group_items = None
-        self._build(sims_per_item)
+        self._build(self.sims_per_item)
         self.config = None
 
-    def _build(self, sims_per_item):
-        """(Re)create the native batch with work items of ``sims_per_item``
-        simulations: one count, or a list indexed by live-window size (the
-        group of size ns takes entry ns; larger sizes take the last entry)."""
-        if isinstance(sims_per_item, (list, tuple)):
-            spi = [int(x) for x in sims_per_item]
-            if not 1 <= len(spi) <= MAX_GROUPS:
-                raise ValueError(f"sims_per_item has 1..{MAX_GROUPS} entries")
-        else:
-            spi = [int(sims_per_item)]
-        if (spi == [self.sims_per_item] if len(spi) == 1 else spi == self.group_items) \
-                and self._handle is not None:
-            return
+    def _build(self, sims_per_item: int):
         nat.release_pending()
+        if self._handle is not None:
+            nat.lib().mbqc_batch_destroy(self._handle)
+            self._handle = None
         arr = (nat.BatchJob * len(self._jobs))()
         for k, (plan, a, s, out) in enumerate(self._jobs):
             sb, sa = _angle_strides(a)
@@ -96,16 +87,10 @@ class BatchedForward:
                                   angle_stride=sb, angle_stride_a=sa, psi_in=s.data_ptr(),
                                   in_stride=s.shape[1], out=out.data_ptr(), B=a.shape[0], S=s.shape[0])
         h = C.c_void_p()
-        sp = (C.c_int32 * len(spi))(*spi)
         with _torch().cuda.device(self.device):
-            nat.check(nat.lib().mbqc_batch_create_groups(arr, len(self._jobs), sp, len(spi),
-                                                         C.byref(h)))
-        # the previous batch stays valid until the new one exists
-        if self._handle is not None:
-            nat.lib().mbqc_batch_destroy(self._handle)
+            nat.check(nat.lib().mbqc_batch_create(arr, len(self._jobs), int(sims_per_item), C.byref(h)))
         self._handle = h
-        self.sims_per_item = spi[0] if len(spi) == 1 else None
-        self.group_items = None if len(spi) == 1 else spi
+        self.sims_per_item = int(sims_per_item)
 
     def __del__(self):
         # queued, not destroyed here: a finalizer may run inside a graph capture
@@ -141,7 +126,7 @@ class BatchedForward:
     def default_config(self):
         torch = _torch()
         cus = torch.cuda.get_device_properties(self.device).multi_processor_count
-        return {"waves": cus * 4 * 4, "wg_waves": 4, "sims_per_item": self.sims_per_item or 64}
+        return {"waves": cus * 4 * 4, "wg_waves": 4, "sims_per_item": self.sims_per_item}
 
     def launch(self, waves: Optional[int] = None, wg_waves: Optional[int] = None,
                group_waves: Optional[Sequence[int]] = None, ref_phase: bool = True,
@@ -227,7 +212,8 @@ def autotune(batch: BatchedForward, cache: Optional[Path] = None, reps: int = 10
             table = {}
     if key in table:
         cfg = table[key]
-        batch._build(cfg.get("group_items") or cfg["sims_per_item"])
+        if cfg["sims_per_item"] != batch.sims_per_item:
+            batch._build(cfg["sims_per_item"])
         batch.config = cfg
         return cfg
     cus = torch.cuda.get_device_properties(batch.device).multi_processor_count
@@ -236,35 +222,27 @@ def autotune(batch: BatchedForward, cache: Optional[Path] = None, reps: int = 10
                       for spi in (64, 128, 256) for occ in (2, 4, 8) for wg in (1, 4)]
     best, best_t = None, float("inf")
     for cfg in candidates:
-        batch._build(cfg["sims_per_item"])
+        if cfg["sims_per_item"] != batch.sims_per_item:
+            batch._build(cfg["sims_per_item"])
         t = _time(lambda: batch.launch(waves=cfg["waves"], wg_waves=cfg["wg_waves"]), reps)
         if verbose:
             print(f"autotune {cfg}: {t:.3f} ms", flush=True)
         if t < best_t:
             best, best_t = dict(cfg), t
-    batch._build(best["sims_per_item"])
-    # second pass, with several window-size groups: each group's work-item
-    # size and resident waves from its solo times.  Short patterns amortise
-    # the per-item fetch over large items, long ones balance better over
-    # small items; resident waves in proportion to the solo times make the
-    # concurrent kernels finish together (equal counts leave the
-    # cheapest-per-wave group running alone).
+    if best["sims_per_item"] != batch.sims_per_item:
+        batch._build(best["sims_per_item"])
+    # second pass: with several window-size groups, size each group's
+    # resident waves by its solo time, so that the concurrent kernels finish
+    # together (equal counts leave the cheapest-per-wave group running alone)
     groups = batch.groups()
     if len(groups) > 1:
         wg, W = best["wg_waves"], best["waves"]
         total = W * len(groups)
-        sizes = sorted({int(c["sims_per_item"]) for c in candidates})
-        solo_at = {}
-        for spi in sizes:
-            batch._build(spi)
-            for ns in groups:
-                gw = [0] * MAX_GROUPS
-                gw[ns] = total
-                solo_at[spi, ns] = _time(lambda: batch.launch(wg_waves=wg, group_waves=gw), reps)
-        pick = {ns: min(sizes, key=lambda s: solo_at[s, ns]) for ns in groups}
-        solo = {ns: solo_at[pick[ns], ns] for ns in groups}
-        items = [pick.get(ns, int(best["sims_per_item"])) for ns in range(MAX_GROUPS)]
-        batch._build(items)
+        solo = {}
+        for ns in groups:
+            gw = [0] * MAX_GROUPS
+            gw[ns] = total
+            solo[ns] = _time(lambda: batch.launch(wg_waves=wg, group_waves=gw), reps)
         norm = sum(solo.values())
         for f in (0.5, 0.75, 1.0, 1.5):
             gw = [0] * MAX_GROUPS
@@ -272,11 +250,10 @@ def autotune(batch: BatchedForward, cache: Optional[Path] = None, reps: int = 10
                 gw[ns] = max(wg, int(round(f * total * t / norm / wg)) * wg)
             t = _time(lambda: batch.launch(wg_waves=wg, group_waves=gw), reps)
             if verbose:
-                print(f"autotune group_items {items} group_waves {gw}: {t:.3f} ms", flush=True)
+                print(f"autotune group_waves {gw}: {t:.3f} ms", flush=True)
             if t < best_t:
-                best = {**best, "group_waves": gw, "group_items": items}
+                best = {**best, "group_waves": gw}
                 best_t = t
-        batch._build(best.get("group_items") or best["sims_per_item"])
     best["ms"] = best_t
     batch.config = best
     table[key] = best

@@ -4242,23 +4242,11 @@ int mbqc_sv_adjoint_backward_c64(const mbqc_plan* plan, const float* angles, int
 
 int mbqc_batch_create(const mbqc_batch_job* jobs, int32_t n_jobs, int32_t sims_per_item,
                       mbqc_batch** out) {
-  return mbqc_batch_create_groups(jobs, n_jobs, &sims_per_item, 1, out);
-}
-
-int mbqc_batch_create_groups(const mbqc_batch_job* jobs, int32_t n_jobs,
-                             const int32_t* sims_per_item, int32_t n_entries, mbqc_batch** out) {
   if (!out) return fail(MBQC_ERR_VALUE, "out is null");
   *out = nullptr;
   if (!jobs || n_jobs <= 0) return fail(MBQC_ERR_VALUE, "no jobs");
-  if (!sims_per_item || n_entries < 1 || n_entries > kMaxBatchSlots + 1)
-    return fail(MBQC_ERR_VALUE, "sims_per_item must have 1.." + std::to_string(kMaxBatchSlots + 1) +
-                                    " entries");
-  int32_t spi[kMaxBatchSlots + 1];
-  for (int ns = 0; ns <= kMaxBatchSlots; ++ns) {
-    spi[ns] = sims_per_item[ns < n_entries ? ns : n_entries - 1];
-    if (spi[ns] <= 0 || spi[ns] % 64 != 0)
-      return fail(MBQC_ERR_VALUE, "sims_per_item must be a positive multiple of 64");
-  }
+  if (sims_per_item <= 0 || sims_per_item % 64 != 0)
+    return fail(MBQC_ERR_VALUE, "sims_per_item must be a positive multiple of 64");
   auto* bt = new mbqc_batch();
   for (bool& sh : bt->share) sh = true;
   for (bool& pe : bt->periodic) pe = true;
@@ -4301,12 +4289,11 @@ int mbqc_batch_create_groups(const mbqc_batch_job* jobs, int32_t n_jobs,
     if (p->ns < 2 || J.S % share_group(p->ns) != 0) bt->share[p->ns] = false;
     if (!p->periodic) bt->periodic[p->ns] = false;
     const int64_t cost = int64_t(p->n_steps) << p->ns;
-    const int32_t per = spi[p->ns];
-    for (int64_t f = 0; f < J.total; f += per) {
+    for (int64_t f = 0; f < J.total; f += sims_per_item) {
       BatchItem it{};
       it.job = i;
       it.first = f;
-      it.count = int32_t(std::min<int64_t>(per, J.total - f));
+      it.count = int32_t(std::min<int64_t>(sims_per_item, J.total - f));
       order.push_back({p->ns, cost, bt->items.size()});
       bt->items.push_back(it);
     }

@@ -201,35 +201,3 @@ def test_batched_lds_groups(rng):
         torch.cuda.synchronize()
         for o, w in zip(bf.outputs, want):
             assert torch.equal(o, w), f"graph replay {rep} differs"
-
-
-def test_batched_group_items(rng):
-    """Work items sized per live-window group (larger for short patterns,
-    smaller for long ones) partition the same simulations: the outputs equal
-    the uniform-item launch's bit for bit; invalid sizes are rejected."""
-    torch = _torch()
-    from mentpy_amd.batched import MAX_GROUPS, BatchedForward
-    jobs = _jobs(rng, n_pat=8, B=70, S=4)
-    bf = BatchedForward([(s, th, st) for s, th, st, _ in jobs], sims_per_item=128)
-    want = [o.clone() for o in bf.launch(waves=512, wg_waves=2)]
-    torch.cuda.synchronize()
-    n128 = bf.n_items
-    for items in ([256, 256, 256, 128, 64, 64], [64] * MAX_GROUPS, [192, 320]):
-        bf._build(items)
-        assert bf.group_items == items
-        for o in bf.outputs:
-            o.fill_(complex(float("nan"), 0.0))
-        got = bf.launch(waves=512, wg_waves=2)
-        torch.cuda.synchronize()
-        for x, y in zip(got, want):
-            assert torch.equal(x, y), f"items {items} differ from the uniform launch"
-    bf._build([64] * MAX_GROUPS)
-    assert bf.n_items >= 2 * n128 - len(jobs)
-    for bad in ([96], [0, 64], [64] * (MAX_GROUPS + 1)):
-        with pytest.raises(ValueError):
-            bf._build(bad)
-    assert bf.group_items == [64] * MAX_GROUPS      # the batch survives a rejected size
-    got = bf.launch(waves=512, wg_waves=2)
-    torch.cuda.synchronize()
-    for x, y in zip(got, want):
-        assert torch.equal(x, y)
