@@ -241,8 +241,7 @@ int mbqc_dm_window_c64(const mbqc_plan* plan, const float* step_angles, int64_t 
  * out is [B*S][2^n_outputs][2^n_outputs].  Live windows up to 10 slots: up to
  * 7 with the density matrix in registers or LDS, 8..10 in a workspace of
  * 4^n_slots complex64 per simulation allocated and freed in stream order
- * (as mbqc_sv_forward_c64).  The window calls below take up to 14 (SV) and
- * 7 (DM) slots. */
+ * (as mbqc_sv_forward_c64). */
 int mbqc_dm_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
                         int64_t angle_stride_a, const void* psi_in, int64_t in_stride, void* out,
                         int64_t B, int64_t S, const mbqc_noise* noise, uint32_t flags, void* stream);

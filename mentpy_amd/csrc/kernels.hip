@@ -2622,15 +2622,15 @@ __device__ __forceinline__ int window_source(const WindowMap& m, int r, uint32_t
 // Reference window after k_steps measurements, statevector: one workgroup per
 // simulation, live state in dynamic LDS (2^n_slots amplitudes), step j at the
 // caller's angle step_angles[b * step_stride + j].
-__global__ __launch_bounds__(64) void k_sv_window(Job J, const float* __restrict__ step_angles,
-                                                  int64_t step_stride, int32_t k_steps,
-                                                  WindowMap map, uint32_t flags) {
-  constexpr int T = 64;
-  extern __shared__ float2 st[];
-  __shared__ float red[4];
+// The body, on the statevector st (LDS or a global-memory slice) of
+// simulation g, T threads, red (4 floats of LDS) as scratch.
+template <int T>
+__device__ __forceinline__ void sv_window_run(float2* st, float* red, const Job& J,
+                                              const float* __restrict__ step_angles,
+                                              int64_t step_stride, int32_t k_steps,
+                                              const WindowMap& map, uint32_t flags, int64_t g,
+                                              int tid) {
   const int ns = J.ns, H = 1 << ns;
-  const int64_t g = blockIdx.x;
-  const int tid = threadIdx.x;
   const int64_t bi = g / J.S, si = g - bi * J.S;
   const float* ang = step_angles + bi * step_stride;
   const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
@@ -2668,17 +2668,36 @@ __global__ __launch_bounds__(64) void k_sv_window(Job J, const float* __restrict
   }
 }
 
+__global__ __launch_bounds__(64) void k_sv_window(Job J, const float* __restrict__ step_angles,
+                                                  int64_t step_stride, int32_t k_steps,
+                                                  WindowMap map, uint32_t flags) {
+  extern __shared__ float2 st[];
+  __shared__ float red[4];
+  sv_window_run<64>(st, red, J, step_angles, step_stride, k_steps, map, flags, blockIdx.x,
+                    threadIdx.x);
+}
+
+// Windows above the LDS: the same on statevectors in a global workspace
+// (ws[(g - g0) * 2^n_slots + j], one slice per workgroup).
+__global__ __launch_bounds__(256) void k_sv_window_global(Job J, const float* __restrict__ step_angles,
+                                                          int64_t step_stride, int32_t k_steps,
+                                                          WindowMap map, uint32_t flags,
+                                                          float2* __restrict__ ws, int64_t g0) {
+  __shared__ float red[4];
+  const int64_t l = blockIdx.x;
+  sv_window_run<256>(ws + (l << J.ns), red, J, step_angles, step_stride, k_steps, map, flags, g0 + l,
+                     threadIdx.x);
+}
+
 // The same for the density matrix (state in dynamic LDS, 4^n_slots entries);
 // the measured qubits' noise is applied as in the full run, output noise is not.
-__global__ __launch_bounds__(64) void k_dm_window(Job J, const float* __restrict__ step_angles,
-                                                  int64_t step_stride, int32_t k_steps,
-                                                  WindowMap map, DmNoise nz) {
-  constexpr int T = 64;
-  extern __shared__ float2 r[];
-  __shared__ float red[4];
+template <int T>
+__device__ __forceinline__ void dm_window_run(float2* r, float* red, const Job& J,
+                                              const float* __restrict__ step_angles,
+                                              int64_t step_stride, int32_t k_steps,
+                                              const WindowMap& map, const DmNoise& nz, int64_t g,
+                                              int tid) {
   const int ns = J.ns, H = 1 << ns;
-  const int64_t g = blockIdx.x;
-  const int tid = threadIdx.x;
   const int64_t bi = g / J.S, si = g - bi * J.S;
   const float* ang = step_angles + bi * step_stride;
   const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
@@ -2717,6 +2736,25 @@ __global__ __launch_bounds__(64) void k_dm_window(Job J, const float* __restrict
     const int jr = window_source(map, row, &sr), jc = window_source(map, col, &sc);
     o[e] = flip(cscale(r[jr * H + jc], scale), sr ^ sc);
   }
+}
+
+__global__ __launch_bounds__(64) void k_dm_window(Job J, const float* __restrict__ step_angles,
+                                                  int64_t step_stride, int32_t k_steps,
+                                                  WindowMap map, DmNoise nz) {
+  extern __shared__ float2 r[];
+  __shared__ float red[4];
+  dm_window_run<64>(r, red, J, step_angles, step_stride, k_steps, map, nz, blockIdx.x, threadIdx.x);
+}
+
+// Windows above the LDS: the same on density matrices in a global workspace.
+__global__ __launch_bounds__(256) void k_dm_window_global(Job J, const float* __restrict__ step_angles,
+                                                          int64_t step_stride, int32_t k_steps,
+                                                          WindowMap map, DmNoise nz,
+                                                          float2* __restrict__ ws, int64_t g0) {
+  __shared__ float red[4];
+  const int64_t l = blockIdx.x;
+  dm_window_run<256>(ws + (l << (2 * J.ns)), red, J, step_angles, step_stride, k_steps, map, nz,
+                     g0 + l, threadIdx.x);
 }
 
 // ------------------------------------------------- measurements in any plane
@@ -2948,14 +2986,16 @@ __global__ __launch_bounds__(64) void k_dm_general(Job J, const float* __restric
 // workspace, ws[(g - g0) * 4^n_slots + e] for simulations g0 + blockIdx (the
 // workgroup's own slice; its threads synchronise through the barrier as in
 // LDS).
-__global__ __launch_bounds__(256) void k_dm_general_global(Job J, DmNoise nz, uint32_t flags,
+__global__ __launch_bounds__(256) void k_dm_general_global(Job J, const float* __restrict__ step_angles,
+                                                           int64_t step_stride, int32_t k_steps,
+                                                           WindowMap map, int32_t window_mode,
+                                                           DmNoise nz, uint32_t flags,
                                                            float* __restrict__ outcomes,
                                                            float2* __restrict__ ws, int64_t g0) {
   __shared__ float red[4];
   const int64_t l = xcd_block(blockIdx.x, gridDim.x);
-  WindowMap none{};
-  dm_general_run<256>(ws + l * (int64_t(1) << (2 * J.ns)), red, J, nullptr, 0, J.n_steps, none, 0,
-                      nz, flags, outcomes, g0 + l, threadIdx.x);
+  dm_general_run<256>(ws + l * (int64_t(1) << (2 * J.ns)), red, J, step_angles, step_stride, k_steps,
+                      map, window_mode, nz, flags, outcomes, g0 + l, threadIdx.x);
 }
 
 
@@ -3156,10 +3196,12 @@ hipError_t launch_sv_global(const Job& J, uint32_t flags, hipStream_t st) {
 
 hipError_t launch_dm_global(const Job& J, const DmNoise& nz, uint32_t flags, float* outcomes,
                             hipStream_t st) {
+  const WindowMap none{};
   return run_global_chunks(J, sizeof(float2) << (2 * J.ns), st,
                            [&](float2* ws, int64_t g0, int64_t n) {
-    hipLaunchKernelGGL(k_dm_general_global, dim3(unsigned(n)), dim3(256), 0, st, J, nz, flags,
-                       outcomes, ws, g0);
+    hipLaunchKernelGGL(k_dm_general_global, dim3(unsigned(n)), dim3(256), 0, st, J,
+                       static_cast<const float*>(nullptr), int64_t(0), int32_t(J.n_steps), none, 0,
+                       nz, flags, outcomes, ws, g0);
   });
 }
 constexpr int kMaxAdjSlots = 5;
@@ -4019,11 +4061,20 @@ int mbqc_sv_window_c64(const mbqc_plan* plan, const float* step_angles, int64_t 
   WindowMap map;
   Plan* p = nullptr;
   int rc = window_job(plan, step_angles, step_stride, psi_in, in_stride, k, out, B, S,
-                      kMaxSvLdsSlots, &J, &map, &p);
+                      kMaxSvGlobalSlots, &J, &map, &p);
   if (rc != MBQC_OK) return rc;
   rc = check_xy_plan(p);
   if (rc != MBQC_OK) return rc;
   if (J.total == 0) return MBQC_OK;
+  hipStream_t gst = static_cast<hipStream_t>(stream);
+  if (p->ns > kMaxSvLdsSlots) {
+    hipError_t e = run_global_chunks(J, sizeof(float2) << p->ns, gst,
+                                     [&](float2* ws, int64_t g0, int64_t n) {
+      hipLaunchKernelGGL(k_sv_window_global, dim3(unsigned(n)), dim3(256), 0, gst, J, step_angles,
+                         step_stride, k, map, flags, ws, g0);
+    });
+    return e == hipSuccess ? MBQC_OK : hip_fail(e, "sv_window launch (global workspace)");
+  }
   const size_t lds = sizeof(float2) << p->ns;
   if (int rl = allow_lds(reinterpret_cast<const void*>(&k_sv_window), lds); rl != MBQC_OK) return rl;
   hipLaunchKernelGGL(k_sv_window, dim3(J.total), dim3(64), lds, static_cast<hipStream_t>(stream), J,
@@ -4039,12 +4090,26 @@ int mbqc_dm_window_c64(const mbqc_plan* plan, const float* step_angles, int64_t 
   WindowMap map;
   Plan* p = nullptr;
   int rc = window_job(plan, step_angles, step_stride, psi_in, in_stride, k, out, B, S,
-                      kMaxDmLdsSlots, &J, &map, &p);
+                      kMaxDmGlobalSlots, &J, &map, &p);
   if (rc != MBQC_OK) return rc;
   DmNoise nz;
   rc = dm_noise(noise, &nz);
   if (rc != MBQC_OK) return rc;
   if (J.total == 0) return MBQC_OK;
+  if (p->ns > kMaxDmLdsSlots) {
+    hipStream_t gst = static_cast<hipStream_t>(stream);
+    const bool general = p->general_planes || outcomes;
+    hipError_t e = run_global_chunks(J, sizeof(float2) << (2 * p->ns), gst,
+                                     [&](float2* ws, int64_t g0, int64_t n) {
+      if (general)
+        hipLaunchKernelGGL(k_dm_general_global, dim3(unsigned(n)), dim3(256), 0, gst, J,
+                           step_angles, step_stride, k, map, 1, nz, 0u, outcomes, ws, g0);
+      else
+        hipLaunchKernelGGL(k_dm_window_global, dim3(unsigned(n)), dim3(256), 0, gst, J,
+                           step_angles, step_stride, k, map, nz, ws, g0);
+    });
+    return e == hipSuccess ? MBQC_OK : hip_fail(e, "dm_window launch (global workspace)");
+  }
   if (J.total > 0x7fffffff) return fail(MBQC_ERR_VALUE, "too many simulations for one launch");
   const size_t lds = sizeof(float2) << (2 * p->ns);
   rc = allow_lds(p->general_planes || outcomes ? reinterpret_cast<const void*>(&k_dm_general)

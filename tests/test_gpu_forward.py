@@ -469,6 +469,56 @@ def test_sv_14_slot_measure_windows_end_at_run_output(rng):
     assert got.shape == want.shape and err < 1e-5
 
 
+@pytest.mark.parametrize("rows", [15, 16])
+def test_sv_measure_windows_beyond_lds_end_at_run_output(rows, rng):
+    """Step-by-step measure() on windows of 15 and 16 slots (the window
+    kernel's body on a stream-ordered global workspace): after the last
+    measurement the window equals run(output_form="sv")."""
+    _torch()
+    import mentpy_amd as mp
+    g = mp.templates.grid_cluster(rows, 3)
+    psi = haar_state(rows, rng)
+    th = rng.uniform(0, 2 * np.pi, len(g.trainable_nodes)).astype(np.float32)
+    sim = mp.PatternSimulator(g, input_state=psi, backend="hip-sv")
+    assert sim.simulator.plan.info["n_slots"] == rows
+    want = sim.run(th, output_form="sv")
+    sim.reset(input_state=psi)
+    tpos = {v: i for i, v in enumerate(g.trainable_nodes)}
+    got = None
+    for node in sim.simulator.schedule_measure:
+        got, out = sim.measure(float(th[tpos[node]]))
+        assert out == 0
+    err = float(np.abs(got - want).max())
+    print(f"{rows}-slot windows (global workspace): final window vs run() {err:.2e}", flush=True)
+    assert got.shape == want.shape and err < 1e-5
+
+
+@pytest.mark.parametrize("xz", [False, True])
+def test_dm_8_slot_measure_windows_end_at_run_output(xz, rng):
+    """Step-by-step measure() on an 8-slot density matrix (global workspace;
+    with an XZ node the general kernel's window mode): without noise the last
+    window is run()'s output."""
+    _torch()
+    import mentpy_amd as mp
+    g = mp.templates.grid_cluster(8, 3)
+    if xz:
+        g[1] = mp.Ment("XZ")
+    psi = haar_state(8, rng)
+    th = rng.uniform(0, 2 * np.pi, len(g.trainable_nodes)).astype(np.float32)
+    sim = mp.PatternSimulator(g, input_state=psi, backend="hip-dm")
+    assert sim.simulator.plan.info["n_slots"] == 8
+    want = sim.run(th)
+    sim.reset(input_state=psi)
+    tpos = {v: i for i, v in enumerate(g.trainable_nodes)}
+    got = None
+    for node in sim.simulator.schedule_measure:
+        got, _ = sim.measure(float(th[tpos[node]]))
+    err = float(np.abs(got - want).max())
+    print(f"8-slot DM windows xz={xz} (global workspace): final window vs run() {err:.2e}",
+          flush=True)
+    assert got.shape == want.shape and err < 1e-5
+
+
 def test_dm_7_slot_measure_windows_end_at_run_output(rng):
     """Step-by-step measure() on a 7-slot density matrix (window kernel: 128 KB
     of dynamic LDS): without noise the last window is run()'s output."""
