@@ -99,6 +99,13 @@ struct RefPhase {
     m_sum += int32_t(uint32_t(__double_as_longlong(mb)));
     th_sum += d;
   }
+  // add(th) where `on`, nothing elsewhere
+  __device__ __forceinline__ void add_if(float th, bool on) {
+    const double d = on ? double(th) : 0.0;
+    const double mb = fma(d, 0.15915494309189533577, kShifter64);
+    m_sum += int32_t(uint32_t(__double_as_longlong(mb)));
+    th_sum += d;
+  }
   __device__ __forceinline__ float2 value() const {
     constexpr double kPi = 3.14159265358979323846;
     double a = 0.5 * th_sum - kPi * double(m_sum);
@@ -204,17 +211,23 @@ __device__ __forceinline__ const float* coef_row(const float* coef, const mbqc_s
 }
 
 // Lanes that share an angle vector in aligned groups of G = kShareGroup<NS>
-// consecutive lanes (S a multiple of G inputs per vector) can share the sine
-// and cosine work of a block: lane j of a group evaluates step j and the
-// values are broadcast inside the group (DPP quad permutations for G <= 4, a
-// ds_swizzle for G = 8; no memory access).
-__host__ __device__ constexpr int share_group(int ns) { return ns <= 2 ? 2 : (ns <= 4 ? 4 : 8); }
+// consecutive lanes (S a multiple of G inputs per vector) can share the sine,
+// cosine and reference-phase work: lane j of a group evaluates step j and the
+// values are broadcast inside the group (DPP quad permutations for G = 4, a
+// ds_swizzle for G = 8, a DPP row broadcast for G = 16; no memory access).
+// Windows of 2 and 3 slots share over 16 lanes, which evaluate all the steps
+// of one angle fetch (8 or 6) at once.
+__host__ __device__ constexpr int share_group(int ns) { return ns <= 3 ? 16 : (ns == 4 ? 4 : 8); }
 template <int NS>
 constexpr int kShareGroup = share_group(NS);
 
 template <int G, int SRC>
 __device__ __forceinline__ float group_bcast(float v) {
-  if constexpr (G == 2) {
+  static_assert(SRC < G, "source lane inside the group");
+  if constexpr (G == 16) {
+    // row_newbcast: lane SRC of each 16-lane row to the whole row
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x150 + SRC, 0xf, 0xf, false));
+  } else if constexpr (G == 2) {
     constexpr int ctrl = SRC | (SRC << 2) | ((2 + SRC) << 4) | ((2 + SRC) << 6);
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xf, 0xf, false));
   } else if constexpr (G == 4) {
@@ -227,8 +240,12 @@ __device__ __forceinline__ float group_bcast(float v) {
   }
 }
 
-template <int NS, int... SUBS>
+// Block [k0, k0 + NS): lane j of the group evaluates step j (and, with REF,
+// adds its reference phase: the group's lanes hold disjoint parts of the sum,
+// reduced by group_reduce_phase at the end).
+template <int NS, bool REF, int... SUBS>
 __device__ __forceinline__ void shared_block_coeffs(const float (&th)[NS], float2 (&bb)[NS],
+                                                    RefPhase* ph, int k0, int n_steps,
                                                     std::integer_sequence<int, SUBS...>) {
   constexpr int G = kShareGroup<NS>;
   static_assert(NS >= 2 && NS <= G, "one step per lane of the group");
@@ -238,7 +255,18 @@ __device__ __forceinline__ void shared_block_coeffs(const float (&th)[NS], float
   for (int sub = 1; sub < NS; ++sub) mine = (j == sub) ? th[sub] : mine;
   float sn, cs;
   sincos_poly(mine, &sn, &cs);
+  if constexpr (REF) ph->add_if(mine, j < NS && k0 + j < n_steps);
   ((bb[SUBS] = make_float2(group_bcast<G, SUBS>(cs), -group_bcast<G, SUBS>(sn))), ...);
+}
+
+// Sum of the reference-phase parts over the G lanes of a sharing group.
+template <int G>
+__device__ __forceinline__ void group_reduce_phase(RefPhase& p) {
+#pragma unroll
+  for (int m = 1; m < G; m <<= 1) {
+    p.m_sum += __shfl_xor(p.m_sum, m, 64);
+    p.th_sum += __shfl_xor(p.th_sum, m, 64);
+  }
 }
 
 template <int NS, bool REF, bool FULL, bool PER_STEP, bool SH, bool PERIODIC, int SUB,
@@ -250,8 +278,7 @@ __device__ __forceinline__ void sv_sub(float2 (&t)[1 << NS], const mbqc_step* st
   if (FULL || k < n_steps) {
     float2 b0;
     if constexpr (SH) {
-      if constexpr (REF) ph->add(th[SUB]);
-      b0 = bb[SUB];
+      b0 = bb[SUB];   // reference phase added where the coefficients were evaluated
     } else {
       b0 = step_coeff<REF>(th[SUB], ph);
     }
@@ -269,9 +296,50 @@ __device__ __forceinline__ void sv_block(float2 (&t)[1 << NS], const mbqc_step* 
                                          const float (&th)[NS], RefPhase* ph, Mod& mod,
                                          std::integer_sequence<int, SUBS...> seq) {
   float2 bb[NS];
-  if constexpr (SH) shared_block_coeffs<NS>(th, bb, seq);
+  if constexpr (SH) shared_block_coeffs<NS, REF>(th, bb, ph, k0, n_steps, seq);
   (sv_sub<NS, REF, FULL, PER_STEP, SH, PERIODIC, SUBS>(t, steps, coef, k0, n_steps, th, bb, ph,
                                                        mod),
+   ...);
+}
+
+// Block B of an angle fetch whose coefficients the group's lanes evaluated at
+// once (lane m of the 16-lane row holds step m's sine and cosine).
+template <int NS, bool REF, bool PER_STEP, bool PERIODIC, int B, typename OnBlock, typename Mod,
+          int... SUBS>
+__device__ __forceinline__ void sv_fetched_block(float2 (&t)[1 << NS], const mbqc_step* steps,
+                                                 const float* coef, int blk, int n_steps, float cs,
+                                                 float sn, RefPhase* ph, OnBlock& on_block,
+                                                 Mod& mod, std::integer_sequence<int, SUBS...>) {
+  float2 bb[NS];
+  ((bb[SUBS] = make_float2(group_bcast<16, B * NS + SUBS>(cs), -group_bcast<16, B * NS + SUBS>(sn))),
+   ...);
+  const float th_unused[NS] = {};
+  on_block(blk + B, t);
+  (sv_sub<NS, REF, true, PER_STEP, true, PERIODIC, SUBS>(t, steps, coef, (blk + B) * NS, n_steps,
+                                                         th_unused, bb, ph, mod),
+   ...);
+}
+
+// The K full blocks of one angle fetch, sharing over 16 lanes: lane m of the
+// row evaluates step m of the fetch (and adds its reference phase).
+template <int NS, int K, bool REF, bool PER_STEP, bool PERIODIC, typename OnBlock, typename Mod,
+          int... BS>
+__device__ __forceinline__ void sv_fetched_blocks(float2 (&t)[1 << NS], const mbqc_step* steps,
+                                                  const float* coef, int blk, int n_steps,
+                                                  const float (&th)[K * NS], RefPhase* ph,
+                                                  OnBlock& on_block, Mod& mod,
+                                                  std::integer_sequence<int, BS...>) {
+  static_assert(K * NS <= 16, "one step per lane of the row");
+  const int j = int(threadIdx.x) & 15;
+  float mine = th[0];
+#pragma unroll
+  for (int m = 1; m < K * NS; ++m) mine = (j == m) ? th[m] : mine;
+  float sn, cs;
+  sincos_poly(mine, &sn, &cs);
+  if constexpr (REF) ph->add_if(mine, j < K * NS);
+  (sv_fetched_block<NS, REF, PER_STEP, PERIODIC, BS>(t, steps, coef, blk, n_steps, cs, sn, ph,
+                                                     on_block, mod,
+                                                     std::make_integer_sequence<int, NS>{}),
    ...);
 }
 
@@ -352,14 +420,19 @@ __device__ __forceinline__ void sv_run_blocks(float2 (&t)[1 << NS], const Job& J
 #pragma unroll
       for (int m = 0; m < K * NS; ++m) th[m] = th_next[m];
       fetch_step_angles<K * NS>(steps, aidx, (blk + K) * NS, n, ang, sa, th_next);
+      if constexpr (SH && kShareGroup<NS> == 16) {
+        sv_fetched_blocks<NS, K, REF, PER_STEP, PERIODIC>(t, steps, coef, blk, n, th, ph, on_block,
+                                                          mod, std::make_integer_sequence<int, K>{});
+      } else {
 #pragma unroll
-      for (int b = 0; b < K; ++b) {
-        float tb[NS];
+        for (int b = 0; b < K; ++b) {
+          float tb[NS];
 #pragma unroll
-        for (int s2 = 0; s2 < NS; ++s2) tb[s2] = th[b * NS + s2];
-        on_block(blk + b, t);
-        sv_block<NS, REF, true, PER_STEP, SH, PERIODIC>(t, steps, coef, (blk + b) * NS, n, tb, ph,
-                                                        mod, Seq{});
+          for (int s2 = 0; s2 < NS; ++s2) tb[s2] = th[b * NS + s2];
+          on_block(blk + b, t);
+          sv_block<NS, REF, true, PER_STEP, SH, PERIODIC>(t, steps, coef, (blk + b) * NS, n, tb,
+                                                          ph, mod, Seq{});
+        }
       }
     }
     if constexpr (K > 1) {
@@ -508,6 +581,8 @@ __device__ __forceinline__ void sv_sim(const Job& J, int64_t g, bool active, int
   RefPhase rp;
   sv_run_blocks<NS, REF, PER_STEP, SH, PERIODIC>(t, J, ang, &rp, 0, (J.n_steps + NS - 1) / NS,
                                        [](int, const float2 (&)[H]) {});
+  // with shared coefficients each lane of a group added the phases of its own steps
+  if constexpr (REF && SH) group_reduce_phase<kShareGroup<NS>>(rp);
   const float2 ph = REF ? rp.value() : make_float2(1.0f, 0.0f);
   // output gather: slot-index j -> output index out_pos[j] (dead slots skipped)
   const int32_t* out_pos = J.tab + J.off_out;

@@ -59,13 +59,14 @@ def test_batched_matches_single_launches(rng, tmp_path):
     print(f"autotune chose {cfg}", flush=True)
 
 
-@pytest.mark.parametrize("S_list", [(8, 8, 16, 8, 16, 8), (8, 3, 16, 8, 4, 8)])
+@pytest.mark.parametrize("S_list", [(16, 16, 16, 8, 16, 16), (8, 3, 16, 8, 4, 8)])
 def test_batched_shared_coefficient_groups(S_list, rng):
     """Window-size groups whose jobs all have S a multiple of the lane-group
-    size (2 lanes for 2-slot windows, 4 for 3-4, 8 for 5-6) evaluate each
-    block's sines/cosines once per group of lanes; a group with one job that
-    cannot share (S = 3, or S = 4 with 6 slots) runs unshared.  Either way
-    every job equals its single launch and the oracle."""
+    size (16 lanes for 2-3-slot windows, which evaluate all steps of an angle
+    fetch at once; 4 for 4 slots, 8 for 5-6) evaluate the sines, cosines and
+    reference phases once per group of lanes; a group with one job that cannot
+    share (S = 3 or 8 with 2-3 slots, S = 4 with 6 slots) runs unshared.  Either
+    way every job equals its single launch and the oracle."""
     torch = _torch()
     import mentpy_amd as mp
     from mentpy_amd.batched import BatchedForward
