@@ -264,11 +264,13 @@ int mbqc_dm_forward_outcomes_c64(const mbqc_plan* plan, const float* angles, int
  * parameter set b are simulations b*C + c.
  * All adjoint entry points need plans in schedule order (rotating slots)
  * and every angle-vector entry used by at most one measured node
- * (angle_index values unique).  These two take live windows of up to 10
+ * (angle_index values unique).  These two take live windows of up to 16
  * qubits: up to 5 with the state in registers, 6..10 with one workgroup per
- * simulation and the state in LDS (the backward pass keeps n_angles floats
- * of per-angle sums in LDS too, so a 10-slot window takes up to about 26,000
- * angles); the segmented ones below take up to 5.  Other plans are
+ * simulation (forward) or partial-sum row (backward) and the state in LDS
+ * (the backward pass keeps n_angles floats of per-angle sums in LDS too, so
+ * a 10-slot window takes up to about 26,000 angles), 11..16 the same with the
+ * states in a workspace the call allocates and frees in stream order
+ * (hipMallocAsync on `stream`); the segmented ones below take up to 5.  Other plans are
  * MBQC_ERR_NOT_IMPLEMENTED. */
 size_t mbqc_sv_adjoint_workspace_bytes(const mbqc_plan* plan, int64_t n_columns);
 int mbqc_sv_adjoint_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,

@@ -1828,12 +1828,9 @@ __global__ __launch_bounds__(64) void k_sv_adj_backward(Job J, const float2* __r
 constexpr int kMaxAdjLdsSlots = 10;
 
 template <int NS, int T>
-__global__ __launch_bounds__(T) void k_sv_adj_forward_lds(Job J, float2* __restrict__ ckpt, int ck) {
+__device__ __forceinline__ void adj_forward_run(const Job& J, float2* __restrict__ ckpt, int ck,
+                                                float2* st, float* th_s, int64_t g, int tid) {
   constexpr int H = 1 << NS;
-  __shared__ float2 st[H];
-  __shared__ float th_s[kAngleChunk];
-  const int64_t g = xcd_block(blockIdx.x, gridDim.x);
-  const int tid = threadIdx.x;
   const int64_t bi = g / J.S, si = g - bi * J.S;
   const float* ang = J.angles + bi * J.angle_stride;
   const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
@@ -1873,6 +1870,24 @@ __global__ __launch_bounds__(T) void k_sv_adj_forward_lds(Job J, float2* __restr
     const int p = out_pos[j];
     if (p >= 0) o[p] = cscale(st[j], scale);
   }
+}
+
+template <int NS, int T>
+__global__ __launch_bounds__(T) void k_sv_adj_forward_lds(Job J, float2* __restrict__ ckpt, int ck) {
+  __shared__ float2 st[1 << NS];
+  __shared__ float th_s[kAngleChunk];
+  adj_forward_run<NS, T>(J, ckpt, ck, st, th_s, xcd_block(blockIdx.x, gridDim.x), threadIdx.x);
+}
+
+// Windows above the LDS: the same with the state in a global workspace
+// (ws[(g - g0) * 2^NS + j] for simulations g = g0 + blockIdx).
+template <int NS>
+__global__ __launch_bounds__(256) void k_sv_adj_forward_global(Job J, float2* __restrict__ ckpt,
+                                                               int ck, float2* __restrict__ ws,
+                                                               int64_t g0) {
+  __shared__ float th_s[kAngleChunk];
+  const int64_t l = xcd_block(blockIdx.x, gridDim.x);
+  adj_forward_run<NS, 256>(J, ckpt, ck, ws + (l << NS), th_s, g0 + l, threadIdx.x);
 }
 
 // The angle of step k of this simulation (its angle-vector entry or its fixed angle).
@@ -1936,19 +1951,13 @@ __device__ void adj_block_lds(float2* cur, float2* lam, float2* ys, float* red, 
 // one after the other; the per-angle sums stay in LDS and the row is stored
 // once.  Dynamic LDS: lam, cur (2^NS each), ys (NS x 2^(NS-1)), acc (n_angles).
 template <int NS, int T>
-__global__ __launch_bounds__(T) void k_sv_adj_backward_lds(Job J, const float2* __restrict__ ckpt,
-                                                           const float2* __restrict__ seed, AdjCtx A,
-                                                           float* __restrict__ gpart) {
-  constexpr int H = 1 << NS, HP = H / 2;
-  extern __shared__ float2 dyn[];
-  float2* lam = dyn;
-  float2* cur = lam + H;
-  float2* ys = cur + H;
-  float* acc = reinterpret_cast<float*>(ys + NS * HP);
-  __shared__ float red[4];
-  const int tid = threadIdx.x;
-  const int64_t row = xcd_block(blockIdx.x, gridDim.x);   // bi * P + r
-  const int64_t bi = row / A.P, r = row - bi * A.P;
+__device__ __forceinline__ void adj_backward_run(const Job& J, const float2* __restrict__ ckpt,
+                                                 const float2* __restrict__ seed, const AdjCtx& A,
+                                                 float* __restrict__ gpart, float2* lam, float2* cur,
+                                                 float2* ys, float* acc, float* red, int64_t row,
+                                                 int tid) {
+  constexpr int H = 1 << NS;
+  const int64_t bi = row / A.P, r = row - bi * A.P;   // row = bi * P + r
   const int ncol = A.red;
   const float* ang = J.angles + bi * J.angle_stride;
   const int64_t sa = J.angle_stride_a;
@@ -2004,6 +2013,40 @@ __global__ __launch_bounds__(T) void k_sv_adj_backward_lds(Job J, const float2* 
   __syncthreads();
   float* prow = gpart + row * A.n_angles;
   for (int a = tid; a < A.n_angles; a += T) prow[a] = acc[a];
+}
+
+template <int NS, int T>
+__global__ __launch_bounds__(T) void k_sv_adj_backward_lds(Job J, const float2* __restrict__ ckpt,
+                                                           const float2* __restrict__ seed, AdjCtx A,
+                                                           float* __restrict__ gpart) {
+  constexpr int H = 1 << NS, HP = H / 2;
+  extern __shared__ float2 dyn[];
+  __shared__ float red[4];
+  float2* lam = dyn;
+  float2* cur = lam + H;
+  float2* ys = cur + H;
+  float* acc = reinterpret_cast<float*>(ys + NS * HP);
+  adj_backward_run<NS, T>(J, ckpt, seed, A, gpart, lam, cur, ys, acc, red,
+                          xcd_block(blockIdx.x, gridDim.x), threadIdx.x);
+}
+
+// Windows above the LDS: lam, cur and the block's measured halves in a global
+// workspace slice per workgroup (ws[(row - r0) * (2 + NS/2) * 2^NS ...]), the
+// per-angle sums in dynamic LDS.
+template <int NS>
+__global__ __launch_bounds__(256) void k_sv_adj_backward_global(Job J, const float2* __restrict__ ckpt,
+                                                                const float2* __restrict__ seed,
+                                                                AdjCtx A, float* __restrict__ gpart,
+                                                                float2* __restrict__ ws, int64_t r0) {
+  constexpr int H = 1 << NS, HP = H / 2;
+  extern __shared__ float acc_dyn[];
+  __shared__ float red[4];
+  const int64_t l = xcd_block(blockIdx.x, gridDim.x);
+  float2* lam = ws + l * int64_t(2 * H + NS * HP);
+  float2* cur = lam + H;
+  float2* ys = cur + H;
+  adj_backward_run<NS, 256>(J, ckpt, seed, A, gpart, lam, cur, ys, acc_dyn, red, r0 + l,
+                            threadIdx.x);
 }
 
 // ---------------------------------------------------------------- segmented adjoint
@@ -3240,6 +3283,7 @@ constexpr int kMaxDmLdsSlots = 7;    // LDS-resident DM kernel (4^7 entries: 128
 // on the caller's stream, freed on it after the launch), at most
 // kGlobalWorkspaceBytes per launch (larger batches run in chunks).
 constexpr int kMaxSvGlobalSlots = 20;   // 2^20 amplitudes: 8 MiB per simulation
+constexpr int kMaxAdjGlobalSlots = 16;  // adjoint: (2 + 16/2) x 2^16 amplitudes per row, 5 MiB
 constexpr int kMaxDmGlobalSlots = 10;   // 4^10 entries: 8 MiB per simulation
 constexpr size_t kGlobalWorkspaceBytes = size_t(4) << 30;
 
@@ -3346,9 +3390,9 @@ int check_xy_plan(const Plan* p) {
 int check_adjoint_plan(const Plan* p) {
   int rc = check_xy_plan(p);
   if (rc != MBQC_OK) return rc;
-  if (p->ns > kMaxAdjLdsSlots)
+  if (p->ns > kMaxAdjGlobalSlots)
     return fail(MBQC_ERR_NOT_IMPLEMENTED, "adjoint kernels support live windows of at most " +
-                                              std::to_string(kMaxAdjLdsSlots) + " qubits");
+                                              std::to_string(kMaxAdjGlobalSlots) + " qubits");
   if (p->shared_angles)
     return fail(MBQC_ERR_NOT_IMPLEMENTED,
                 "adjoint gradients need every angle-vector entry to belong to at most one "
@@ -3526,6 +3570,46 @@ hipError_t launch_adj_fwd_lds(const Job& J, float2* ckpt, int ck, hipStream_t st
   hipLaunchKernelGGL((k_sv_adj_forward_lds<NS, T>), dim3(unsigned(J.total)), dim3(T), 0, st, J, ckpt,
                      ck);
   return hipGetLastError();
+}
+
+template <int NS>
+hipError_t launch_adj_fwd_global(const Job& J, float2* ckpt, int ck, hipStream_t st) {
+  return run_global_chunks(J, sizeof(float2) << NS, st, [&](float2* ws, int64_t g0, int64_t n) {
+    hipLaunchKernelGGL((k_sv_adj_forward_global<NS>), dim3(unsigned(n)), dim3(256), 0, st, J, ckpt,
+                       ck, ws, g0);
+  });
+}
+
+// Backward sweep above the LDS: per partial-sum row a global slice of
+// (2 + NS/2) 2^NS amplitudes (allocated and freed in stream order, at most
+// kGlobalWorkspaceBytes per launch, rows in chunks) and n_angles floats of
+// dynamic LDS.
+template <int NS>
+hipError_t launch_adj_bwd_global(const Job& J, const float2* ckpt, const float2* seed,
+                                 const AdjCtx& A, hipStream_t st) {
+  constexpr int64_t H = int64_t(1) << NS;
+  const size_t row_bytes = size_t(2 * H + NS * (H / 2)) * sizeof(float2);
+  const size_t lds = size_t(A.n_angles) * sizeof(float);
+  const void* kern = reinterpret_cast<const void*>(&k_sv_adj_backward_global<NS>);
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (e != hipSuccess) return e;
+  }
+  const int64_t rows = (J.total / A.C) * A.P;
+  const int64_t per = std::max<int64_t>(
+      1, std::min<int64_t>(int64_t(kGlobalWorkspaceBytes / row_bytes), 0x7fffffff));
+  const int64_t n0 = std::min<int64_t>(per, rows);
+  void* ws = nullptr;
+  hipError_t e = hipMallocAsync(&ws, size_t(n0) * row_bytes, st);
+  if (e != hipSuccess) return e;
+  for (int64_t r0 = 0; r0 < rows && e == hipSuccess; r0 += n0) {
+    const int64_t n = std::min<int64_t>(n0, rows - r0);
+    hipLaunchKernelGGL((k_sv_adj_backward_global<NS>), dim3(unsigned(n)), dim3(256), lds, st, J,
+                       ckpt, seed, A, A.gpart, static_cast<float2*>(ws), r0);
+    e = hipGetLastError();
+  }
+  const hipError_t ef = hipFreeAsync(ws, st);
+  return e != hipSuccess ? e : ef;
 }
 
 // Dynamic LDS of k_sv_adj_backward_lds: lam and cur (2^ns each), a block's
@@ -4329,6 +4413,12 @@ int mbqc_sv_adjoint_forward_c64(const mbqc_plan* plan, const float* angles, int6
     case 8: e = launch_adj_fwd_lds<8>(J, ck, adj_ckpt_blocks(B * C), st); break;
     case 9: e = launch_adj_fwd_lds<9>(J, ck, adj_ckpt_blocks(B * C), st); break;
     case 10: e = launch_adj_fwd_lds<10>(J, ck, adj_ckpt_blocks(B * C), st); break;
+    case 11: e = launch_adj_fwd_global<11>(J, ck, adj_ckpt_blocks(B * C), st); break;
+    case 12: e = launch_adj_fwd_global<12>(J, ck, adj_ckpt_blocks(B * C), st); break;
+    case 13: e = launch_adj_fwd_global<13>(J, ck, adj_ckpt_blocks(B * C), st); break;
+    case 14: e = launch_adj_fwd_global<14>(J, ck, adj_ckpt_blocks(B * C), st); break;
+    case 15: e = launch_adj_fwd_global<15>(J, ck, adj_ckpt_blocks(B * C), st); break;
+    case 16: e = launch_adj_fwd_global<16>(J, ck, adj_ckpt_blocks(B * C), st); break;
   }
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "adjoint forward launch");
 }
@@ -4357,7 +4447,9 @@ int mbqc_sv_adjoint_backward_c64(const mbqc_plan* plan, const float* angles, int
   A.red = (C <= 64 && (C & (C - 1)) == 0) ? int32_t(C) : (C % 64 == 0 ? 64 : 1);
   A.n_angles = p->n_angles;
   A.ck = adj_ckpt_blocks(B * C);
-  if (p->ns > kMaxAdjSlots && adj_lds_bytes(p->ns, p->n_angles) > kMaxDynLds)
+  if (p->ns > kMaxAdjSlots &&
+      (p->ns > kMaxAdjLdsSlots ? size_t(p->n_angles) * sizeof(float)
+                               : adj_lds_bytes(p->ns, p->n_angles)) > kMaxDynLds)
     return fail(MBQC_ERR_NOT_IMPLEMENTED,
                 "adjoint of a " + std::to_string(p->ns) + "-slot window with " +
                     std::to_string(p->n_angles) + " angles exceeds the workgroup's LDS");
@@ -4376,6 +4468,12 @@ int mbqc_sv_adjoint_backward_c64(const mbqc_plan* plan, const float* angles, int
     case 8: e = launch_adj_bwd_lds<8>(J, ck, sd, A, st); break;
     case 9: e = launch_adj_bwd_lds<9>(J, ck, sd, A, st); break;
     case 10: e = launch_adj_bwd_lds<10>(J, ck, sd, A, st); break;
+    case 11: e = launch_adj_bwd_global<11>(J, ck, sd, A, st); break;
+    case 12: e = launch_adj_bwd_global<12>(J, ck, sd, A, st); break;
+    case 13: e = launch_adj_bwd_global<13>(J, ck, sd, A, st); break;
+    case 14: e = launch_adj_bwd_global<14>(J, ck, sd, A, st); break;
+    case 15: e = launch_adj_bwd_global<15>(J, ck, sd, A, st); break;
+    case 16: e = launch_adj_bwd_global<16>(J, ck, sd, A, st); break;
   }
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "adjoint backward launch");
 }

@@ -1,6 +1,7 @@
-"""GPU parity of the adjoint angle gradient for live windows of 6..10 slots
-(the LDS-resident sweeps: one workgroup per simulation in the forward pass,
-one per partial-sum row in the backward pass).
+"""GPU parity of the adjoint angle gradient for live windows of 6..16 slots
+(one workgroup per simulation in the forward pass, one per partial-sum row in
+the backward pass; states in LDS up to 10 slots, in a stream-ordered global
+workspace above).
 
 References: the exact shift rule (s = pi/2) of the complex128 oracle cost,
 exact for these costs (a + b cos t + c sin t in every angle), and the
@@ -44,7 +45,8 @@ def _oracle_cost(g, X, Y):
 @pytest.mark.parametrize("rows,cols,S,mode,B", [
     (6, 3, 8, "direct", 3), (6, 3, 64, "transfer", 2), (7, 3, 5, "direct", 2),
     (7, 3, 130, "transfer", 2), (8, 3, 6, "direct", 2), (9, 3, 3, "direct", 2),
-    (10, 3, 4, "direct", 2)])
+    (10, 3, 4, "direct", 2), (11, 3, 3, "direct", 2), (12, 3, 2, "direct", 1),
+    (16, 3, 1, "direct", 1)])
 def test_lds_adjoint_matches_exact_shift(rows, cols, S, mode, B, rng):
     """Column counts 3..128 cover the three partial-row layouts (one row of
     C <= 64 columns, rows of 64 columns, one row per column)."""
@@ -123,7 +125,7 @@ def test_lds_adjoint_wide_window_matches_register_path(rows, cols, w, S, mode, r
 
 def test_lds_adjoint_refusals(rng):
     """Time segments stay with the register sweeps (at most 5 slots); windows
-    above 10 slots have no adjoint."""
+    above 16 slots have no adjoint."""
     _torch()
     import mentpy_amd as mp
     from mentpy_amd import _native as nat
@@ -135,9 +137,9 @@ def test_lds_adjoint_refusals(rng):
     with pytest.raises((NotImplementedError, ValueError)):
         MBQMLStep(sim, X, Y, n_sets=4, mode="transfer", segments=2).run(
             np.zeros((4, len(g.trainable_nodes)), np.float32))
-    g11 = mp.templates.grid_cluster(11, 3)
-    sim11 = mp.PatternSimulator(g11, backend="hip-sv")
-    X, Y = _dataset(11, 2, rng)
+    g17 = mp.templates.grid_cluster(17, 3)
+    sim17 = mp.PatternSimulator(g17, backend="hip-sv")
+    X, Y = _dataset(17, 1, rng)
     with pytest.raises(NotImplementedError):
-        MBQMLStep(sim11, X, Y, n_sets=1, mode="direct").run(
-            np.zeros((1, len(g11.trainable_nodes)), np.float32))
+        MBQMLStep(sim17, X, Y, n_sets=1, mode="direct").run(
+            np.zeros((1, len(g17.trainable_nodes)), np.float32))

@@ -182,8 +182,9 @@ def test_kernel_limits_are_reported(native):
     rc = L.mbqc_sv_segmented_forward_c64(plan.handle, None, 0, 1, None, 1, 2, None, 0, None)
     assert rc == native.MBQC_ERR_NOT_IMPLEMENTED          # time segments: at most 5 slots
     assert L.mbqc_sv_segments_auto(plan.handle, 1) == 1
-    rc = L.mbqc_sv_adjoint_forward_c64(plan11.handle, None, 0, 1, None, 0, None, 1, 1, None, 0, None)
-    assert rc == native.MBQC_ERR_NOT_IMPLEMENTED          # adjoint: at most 10 slots
+    plan17, *_ = compile_pattern(mp.templates.grid_cluster(17, 3))
+    rc = L.mbqc_sv_adjoint_forward_c64(plan17.handle, None, 0, 1, None, 0, None, 1, 1, None, 0, None)
+    assert rc == native.MBQC_ERR_NOT_IMPLEMENTED          # adjoint: at most 16 slots
     plan20, *_ = compile_pattern(mp.templates.grid_cluster(20, 3))
     assert plan20.info["n_slots"] == 20                     # statevectors: up to 20 slots
     with pytest.raises(NotImplementedError):
