@@ -205,9 +205,10 @@ int mbqc_sv_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
  * word k/32 = outcome of the k-th measured node), words W and W+1 the X and Z
  * frame applied to the output (bit n_outputs-1-m = output node m).  Draws are
  * a counter-based hash of (seed, simulation, step).  Needs the flow in the
- * pattern description, no dropped edges and n_slots <= 14 (else
+ * pattern description, no dropped edges and n_slots <= 20 (else
  * MBQC_ERR_NOT_IMPLEMENTED with the reason); windows of 7..14 slots run one
- * workgroup per simulation with the state in LDS. */
+ * workgroup per simulation with the state in LDS, 15..20 the same with the
+ * state in a workspace allocated and freed in stream order. */
 int mbqc_sv_sample_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
                        int64_t angle_stride_a, const void* psi_in, int64_t in_stride, void* out,
                        int32_t* outcomes, int64_t B, int64_t S, uint64_t seed, void* stream);

@@ -1190,17 +1190,13 @@ __global__ __launch_bounds__(256) void k_sv_forward_global(Job J, uint32_t flags
 // k_sv_forward_lds.  Raw phase; the output is renormalised and leaves with the
 // output frame applied, as in k_sv_sample.
 template <int NS, int T>
-__global__ __launch_bounds__(T) void k_sv_sample_lds(Job J, const uint64_t* __restrict__ xm,
-                                                     const uint64_t* __restrict__ zm,
-                                                     const int32_t* __restrict__ out_off,
-                                                     int32_t* __restrict__ outcomes, int32_t words,
-                                                     uint64_t seed) {
+__device__ __forceinline__ void sample_run(const Job& J, const uint64_t* __restrict__ xm,
+                                           const uint64_t* __restrict__ zm,
+                                           const int32_t* __restrict__ out_off,
+                                           int32_t* __restrict__ outcomes, int32_t words,
+                                           uint64_t seed, float2* st, float* red, float* th_s,
+                                           int64_t g, int tid) {
   constexpr int H = 1 << NS;
-  __shared__ float2 st[H];
-  __shared__ float red[4];
-  __shared__ float th_s[kAngleChunk];
-  const int64_t g = xcd_block(blockIdx.x, gridDim.x);
-  const int tid = threadIdx.x;
   const int64_t bi = g / J.S, si = g - bi * J.S;
   const float* ang = J.angles + bi * J.angle_stride;
   const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
@@ -1257,6 +1253,34 @@ __global__ __launch_bounds__(T) void k_sv_sample_lds(Job J, const uint64_t* __re
     row[words] = int32_t(xo);
     row[words + 1] = int32_t(zo);
   }
+}
+
+template <int NS, int T>
+__global__ __launch_bounds__(T) void k_sv_sample_lds(Job J, const uint64_t* __restrict__ xm,
+                                                     const uint64_t* __restrict__ zm,
+                                                     const int32_t* __restrict__ out_off,
+                                                     int32_t* __restrict__ outcomes, int32_t words,
+                                                     uint64_t seed) {
+  __shared__ float2 st[1 << NS];
+  __shared__ float red[4];
+  __shared__ float th_s[kAngleChunk];
+  sample_run<NS, T>(J, xm, zm, out_off, outcomes, words, seed, st, red, th_s,
+                    xcd_block(blockIdx.x, gridDim.x), threadIdx.x);
+}
+
+// Windows above the LDS: the same with the state in a global workspace.
+template <int NS>
+__global__ __launch_bounds__(256) void k_sv_sample_global(Job J, const uint64_t* __restrict__ xm,
+                                                          const uint64_t* __restrict__ zm,
+                                                          const int32_t* __restrict__ out_off,
+                                                          int32_t* __restrict__ outcomes,
+                                                          int32_t words, uint64_t seed,
+                                                          float2* __restrict__ ws, int64_t g0) {
+  __shared__ float red[4];
+  __shared__ float th_s[kAngleChunk];
+  const int64_t l = xcd_block(blockIdx.x, gridDim.x);
+  sample_run<NS, 256>(J, xm, zm, out_off, outcomes, words, seed, ws + (l << NS), red, th_s, g0 + l,
+                      threadIdx.x);
 }
 
 // Sub-step SUB of the block starting at step k0 (measured slot NS - 1 - SUB);
@@ -3776,6 +3800,17 @@ hipError_t launch_sv_lds(const Job& J, uint32_t flags, hipStream_t st) {
 }
 
 template <int NS>
+hipError_t launch_sample_global(const Job& J, const int32_t* tab, const TableOffsets& o,
+                                int32_t* outcomes, int32_t words, uint64_t seed, hipStream_t st) {
+  const uint64_t* xm = reinterpret_cast<const uint64_t*>(tab + o.xmask);
+  const uint64_t* zm = reinterpret_cast<const uint64_t*>(tab + o.zmask);
+  return run_global_chunks(J, sizeof(float2) << NS, st, [&](float2* ws, int64_t g0, int64_t n) {
+    hipLaunchKernelGGL((k_sv_sample_global<NS>), dim3(unsigned(n)), dim3(256), 0, st, J, xm, zm,
+                       tab + o.out_offset, outcomes, words, seed, ws, g0);
+  });
+}
+
+template <int NS>
 hipError_t launch_sample_lds(const Job& J, const int32_t* tab, const TableOffsets& o,
                              int32_t* outcomes, int32_t words, uint64_t seed, hipStream_t st) {
   constexpr int T = (1 << (NS - 1)) < 256 ? (1 << (NS - 1)) : 256;
@@ -4187,7 +4222,13 @@ int mbqc_sv_sample_c64(const mbqc_plan* plan, const float* angles, int64_t angle
     case 12: e = launch_sample_lds<12>(J, tab, o, outcomes, words, seed, st); break;
     case 13: e = launch_sample_lds<13>(J, tab, o, outcomes, words, seed, st); break;
     case 14: e = launch_sample_lds<14>(J, tab, o, outcomes, words, seed, st); break;
-    default: return fail(MBQC_ERR_NOT_IMPLEMENTED, "sampled outcomes need n_slots <= 14");
+    case 15: e = launch_sample_global<15>(J, tab, o, outcomes, words, seed, st); break;
+    case 16: e = launch_sample_global<16>(J, tab, o, outcomes, words, seed, st); break;
+    case 17: e = launch_sample_global<17>(J, tab, o, outcomes, words, seed, st); break;
+    case 18: e = launch_sample_global<18>(J, tab, o, outcomes, words, seed, st); break;
+    case 19: e = launch_sample_global<19>(J, tab, o, outcomes, words, seed, st); break;
+    case 20: e = launch_sample_global<20>(J, tab, o, outcomes, words, seed, st); break;
+    default: return fail(MBQC_ERR_NOT_IMPLEMENTED, "sampled outcomes need n_slots <= 20");
   }
   return e == hipSuccess ? MBQC_OK : hip_fail(e, "sv_sample launch");
 }

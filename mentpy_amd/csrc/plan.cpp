@@ -320,7 +320,7 @@ int compile_plan(const mbqc_pattern_desc* d, Plan* p) {
   } else if (p->n_dropped > 0) {
     p->sample_reason = "the window drops CZ edges";
   } else if (ns > kMaxSampleSlots) {
-    p->sample_reason = "sampled outcomes support live windows of at most 14 qubits";
+    p->sample_reason = "sampled outcomes support live windows of at most 20 qubits";
   } else {
     bool ok = true;
     p->xmask.assign(nm, 0);

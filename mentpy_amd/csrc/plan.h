@@ -11,7 +11,7 @@ namespace mbqc {
 constexpr int kMaxSlots = 20;   // host tables; kernels support fewer (see launchers)
 constexpr int kMaxDevices = 16;
 constexpr int kMaxCoefSlots = 6;
-constexpr int kMaxSampleSlots = 14;   // sampled outcomes (LDS kernel above 6 slots)
+constexpr int kMaxSampleSlots = 20;   // sampled outcomes (LDS kernel above 6 slots, global above 14)
 // largest pair-coefficient table kept with one row per step
 constexpr size_t kCoefPerStepBytes = 8192;
 

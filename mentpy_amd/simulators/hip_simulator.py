@@ -360,7 +360,7 @@ class _HipBackend(BaseSimulator):
         if not self.force0 and not plan.info["can_sample"]:
             raise NotImplementedError(
                 "force0=False needs a pattern with a causal flow, no dropped CZ edges and a "
-                "live window of at most 14 qubits")
+                "live window of at most 20 qubits")
         self._rng = np.random.default_rng(seed)
         self.last_outcomes = None
         self._step_angles = []

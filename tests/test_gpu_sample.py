@@ -123,9 +123,11 @@ def test_force0_false_api(rng):
                             schedule=list(g3.measurement_order))
 
 
-@pytest.mark.parametrize("rows,cols,w", [(3, 5, 8), (4, 6, 9), (2, 12, 12), (5, 6, 15)])
+@pytest.mark.parametrize("rows,cols,w", [(3, 5, 8), (4, 6, 9), (2, 12, 12), (5, 6, 15), (5, 6, 17),
+                                         (4, 8, 21)])
 def test_sampled_lds_windows_match_register_kernel(rows, cols, w, rng):
-    """Windows of 7..14 slots run the LDS kernel: with the same seed the
+    """Windows of 7..14 slots run the LDS kernel, 15..20 the same code on a
+    stream-ordered global workspace: with the same seed the
     outcome rows (bits and output frames) equal the register kernel's on the
     minimal window bit for bit, and the outputs agree (the draws depend on
     the seed, the simulation and the step only)."""
