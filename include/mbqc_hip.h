@@ -338,15 +338,17 @@ int64_t mbqc_batch_items(const mbqc_batch* batch);
  * in a hipGraph), `wg_waves` wavefronts per workgroup. */
 int mbqc_batch_launch(mbqc_batch* batch, int32_t waves, int32_t wg_waves, uint32_t flags,
                       void* stream);
-/* Work items of the group of live-window size n_slots (0..14; -1 for a null
+/* Work items of the group of live-window size n_slots (0..20; -1 for a null
  * batch or n_slots out of range).  Each non-empty group runs its own
  * persistent kernel, concurrently with the others: up to 6 slots one
  * simulation per lane, 7..14 slots one simulation at a time per workgroup
  * with the state in LDS (2^(n_slots-1) threads, at most 256; its resident
- * waves make waves / (threads / 64) workgroups). */
+ * waves make waves / (threads / 64) workgroups), 15..20 the same with the
+ * states in a workspace allocated and freed in stream order (256 threads,
+ * at most 4 GiB of states). */
 int64_t mbqc_batch_group_items(const mbqc_batch* batch, int32_t n_slots);
 /* mbqc_batch_launch with the resident wavefront count chosen per group:
- * group_waves[ns] (n_entries <= 15 entries, indexed by live-window size) for
+ * group_waves[ns] (n_entries <= 21 entries, indexed by live-window size) for
  * the group of size ns; 0 skips the group (its outputs are not written), a
  * positive count must be at least wg_waves.  Sizing the groups by their
  * work makes the concurrent kernels finish together. */

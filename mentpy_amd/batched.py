@@ -29,9 +29,10 @@ from .simulators.hip_simulator import _angle_strides, _as_angles, _as_states, _r
 
 __all__ = ["BatchedForward", "autotune", "default_cache_path"]
 
-# live-window sizes 0..14 of the batched statevector kernels (one simulation per
-# lane up to 6 slots, one per workgroup with the state in LDS above)
-MAX_GROUPS = 15
+# live-window sizes 0..20 of the batched statevector kernels (one simulation per
+# lane up to 6 slots, one per workgroup with the state in LDS up to 14, in a
+# global workspace above)
+MAX_GROUPS = 21
 # tag of the cached tuning results (entries timed another way are not reused)
 _TUNE_VERSION = "graph-replay"
 

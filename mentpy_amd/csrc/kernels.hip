@@ -3266,6 +3266,35 @@ __global__ __launch_bounds__(T) void k_batch_lds(const Job* __restrict__ jobs,
   }
 }
 
+// Groups above the LDS (15..20 slots): the same with each persistent
+// workgroup's state in its own slice of a global workspace,
+// ws[blockIdx * 2^NS + j].
+template <int NS>
+__global__ __launch_bounds__(256) void k_batch_global(const Job* __restrict__ jobs,
+                                                      const BatchItem* __restrict__ items,
+                                                      int64_t n_items, int* counter, uint32_t flags,
+                                                      float2* __restrict__ ws) {
+  __shared__ float red[4];
+  __shared__ float th_s[kAngleChunk];
+  __shared__ int item_s;
+  const int tid = threadIdx.x;
+  float2* st = ws + (int64_t(blockIdx.x) << NS);
+  for (;;) {
+    __syncthreads();   // the previous item's last reads of item_s and st are done
+    if (tid == 0) item_s = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = item_s;
+    if (it >= n_items) break;
+    const BatchItem I = items[it];
+    const Job J = load_job(jobs, I.job);
+    const int64_t end = std::min<int64_t>(I.first + I.count, J.total);
+    for (int64_t g = I.first; g < end; ++g) {
+      __syncthreads();
+      sv_lds_run<NS, 256>(J, g, flags, st, red, th_s, tid);
+    }
+  }
+}
+
 // group counters live kCounterStride ints apart (separate cache lines)
 constexpr int kCounterStride = 64;
 
@@ -3288,7 +3317,7 @@ static __device__ __align__(256) float scratch[kScratchFloats];
 namespace {
 
 constexpr int kMaxSvSlots = 6;       // register-resident SV kernels
-constexpr int kMaxBatchSlots = 14;   // batched groups (LDS-resident above kMaxSvSlots)
+constexpr int kMaxBatchSlots = 20;   // batched groups (LDS above kMaxSvSlots, global above 14)
 // below this many simulations the DM path keeps each density matrix in LDS
 // (one quartet per thread, several simulations per workgroup); the two
 // kernels take the same time at 32,768 simulations of the 3x12 grid
@@ -4011,6 +4040,24 @@ void launch_batch_lds_group(int64_t waves, const BatchGroupLaunch& L) {
                      L.items, L.n_items, L.counter, L.flags);
 }
 
+// Groups above the LDS: `waves` / 4 persistent 256-thread workgroups (at least
+// one, at most one per item, at most kGlobalWorkspaceBytes of states), each
+// with its slice of a workspace allocated and freed in stream order.
+template <int NS>
+hipError_t launch_batch_global_group(int64_t waves, const BatchGroupLaunch& L) {
+  const size_t state_bytes = sizeof(float2) << NS;
+  const int64_t cap = std::max<int64_t>(1, int64_t(kGlobalWorkspaceBytes / state_bytes));
+  const int64_t wgs = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(waves / 4, L.n_items), cap));
+  void* ws = nullptr;
+  hipError_t e = hipMallocAsync(&ws, size_t(wgs) * state_bytes, L.stream);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_batch_global<NS>), dim3(unsigned(wgs)), dim3(256), 0, L.stream, L.jobs,
+                     L.items, L.n_items, L.counter, L.flags, static_cast<float2*>(ws));
+  e = hipGetLastError();
+  const hipError_t ef = hipFreeAsync(ws, L.stream);
+  return e != hipSuccess ? e : ef;
+}
+
 }  // namespace
 
 struct mbqc_batch {
@@ -4090,9 +4137,15 @@ static int batch_launch(mbqc_batch* bt, const int32_t* group_waves, int32_t wg_w
       case 11: launch_batch_lds_group<11>(group_waves[ns], L); break;
       case 12: launch_batch_lds_group<12>(group_waves[ns], L); break;
       case 13: launch_batch_lds_group<13>(group_waves[ns], L); break;
-      default: launch_batch_lds_group<14>(group_waves[ns], L); break;
+      case 14: launch_batch_lds_group<14>(group_waves[ns], L); break;
+      case 15: e = launch_batch_global_group<15>(group_waves[ns], L); break;
+      case 16: e = launch_batch_global_group<16>(group_waves[ns], L); break;
+      case 17: e = launch_batch_global_group<17>(group_waves[ns], L); break;
+      case 18: e = launch_batch_global_group<18>(group_waves[ns], L); break;
+      case 19: e = launch_batch_global_group<19>(group_waves[ns], L); break;
+      default: e = launch_batch_global_group<20>(group_waves[ns], L); break;
     }
-    e = hipGetLastError();
+    if (e == hipSuccess) e = hipGetLastError();
     if (e == hipSuccess && concurrent) {
       e = hipEventRecord(bt->join[ns], gs);
       if (e == hipSuccess) e = hipStreamWaitEvent(st, bt->join[ns], 0);
