@@ -2619,6 +2619,29 @@ __device__ __forceinline__ void chain_tile_load(float2* lds, const float2* src, 
   }
 }
 
+// The same tile into four registers per lane (entries l + 64 i), so the
+// next step's tiles load while the current product runs.
+struct ChainTileRegs {
+  float2 v[4];
+  __device__ __forceinline__ void load(const float2* src, int rows, int cols) {
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = l + 64 * i;
+      const int c = e >> 4, r = e & 15;
+      v[i] = (r < rows && c < cols) ? src[c * rows + r] : make_float2(0.f, 0.f);
+    }
+  }
+  __device__ __forceinline__ void put(float2* lds) const {
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = l + 64 * i;
+      lds[(e >> 4) * kChainLd + (e & 15)] = v[i];
+    }
+  }
+};
+
 // D tile of this lane into LDS (column-major) and, for rows < rows and
 // columns < cols, into global memory (ld = rows).
 __device__ __forceinline__ void chain_tile_store(float2* lds, float2* dst, const f32x4& dr,
@@ -2643,11 +2666,16 @@ __global__ __launch_bounds__(64) void k_seg_chain_forward_mfma(SegInfo G, float2
   const int64_t b = blockIdx.x;
   const int l = threadIdx.x;
   const int C0 = G.cols0;
+  auto rows_of = [&](int p) { return p + 1 == G.n ? G.rows_last : 16; };
+  ChainTileRegs tn;
+  tn.load(ws + G.tmat[1] + b * 16 * rows_of(1), rows_of(1), 16);
   chain_tile_load(Sm, ws + G.tmat[0] + b * C0 * 16, 16, C0);
   for (int p = 1; p < G.n; ++p) {
     const bool last = p + 1 == G.n;
-    const int R = last ? G.rows_last : 16;
-    chain_tile_load(Tm, ws + G.tmat[p] + b * 16 * R, R, 16);
+    const int R = rows_of(p);
+    tn.put(Tm);
+    // the next step's tile loads while this product runs
+    if (!last) tn.load(ws + G.tmat[p + 1] + b * 16 * rows_of(p + 1), rows_of(p + 1), 16);
     wave_lds_sync();
     f32x4 dr = {0.f, 0.f, 0.f, 0.f}, di = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -2672,11 +2700,20 @@ __global__ __launch_bounds__(64) void k_seg_chain_backward_mfma(SegInfo G, float
   const int l = threadIdx.x;
   const int C0 = G.cols0;
   int R = G.rows_last;                                   // rows of L
+  auto s_src = [&](int p) { return (p == 1 ? ws + G.tmat[0] : ws + G.smat[p]) + b * C0 * 16; };
+  ChainTileRegs sn, tn;
+  sn.load(s_src(G.n - 1), 16, C0);
+  tn.load(ws + G.tmat[G.n - 1] + b * 16 * R, R, 16);
   chain_tile_load(Lm, seed + b * C0 * R, R, C0);
   const int kk0 = l >> 4, rc = l & 15;
   for (int p = G.n - 1; p >= 1; --p) {
-    chain_tile_load(Sm, (p == 1 ? ws + G.tmat[0] : ws + G.smat[p]) + b * C0 * 16, 16, C0);
-    chain_tile_load(Tm, ws + G.tmat[p] + b * 16 * R, R, 16);
+    sn.put(Sm);
+    tn.put(Tm);
+    // the next step's tiles load while these products run
+    if (p > 1) {
+      sn.load(s_src(p - 1), 16, C0);
+      tn.load(ws + G.tmat[p - 1] + b * 16 * 16, 16, 16);
+    }
     wave_lds_sync();
     // G_p[r][j] = sum_c L[r][c] conj(S_p[j][c]):  A = L, B[c][j] = conj(S_p[j][c])
     f32x4 gr = {0.f, 0.f, 0.f, 0.f}, gi = {0.f, 0.f, 0.f, 0.f};
