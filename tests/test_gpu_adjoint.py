@@ -357,7 +357,8 @@ def test_fused_gram_loss_matches_library_gemm(rows, cols, S, B, rng):
 @pytest.mark.parametrize("rows,cols,S,segs,mode", [
     (4, 16, 256, 8, "gram"), (4, 16, 40, 2, "transfer"), (2, 7, 9, 3, "gram"), (3, 6, 20, 4, "gram"),
     (5, 5, 40, 2, "transfer"), (1, 9, 5, 4, "gram"), (4, 6, 30, 5, "gram"), (3, 11, 24, 10, "gram"),
-    (4, 20, 40, 15, "gram"), (4, 20, 40, 16, "transfer")])
+    (4, 20, 40, 15, "gram"), (4, 20, 40, 16, "transfer"), (4, 16, 64, 4, "gram"),
+    (4, 20, 40, 4, "transfer")])
 def test_segmented_adjoint_matches_unsegmented(rows, cols, S, segs, mode, rng):
     """The time-segmented transfer-matrix adjoint (segments' transfer matrices
     computed concurrently, chained per parameter set) gives the transfer
@@ -428,7 +429,8 @@ def test_adjoint_checkpoint_pairs(rows, cols, mode, S, B, rng):
 
 @pytest.mark.parametrize("rows,cols,w,S,mode,segs", [
     (3, 5, 5, 16, "transfer", 1), (2, 6, 4, 9, "gram", 1), (3, 5, 5, 4, "direct", 1),
-    (3, 6, 5, 20, "gram", 3), (2, 7, 4, 9, "transfer", 2)])
+    (3, 6, 5, 20, "gram", 3), (2, 7, 4, 9, "transfer", 2), (3, 8, 5, 20, "gram", 4),
+    (3, 8, 5, 12, "transfer", 4)])
 def test_adjoint_table_path_wider_window(rows, cols, w, S, mode, segs, rng):
     """Grids on a window above the minimum are not periodic: the adjoint
     kernels (single sweep and time-segmented) read the pair coefficients from
