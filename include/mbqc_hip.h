@@ -9,6 +9,10 @@
  * and the message is available from mbqc_last_error() (thread-local).  A
  * forward, density-matrix or sampled-outcome launch over B = 0 angle vectors
  * does nothing and returns MBQC_OK; its data pointers may then be null.
+ * Launches whose live window is beyond the LDS tiers allocate their workspace
+ * in stream order on every call; on a stream under HIP graph capture they
+ * return MBQC_ERR_NOT_IMPLEMENTED before recording anything (replays of a
+ * recorded stream-ordered allocation were measured to read a stale workspace).
  *
  * The reference functions each entry point replaces are cited below as
  * file:line into mentpy 0.1.0a9.

@@ -3377,6 +3377,28 @@ constexpr int kMaxAdjGlobalSlots = 16;  // adjoint: (2 + 16/2) x 2^16 amplitudes
 constexpr int kMaxDmGlobalSlots = 10;   // 4^10 entries: 8 MiB per simulation
 constexpr size_t kGlobalWorkspaceBytes = size_t(4) << 30;
 
+// Stream-ordered workspace of the launches beyond the LDS.  Under stream
+// capture a HIP graph records it as allocation and free nodes, and replays of
+// such a graph were measured to hand the kernels a stale workspace (the first
+// replay right, later ones wrong), so these launches refuse capture.
+hipError_t global_ws_alloc(void** ws, size_t bytes, hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  hipError_t e = hipStreamIsCapturing(st, &cs);
+  if (e != hipSuccess) return e;
+  if (cs != hipStreamCaptureStatusNone) return hipErrorStreamCaptureUnsupported;
+  return hipMallocAsync(ws, bytes, st);
+}
+
+int hip_fail(hipError_t e, const char* what);
+
+int global_fail(hipError_t e, const char* what) {
+  if (e == hipErrorStreamCaptureUnsupported)
+    return fail(MBQC_ERR_NOT_IMPLEMENTED,
+                std::string(what) + ": live windows beyond the LDS use a workspace allocated per "
+                                    "call and cannot be captured in a HIP graph");
+  return hip_fail(e, what);
+}
+
 // Launch `kernel_chunk(ws, g0, n)` over the job's simulations in chunks whose
 // states (state_bytes each) fit the workspace.
 template <typename F>
@@ -3385,7 +3407,7 @@ hipError_t run_global_chunks(const Job& J, size_t state_bytes, hipStream_t st, F
       1, std::min<int64_t>(int64_t(kGlobalWorkspaceBytes / state_bytes), 0x7fffffff));
   const int64_t n0 = std::min<int64_t>(per, J.total);
   void* ws = nullptr;
-  hipError_t e = hipMallocAsync(&ws, size_t(n0) * state_bytes, st);
+  hipError_t e = global_ws_alloc(&ws, size_t(n0) * state_bytes, st);
   if (e != hipSuccess) return e;
   for (int64_t g0 = 0; g0 < J.total && e == hipSuccess; g0 += n0) {
     kernel_chunk(static_cast<float2*>(ws), g0, std::min<int64_t>(n0, J.total - g0));
@@ -3690,7 +3712,7 @@ hipError_t launch_adj_bwd_global(const Job& J, const float2* ckpt, const float2*
       1, std::min<int64_t>(int64_t(kGlobalWorkspaceBytes / row_bytes), 0x7fffffff));
   const int64_t n0 = std::min<int64_t>(per, rows);
   void* ws = nullptr;
-  hipError_t e = hipMallocAsync(&ws, size_t(n0) * row_bytes, st);
+  hipError_t e = global_ws_alloc(&ws, size_t(n0) * row_bytes, st);
   if (e != hipSuccess) return e;
   for (int64_t r0 = 0; r0 < rows && e == hipSuccess; r0 += n0) {
     const int64_t n = std::min<int64_t>(n0, rows - r0);
@@ -3973,7 +3995,7 @@ int dm_forward(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
   if (p->ns > kMaxDmLdsSlots) {
     // density matrices beyond the LDS: the general kernel on a global workspace
     e = launch_dm_global(J, nz, flags, outcomes, st);
-    return e == hipSuccess ? MBQC_OK : hip_fail(e, "dm_forward launch (global workspace)");
+    return e == hipSuccess ? MBQC_OK : global_fail(e, "dm_forward launch (global workspace)");
   }
   // other planes, per-step probabilities or a measurement order off the
   // slot rotation: the kernel that reads each step's slot and plane
@@ -4086,7 +4108,7 @@ hipError_t launch_batch_global_group(int64_t waves, const BatchGroupLaunch& L) {
   const int64_t cap = std::max<int64_t>(1, int64_t(kGlobalWorkspaceBytes / state_bytes));
   const int64_t wgs = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(waves / 4, L.n_items), cap));
   void* ws = nullptr;
-  hipError_t e = hipMallocAsync(&ws, size_t(wgs) * state_bytes, L.stream);
+  hipError_t e = global_ws_alloc(&ws, size_t(wgs) * state_bytes, L.stream);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL((k_batch_global<NS>), dim3(unsigned(wgs)), dim3(256), 0, L.stream, L.jobs,
                      L.items, L.n_items, L.counter, L.flags, static_cast<float2*>(ws));
@@ -4188,7 +4210,7 @@ static int batch_launch(mbqc_batch* bt, const int32_t* group_waves, int32_t wg_w
       if (e == hipSuccess) e = hipStreamWaitEvent(st, bt->join[ns], 0);
     }
   }
-  return e == hipSuccess ? MBQC_OK : hip_fail(e, "batch launch");
+  return e == hipSuccess ? MBQC_OK : global_fail(e, "batch launch");
 }
 
 extern "C" {
@@ -4237,7 +4259,7 @@ int mbqc_sv_forward_c64(const mbqc_plan* plan, const float* angles, int64_t angl
       case 19: e = launch_sv_global<19>(J, flags, st); break;
       case 20: e = launch_sv_global<20>(J, flags, st); break;
     }
-    return e == hipSuccess ? MBQC_OK : hip_fail(e, "sv_forward launch (global workspace)");
+    return e == hipSuccess ? MBQC_OK : global_fail(e, "sv_forward launch (global workspace)");
   }
   if (!p->rotating) {
     // measured slots off the rotation: the LDS kernel reads them per step
@@ -4320,7 +4342,7 @@ int mbqc_sv_sample_c64(const mbqc_plan* plan, const float* angles, int64_t angle
     case 20: e = launch_sample_global<20>(J, tab, o, outcomes, words, seed, st); break;
     default: return fail(MBQC_ERR_NOT_IMPLEMENTED, "sampled outcomes need n_slots <= 20");
   }
-  return e == hipSuccess ? MBQC_OK : hip_fail(e, "sv_sample launch");
+  return e == hipSuccess ? MBQC_OK : global_fail(e, "sv_sample launch");
 }
 
 // Shared set-up of the window entry points.
@@ -4363,7 +4385,7 @@ int mbqc_sv_window_c64(const mbqc_plan* plan, const float* step_angles, int64_t 
       hipLaunchKernelGGL(k_sv_window_global, dim3(unsigned(n)), dim3(256), 0, gst, J, step_angles,
                          step_stride, k, map, flags, ws, g0);
     });
-    return e == hipSuccess ? MBQC_OK : hip_fail(e, "sv_window launch (global workspace)");
+    return e == hipSuccess ? MBQC_OK : global_fail(e, "sv_window launch (global workspace)");
   }
   const size_t lds = sizeof(float2) << p->ns;
   if (int rl = allow_lds(reinterpret_cast<const void*>(&k_sv_window), lds); rl != MBQC_OK) return rl;
@@ -4398,7 +4420,7 @@ int mbqc_dm_window_c64(const mbqc_plan* plan, const float* step_angles, int64_t 
         hipLaunchKernelGGL(k_dm_window_global, dim3(unsigned(n)), dim3(256), 0, gst, J,
                            step_angles, step_stride, k, map, nz, ws, g0);
     });
-    return e == hipSuccess ? MBQC_OK : hip_fail(e, "dm_window launch (global workspace)");
+    return e == hipSuccess ? MBQC_OK : global_fail(e, "dm_window launch (global workspace)");
   }
   if (J.total > 0x7fffffff) return fail(MBQC_ERR_VALUE, "too many simulations for one launch");
   const size_t lds = sizeof(float2) << (2 * p->ns);
@@ -4551,7 +4573,7 @@ int mbqc_sv_adjoint_forward_c64(const mbqc_plan* plan, const float* angles, int6
     case 15: e = launch_adj_fwd_global<15>(J, ck, adj_ckpt_blocks(B * C), st); break;
     case 16: e = launch_adj_fwd_global<16>(J, ck, adj_ckpt_blocks(B * C), st); break;
   }
-  return e == hipSuccess ? MBQC_OK : hip_fail(e, "adjoint forward launch");
+  return e == hipSuccess ? MBQC_OK : global_fail(e, "adjoint forward launch");
 }
 
 int mbqc_sv_adjoint_backward_c64(const mbqc_plan* plan, const float* angles, int64_t angle_stride,
@@ -4606,7 +4628,7 @@ int mbqc_sv_adjoint_backward_c64(const mbqc_plan* plan, const float* angles, int
     case 15: e = launch_adj_bwd_global<15>(J, ck, sd, A, st); break;
     case 16: e = launch_adj_bwd_global<16>(J, ck, sd, A, st); break;
   }
-  return e == hipSuccess ? MBQC_OK : hip_fail(e, "adjoint backward launch");
+  return e == hipSuccess ? MBQC_OK : global_fail(e, "adjoint backward launch");
 }
 
 int mbqc_batch_create(const mbqc_batch_job* jobs, int32_t n_jobs, int32_t sims_per_item,

@@ -446,6 +446,37 @@ def test_fresh_plan_first_launch_inside_graph_capture(rng):
         _ = (want * 2).sum().item()          # eager work between replays
 
 
+def test_global_workspace_launches_refuse_graph_capture(rng):
+    """Windows beyond the LDS allocate their workspace per call, in stream
+    order; replays of a graph that recorded such an allocation were measured
+    to compute with a stale workspace after the first replay, so these launches
+    refuse stream capture with NotImplementedError (before any work is
+    recorded), and the same call runs eagerly afterwards."""
+    torch = _torch()
+    import mentpy_amd as mp
+    from mentpy_amd.simulators.hip_simulator import _as_angles, _as_states, dm_forward, sv_forward
+    for rows, backend in ((15, "hip-sv"), (8, "hip-dm")):
+        g = mp.templates.grid_cluster(rows, 3)
+        core = mp.PatternSimulator(g, backend=backend).simulator
+        th = _as_angles(rng.uniform(0, 2 * np.pi, (3, len(g.trainable_nodes))),
+                        core.plan.info["n_angles"], core.device)
+        st = _as_states(haar_state(rows, rng)[None], rows, core.device)
+        run = (lambda c=core, a=th, s=st: sv_forward(c.plan, a, s)) if backend == "hip-sv" else \
+              (lambda c=core, a=th, s=st: dm_forward(c.plan, a, s, noise=c.noise))
+        want = run().clone()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        graph = torch.cuda.CUDAGraph()
+        with pytest.raises(NotImplementedError, match="cannot be captured"):
+            with torch.cuda.stream(side):
+                with torch.cuda.graph(graph):
+                    run()
+        torch.cuda.synchronize()
+        again = run()
+        torch.cuda.synchronize()
+        assert torch.equal(again, want), f"{rows} slots: eager call after the refused capture"
+
+
 def test_sv_14_slot_measure_windows_end_at_run_output(rng):
     """Step-by-step measure() on a 14-slot window (the window kernel's state
     takes 128 KB of dynamic LDS): after the last measurement the window is the
