@@ -9,10 +9,12 @@
  * and the message is available from mbqc_last_error() (thread-local).  A
  * forward, density-matrix or sampled-outcome launch over B = 0 angle vectors
  * does nothing and returns MBQC_OK; its data pointers may then be null.
- * Launches whose live window is beyond the LDS tiers allocate their workspace
- * in stream order on every call; on a stream under HIP graph capture they
- * return MBQC_ERR_NOT_IMPLEMENTED before recording anything (replays of a
- * recorded stream-ordered allocation were measured to read a stale workspace).
+ * Single-plan launches whose live window is beyond the LDS tiers allocate
+ * their workspace in stream order on every call; on a stream under HIP graph
+ * capture they return MBQC_ERR_NOT_IMPLEMENTED before recording anything
+ * (replays of a recorded stream-ordered allocation were measured to read a
+ * stale workspace).  Batch launches use workspaces owned by the batch and can
+ * be captured at every window size.
  *
  * The reference functions each entry point replaces are cited below as
  * file:line into mentpy 0.1.0a9.
@@ -348,8 +350,11 @@ int mbqc_batch_launch(mbqc_batch* batch, int32_t waves, int32_t wg_waves, uint32
  * simulation per lane, 7..14 slots one simulation at a time per workgroup
  * with the state in LDS (2^(n_slots-1) threads, at most 256; its resident
  * waves make waves / (threads / 64) workgroups), 15..20 the same with the
- * states in a workspace allocated and freed in stream order (256 threads,
- * at most 4 GiB of states). */
+ * states in a workspace that mbqc_batch_create allocates with the batch
+ * (256 threads, at most 256 resident workgroups per group: up to 256 * 2^n_slots
+ * complex64, 2 GiB for a 20-slot group), so every batch launch allocates
+ * nothing and can be captured.  Launches of one batch must not overlap (they
+ * share its counters and workspaces). */
 int64_t mbqc_batch_group_items(const mbqc_batch* batch, int32_t n_slots);
 /* mbqc_batch_launch with the resident wavefront count chosen per group:
  * group_waves[ns] (n_entries <= 21 entries, indexed by live-window size) for

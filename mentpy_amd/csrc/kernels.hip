@@ -4103,19 +4103,21 @@ void launch_batch_lds_group(int64_t waves, const BatchGroupLaunch& L) {
 // one, at most one per item, at most kGlobalWorkspaceBytes of states), each
 // with its slice of a workspace allocated and freed in stream order.
 template <int NS>
-hipError_t launch_batch_global_group(int64_t waves, const BatchGroupLaunch& L) {
-  const size_t state_bytes = sizeof(float2) << NS;
-  const int64_t cap = std::max<int64_t>(1, int64_t(kGlobalWorkspaceBytes / state_bytes));
-  const int64_t wgs = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(waves / 4, L.n_items), cap));
-  void* ws = nullptr;
-  hipError_t e = global_ws_alloc(&ws, size_t(wgs) * state_bytes, L.stream);
-  if (e != hipSuccess) return e;
+hipError_t launch_batch_global_group(int64_t waves, const BatchGroupLaunch& L, void* ws,
+                                     int64_t ws_wgs) {
+  // the batch's own workspace (allocated with the batch, so that a launch
+  // allocates nothing and can be captured in a HIP graph): one state per
+  // resident workgroup
+  const int64_t wgs =
+      std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(waves / 4, L.n_items), ws_wgs));
   hipLaunchKernelGGL((k_batch_global<NS>), dim3(unsigned(wgs)), dim3(256), 0, L.stream, L.jobs,
                      L.items, L.n_items, L.counter, L.flags, static_cast<float2*>(ws));
-  e = hipGetLastError();
-  const hipError_t ef = hipFreeAsync(ws, L.stream);
-  return e != hipSuccess ? e : ef;
+  return hipGetLastError();
 }
+
+// Resident workgroups of a batch group of 15..20 slots (one per CU); the batch
+// allocates its workspace for this many states per group at creation.
+constexpr int64_t kBatchGlobalWgs = 256;
 
 }  // namespace
 
@@ -4139,6 +4141,9 @@ struct mbqc_batch {
   bool share[kMaxBatchSlots + 1] = {};
   // every plan of the group is periodic (Plan::periodic)
   bool periodic[kMaxBatchSlots + 1] = {};
+  // groups of 15..20 slots: the states of up to gws_wgs[ns] resident workgroups
+  void* gws[kMaxBatchSlots + 1] = {};
+  int64_t gws_wgs[kMaxBatchSlots + 1] = {};
 };
 
 // One launch of a batch: the counter reset, then one persistent kernel per
@@ -4197,12 +4202,12 @@ static int batch_launch(mbqc_batch* bt, const int32_t* group_waves, int32_t wg_w
       case 12: launch_batch_lds_group<12>(group_waves[ns], L); break;
       case 13: launch_batch_lds_group<13>(group_waves[ns], L); break;
       case 14: launch_batch_lds_group<14>(group_waves[ns], L); break;
-      case 15: e = launch_batch_global_group<15>(group_waves[ns], L); break;
-      case 16: e = launch_batch_global_group<16>(group_waves[ns], L); break;
-      case 17: e = launch_batch_global_group<17>(group_waves[ns], L); break;
-      case 18: e = launch_batch_global_group<18>(group_waves[ns], L); break;
-      case 19: e = launch_batch_global_group<19>(group_waves[ns], L); break;
-      default: e = launch_batch_global_group<20>(group_waves[ns], L); break;
+      case 15: e = launch_batch_global_group<15>(group_waves[ns], L, bt->gws[ns], bt->gws_wgs[ns]); break;
+      case 16: e = launch_batch_global_group<16>(group_waves[ns], L, bt->gws[ns], bt->gws_wgs[ns]); break;
+      case 17: e = launch_batch_global_group<17>(group_waves[ns], L, bt->gws[ns], bt->gws_wgs[ns]); break;
+      case 18: e = launch_batch_global_group<18>(group_waves[ns], L, bt->gws[ns], bt->gws_wgs[ns]); break;
+      case 19: e = launch_batch_global_group<19>(group_waves[ns], L, bt->gws[ns], bt->gws_wgs[ns]); break;
+      default: e = launch_batch_global_group<20>(group_waves[ns], L, bt->gws[ns], bt->gws_wgs[ns]); break;
     }
     if (e == hipSuccess) e = hipGetLastError();
     if (e == hipSuccess && concurrent) {
@@ -4210,7 +4215,7 @@ static int batch_launch(mbqc_batch* bt, const int32_t* group_waves, int32_t wg_w
       if (e == hipSuccess) e = hipStreamWaitEvent(st, bt->join[ns], 0);
     }
   }
-  return e == hipSuccess ? MBQC_OK : global_fail(e, "batch launch");
+  return e == hipSuccess ? MBQC_OK : hip_fail(e, "batch launch");
 }
 
 extern "C" {
@@ -4708,6 +4713,13 @@ int mbqc_batch_create(const mbqc_batch_job* jobs, int32_t n_jobs, int32_t sims_p
   if (e == hipSuccess && !bt->items.empty())
     e = hipMalloc(&bt->d_items, sizeof(BatchItem) * bt->items.size());
   if (e == hipSuccess) e = hipMalloc(&bt->d_counter, sizeof(int) * kCounterStride * (kMaxBatchSlots + 1));
+  for (int ns = kMaxSvLdsSlots + 1; ns <= kMaxBatchSlots && e == hipSuccess; ++ns) {
+    if (bt->group_count[ns] == 0) continue;
+    const size_t state_bytes = sizeof(float2) << ns;
+    bt->gws_wgs[ns] = std::min<int64_t>(std::min<int64_t>(bt->group_count[ns], kBatchGlobalWgs),
+                                        int64_t(kGlobalWorkspaceBytes / state_bytes));
+    e = hipMalloc(&bt->gws[ns], size_t(bt->gws_wgs[ns]) * state_bytes);
+  }
   int n_groups = 0;
   for (int ns = 1; ns <= kMaxBatchSlots; ++ns) n_groups += bt->group_count[ns] > 0;
   if (e == hipSuccess && n_groups > 1) {
@@ -4737,6 +4749,7 @@ void mbqc_batch_destroy(mbqc_batch* bt) {
   if (bt->d_items) (void)hipFree(bt->d_items);
   if (bt->d_counter) (void)hipFree(bt->d_counter);
   for (int ns = 0; ns <= kMaxBatchSlots; ++ns) {
+    if (bt->gws[ns]) (void)hipFree(bt->gws[ns]);
     if (bt->side[ns]) (void)hipStreamDestroy(bt->side[ns]);
     if (bt->join[ns]) (void)hipEventDestroy(bt->join[ns]);
   }

@@ -155,7 +155,8 @@ def test_batched_group_waves(rng):
 
 def test_batched_lds_groups(rng):
     """Groups of 7..14 live slots run persistent workgroups with the state in
-    LDS (15..20: in a stream-ordered global workspace), concurrently with the
+    LDS (15..20: in a global workspace the batch allocates when it is created,
+    so its launches stay capturable), concurrently with the
     register-kernel groups: every job equals its
     single launch bit for bit (the same LDS simulation code), a sample matches
     the oracle, every group sizing gives the same outputs, and a graph replay
