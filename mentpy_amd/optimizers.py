@@ -193,7 +193,10 @@ class MBQMLTrainer:
 
     def capture(self):
         """Capture one iteration in a HIP graph (the parameters advance by one
-        optimizer step during capture warm-up; use a fresh trainer to time)."""
+        optimizer step during capture warm-up; use a fresh trainer to time).
+        Live windows of 11..16 qubits allocate the adjoint's workspace on every
+        call, which graph capture refuses (NotImplementedError): run those
+        without a graph."""
         import torch
         self._iteration()
         torch.cuda.synchronize(self.params.device)
