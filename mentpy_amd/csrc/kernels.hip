@@ -184,6 +184,9 @@ struct Job {
   int32_t off_in, off_sign, off_out, off_svres, off_dmres, e_sv, e_dm;
   int32_t ns, n_steps, n_out, live_mask, off_coef;
   int32_t off_aidx, off_run;  // contiguous angle indices, stretch ends
+  // 1: no step has a fixed angle and every angle offset angle_index *
+  // angle_stride_a fits in 32 bits (the statevector kernels' short fetch)
+  int32_t fast_fetch;
   const float* angles;
   int64_t angle_stride;     // elements between angle vectors (0: shared)
   int64_t angle_stride_a;   // elements between the angles of one vector
@@ -353,10 +356,14 @@ __device__ __forceinline__ void sv_fetched_blocks(float2 (&t)[1 << NS], const mb
 // The angle indices come from the plan's contiguous, padded index table
 // (aidx, M <= kAngleIdxPad entries past the last step at most), so they are
 // adjacent scalar loads.
+// Jobs with Job::fast_fetch (no fixed angle, 32-bit angle offsets: the usual
+// case) take one uniform branch per fetch to loads whose addresses are a
+// 32-bit scalar product added to the lane's angle vector; otherwise each step
+// chooses its source (a uniform branch and a 64-bit product per step).
 template <int M>
 __device__ __forceinline__ void fetch_step_angles(const mbqc_step* steps, const int32_t* aidx,
                                                   int k0, int n_steps, const float* ang,
-                                                  int64_t sa, float (&th)[M]) {
+                                                  int64_t sa, bool fast, float (&th)[M]) {
   static_assert(M <= kAngleIdxPad, "index table padding");
   // angles live in device memory: a global-typed pointer keeps the loads
   // global (a descriptor read from memory otherwise leaves them flat)
@@ -365,6 +372,11 @@ __device__ __forceinline__ void fetch_step_angles(const mbqc_step* steps, const 
   int ai[M];
 #pragma unroll
   for (int m = 0; m < M; ++m) ai[m] = aidx[k0 + m];
+  if (fast) {
+#pragma unroll
+    for (int m = 0; m < M; ++m) th[m] = gang[uint32_t(ai[m]) * uint32_t(sa)];
+    return;
+  }
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     const int k = k0 + m < n_steps ? k0 + m : n_steps - 1;
@@ -406,7 +418,8 @@ __device__ __forceinline__ void sv_run_blocks(float2 (&t)[1 << NS], const Job& J
   // th_next holds the angles of blocks blk .. blk + K - 1
   float th_next[K * NS];
   const int64_t sa = J.angle_stride_a;
-  fetch_step_angles<K * NS>(steps, aidx, blk0 * NS, n, ang, sa, th_next);
+  const bool fast = J.fast_fetch != 0;
+  fetch_step_angles<K * NS>(steps, aidx, blk0 * NS, n, ang, sa, fast, th_next);
   int blk = blk0;
   while (blk < full_end) {
     const int re = run_end[blk];
@@ -419,7 +432,7 @@ __device__ __forceinline__ void sv_run_blocks(float2 (&t)[1 << NS], const Job& J
       float th[K * NS];
 #pragma unroll
       for (int m = 0; m < K * NS; ++m) th[m] = th_next[m];
-      fetch_step_angles<K * NS>(steps, aidx, (blk + K) * NS, n, ang, sa, th_next);
+      fetch_step_angles<K * NS>(steps, aidx, (blk + K) * NS, n, ang, sa, fast, th_next);
       if constexpr (SH && kShareGroup<NS> == 16) {
         sv_fetched_blocks<NS, K, REF, PER_STEP, PERIODIC>(t, steps, coef, blk, n, th, ph, on_block,
                                                           mod, std::make_integer_sequence<int, K>{});
@@ -451,7 +464,7 @@ __device__ __forceinline__ void sv_run_blocks(float2 (&t)[1 << NS], const Job& J
           }
         }
         blk = end;
-        fetch_step_angles<K * NS>(steps, aidx, blk * NS, n, ang, sa, th_next);
+        fetch_step_angles<K * NS>(steps, aidx, blk * NS, n, ang, sa, fast, th_next);
       }
     }
   }
@@ -3202,6 +3215,7 @@ struct GlobalJob {
   int32_t off_in, off_sign, off_out, off_svres, off_dmres, e_sv, e_dm;
   int32_t ns, n_steps, n_out, live_mask, off_coef;
   int32_t off_aidx, off_run;
+  int32_t fast_fetch;
   const __attribute__((address_space(1))) float* angles;
   int64_t angle_stride;
   int64_t angle_stride_a;
@@ -3231,6 +3245,7 @@ __device__ __forceinline__ Job load_job(const Job* jobs, int i) {
   j.off_coef = g.off_coef;
   j.off_aidx = g.off_aidx;
   j.off_run = g.off_run;
+  j.fast_fetch = g.fast_fetch;
   j.angles = (const float*)g.angles;
   j.angle_stride = g.angle_stride;
   j.angle_stride_a = g.angle_stride_a;
@@ -3554,6 +3569,10 @@ int make_job(Plan* p, const float* angles, int64_t angle_stride, int64_t angle_s
   j.angles = angles;
   j.angle_stride = angle_stride;
   j.angle_stride_a = angle_stride_a;
+  bool fixed = false;
+  for (const mbqc_step& st : p->steps) fixed = fixed || st.angle_index < 0;
+  j.fast_fetch = !fixed && p->n_angles > 0 &&
+                 uint64_t(p->n_angles - 1) * uint64_t(angle_stride_a) <= 0xffffffffull;
   j.psi = static_cast<const float2*>(psi);
   j.in_stride = in_stride;
   j.out = static_cast<float2*>(out);
