@@ -2162,10 +2162,11 @@ __global__ __launch_bounds__(64) void k_sv_seg_forward(Job J, SegInfo G, float2*
   const int b0 = G.blk[p];
   sv_run_blocks<NS, false, false, false, PERIODIC>(t, J, ang, &ph, b0, G.blk[p + 1],
                                                    [&](int blk, const float2 (&tt)[H]) {
-    if (active) {
+    // every lane stores (inactive lanes, l < Np, into the padding of the
+    // segment's checkpoint rows): a store branch that can be skipped would
+    // make the wait for the next block's angle loads wait for these stores
 #pragma unroll
-      for (int j = 0; j < H; ++j) ck[(int64_t(blk - b0) * H + j) * Np + l] = tt[j];
-    }
+    for (int j = 0; j < H; ++j) ck[(int64_t(blk - b0) * H + j) * Np + l] = tt[j];
   });
   // the wave's rows (64 consecutive lanes of the segment) leave through its
   // LDS column tile
