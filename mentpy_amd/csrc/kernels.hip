@@ -399,7 +399,7 @@ constexpr int kFetchBlocks = NS <= 2 ? 4 : (NS == 3 ? 2 : 1);
 // per block).  PERIODIC: the full blocks take the compile-time coefficients
 // of a periodic plan (Plan::periodic), the tail block the table.
 template <int NS, bool REF, bool PER_STEP, bool SH = false, bool PERIODIC = false,
-          typename OnBlock, typename Mod = NoModify>
+          bool SETTLE = true, typename OnBlock, typename Mod = NoModify>
 __device__ __forceinline__ void sv_run_blocks(float2 (&t)[1 << NS], const Job& J, const float* ang,
                                               RefPhase* ph, int blk0, int blk1, OnBlock on_block,
                                               Mod&& mod = Mod()) {
@@ -422,6 +422,18 @@ __device__ __forceinline__ void sv_run_blocks(float2 (&t)[1 << NS], const Job& J
   fetch_step_angles<K * NS>(steps, aidx, blk0 * NS, n, ang, sa, fast, th_next);
   int blk = blk0;
   while (blk < full_end) {
+    // SETTLE: the angles in flight enter the block loop through an empty
+    // asm that reads and writes them.  The compiler places its memory-counter
+    // waits per loop header, and angles arriving there straight from loads
+    // (the first fetch) would put waits inside the loop body, where they also
+    // wait for the loads just issued for the next fetch; with the asm the
+    // wait is once per stretch of blocks (4x20 forward: 0.344 -> 0.332 and
+    // 0.348 -> 0.343 ms per step on two boxes).  The batched kernel, which
+    // starts a stretch per simulation, runs without it.
+    if constexpr (SETTLE) {
+#pragma unroll
+      for (int m = 0; m < K * NS; ++m) asm("" : "+v"(th_next[m]));
+    }
     const int re = run_end[blk];
     const int end = re < full_end ? re : full_end;
     if (resc[blk]) {
@@ -582,7 +594,7 @@ enum StoreMode {
 // rows_valid are written (kStoreDirect: active lanes write).  Inactive lanes
 // run on simulation 0.
 template <int NS, bool REF, int STORE, bool PER_STEP = false, bool SH = false,
-          bool PERIODIC = false>
+          bool PERIODIC = false, bool SETTLE = true>
 __device__ __forceinline__ void sv_sim(const Job& J, int64_t g, bool active, int rows_valid,
                                        uint32_t flags) {
   constexpr int H = 1 << NS;
@@ -592,7 +604,8 @@ __device__ __forceinline__ void sv_sim(const Job& J, int64_t g, bool active, int
   float2 t[H];
   sv_load_input<NS>(J, J.psi + si * J.in_stride, t);
   RefPhase rp;
-  sv_run_blocks<NS, REF, PER_STEP, SH, PERIODIC>(t, J, ang, &rp, 0, (J.n_steps + NS - 1) / NS,
+  sv_run_blocks<NS, REF, PER_STEP, SH, PERIODIC, SETTLE>(t, J, ang, &rp, 0,
+                                                         (J.n_steps + NS - 1) / NS,
                                        [](int, const float2 (&)[H]) {});
   // with shared coefficients each lane of a group added the phases of its own steps
   if constexpr (REF && SH) group_reduce_phase<kShareGroup<NS>>(rp);
@@ -3283,7 +3296,7 @@ __global__ __launch_bounds__(256) void k_batch(const Job* __restrict__ jobs,
       const int rows = int(std::min<int64_t>(std::min<int64_t>(I.count - off, left), 64));
       // direct stores: staging tiles would cost the concurrently running
       // window-size groups LDS occupancy (measured slower)
-      sv_sim<NS, REF, kBatchStore<NS>, false, SH, PERIODIC>(
+      sv_sim<NS, REF, kBatchStore<NS>, false, SH, PERIODIC, false>(
           J, g, (off + lane) < I.count && g < J.total, rows, flags);
     }
   }
