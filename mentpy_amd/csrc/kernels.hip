@@ -1813,10 +1813,13 @@ __global__ __launch_bounds__(64) void k_sv_adj_backward(Job J, const float2* __r
   const float scale = sv_final_scale(J.e_sv);
   float2 lam[H];
   const float2* sd = seed + gg * (int64_t(1) << J.n_out);
+  // every lane loads (inactive lanes simulation 0's seed) and a select zeroes
+  // what is not used: loads under a lane guard each wait for their own round trip
 #pragma unroll
   for (int j = 0; j < H; ++j) {
     const int p = out_pos[j];
-    lam[j] = (active && p >= 0) ? cscale(sd[p], scale) : make_float2(0.f, 0.f);
+    const float2 v = cscale(sd[p >= 0 ? p : 0], scale);
+    lam[j] = (active && p >= 0) ? v : make_float2(0.f, 0.f);
   }
   const int n = J.n_steps;
   const int nblk = (n + NS - 1) / NS;
@@ -2351,6 +2354,9 @@ __global__ __launch_bounds__(64) void k_sv_seg_backward(Job J, SegInfo G, const 
   const mbqc_step* steps = reinterpret_cast<const mbqc_step*>(J.tab);
   const int32_t* resc = J.tab + J.off_svres;
   const float* coef = reinterpret_cast<const float*>(J.tab + J.off_coef);
+  // The seeds are loaded by every lane (inactive lanes read lane 0's) and
+  // zeroed by a select: loads under a lane guard each wait for their own
+  // round trip.
   float2 lam[H];
   if (p + 1 == G.n) {
     const int32_t* out_pos = J.tab + J.off_out;
@@ -2359,12 +2365,16 @@ __global__ __launch_bounds__(64) void k_sv_seg_backward(Job J, SegInfo G, const 
 #pragma unroll
     for (int j = 0; j < H; ++j) {
       const int q = out_pos[j];
-      lam[j] = (active && q >= 0) ? cscale(sd[q], scale) : make_float2(0.f, 0.f);
+      const float2 v = cscale(sd[q >= 0 ? q : 0], scale);
+      lam[j] = (active && q >= 0) ? v : make_float2(0.f, 0.f);
     }
   } else {
     const float2* sd = ws + G.gmat[p] + ll * H;
 #pragma unroll
-    for (int j = 0; j < H; ++j) lam[j] = active ? sd[j] : make_float2(0.f, 0.f);
+    for (int j = 0; j < H; ++j) {
+      const float2 v = sd[j];
+      lam[j] = active ? v : make_float2(0.f, 0.f);
+    }
   }
   AdjCtx A{};
   A.red = C;
